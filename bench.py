@@ -1,0 +1,210 @@
+#!/usr/bin/env python
+"""J+K Fock-build benchmark on one MI355X.
+
+Workload (default): 64 H2O on a 4x4x4 grid (3.0 A spacing, seeded random
+orientations), cc-pVDZ (spherical d), 1536 basis functions, a seeded
+S-orthonormal density of 320 occupied orbitals (D = 2 C C^T), int2e_cutoff
+5e-11.  One step = one J+K build (G = J - K/2 for that density) with the
+density and result resident in HBM.  The plain FP64 build and the mixed
+build (FP32 for quartets whose Schwarz x density bound is below the FP32
+cutoff, FP64 accumulation) are both timed; ``value`` is the mixed rate when
+its energy error |dE2| = |0.5 Tr D (G_mixed - G_fp64)| is within 1 uEh,
+otherwise the FP64 rate.
+
+Prints one JSON line (rank 0, N = 1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "J+K Fock builds/s, mixed vs FP64 GPU, at |ΔE|≤1 µEh; gradient evals/s"
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) dense peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector dense peak (packed)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def trace_energy(dp, fp, nbf):
+    """0.5 Tr(D G) for packed symmetric D, G: 2*dot - diagonal (mathlib.F90:69-85)."""
+    idx = np.arange(1, nbf + 1)
+    diag = idx * (idx + 1) // 2 - 1
+    return 0.5 * (2.0 * np.dot(dp, fp) - np.dot(dp[diag], fp[diag]))
+
+
+def cpu_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name"):
+                return ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(bas, dp, gpu_quartets, target_s=15.0):
+    """Time the CPU FP64 restatement (oracle) on OpenQP's round-robin 1/size
+    share of shell pairs (int2.F90:759-761) and scale to builds/s."""
+    from oracle import oracle as O
+    Q = O.schwarz(bas)
+    size = 4096
+    t0 = time.perf_counter()
+    _, st = O.fock_jk(bas, dp, Q=Q, rank=0, size=size)
+    dt = time.perf_counter() - t0
+    if dt < target_s / 4:
+        size = max(1, int(size * dt / target_s))
+        t0 = time.perf_counter()
+        _, st = O.fock_jk(bas, dp, Q=Q, rank=0, size=size)
+        dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", cpu_cores()))
+    return {"value": round(1.0 / (dt * size), 6), "unit": "builds/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle FP64 J+K on shell-pair share rank 0 of {size} (int2.F90:759-761 round robin), "
+                      f"{int(st[1])} quartets in {dt:.2f} s on {threads} threads of {cpu_model()}; "
+                      f"x{size} for one build",
+            "quartets_per_s": round(int(st[1]) / dt, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="h2o64_ccpvdz")
+    ap.add_argument("--fp32-cutoff", type=float, default=1e-5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {a.gpus} requested); "
+              "the multi-GPU decomposition is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from openqp_amd import basis as B, molecules as M
+    from openqp_amd import lib as L
+
+    bld, bname, nocc = M.WORKLOADS[a.workload]
+    Z, xyz = bld(0)
+    bas = B.build_basis(bname, Z, xyz)
+    log(f"[bench] workload {a.workload}: nbf {bas.nbf}, nsh {bas.nsh}, natm {bas.natm}")
+    t0 = time.perf_counter()
+    S, _, _ = L.int1e(bas)
+    D = M.random_density(S, nocc, seed=0)
+    dp = B.pack_tri(D)
+    log(f"[bench] density ready ({time.perf_counter() - t0:.1f} s)")
+
+    dev = torch.device("cuda:0")
+    d_dev = torch.tensor(dp, dtype=torch.float64, device=dev)
+    f64 = torch.zeros_like(d_dev)
+    fmx = torch.zeros_like(d_dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    fb = L.FockBuilder(bas)
+    t0 = time.perf_counter()
+    fb.schwarz()
+    log(f"[bench] geometry setup (pairs + Schwarz) {time.perf_counter() - t0:.2f} s")
+
+    def run(fout, steps, warm, precision):
+        fb.set_precision(precision, a.fp32_cutoff)
+        for _ in range(warm):
+            fb.fock_jk_device(d_dev.data_ptr(), fout.data_ptr(), stream=stream)
+        torch.cuda.synchronize()
+        fb.set_profiling(True)
+        prof = {}
+        st_last = None
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            st_last = fb.fock_jk_device(d_dev.data_ptr(), fout.data_ptr(), stream=stream)
+            for p in fb.last_profile():
+                q = prof.setdefault(p["cls"], {"ms": 0.0, "flops": 0.0, "quartets": 0, "prims": 0, "n": 0})
+                q["ms"] += p["ms"]; q["flops"] += p["flops"]; q["quartets"] += p["quartets"]
+                q["prims"] += p["prims"]; q["n"] += 1
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        fb.set_profiling(False)
+        return dt, st_last, prof
+
+    log("[bench] FP64 builds")
+    dt64, st64, prof64 = run(f64, a.steps, a.warmup, 0)
+    log(f"[bench] FP64 {dt64 * 1e3:.1f} ms/build, quartets {st64.quartets_computed}, "
+        f"model {st64.model_flops / 1e12:.3f} TFLOP")
+    log("[bench] mixed builds")
+    dtmx, stmx, profmx = run(fmx, a.steps, a.warmup, 1)
+    log(f"[bench] mixed {dtmx * 1e3:.1f} ms/build, fp32 quartets {stmx.quartets_fp32}")
+
+    g64 = f64.cpu().numpy()
+    gmx = fmx.cpu().numpy()
+    e64 = trace_energy(dp, g64, bas.nbf)
+    emx = trace_energy(dp, gmx, bas.nbf)
+    dE = abs(emx - e64)
+    dG = float(np.abs(gmx - g64).max())
+    mixed_ok = dE <= 1e-6
+    dt = dtmx if mixed_ok else dt64
+    st = stmx if mixed_ok else st64
+    prof = profmx if mixed_ok else prof64
+
+    # dominant kernel: the class launch with the largest device time
+    dom_cls, dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    dom_ms = dom["ms"] / dom["n"]
+    dom_flops = dom["flops"] / dom["n"]
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    build_tflops = st.model_flops / (st.kernel_ms * 1e-3) / 1e12 if st.kernel_ms > 0 else 0.0
+    roof = {"bound": "valu", "kernel": f"k_jk<{dom_cls // 1000},{dom_cls // 100 % 10},{dom_cls // 10 % 10},"
+                                       f"{dom_cls % 10}>",
+            "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "launch_ms": round(dom_ms, 3), "flops_per_launch": dom_flops,
+            "build_model_tflops": round(build_tflops, 3)}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle restatement, sampled)")
+        try:
+            cpu = cpu_baseline(bas, dp, st64.quartets_computed, a.cpu_seconds)
+        except Exception as e:  # reported baseline only
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": METRIC, "value": round(1.0 / dt, 4), "unit": "builds/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if mixed_ok else "f64",
+        "data": "synthetic (seeded geometry and S-orthonormal random density)",
+        "config": {"workload": a.workload, "basis": bname, "nbf": bas.nbf, "nshell": bas.nsh, "nocc": nocc,
+                   "int2e_cutoff": 5e-11, "fp32_cutoff": a.fp32_cutoff,
+                   "precision": "mixed" if mixed_ok else "fp64"},
+        "fp64_builds_per_s": round(1.0 / dt64, 4), "mixed_builds_per_s": round(1.0 / dtmx, 4),
+        "mixed_speedup": round(dt64 / dtmx, 3), "mixed_dE2_hartree": dE, "mixed_max_dG": dG,
+        "quartets_per_build": st.quartets_computed, "quartets_per_s": round(st.quartets_computed / dt, 1),
+        "prim_quartets_per_build": st.prim_quartets, "fp32_quartets": stmx.quartets_fp32,
+        "E2_fp64_hartree": e64,
+        "gradient_evals_per_s": None,
+        "roofline": roof, "cpu_baseline": cpu,
+        "per_class_ms": {str(k): round(v["ms"] / v["n"], 3) for k, v in sorted(prof.items())},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,127 @@
+/*
+ * oqph.h -- C ABI of liboqph, the MI355X (gfx950) direct-SCF two-electron
+ * Fock builder (J/K) and two-electron nuclear gradient.
+ *
+ * Drop-in boundary: OpenQP's Route-C seam (source/modules/routec_bridge.F90)
+ * dlopens $OQP_ROUTEC_LIB / $OQP_ROUTEC_GRAD_LIB and binds routec_fock_jk and
+ * routec_grad2 by name; liboqph.so exports both with the seam's exact
+ * by-reference signatures.  The seam carries no basis, geometry or
+ * thresholds, so a session (oqph_session_*) supplies them out of band; the
+ * routec_* entry points use the process-default session.
+ *
+ * Conventions: AO order, normalisation and packing are OpenQP's
+ * (source/constants.F90:34-60, source/basis_tools.F90:277-303,
+ * source/integrals/int2.F90:1436-1447): packed lower triangle, row-packed,
+ * ij = i(i-1)/2 + j (1-based).  Densities are total RHF densities.
+ * Lengths in bohr, energies in hartree.
+ *
+ * Errors: every entry point returns (or writes to *info) 0 on success and a
+ * positive code when it declines: 1 unsupported request or no session,
+ * 2 device error, 3 out of memory, 4 invalid argument.  Nothing aborts the
+ * process; oqph_last_error() describes the last failure.
+ *
+ * Threading: a session is not re-entrant; distinct sessions may be used from
+ * distinct threads.  Each session owns one HIP stream on its device.
+ */
+#ifndef OQPH_H
+#define OQPH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Route-C drop-ins ------------------------------------------------ */
+
+/* J/K Fock build.  Replaces routec_fock_jk (source/modules/routec_bridge.F90:33-40,
+ * caller source/scf_addons.F90:1138-1146).  d, f: packed (ntri, nfocks); f is
+ * returned ready to use, i.e. f = sc*J[D] - 0.5*se*K[D] exactly as fock_jk's
+ * native path leaves it after its scaling (source/scf_addons.F90:1178-1185).
+ * nfocks must be 1 (RHF); other values decline with info = 1. */
+void routec_fock_jk(const double *d, double *f, const int *nbf, const int *nfocks, const double *se,
+                    const double *sc, int *info);
+
+/* 2e nuclear gradient.  Replaces routec_grad2 (source/modules/routec_bridge.F90:51-61,
+ * caller source/modules/hf_gradient.F90:274-290).  d: packed total density;
+ * xyz: (3, natm) bohr; de: (3, natm) receives dE2e/dR = Coulomb*coulscale +
+ * exchange*hfscale at fixed density. */
+void routec_grad2(const double *d, const double *xyz, double *de, const int *nbf, const int *natm,
+                  const double *hfscale, const double *coulscale, int *info);
+
+/* ---- Session (no counterpart in the seam, which passes no basis) ------- */
+
+typedef struct oqph_session oqph_session;
+
+typedef struct {
+  long long quartets_computed;   /* shell quartets evaluated on the device          */
+  long long quartets_screened;   /* shell quartets skipped by the density screen    */
+  long long quartets_fp32;       /* of the computed ones, evaluated in FP32          */
+  double kernel_ms;              /* device time of the build (HIP events)             */
+  double model_flops;            /* algorithmic FLOPs of the build (Rys model)        */
+  long long prim_quartets;       /* primitive quartets that passed the screens        */
+} oqph_stats;
+
+/* Create a session on HIP device `device`.  Fields mirror oqp_get_basis
+ * (source/c_interop.F90:237-280). */
+int oqph_session_create(oqph_session **s, int device);
+int oqph_session_destroy(oqph_session *s);
+
+/* Basis: nsh shells; am, harmonic (1 = pure for l >= 2), atom (0-based),
+ * nprim per shell; ex and cc concatenated per shell in shell order, cc
+ * carrying OpenQP's primitive normalisation (basis_tools.F90:277-303). */
+int oqph_session_set_basis(oqph_session *s, int nsh, const int *am, const int *harmonic, const int *atom,
+                           const int *nprim, const double *ex, const double *cc);
+/* Geometry: natm atoms, xyz (3, natm) bohr, nuclear charges. */
+int oqph_session_set_geometry(oqph_session *s, int natm, const double *xyz, const double *zn);
+/* int2e_cutoff (default 5e-11, source/types.F90:121) and the gradient cutoff
+ * (default 1e-10, pyoqp perf_levels.py:37). */
+int oqph_session_set_thresholds(oqph_session *s, double int2e_cutoff, double grad_cutoff);
+/* policy 0 = FP64 everywhere; 1 = mixed: quartets whose bound
+ * Q_ij Q_kl max|D| is below fp32_cutoff run in FP32 (FP64 accumulation). */
+int oqph_session_set_precision(oqph_session *s, int policy, double fp32_cutoff);
+/* Make s the session used by routec_fock_jk / routec_grad2. */
+int oqph_set_default_session(oqph_session *s);
+
+/* Host-buffer builds (copy in, build, copy out). */
+int oqph_fock_jk(oqph_session *s, const double *d, double *f, int nfocks, double se, double sc,
+                 oqph_stats *stats);
+/* Incremental build (source/scf_addons.F90:1207-1210, 2036-2042):
+ * f = G[d - d_old] + f_old; then f_old <- f and d_old <- d. */
+int oqph_fock_jk_inc(oqph_session *s, const double *d, double *d_old, double *f, double *f_old, int nfocks,
+                     double se, double sc, oqph_stats *stats);
+int oqph_grad2(oqph_session *s, const double *d, double *de, double hfscale, double coulscale,
+               oqph_stats *stats);
+
+/* Device-resident builds: d_dev, f_dev are device pointers (ntri doubles) on
+ * the session's device; stream 0 means the session stream.  f_dev is
+ * overwritten.  Inputs already in HBM: no host transfer in the call. */
+int oqph_fock_jk_device(oqph_session *s, const double *d_dev, double *f_dev, int nfocks, double se, double sc,
+                        oqph_stats *stats, void *stream);
+
+/* Per-launch profile of the next builds (HIP events around each class
+ * launch); oqph_last_profile copies up to maxn entries of the last build:
+ * class code la*1000+lb*100+lc*10+ld, device ms, model FLOPs, computed
+ * quartets, primitive quartets.  Returns the number of launches.
+ * No counterpart in the seam: measurement hooks for benchmarks. */
+int oqph_session_set_profiling(oqph_session *s, int on);
+int oqph_last_profile(oqph_session *s, int maxn, int *cls, double *ms, double *flops, long long *quartets,
+                      long long *prims);
+
+/* Schwarz table (nsh*nsh, symmetric) of the current geometry. */
+int oqph_schwarz(oqph_session *s, double *q);
+
+/* One-electron integrals S, T, V (full nbf x nbf, row-major, AO
+ * representation as above) for the host SCF loop and test densities;
+ * any of S, T, V may be NULL.  Host code (OpenMP); the counterpart of
+ * OpenQP's omp_hst (source/integrals/int1.F90:147).  l <= 3. */
+int oqph_int1e(int nsh, const int *am, const int *harmonic, const int *atom, const int *nprim, const double *ex,
+               const double *cc, int natm, const double *xyz, const double *zn, double *S, double *T, double *V);
+
+/* Basis dimensions of the session. */
+int oqph_session_nbf(oqph_session *s);
+
+const char *oqph_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

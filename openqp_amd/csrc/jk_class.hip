@@ -1,0 +1,40 @@
+// jk_class.hip -- instantiates the J/K kernels of one quartet class
+// (XA XB | XC XD), both Cartesian and pure AO variants, and for the diagonal
+// classes (XA XB | XA XB) the Schwarz kernel.  Built once per class with
+// -DXA=.. -DXB=.. -DXC=.. -DXD=.. so the classes compile in parallel.
+#include "jk_device.h"
+
+#define OQPH_CAT2(a, b, c, d) a##b##c##d
+#define OQPH_NAME(p, a, b, c, d) OQPH_CAT2(p, a, b, c) ## d
+
+namespace oqph {
+
+#define OQPH_JK_LAUNCHER_(a, b, c, d) launch_jk_##a##b##c##d
+#define OQPH_JK_LAUNCHER(a, b, c, d) OQPH_JK_LAUNCHER_(a, b, c, d)
+#define OQPH_SCH_LAUNCHER_(a, b) launch_schwarz_##a##b
+#define OQPH_SCH_LAUNCHER(a, b) OQPH_SCH_LAUNCHER_(a, b)
+
+void OQPH_JK_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const ClassWork& cw, hipStream_t s) {
+  const int bs = 256;
+  const long long nb = (cw.nwork + bs - 1) / bs;
+  if (pure)
+    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, true>), dim3((unsigned)nb), dim3(bs), 0, s, p, cw);
+  else
+    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, false>), dim3((unsigned)nb), dim3(bs), 0, s, p, cw);
+}
+
+#if XA == XC && XB == XD
+void OQPH_SCH_LAUNCHER(XA, XB)(bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+                               const double* tab, const double* herm, const double* xf, double qcut, double* Q,
+                               hipStream_t s) {
+  const int bs = 64;
+  if (pure)
+    hipLaunchKernelGGL((k_schwarz<XA, XB, true>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, pairs, prims, list, n, tab,
+                       herm, xf, qcut, Q);
+  else
+    hipLaunchKernelGGL((k_schwarz<XA, XB, false>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, pairs, prims, list, n,
+                       tab, herm, xf, qcut, Q);
+}
+#endif
+
+}  // namespace oqph

@@ -1,0 +1,65 @@
+// oqph_common.h -- data layout shared by the host session and the device kernels.
+//
+// Device-resident layout (all in HBM, built once per basis+geometry):
+//   ShellPair  pairs[npair]      one record per significant shell pair, the
+//                                 higher-L shell first (swap recorded);
+//   PrimPair   prims[nprimpair]  primitive pairs of every shell pair, 64 B each,
+//                                 screened as OpenQP screens them
+//                                 (source/integrals/int2_pairs.F90:179-266);
+//   double     rys[...]           Rys node/weight fit table (rys_table.inc).
+// Per build:
+//   double     D[ntri]            packed lower-triangular density, row-packed;
+//   double     F[ntri]            packed result (atomics);
+//   double     dsh[nsh*nsh]       shell-pair max |D| (OpenQP shlden).
+#pragma once
+#include <stdint.h>
+
+#define OQPH_LMAX 3            // largest shell angular momentum handled on the device
+#define OQPH_MAXNAO 10         // Cartesian components of an f shell
+
+struct ShellPair {
+  double A[3];                 // centre of shell a (the higher-L shell of the pair)
+  double B[3];                 // centre of shell b
+  double Q;                    // Schwarz bound sqrt(max|(ab|ab)|)
+  int sa, sb;                  // shell indices (0-based) of a and b
+  int ao_a, ao_b;              // AO offsets of a and b
+  int prim0, nprim;            // primitive-pair range in prims[]
+  int si, sj;                  // reference order: si >= sj (shell indices)
+  int la, lb;                  // angular momenta of a, b (la >= lb)
+};
+
+struct PrimPair {              // 64 bytes: one cache line per primitive pair
+  double g;                    // zeta = alpha + beta
+  double Kg;                   // K / zeta, K = sqrt(2) pi^(5/4) c_a c_b exp(-alpha beta/zeta AB^2)
+  double P[3];                 // Gaussian product centre
+  double a, b;                 // exponents of shell a and shell b
+  double pad;
+};
+
+// Per-class-pair work description for one launch.
+struct ClassWork {
+  const int* bra;              // pair indices of the bra class list, sorted by Q descending
+  const int* ket;              // pair indices of the ket class list, sorted by Q descending
+  const long long* off;        // prefix offsets of the ket counts per bra (nbra+1)
+  int nbra;
+  long long nwork;
+};
+
+// Arguments of one J/K build launch (shared by host and device code).
+struct JKParams {
+  const ShellPair* pairs;
+  const PrimPair* prims;
+  const double* rys;
+  const double* herm;
+  const double* xform;  // [2][2][10][10] canonical Cartesian -> AO transforms (L = 2, 3)
+  const double* D;      // packed density (ntri)
+  double* F;            // packed G (ntri), accumulated with atomics
+  const double* dsh;    // nsh*nsh shell-pair max |D|
+  unsigned long long* counters;  // [0] computed quartets, [1] skipped by fine screen, [2] fp32 quartets
+  int nsh;
+  double cutoff;        // int2e_cutoff c
+  double maxden;
+  double sc, se;
+  int mixed;            // 1: quartets with bound below fp32_cut run in FP32
+  double fp32_cut;
+};

@@ -1,0 +1,732 @@
+// session.hip -- host side of liboqph: session state, pair data, screening
+// lists, launches, and the C ABI declared in include/oqph.h.
+//
+// Per geometry: primitive-pair data screened as OpenQP screens it
+// (source/integrals/int2_pairs.F90:74-266, cutoffs from int2.F90:266-272),
+// the Schwarz table on the device with the tight cutoffs of ints_exchange
+// (int2.F90:1582-1737, :1600-1604), and per-class shell-pair lists sorted by
+// Schwarz bound.  Per build: the shell-pair density maxima on the device,
+// the coarse per-bra ket counts (a prefix of each sorted ket list survives
+// Q_bra Q_ket 4 maxD >= c), and one launch per class pair; the exact OpenQP
+// screens run inside the kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/oqph.h"
+#include "oqph_common.h"
+
+namespace oqph {
+bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const ClassWork& cw, hipStream_t s);
+bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+                    const double* tab, const double* herm, const double* xf, double qcut, double* Q, hipStream_t s);
+void launch_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh, hipStream_t s);
+void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
+bool kernels_support(int lmax);
+const double* rys_table_host(size_t* n);
+const double* rys_herm_host(size_t* n);
+}  // namespace oqph
+
+using namespace oqph;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+static void set_err(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+#define HIPCHK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      set_err("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+      return e_ == hipErrorOutOfMemory ? 3 : 2;                                \
+    }                                                                          \
+  } while (0)
+
+// fault injection for host-side tests of the caller's fallback:
+// OQPH_FAULT=decline -> every build declines with 1; =oom -> with 3.
+static int fault_code() {
+  const char* f = getenv("OQPH_FAULT");
+  if (!f || !*f) return 0;
+  if (!strcmp(f, "decline")) return 1;
+  if (!strcmp(f, "oom")) return 3;
+  return 0;
+}
+
+// ------------------------------------------------------------------ device buffers
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t m) {
+    if (m <= n && p) return 0;
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (m == 0) m = 1;
+    hipError_t e = hipMalloc((void**)&p, m * sizeof(T));
+    if (e != hipSuccess) {
+      set_err("hipMalloc(%zu bytes): %s", m * sizeof(T), hipGetErrorString(e));
+      p = nullptr;
+      return e == hipErrorOutOfMemory ? 3 : 2;
+    }
+    n = m;
+    return 0;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+static inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+static inline int class_of(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+
+struct oqph_session {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // basis
+  int nsh = 0, nbf = 0, natm = 0, lmax = 0;
+  bool pure = false;
+  std::vector<int> am, harm, atom, nprim, poff, aooff, nao;
+  std::vector<double> ex, cc, xyz, zn;
+  double cutoff = 5.0e-11, grad_cutoff = 1.0e-10;
+  int policy = 0;
+  double fp32_cut = 0.0;
+  // geometry-derived state
+  bool ready = false;
+  std::vector<ShellPair> pairs;              // regular cutoffs, indexed by pair id i(i+1)/2+j
+  std::vector<double> Q;                     // Schwarz per pair id
+  std::vector<std::vector<int>> cls;         // per class: pair ids sorted by Q desc
+  std::vector<std::vector<double>> clsQ;     // matching Q values
+  DBuf<ShellPair> d_pairs;
+  DBuf<PrimPair> d_prims;
+  DBuf<double> d_rys, d_herm, d_xform;
+  DBuf<int> d_cls;                           // concatenated class lists
+  std::vector<size_t> cls_off;
+  DBuf<int> d_aooff, d_nao;
+  // per build
+  DBuf<double> d_D, d_F, d_dsh, d_tmp;
+  DBuf<long long> d_off;
+  DBuf<unsigned long long> d_ctr;
+  double cached_maxden = -1.0;
+  std::vector<long long> h_off;
+  struct Launch {
+    int la, lb, lc, ld;
+    ClassWork cw;
+  };
+  std::vector<Launch> launches;
+  // optional per-launch profile of the last build (HIP events on the build stream)
+  bool profile = false;
+  std::vector<hipEvent_t> pev;
+  std::vector<float> prof_ms;
+  std::vector<unsigned long long> prof_ctr;   // 8 per launch
+};
+
+static oqph_session* g_default = nullptr;
+static std::mutex g_mtx;
+
+#include "ao_xform.h"
+using oqph_host::build_xform;
+
+// ------------------------------------------------------------------ pair data
+struct Cut {
+  double quartet, expo;
+};
+
+static void build_pairs(const oqph_session& s, const Cut& cut, std::vector<ShellPair>& pairs,
+                        std::vector<PrimPair>& prims) {
+  const double sqrtpi52 = std::sqrt(2.0) * std::pow(M_PI, 1.25);
+  const size_t np = (size_t)s.nsh * (s.nsh + 1) / 2;
+  pairs.assign(np, ShellPair{});
+  prims.clear();
+  for (int i = 0; i < s.nsh; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const size_t pid = (size_t)i * (i + 1) / 2 + j;
+      int a = i, b = j;
+      if (s.am[j] > s.am[i]) std::swap(a, b);
+      ShellPair sp{};
+      const double* A = &s.xyz[3 * s.atom[a]];
+      const double* B = &s.xyz[3 * s.atom[b]];
+      double ab2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        sp.A[d] = A[d];
+        sp.B[d] = B[d];
+        ab2 += (A[d] - B[d]) * (A[d] - B[d]);
+      }
+      sp.sa = a; sp.sb = b;
+      sp.ao_a = s.aooff[a]; sp.ao_b = s.aooff[b];
+      sp.si = i; sp.sj = j;
+      sp.la = s.am[a]; sp.lb = s.am[b];
+      sp.prim0 = (int)prims.size();
+      for (int p = 0; p < s.nprim[a]; ++p)
+        for (int q = 0; q < s.nprim[b]; ++q) {
+          const double al = s.ex[s.poff[a] + p], be = s.ex[s.poff[b] + q];
+          const double g = al + be;
+          double e12 = al * be * ab2;
+          if (e12 > g * cut.expo) continue;          // int2_pairs.F90:248
+          e12 /= g;
+          const double k1 = s.cc[s.poff[a] + p] * s.cc[s.poff[b] + q] * std::exp(-e12);
+          if (std::fabs(k1) < cut.quartet) continue;  // int2_pairs.F90:252
+          PrimPair pp{};
+          pp.g = g;
+          pp.Kg = sqrtpi52 * k1 / g;
+          for (int d = 0; d < 3; ++d) pp.P[d] = (al * A[d] + be * B[d]) / g;
+          pp.a = al;
+          pp.b = be;
+          prims.push_back(pp);
+        }
+      sp.nprim = (int)prims.size() - sp.prim0;
+      pairs[pid] = sp;
+    }
+}
+
+static int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  return 0;
+}
+
+static int prepare_geometry(oqph_session* s) {
+  if (s->ready) return 0;
+  if (s->nsh == 0 || s->natm == 0) {
+    set_err("basis or geometry not set");
+    return 4;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  int rc;
+  // Rys tables and AO transforms
+  size_t nt, nh;
+  const double* tab = rys_table_host(&nt);
+  const double* herm = rys_herm_host(&nh);
+  if ((rc = s->d_rys.ensure(nt)) || (rc = s->d_herm.ensure(nh))) return rc;
+  if ((rc = upload(s->d_rys.p, tab, nt * sizeof(double), s->stream))) return rc;
+  if ((rc = upload(s->d_herm.p, herm, nh * sizeof(double), s->stream))) return rc;
+  double xf[2][2][10][10];
+  build_xform(xf);
+  if ((rc = s->d_xform.ensure(400))) return rc;
+  if ((rc = upload(s->d_xform.p, &xf[0][0][0][0], 400 * sizeof(double), s->stream))) return rc;
+  // Schwarz with the tight cutoffs of ints_exchange: exponent 50, quartet 1e-17
+  std::vector<ShellPair> tp;
+  std::vector<PrimPair> tpr;
+  build_pairs(*s, Cut{1.0e-17, 50.0}, tp, tpr);
+  const size_t np = tp.size();
+  DBuf<ShellPair> dtp;
+  DBuf<PrimPair> dtpr;
+  DBuf<int> dlist;
+  DBuf<double> dQ;
+  if ((rc = dtp.ensure(np)) || (rc = dtpr.ensure(tpr.size())) || (rc = dlist.ensure(np)) || (rc = dQ.ensure(np)))
+    return rc;
+  if ((rc = upload(dtp.p, tp.data(), np * sizeof(ShellPair), s->stream))) return rc;
+  if (!tpr.empty() && (rc = upload(dtpr.p, tpr.data(), tpr.size() * sizeof(PrimPair), s->stream))) return rc;
+  std::vector<std::vector<int>> byc(10);
+  for (size_t pid = 0; pid < np; ++pid) byc[class_of(tp[pid].la, tp[pid].lb)].push_back((int)pid);
+  std::vector<int> flat;
+  std::vector<size_t> foff(11, 0);
+  for (int c = 0; c < 10; ++c) {
+    foff[c] = flat.size();
+    flat.insert(flat.end(), byc[c].begin(), byc[c].end());
+  }
+  foff[10] = flat.size();
+  if ((rc = upload(dlist.p, flat.data(), flat.size() * sizeof(int), s->stream))) return rc;
+  HIPCHK(hipMemsetAsync(dQ.p, 0, np * sizeof(double), s->stream));
+  for (int la = 0; la <= s->lmax; ++la)
+    for (int lb = 0; lb <= la; ++lb) {
+      const int c = class_of(la, lb);
+      if (!launch_schwarz(la, lb, s->pure, dtp.p, dtpr.p, dlist.p + foff[c], (int)byc[c].size(), s->d_rys.p,
+                          s->d_herm.p, s->d_xform.p, 1.0e-17, dQ.p, s->stream)) {
+        set_err("no Schwarz kernel for class (%d %d)", la, lb);
+        return 1;
+      }
+    }
+  HIPCHK(hipGetLastError());
+  s->Q.resize(np);
+  HIPCHK(hipMemcpyAsync(s->Q.data(), dQ.p, np * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  dtp.release(); dtpr.release(); dlist.release(); dQ.release();
+  // regular pair data (int2.F90:266-272): quartet 1e-4 c, exponent 25 ln 10
+  std::vector<PrimPair> prims;
+  build_pairs(*s, Cut{1.0e-4 * s->cutoff, 25.0 * std::log(10.0)}, s->pairs, prims);
+  for (size_t pid = 0; pid < np; ++pid) s->pairs[pid].Q = s->Q[pid];
+  if ((rc = s->d_pairs.ensure(np)) || (rc = s->d_prims.ensure(prims.size()))) return rc;
+  if ((rc = upload(s->d_pairs.p, s->pairs.data(), np * sizeof(ShellPair), s->stream))) return rc;
+  if (!prims.empty() && (rc = upload(s->d_prims.p, prims.data(), prims.size() * sizeof(PrimPair), s->stream)))
+    return rc;
+  // class lists of significant pairs, Q descending
+  s->cls.assign(10, {});
+  s->clsQ.assign(10, {});
+  for (size_t pid = 0; pid < np; ++pid) {
+    const ShellPair& sp = s->pairs[pid];
+    if (sp.nprim == 0 || sp.Q <= 0.0) continue;
+    s->cls[class_of(sp.la, sp.lb)].push_back((int)pid);
+  }
+  std::vector<int> all;
+  s->cls_off.assign(11, 0);
+  for (int c = 0; c < 10; ++c) {
+    auto& v = s->cls[c];
+    std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return s->Q[x] > s->Q[y]; });
+    for (int x : v) s->clsQ[c].push_back(s->Q[x]);
+    s->cls_off[c] = all.size();
+    all.insert(all.end(), v.begin(), v.end());
+  }
+  s->cls_off[10] = all.size();
+  if ((rc = s->d_cls.ensure(all.size()))) return rc;
+  if (!all.empty() && (rc = upload(s->d_cls.p, all.data(), all.size() * sizeof(int), s->stream))) return rc;
+  if ((rc = s->d_aooff.ensure(s->nsh)) || (rc = s->d_nao.ensure(s->nsh))) return rc;
+  if ((rc = upload(s->d_aooff.p, s->aooff.data(), s->nsh * sizeof(int), s->stream))) return rc;
+  if ((rc = upload(s->d_nao.p, s->nao.data(), s->nsh * sizeof(int), s->stream))) return rc;
+  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+  if ((rc = s->d_D.ensure(ntri)) || (rc = s->d_F.ensure(ntri)) || (rc = s->d_dsh.ensure((size_t)s->nsh * s->nsh)) ||
+      (rc = s->d_ctr.ensure(8)))
+    return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->cached_maxden = -1.0;
+  s->ready = true;
+  return 0;
+}
+
+// per-bra ket counts and launch list for a given maxden
+static int prepare_lists(oqph_session* s, double maxden) {
+  if (maxden == s->cached_maxden) return 0;
+  s->launches.clear();
+  s->h_off.clear();
+  struct Pend {
+    int la, lb, lc, ld;
+    size_t off0;
+    int cb, ck, nbra;
+    long long nwork;
+  };
+  std::vector<Pend> pend;
+  const double c = s->cutoff;
+  const int ncls = class_of(s->lmax, s->lmax) + 1;
+  for (int cb = 0; cb < ncls; ++cb)
+    for (int ck = 0; ck <= cb; ++ck) {
+      const auto& QB = s->clsQ[cb];
+      const auto& QK = s->clsQ[ck];
+      if (QB.empty() || QK.empty()) continue;
+      const size_t off0 = s->h_off.size();
+      s->h_off.push_back(0);
+      long long tot = 0;
+      for (size_t p = 0; p < QB.size(); ++p) {
+        const double thr = c / (4.0 * maxden * QB[p]);
+        // number of leading ket entries with Q >= thr
+        size_t k = std::lower_bound(QK.begin(), QK.end(), thr, [](double a, double t) { return a >= t; }) - QK.begin();
+        if (cb == ck) k = std::min(k, p + 1);
+        tot += (long long)k;
+        s->h_off.push_back(tot);
+      }
+      int la = 0, lb = 0, lc = 0, ld = 0;
+      for (int x = 0; x <= 3; ++x)
+        for (int y = 0; y <= x; ++y) {
+          if (class_of(x, y) == cb) { la = x; lb = y; }
+          if (class_of(x, y) == ck) { lc = x; ld = y; }
+        }
+      pend.push_back({la, lb, lc, ld, off0, cb, ck, (int)QB.size(), tot});
+    }
+  int rc;
+  if ((rc = s->d_off.ensure(s->h_off.size()))) return rc;
+  if ((rc = upload(s->d_off.p, s->h_off.data(), s->h_off.size() * sizeof(long long), s->stream))) return rc;
+  for (auto& p : pend) {
+    oqph_session::Launch L;
+    L.la = p.la; L.lb = p.lb; L.lc = p.lc; L.ld = p.ld;
+    L.cw.bra = s->d_cls.p + s->cls_off[p.cb];
+    L.cw.ket = s->d_cls.p + s->cls_off[p.ck];
+    L.cw.off = s->d_off.p + p.off0;
+    L.cw.nbra = p.nbra;
+    L.cw.nwork = p.nwork;
+    s->launches.push_back(L);
+  }
+  s->cached_maxden = maxden;
+  return 0;
+}
+
+// G[D] on device buffers; D already on the device
+static int build_device(oqph_session* s, const double* dD, double* dF, double se, double sc, oqph_stats* st,
+                        hipStream_t stream) {
+  int rc;
+  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+  launch_shell_density(dD, s->d_aooff.p, s->d_nao.p, s->nsh, s->d_dsh.p, stream);
+  HIPCHK(hipMemsetAsync(s->d_ctr.p, 0, 8 * sizeof(unsigned long long), stream));
+  launch_max_reduce(s->d_dsh.p, (long long)s->nsh * s->nsh, s->d_ctr.p + 4, stream);
+  unsigned long long mbits = 0;
+  HIPCHK(hipMemcpyAsync(&mbits, s->d_ctr.p + 4, sizeof(mbits), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  double maxden;
+  memcpy(&maxden, &mbits, sizeof(double));
+  HIPCHK(hipMemsetAsync(dF, 0, ntri * sizeof(double), stream));
+  if (maxden <= 0.0) {
+    if (st) memset(st, 0, sizeof(*st));
+    return 0;
+  }
+  if ((rc = prepare_lists(s, maxden))) return rc;
+  const size_t nl = s->launches.size();
+  if ((rc = s->d_ctr.ensure(8 + 8 * nl))) return rc;
+  HIPCHK(hipMemsetAsync(s->d_ctr.p + 8, 0, 8 * nl * sizeof(unsigned long long), stream));
+  if (s->profile && s->pev.size() < 2 * nl) {
+    for (size_t i = s->pev.size(); i < 2 * nl; ++i) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      s->pev.push_back(e);
+    }
+  }
+  JKParams p{};
+  p.pairs = s->d_pairs.p;
+  p.prims = s->d_prims.p;
+  p.rys = s->d_rys.p;
+  p.herm = s->d_herm.p;
+  p.xform = s->d_xform.p;
+  p.D = dD;
+  p.F = dF;
+  p.dsh = s->d_dsh.p;
+  p.counters = s->d_ctr.p;
+  p.nsh = s->nsh;
+  p.cutoff = s->cutoff;
+  p.maxden = maxden;
+  p.sc = sc;
+  p.se = se;
+  p.mixed = s->policy;
+  p.fp32_cut = s->fp32_cut;
+  HIPCHK(hipEventRecord(s->ev0, stream));
+  for (size_t i = 0; i < nl; ++i) {
+    auto& L = s->launches[i];
+    p.counters = s->d_ctr.p + 8 + 8 * i;
+    if (s->profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
+    if (!launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.cw, stream)) {
+      set_err("no J/K kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
+      return 1;
+    }
+    if (s->profile) HIPCHK(hipEventRecord(s->pev[2 * i + 1], stream));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(s->ev1, stream));
+  if (st || s->profile) {
+    s->prof_ctr.assign(8 * nl, 0ull);
+    if (nl) HIPCHK(hipMemcpyAsync(s->prof_ctr.data(), s->d_ctr.p + 8, 8 * nl * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    s->prof_ms.assign(nl, 0.f);
+    if (s->profile)
+      for (size_t i = 0; i < nl; ++i) HIPCHK(hipEventElapsedTime(&s->prof_ms[i], s->pev[2 * i], s->pev[2 * i + 1]));
+  }
+  if (st) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    memset(st, 0, sizeof(*st));
+    for (size_t i = 0; i < nl; ++i) {
+      const unsigned long long* c = &s->prof_ctr[8 * i];
+      st->quartets_computed += (long long)c[0];
+      st->quartets_screened += (long long)c[1];
+      st->quartets_fp32 += (long long)c[2];
+      st->model_flops += (double)c[3];
+      st->prim_quartets += (long long)c[4];
+    }
+    st->kernel_ms = ms;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char* oqph_last_error(void) { return g_err.c_str(); }
+
+int oqph_session_create(oqph_session** out, int device) {
+  if (!out) return 4;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_err("no HIP device");
+    return 2;
+  }
+  if (device < 0 || device >= ndev) {
+    set_err("device %d out of range (%d devices)", device, ndev);
+    return 4;
+  }
+  HIPCHK(hipSetDevice(device));
+  oqph_session* s = new oqph_session();
+  s->device = device;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
+    set_err("stream/event creation failed");
+    delete s;
+    return 2;
+  }
+  *out = s;
+  std::lock_guard<std::mutex> lk(g_mtx);
+  if (!g_default) g_default = s;
+  return 0;
+}
+
+int oqph_session_destroy(oqph_session* s) {
+  if (!s) return 0;
+  {
+    std::lock_guard<std::mutex> lk(g_mtx);
+    if (g_default == s) g_default = nullptr;
+  }
+  hipSetDevice(s->device);
+  s->d_pairs.release(); s->d_prims.release(); s->d_rys.release(); s->d_herm.release(); s->d_cls.release();
+  s->d_xform.release();
+  s->d_aooff.release(); s->d_nao.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
+  s->d_tmp.release(); s->d_off.release(); s->d_ctr.release();
+  for (auto e : s->pev) hipEventDestroy(e);
+  if (s->ev0) hipEventDestroy(s->ev0);
+  if (s->ev1) hipEventDestroy(s->ev1);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return 0;
+}
+
+int oqph_set_default_session(oqph_session* s) {
+  std::lock_guard<std::mutex> lk(g_mtx);
+  g_default = s;
+  return 0;
+}
+
+int oqph_session_set_basis(oqph_session* s, int nsh, const int* am, const int* harmonic, const int* atom,
+                           const int* nprim, const double* ex, const double* cc) {
+  if (!s || nsh <= 0 || !am || !harmonic || !atom || !nprim || !ex || !cc) {
+    set_err("set_basis: invalid argument");
+    return 4;
+  }
+  s->nsh = nsh;
+  s->am.assign(am, am + nsh);
+  s->harm.assign(harmonic, harmonic + nsh);
+  s->atom.assign(atom, atom + nsh);
+  s->nprim.assign(nprim, nprim + nsh);
+  s->poff.resize(nsh);
+  s->aooff.resize(nsh);
+  s->nao.resize(nsh);
+  int p = 0, a = 0, lmax = 0, npure = 0, nhigh = 0;
+  for (int i = 0; i < nsh; ++i) {
+    if (am[i] < 0 || nprim[i] <= 0) {
+      set_err("set_basis: shell %d invalid", i);
+      return 4;
+    }
+    s->poff[i] = p;
+    s->aooff[i] = a;
+    const bool pu = harmonic[i] && am[i] >= 2;
+    s->nao[i] = pu ? 2 * am[i] + 1 : ncart(am[i]);
+    if (am[i] >= 2) {
+      ++nhigh;
+      if (pu) ++npure;
+    }
+    p += nprim[i];
+    a += s->nao[i];
+    lmax = std::max(lmax, am[i]);
+  }
+  s->nbf = a;
+  s->lmax = lmax;
+  s->ex.assign(ex, ex + p);
+  s->cc.assign(cc, cc + p);
+  if (npure != 0 && npure != nhigh) {
+    set_err("set_basis: mixed pure/Cartesian shells of l >= 2 are not supported");
+    return 1;
+  }
+  s->pure = npure > 0;
+  if (!kernels_support(lmax)) {
+    set_err("set_basis: angular momentum %d not supported by the device kernels", lmax);
+    return 1;
+  }
+  s->ready = false;
+  return 0;
+}
+
+int oqph_session_set_geometry(oqph_session* s, int natm, const double* xyz, const double* zn) {
+  if (!s || natm <= 0 || !xyz) {
+    set_err("set_geometry: invalid argument");
+    return 4;
+  }
+  for (int i = 0; i < s->nsh; ++i)
+    if (s->atom[i] < 0 || s->atom[i] >= natm) {
+      set_err("set_geometry: shell %d refers to atom %d of %d", i, s->atom[i], natm);
+      return 4;
+    }
+  s->natm = natm;
+  s->xyz.assign(xyz, xyz + 3 * natm);
+  s->zn.assign(natm, 0.0);
+  if (zn) s->zn.assign(zn, zn + natm);
+  s->ready = false;
+  return 0;
+}
+
+int oqph_session_set_thresholds(oqph_session* s, double c, double gc) {
+  if (!s || !(c > 0.0) || !(gc > 0.0)) {
+    set_err("set_thresholds: invalid argument");
+    return 4;
+  }
+  if (c != s->cutoff) s->ready = false;
+  s->cutoff = c;
+  s->grad_cutoff = gc;
+  return 0;
+}
+
+int oqph_session_set_precision(oqph_session* s, int policy, double fp32_cut) {
+  if (!s || policy < 0 || policy > 1 || fp32_cut < 0.0) {
+    set_err("set_precision: invalid argument");
+    return 4;
+  }
+  s->policy = policy;
+  s->fp32_cut = fp32_cut;
+  return 0;
+}
+
+int oqph_session_nbf(oqph_session* s) { return s ? s->nbf : 0; }
+
+int oqph_session_set_profiling(oqph_session* s, int on) {
+  if (!s) return 4;
+  s->profile = on != 0;
+  return 0;
+}
+
+int oqph_last_profile(oqph_session* s, int maxn, int* cls, double* ms, double* flops, long long* quartets,
+                      long long* prims) {
+  if (!s) return -4;
+  const int n = (int)std::min<size_t>((size_t)maxn, s->launches.size());
+  for (int i = 0; i < n; ++i) {
+    const auto& L = s->launches[i];
+    const unsigned long long* c = i * 8 + 8 <= (int)s->prof_ctr.size() ? &s->prof_ctr[8 * i] : nullptr;
+    if (cls) cls[i] = L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
+    if (ms) ms[i] = i < (int)s->prof_ms.size() ? s->prof_ms[i] : 0.0;
+    if (flops) flops[i] = c ? (double)c[3] : 0.0;
+    if (quartets) quartets[i] = c ? (long long)c[0] : 0;
+    if (prims) prims[i] = c ? (long long)c[4] : 0;
+  }
+  return n;
+}
+
+int oqph_schwarz(oqph_session* s, double* q) {
+  if (!s || !q) return 4;
+  int rc = prepare_geometry(s);
+  if (rc) return rc;
+  for (int i = 0; i < s->nsh; ++i)
+    for (int j = 0; j <= i; ++j) q[i * s->nsh + j] = q[j * s->nsh + i] = s->Q[(size_t)i * (i + 1) / 2 + j];
+  return 0;
+}
+
+int oqph_fock_jk_device(oqph_session* s, const double* d_dev, double* f_dev, int nfocks, double se, double sc,
+                        oqph_stats* st, void* stream) {
+  if (!s || !d_dev || !f_dev) return 4;
+  if (int fc = fault_code()) {
+    set_err("OQPH_FAULT injected decline");
+    return fc;
+  }
+  if (nfocks != 1) {
+    set_err("nfocks=%d: only RHF (nfocks=1) is built on the device", nfocks);
+    return 1;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  int rc = prepare_geometry(s);
+  if (rc) return rc;
+  hipStream_t st_ = stream ? (hipStream_t)stream : s->stream;
+  return build_device(s, d_dev, f_dev, se, sc, st, st_);
+}
+
+int oqph_fock_jk(oqph_session* s, const double* d, double* f, int nfocks, double se, double sc, oqph_stats* st) {
+  if (!s || !d || !f) return 4;
+  if (int fc = fault_code()) {
+    set_err("OQPH_FAULT injected decline");
+    return fc;
+  }
+  if (nfocks != 1) {
+    set_err("nfocks=%d: only RHF (nfocks=1) is built on the device", nfocks);
+    return 1;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  int rc = prepare_geometry(s);
+  if (rc) return rc;
+  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+  if ((rc = upload(s->d_D.p, d, ntri * sizeof(double), s->stream))) return rc;
+  if ((rc = build_device(s, s->d_D.p, s->d_F.p, se, sc, st, s->stream))) return rc;
+  HIPCHK(hipMemcpyAsync(f, s->d_F.p, ntri * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int oqph_fock_jk_inc(oqph_session* s, const double* d, double* d_old, double* f, double* f_old, int nfocks, double se,
+                     double sc, oqph_stats* st) {
+  if (!s || !d || !d_old || !f || !f_old) return 4;
+  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+  std::vector<double> dd(ntri * (size_t)std::max(nfocks, 1));
+  for (size_t i = 0; i < dd.size(); ++i) dd[i] = d[i] - d_old[i];
+  int rc = oqph_fock_jk(s, dd.data(), f, nfocks, se, sc, st);
+  if (rc) return rc;
+  for (size_t i = 0; i < dd.size(); ++i) {
+    f[i] += f_old[i];
+    f_old[i] = f[i];
+    d_old[i] = d[i];
+  }
+  return 0;
+}
+
+int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, double coulscale, oqph_stats* st) {
+  (void)d; (void)de; (void)hfscale; (void)coulscale; (void)st;
+  if (!s) return 4;
+  set_err("grad2: not available in this build");
+  return 1;
+}
+
+void routec_fock_jk(const double* d, double* f, const int* nbf, const int* nfocks, const double* se,
+                    const double* sc, int* info) {
+  if (!info) return;
+  *info = 1;
+  oqph_session* s;
+  {
+    std::lock_guard<std::mutex> lk(g_mtx);
+    s = g_default;
+  }
+  if (!s || !d || !f || !nbf || !nfocks || !se || !sc) {
+    set_err("routec_fock_jk: no default session");
+    return;
+  }
+  if (*nbf != s->nbf) {
+    set_err("routec_fock_jk: nbf %d does not match the session basis (%d)", *nbf, s->nbf);
+    *info = 4;
+    return;
+  }
+  *info = oqph_fock_jk(s, d, f, *nfocks, *se, *sc, nullptr);
+}
+
+void routec_grad2(const double* d, const double* xyz, double* de, const int* nbf, const int* natm,
+                  const double* hfscale, const double* coulscale, int* info) {
+  if (!info) return;
+  *info = 1;
+  oqph_session* s;
+  {
+    std::lock_guard<std::mutex> lk(g_mtx);
+    s = g_default;
+  }
+  if (!s || !d || !xyz || !de || !nbf || !natm || !hfscale || !coulscale) {
+    set_err("routec_grad2: no default session");
+    return;
+  }
+  if (*nbf != s->nbf || *natm != s->natm) {
+    set_err("routec_grad2: dimensions do not match the session");
+    *info = 4;
+    return;
+  }
+  // the caller passes the current geometry: refresh the session's copy
+  bool moved = false;
+  for (int i = 0; i < 3 * s->natm; ++i)
+    if (xyz[i] != s->xyz[i]) moved = true;
+  if (moved) {
+    s->xyz.assign(xyz, xyz + 3 * s->natm);
+    s->ready = false;
+  }
+  *info = oqph_grad2(s, d, de, *hfscale, *coulscale, nullptr);
+}
+
+}  // extern "C"

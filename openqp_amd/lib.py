@@ -1,0 +1,198 @@
+"""ctypes binding of liboqph (include/oqph.h) and the host-side mirror of
+OpenQP's Fock-build interface.
+
+``FockBuilder`` plays the role of ``int2_compute_t`` + ``fock_jk``
+(source/integrals/int2.F90:137-185, source/scf_addons.F90:1063-1214): it is
+initialised with a basis, computes the Schwarz table, and returns the packed
+2e Fock contribution for a packed density.  The device library is required:
+every call goes to liboqph and a missing library is an error, never a CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .basis import Basis
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "liboqph.so")
+_lib = None
+
+_pi = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_pd = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_session_destroy",
+           "oqph_session_set_basis", "oqph_session_set_geometry", "oqph_session_set_thresholds",
+           "oqph_session_set_precision", "oqph_set_default_session", "oqph_fock_jk", "oqph_fock_jk_inc",
+           "oqph_grad2", "oqph_fock_jk_device", "oqph_schwarz", "oqph_session_nbf", "oqph_last_error",
+           "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e"]
+
+
+class OqphError(RuntimeError):
+    pass
+
+
+class Stats(C.Structure):
+    _fields_ = [("quartets_computed", C.c_longlong), ("quartets_screened", C.c_longlong),
+                ("quartets_fp32", C.c_longlong), ("kernel_ms", C.c_double),
+                ("model_flops", C.c_double), ("prim_quartets", C.c_longlong)]
+
+
+def load(path: str | None = None):
+    """Load liboqph.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OqphError(f"liboqph not built: {p} missing (run openqp_amd/build.py)")
+    L = C.CDLL(p)
+    vp = C.c_void_p
+    L.oqph_session_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.oqph_session_destroy.argtypes = [vp]
+    L.oqph_session_set_basis.argtypes = [vp, C.c_int, _pi, _pi, _pi, _pi, _pd, _pd]
+    L.oqph_session_set_geometry.argtypes = [vp, C.c_int, _pd, _pd]
+    L.oqph_session_set_thresholds.argtypes = [vp, C.c_double, C.c_double]
+    L.oqph_session_set_precision.argtypes = [vp, C.c_int, C.c_double]
+    L.oqph_set_default_session.argtypes = [vp]
+    L.oqph_fock_jk.argtypes = [vp, _pd, _pd, C.c_int, C.c_double, C.c_double, C.POINTER(Stats)]
+    L.oqph_fock_jk_inc.argtypes = [vp, _pd, _pd, _pd, _pd, C.c_int, C.c_double, C.c_double, C.POINTER(Stats)]
+    L.oqph_grad2.argtypes = [vp, _pd, _pd, C.c_double, C.c_double, C.POINTER(Stats)]
+    L.oqph_fock_jk_device.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                      C.POINTER(Stats), C.c_void_p]
+    L.oqph_schwarz.argtypes = [vp, _pd]
+    L.oqph_session_nbf.argtypes = [vp]
+    L.oqph_last_error.restype = C.c_char_p
+    L.oqph_session_set_profiling.argtypes = [vp, C.c_int]
+    _pl = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+    L.oqph_last_profile.argtypes = [vp, C.c_int, _pi, _pd, _pd, _pl, _pl]
+    L.oqph_int1e.argtypes = [C.c_int, _pi, _pi, _pi, _pi, _pd, _pd, C.c_int, _pd, _pd, C.c_void_p, C.c_void_p,
+                             C.c_void_p]
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    L.routec_fock_jk.argtypes = [dp, dp, ip, ip, dp, dp, ip]
+    L.routec_grad2.argtypes = [dp, dp, dp, ip, ip, dp, dp, ip]
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise OqphError(f"{what} failed (code {rc}): {load().oqph_last_error().decode()}")
+
+
+class FockBuilder:
+    """Device J/K builder for one basis + geometry (one liboqph session)."""
+
+    def __init__(self, bas: Basis, cutoff: float = 5e-11, grad_cutoff: float = 1e-10, device: int = 0,
+                 precision: int = 0, fp32_cutoff: float = 0.0):
+        L = load()
+        self.bas = bas
+        self._s = C.c_void_p()
+        _check(L.oqph_session_create(C.byref(self._s), device), "session_create")
+        _check(L.oqph_session_set_basis(self._s, bas.nsh, np.ascontiguousarray(bas.am, np.int32),
+                                        np.ascontiguousarray(bas.harmonic, np.int32),
+                                        np.ascontiguousarray(bas.atom, np.int32),
+                                        np.ascontiguousarray(bas.nprim, np.int32),
+                                        np.ascontiguousarray(bas.ex, np.float64),
+                                        np.ascontiguousarray(bas.cc, np.float64)), "set_basis")
+        self.set_geometry(bas.xyz)
+        _check(L.oqph_session_set_thresholds(self._s, cutoff, grad_cutoff), "set_thresholds")
+        _check(L.oqph_session_set_precision(self._s, precision, fp32_cutoff), "set_precision")
+        self.last_stats = Stats()
+
+    def set_geometry(self, xyz):
+        self.bas = self.bas.with_coords(np.asarray(xyz, np.float64).reshape(-1, 3))
+        _check(load().oqph_session_set_geometry(self._s, self.bas.natm, np.ascontiguousarray(self.bas.xyz.reshape(-1)),
+                                                np.ascontiguousarray(self.bas.zn, np.float64)), "set_geometry")
+
+    def set_precision(self, policy: int, fp32_cutoff: float = 0.0):
+        _check(load().oqph_session_set_precision(self._s, policy, fp32_cutoff), "set_precision")
+
+    def make_default(self):
+        _check(load().oqph_set_default_session(self._s), "set_default_session")
+
+    def set_profiling(self, on: bool = True):
+        _check(load().oqph_session_set_profiling(self._s, 1 if on else 0), "set_profiling")
+
+    def last_profile(self):
+        """Per-launch (class code, ms, model FLOPs, quartets, primitive quartets) of the last build."""
+        n = 64
+        cls = np.zeros(n, np.int32); ms = np.zeros(n); fl = np.zeros(n)
+        q = np.zeros(n, np.int64); pq = np.zeros(n, np.int64)
+        m = load().oqph_last_profile(self._s, n, cls, ms, fl, q, pq)
+        return [dict(cls=int(cls[i]), ms=float(ms[i]), flops=float(fl[i]), quartets=int(q[i]), prims=int(pq[i]))
+                for i in range(max(m, 0))]
+
+    def schwarz(self) -> np.ndarray:
+        q = np.zeros(self.bas.nsh * self.bas.nsh)
+        _check(load().oqph_schwarz(self._s, q), "schwarz")
+        return q.reshape(self.bas.nsh, self.bas.nsh)
+
+    def fock_jk(self, d_packed, se: float = 1.0, sc: float = 1.0) -> np.ndarray:
+        """Packed G = sc J - se K/2 for a packed total density (fock_jk semantics)."""
+        d = np.ascontiguousarray(d_packed, np.float64)
+        f = np.zeros_like(d)
+        _check(load().oqph_fock_jk(self._s, d, f, 1, se, sc, C.byref(self.last_stats)), "fock_jk")
+        return f
+
+    def fock_jk_inc(self, d, d_old, f_old, se: float = 1.0, sc: float = 1.0):
+        """Incremental build: returns f = G[d - d_old] + f_old and updates d_old, f_old in place."""
+        d = np.ascontiguousarray(d, np.float64)
+        f = np.zeros_like(d)
+        _check(load().oqph_fock_jk_inc(self._s, d, d_old, f, f_old, 1, se, sc, C.byref(self.last_stats)),
+               "fock_jk_inc")
+        return f
+
+    def fock_jk_device(self, d_dev_ptr: int, f_dev_ptr: int, se: float = 1.0, sc: float = 1.0,
+                       stream: int = 0) -> Stats:
+        st = Stats()
+        _check(load().oqph_fock_jk_device(self._s, C.c_void_p(d_dev_ptr), C.c_void_p(f_dev_ptr), 1, se, sc,
+                                          C.byref(st), C.c_void_p(stream)), "fock_jk_device")
+        return st
+
+    def grad2(self, d_packed, hfscale: float = 1.0, coulscale: float = 1.0) -> np.ndarray:
+        d = np.ascontiguousarray(d_packed, np.float64)
+        de = np.zeros(self.bas.natm * 3)
+        _check(load().oqph_grad2(self._s, d, de, hfscale, coulscale, C.byref(self.last_stats)), "grad2")
+        return de.reshape(self.bas.natm, 3)
+
+    def close(self):
+        if self._s:
+            load().oqph_session_destroy(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def int1e(bas: Basis):
+    """S, T, V (nbf x nbf) in OpenQP's AO representation (host code in liboqph)."""
+    L = load()
+    n = bas.nbf
+    S, T, V = (np.zeros((n, n)) for _ in range(3))
+    rc = L.oqph_int1e(bas.nsh, np.ascontiguousarray(bas.am, np.int32), np.ascontiguousarray(bas.harmonic, np.int32),
+                      np.ascontiguousarray(bas.atom, np.int32), np.ascontiguousarray(bas.nprim, np.int32),
+                      np.ascontiguousarray(bas.ex, np.float64), np.ascontiguousarray(bas.cc, np.float64), bas.natm,
+                      np.ascontiguousarray(bas.xyz.reshape(-1)), np.ascontiguousarray(bas.zn, np.float64),
+                      S.ctypes.data, T.ctypes.data, V.ctypes.data)
+    _check(rc, "int1e")
+    return S, T, V
+
+
+def routec_fock_jk(d_packed, nbf: int, se: float = 1.0, sc: float = 1.0):
+    """Call the Route-C symbol exactly as OpenQP's seam does (by reference); returns (f, info)."""
+    L = load()
+    d = np.ascontiguousarray(d_packed, np.float64)
+    f = np.zeros_like(d)
+    nb, nf, info = C.c_int(nbf), C.c_int(1), C.c_int(-1)
+    se_, sc_ = C.c_double(se), C.c_double(sc)
+    L.routec_fock_jk(d.ctypes.data_as(C.POINTER(C.c_double)), f.ctypes.data_as(C.POINTER(C.c_double)),
+                     C.byref(nb), C.byref(nf), C.byref(se_), C.byref(sc_), C.byref(info))
+    return f, info.value

@@ -1,0 +1,980 @@
+/*
+ * oqo.c -- CPU FP64 oracle for the direct-SCF two-electron Fock build (J/K)
+ *          and its adjoint, the two-electron nuclear gradient.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is linked by the
+ * tests, by the package's smoke check and by the CPU-baseline leg of the
+ * benchmark, and by nothing on the product path.  The product (liboqph)
+ * never links or calls it.
+ *
+ * What it restates (OpenQP, files relative to the source tree):
+ *   - primitive-pair data and screening      source/integrals/int2_pairs.F90:74-266
+ *   - cutoff derivation from int2e_cutoff    source/integrals/int2.F90:266-272
+ *   - Schwarz table (ints_exchange)          source/integrals/int2.F90:1582-1737
+ *   - shell-pair density (shlden), screens   source/integrals/int2.F90:963-1047, 1338-1346
+ *   - canonical quartet loop (int2_twoei)    source/integrals/int2.F90:755-821
+ *   - primitive-quartet screen (rotated axis) source/integrals/int_rotaxis.F90:292-306, 629
+ *   - element cutoff + canonicalisation      source/integrals/int2.F90:1741-1865
+ *   - RHF 6-scatter digestion                source/integrals/int2.F90:1414-1484
+ *   - 0.5 scaling and diagonal doubling      source/scf_addons.F90:1178-1185
+ *   - Cartesian order / component norms      source/constants.F90:34-60, 110+
+ *   - Cartesian -> real solid harmonics      source/integrals/cart2sph.F90:111-160
+ *   - 2e gradient contraction                source/modules/hf_gradient.F90:428-495,
+ *                                            source/integrals/grd2.F90:145-391
+ *
+ * The ERI values themselves are computed with the Obara-Saika vertical
+ * recursion and the Head-Gordon-Pople horizontal recursion over a tabulated
+ * Boys function -- a formula independent of the Rys quadrature that the
+ * device kernels use, so that the two agree only if both are right.  The
+ * reference's own engines (rotated axis for L<=2, Rys or libint above) give
+ * the same integrals to rounding; only the screening decisions listed above
+ * shape which integrals enter J/K, and those are restated exactly.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define LMAXB 4          /* largest shell angular momentum accepted      */
+#define LINT  10         /* largest internal Cartesian level             */
+#define NCUM(L) (((L) + 1) * ((L) + 2) * ((L) + 3) / 6)
+#define NCART(L) (((L) + 1) * ((L) + 2) / 2)
+#define MAXCOMP NCUM(LINT)
+
+/* ------------------------------------------------------------------ */
+/* Cartesian components                                                */
+/* ------------------------------------------------------------------ */
+static int comp_l[MAXCOMP][3];          /* global index -> (lx,ly,lz)   */
+static int gidx_tab[LINT + 1][LINT + 1][LINT + 1];
+/* OpenQP component order within a shell (constants.F90:34-60) */
+static const int ORDER_D[6][3] = {{2,0,0},{0,2,0},{0,0,2},{1,1,0},{1,0,1},{0,1,1}};
+static const int ORDER_F[10][3] = {{3,0,0},{0,3,0},{0,0,3},{2,1,0},{2,0,1},
+                                   {1,2,0},{0,2,1},{1,0,2},{0,1,2},{1,1,1}};
+static const int ORDER_G[15][3] = {{4,0,0},{0,4,0},{0,0,4},{3,1,0},{3,0,1},
+                                   {1,3,0},{0,3,1},{1,0,3},{0,1,3},{2,2,0},
+                                   {2,0,2},{0,2,2},{2,1,1},{1,2,1},{1,1,2}};
+static int out_comp[LMAXB + 1][15][3];
+static double c2s_mat[LMAXB + 1][15][9];  /* [l][cart][sph], unit-normalised Cartesians */
+static int tables_ready = 0;
+static void boys_init(void);
+
+static int gidx(int x, int y, int z) { return gidx_tab[x][y][z]; }
+
+static double dfact(int n) { /* (n)!! with (-1)!! = 1 */
+  double r = 1.0;
+  for (int k = n; k > 1; k -= 2) r *= k;
+  return r;
+}
+
+static double binom(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * (n - i + 1) / i;
+  return r;
+}
+
+/* overlap of two same-exponent monomials, relative units */
+static double mono_ovl(const int *a, const int *b) {
+  double r = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    int s = a[d] + b[d];
+    if (s & 1) return 0.0;
+    r *= dfact(s - 1);
+  }
+  return r;
+}
+
+static void build_tables(void) {
+  if (tables_ready) return;
+  int g = 0;
+  for (int L = 0; L <= LINT; ++L)
+    for (int lx = L; lx >= 0; --lx)
+      for (int ly = L - lx; ly >= 0; --ly) {
+        int lz = L - lx - ly;
+        comp_l[g][0] = lx; comp_l[g][1] = ly; comp_l[g][2] = lz;
+        gidx_tab[lx][ly][lz] = g;
+        ++g;
+      }
+  memset(out_comp, 0, sizeof(out_comp));
+  out_comp[0][0][0] = 0;
+  out_comp[1][0][0] = 1; out_comp[1][1][1] = 1; out_comp[1][2][2] = 1;
+  memcpy(out_comp[2], ORDER_D, sizeof(ORDER_D));
+  memcpy(out_comp[3], ORDER_F, sizeof(ORDER_F));
+  memcpy(out_comp[4], ORDER_G, sizeof(ORDER_G));
+  /* real solid harmonics, m = -l..l, on unit-normalised Cartesians:
+   * Helgaker/Jorgensen/Olsen eq. (6.4.47)-(6.4.50); normalised in the
+   * intra-shell Cartesian metric (cart2sph.F90:111-160 states B S B^T = I). */
+  memset(c2s_mat, 0, sizeof(c2s_mat));
+  for (int l = 2; l <= LMAXB; ++l) {
+    int nc = NCART(l);
+    for (int m = -l; m <= l; ++m) {
+      int am = abs(m);
+      double row[15] = {0};
+      int k0 = (m >= 0) ? 0 : 1;
+      for (int t = 0; t <= (l - am) / 2; ++t)
+        for (int u = 0; u <= t; ++u)
+          for (int k = k0; k <= am; k += 2) {
+            int sp = t + (k - k0) / 2;
+            double c = ((sp & 1) ? -1.0 : 1.0) * pow(0.25, t) * binom(l, t) *
+                       binom(l - t, am + t) * binom(t, u) * binom(am, k);
+            if (c == 0.0) continue;
+            int ax = 2 * t + am - 2 * u - k, ay = 2 * u + k, az = l - 2 * t - am;
+            if (ax < 0 || ay < 0 || az < 0) continue;
+            int ci = -1;
+            for (int q = 0; q < nc; ++q)
+              if (out_comp[l][q][0] == ax && out_comp[l][q][1] == ay && out_comp[l][q][2] == az) ci = q;
+            /* monomial = unit component * sqrt(self-overlap of monomial) */
+            row[ci] += c * sqrt(dfact(2 * ax - 1) * dfact(2 * ay - 1) * dfact(2 * az - 1));
+          }
+      double n2 = 0.0;
+      for (int p = 0; p < nc; ++p)
+        for (int q = 0; q < nc; ++q) {
+          double sp_ = mono_ovl(out_comp[l][p], out_comp[l][q]) /
+                       sqrt(mono_ovl(out_comp[l][p], out_comp[l][p]) * mono_ovl(out_comp[l][q], out_comp[l][q]));
+          n2 += row[p] * row[q] * sp_;
+        }
+      for (int p = 0; p < nc; ++p) c2s_mat[l][p][m + l] = row[p] / sqrt(n2);
+    }
+  }
+  boys_init();
+  tables_ready = 1;
+}
+
+/* component norm of a unit-normalised Cartesian relative to x^l (shells_pnrm2) */
+static double pnrm2(int l, int q) {
+  const int *c = out_comp[l][q];
+  return sqrt(dfact(2 * l - 1) / (dfact(2 * c[0] - 1) * dfact(2 * c[1] - 1) * dfact(2 * c[2] - 1)));
+}
+
+void oqo_c2s(int l, double *out /* [ncart][2l+1] */) {
+  build_tables();
+  int nc = NCART(l), ns = 2 * l + 1;
+  for (int p = 0; p < nc; ++p)
+    for (int s = 0; s < ns; ++s) out[p * ns + s] = (l >= 2) ? c2s_mat[l][p][s] : (p == s ? 1.0 : 0.0);
+}
+
+void oqo_cart_order(int l, int *out /* [ncart][3] */) {
+  build_tables();
+  for (int q = 0; q < NCART(l); ++q)
+    for (int d = 0; d < 3; ++d) out[3 * q + d] = out_comp[l][q][d];
+}
+
+/* ------------------------------------------------------------------ */
+/* Boys function F_m(T) = int_0^1 t^2m exp(-T t^2) dt                  */
+/* ------------------------------------------------------------------ */
+#define BOYS_MMAX 32
+#define BOYS_NTAY 8
+#define BOYS_H 0.05
+#define BOYS_TMAX 45.0
+#define BOYS_NGRID ((int)(BOYS_TMAX / BOYS_H) + 2)
+static double *boys_tab = NULL;   /* [NGRID][MMAX+NTAY+1] */
+
+static void boys_series(int mtop, long double T, long double *F) {
+  /* F_m(T) = exp(-T) sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)); all terms positive */
+  for (int m = 0; m <= mtop; ++m) {
+    long double term = 1.0L / (2 * m + 1), sum = term;
+    for (int k = 1; k < 2000; ++k) {
+      term *= 2.0L * T / (2 * m + 2 * k + 1);
+      sum += term;
+      if (term < sum * 1e-21L) break;
+    }
+    F[m] = expl(-T) * sum;
+  }
+}
+
+static void boys_init(void) {
+  if (boys_tab) return;
+  int ncol = BOYS_MMAX + BOYS_NTAY + 1;
+  double *tab = (double *)malloc(sizeof(double) * BOYS_NGRID * ncol);
+  long double F[BOYS_MMAX + BOYS_NTAY + 2];
+  for (int k = 0; k < BOYS_NGRID; ++k) {
+    boys_series(ncol - 1, (long double)(k * BOYS_H), F);
+    for (int m = 0; m < ncol; ++m) tab[k * ncol + m] = (double)F[m];
+  }
+  boys_tab = tab;
+}
+
+void oqo_boys(int mmax, double T, double *F) {
+  boys_init();
+  if (T < BOYS_TMAX - 1.0) {
+    int ncol = BOYS_MMAX + BOYS_NTAY + 1;
+    int k = (int)(T / BOYS_H + 0.5);
+    double dT = T - k * BOYS_H;
+    const double *row = boys_tab + k * ncol;
+    double s = 0.0, fac = 1.0;
+    for (int j = 0; j < BOYS_NTAY; ++j) {
+      s += row[mmax + j] * fac;
+      fac *= -dT / (j + 1);
+    }
+    F[mmax] = s;
+    double e = exp(-T);
+    for (int m = mmax; m > 0; --m) F[m - 1] = (2.0 * T * F[m] + e) / (2 * m - 1);
+  } else {
+    double e = exp(-T);
+    F[0] = 0.5 * sqrt(M_PI / T);
+    for (int m = 0; m < mmax; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) / (2.0 * T);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Basis                                                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int nsh, nbf, natm;
+  const int *am, *harm, *atom, *nprim, *poff, *aooff;
+  const double *ex, *cc, *xyz;   /* xyz: [natm][3] bohr */
+} oqo_basis;
+
+static int nao_of(const oqo_basis *b, int s) {
+  int l = b->am[s];
+  return (b->harm[s] && l >= 2) ? 2 * l + 1 : NCART(l);
+}
+
+typedef struct {
+  double integral, pair, quartet, expo;
+} oqo_cut;
+
+static oqo_cut make_cut(double c) { /* int2.F90:266-272 */
+  oqo_cut r = {c, 1.0e-2 * c, 1.0e-4 * c, 25.0 * log(10.0)};
+  return r;
+}
+
+/* primitive pair (int2_pairs.F90:179-266) */
+typedef struct {
+  double a, b, g, ginv, K, P[3];
+} ppair;
+
+static int make_pairs(const oqo_basis *bs, int sa, int sb, const oqo_cut *cut, ppair *out) {
+  const double *A = bs->xyz + 3 * bs->atom[sa], *B = bs->xyz + 3 * bs->atom[sb];
+  double ab2 = 0.0;
+  for (int d = 0; d < 3; ++d) ab2 += (A[d] - B[d]) * (A[d] - B[d]);
+  const double sqrtpi52 = sqrt(2.0) * pow(M_PI, 1.25);
+  int n = 0;
+  for (int p = 0; p < bs->nprim[sa]; ++p)
+    for (int q = 0; q < bs->nprim[sb]; ++q) {
+      double a = bs->ex[bs->poff[sa] + p], b = bs->ex[bs->poff[sb] + q];
+      double g = a + b, e12 = a * b * ab2;
+      if (e12 > g * cut->expo) continue;
+      double gi = 1.0 / g;
+      e12 *= gi;
+      double k1 = bs->cc[bs->poff[sa] + p] * bs->cc[bs->poff[sb] + q] * exp(-e12);
+      if (fabs(k1) < cut->quartet) continue;
+      ppair *pp = out + n++;
+      pp->a = a; pp->b = b; pp->g = g; pp->ginv = gi; pp->K = sqrtpi52 * k1;
+      for (int d = 0; d < 3; ++d) pp->P[d] = (a * A[d] + b * B[d]) * gi;
+    }
+  return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Contracted shell quartet via OS-VRR + HGP-HRR                       */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  double *W;       /* VRR work [ncum(Lab)][ncum(Lcd)][Mtot+1] */
+  double *C;       /* contracted [e][f] */
+  double *H1, *H2; /* HRR buffers */
+  size_t nW, nC, nH;
+} work_t;
+
+static void work_init(work_t *w) {
+  int ne = NCUM(LINT - 1);
+  w->nW = (size_t)ne * ne * (2 * LINT + 1);
+  w->W = (double *)malloc(sizeof(double) * w->nW);
+  w->nC = (size_t)ne * ne;
+  w->C = (double *)malloc(sizeof(double) * w->nC);
+  w->nH = (size_t)ne * ne * 16;
+  w->H1 = (double *)malloc(sizeof(double) * w->nH);
+  w->H2 = (double *)malloc(sizeof(double) * w->nH);
+}
+static void work_free(work_t *w) { free(w->W); free(w->C); free(w->H1); free(w->H2); }
+
+/* weight mode for the primitive contraction: 0 -> 1, 1 -> 2a, 2 -> 2b, 3 -> 2c, 4 -> 2d */
+static double wfac(int mode, const ppair *p, const ppair *q) {
+  switch (mode) {
+    case 1: return 2.0 * p->a;
+    case 2: return 2.0 * p->b;
+    case 3: return 2.0 * q->a;
+    case 4: return 2.0 * q->b;
+    default: return 1.0;
+  }
+}
+
+/* first axis along which a component can be lowered */
+static int pick_dir(const int *c) { return c[0] > 0 ? 0 : (c[1] > 0 ? 1 : 2); }
+
+static int lower(int g, int d) {
+  int c[3] = {comp_l[g][0], comp_l[g][1], comp_l[g][2]};
+  c[d]--;
+  return gidx(c[0], c[1], c[2]);
+}
+static int raise_(int g, int d) {
+  int c[3] = {comp_l[g][0], comp_l[g][1], comp_l[g][2]};
+  c[d]++;
+  return gidx(c[0], c[1], c[2]);
+}
+static int lev(int g) { return comp_l[g][0] + comp_l[g][1] + comp_l[g][2]; }
+
+/*
+ * Raw Cartesian block (ab|cd) for angular momenta La..Ld on centres A..D,
+ * canonical internal component order, x^l primitive normalisation
+ * (i.e. before the shells_pnrm2 factors).  Output out[a][b][c][d] with
+ * local canonical indices.  np/nq primitive pairs.  Returns 0 when every
+ * primitive quartet was screened.
+ */
+static int raw_block(int La, int Lb, int Lc, int Ld, const double *A, const double *B,
+                     const double *Cc, const double *D, const ppair *pp, int np, const ppair *qq,
+                     int nq, const oqo_cut *cut, int mode, work_t *w, double *out) {
+  int Lab = La + Lb, Lcd = Lc + Ld, Mt = Lab + Lcd;
+  int ne = NCUM(Lab), nf = NCUM(Lcd), nm = Mt + 1;
+  double *W = w->W, *Cn = w->C;
+  memset(Cn, 0, sizeof(double) * ne * nf);
+  double F[4 * LINT + 2];
+  int any = 0;
+  double qc2 = cut->quartet * cut->quartet;
+#define WI(e, f, m) W[((size_t)(e) * nf + (f)) * nm + (m)]
+  for (int iq = 0; iq < nq; ++iq) {
+    const ppair *q = qq + iq;
+    double sq = q->K * q->ginv;
+    if (fabs(sq) < cut->quartet) continue;               /* int_rotaxis.F90:306 */
+    for (int ip = 0; ip < np; ++ip) {
+      const ppair *p = pp + ip;
+      double sp = p->K * p->ginv;
+      if (fabs(sp) < cut->quartet) continue;             /* int_rotaxis.F90:295 */
+      double fqz = sp * sq, zeta = p->g, eta = q->g, ze = zeta + eta;
+      if (fqz * fqz < qc2 * ze) continue;                /* int_rotaxis.F90:629 */
+      any = 1;
+      double rho = zeta * eta / ze;
+      double Wc[3], PA[3], WP[3], QC[3], WQ[3], pq2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        Wc[d] = (zeta * p->P[d] + eta * q->P[d]) / ze;
+        PA[d] = p->P[d] - A[d];
+        WP[d] = Wc[d] - p->P[d];
+        QC[d] = q->P[d] - Cc[d];
+        WQ[d] = Wc[d] - q->P[d];
+        pq2 += (p->P[d] - q->P[d]) * (p->P[d] - q->P[d]);
+      }
+      /* K includes sqrt(2) pi^(5/4) c c exp(); product/(zeta eta sqrt(ze)) = 2 pi^(5/2)... */
+      double pref = fqz / sqrt(ze) * wfac(mode, p, q);
+      oqo_boys(Mt, rho * pq2, F);
+      for (int m = 0; m <= Mt; ++m) WI(0, 0, m) = pref * F[m];
+      double i2z = 0.5 / zeta, i2e = 0.5 / eta, i2ze = 0.5 / ze, rz = rho / zeta, re = rho / eta;
+      /* bra VRR: [e0|00]^(m) */
+      for (int e = 1; e < ne; ++e) {
+        int d = pick_dir(comp_l[e]);
+        int e1 = lower(e, d);
+        int ed = comp_l[e][d] - 1;
+        int e2 = ed > 0 ? lower(e1, d) : -1;
+        int mtop = Mt - lev(e);
+        for (int m = 0; m <= mtop; ++m) {
+          double v = PA[d] * WI(e1, 0, m) + WP[d] * WI(e1, 0, m + 1);
+          if (e2 >= 0) v += ed * i2z * (WI(e2, 0, m) - rz * WI(e2, 0, m + 1));
+          WI(e, 0, m) = v;
+        }
+      }
+      /* ket VRR: [e0|f0]^(m) */
+      for (int f = 1; f < nf; ++f) {
+        int d = pick_dir(comp_l[f]);
+        int f1 = lower(f, d);
+        int fd = comp_l[f][d] - 1;
+        int f2 = fd > 0 ? lower(f1, d) : -1;
+        for (int e = 0; e < ne; ++e) {
+          int mtop = Mt - lev(e) - lev(f);
+          if (mtop < 0) continue;
+          int ei = comp_l[e][d];
+          int em = ei > 0 ? lower(e, d) : -1;
+          for (int m = 0; m <= mtop; ++m) {
+            double v = QC[d] * WI(e, f1, m) + WQ[d] * WI(e, f1, m + 1);
+            if (f2 >= 0) v += fd * i2e * (WI(e, f2, m) - re * WI(e, f2, m + 1));
+            if (em >= 0) v += ei * i2ze * WI(em, f1, m + 1);
+            WI(e, f, m) = v;
+          }
+        }
+      }
+      for (int e = NCUM(La - 1 < 0 ? -1 : La - 1); e < ne; ++e)
+        for (int f = NCUM(Lc - 1 < 0 ? -1 : Lc - 1); f < nf; ++f) Cn[e * nf + f] += WI(e, f, 0);
+    }
+  }
+#undef WI
+  if (!any) return 0;
+  /* HRR on the bra: (a b| from (e 0|, e in [La, Lab]) */
+  double AB[3], CD[3];
+  for (int d = 0; d < 3; ++d) { AB[d] = A[d] - B[d]; CD[d] = Cc[d] - D[d]; }
+  int e0 = NCUM(La - 1 < 0 ? -1 : La - 1), f0 = NCUM(Lc - 1 < 0 ? -1 : Lc - 1);
+  int nec = ne - e0, nfc = nf - f0;
+  /* stage buffers: T[a_g - e0][b_local][f - f0] */
+  double *T = w->H1, *U = w->H2;
+  for (int e = 0; e < nec; ++e)
+    for (int f = 0; f < nfc; ++f) T[(size_t)e * nfc + f] = Cn[(e + e0) * nf + (f + f0)];
+  int nbcur = 1; /* level-0 b */
+  for (int s = 1; s <= Lb; ++s) {
+    int nbs = NCART(s), bo = NCUM(s - 1), bpo = NCUM(s - 2 < 0 ? -1 : s - 2);
+    int atop = NCUM(Lab - s) - e0; /* a rows valid at this stage */
+    for (int a = 0; a < atop; ++a)
+      for (int bl = 0; bl < nbs; ++bl) {
+        int bg = bo + bl, d = pick_dir(comp_l[bg]);
+        int b1 = lower(bg, d) - bpo;
+        int ar = raise_(a + e0, d) - e0;
+        for (int f = 0; f < nfc; ++f)
+          U[((size_t)a * nbs + bl) * nfc + f] =
+              T[((size_t)ar * nbcur + b1) * nfc + f] + AB[d] * T[((size_t)a * nbcur + b1) * nfc + f];
+      }
+    double *t = T; T = U; U = t;
+    nbcur = nbs;
+  }
+  /* now T[a (rows of level La only: first NCART(La) of the La block)][b][f] */
+  int na = NCART(La), nb = NCART(Lb), nc = NCART(Lc), nd = NCART(Ld);
+  /* ket HRR on each (a,b) */
+  int nab = na * nb;
+  /* reorganise: V[ab][f] */
+  double *V = U;
+  for (int a = 0; a < na; ++a)
+    for (int b = 0; b < nb; ++b)
+      for (int f = 0; f < nfc; ++f) V[((size_t)(a * nb + b)) * nfc + f] = T[((size_t)a * nbcur + b) * nfc + f];
+  /* T as ket stage buffer [ab][c_g - f0][d_local] */
+  double *S0 = V, *S1 = T;
+  int ndcur = 1;
+  for (int s = 1; s <= Ld; ++s) {
+    int nds = NCART(s), dof = NCUM(s - 1), dpo = NCUM(s - 2 < 0 ? -1 : s - 2);
+    int ctop = NCUM(Lcd - s) - f0;
+    for (int ab = 0; ab < nab; ++ab)
+      for (int c = 0; c < ctop; ++c)
+        for (int dl = 0; dl < nds; ++dl) {
+          int dg = dof + dl, dd = pick_dir(comp_l[dg]);
+          int d1 = lower(dg, dd) - dpo;
+          int cr = raise_(c + f0, dd) - f0;
+          S1[((size_t)ab * nfc + c) * nds + dl] =
+              S0[((size_t)ab * nfc + cr) * ndcur + d1] + CD[dd] * S0[((size_t)ab * nfc + c) * ndcur + d1];
+        }
+    double *t = S0; S0 = S1; S1 = t;
+    ndcur = nds;
+  }
+  for (int ab = 0; ab < nab; ++ab)
+    for (int c = 0; c < nc; ++c)
+      for (int d = 0; d < nd; ++d) out[((size_t)ab * nc + c) * nd + d] = S0[((size_t)ab * nfc + c) * ndcur + d];
+  return 1;
+}
+
+/* map canonical-local index of level l to OpenQP output position */
+static int out_pos(int l, int local) {
+  int g = NCUM(l - 1 < 0 ? -1 : l - 1) + local;
+  for (int q = 0; q < NCART(l); ++q)
+    if (out_comp[l][q][0] == comp_l[g][0] && out_comp[l][q][1] == comp_l[g][1] &&
+        out_comp[l][q][2] == comp_l[g][2])
+      return q;
+  return -1;
+}
+
+/*
+ * Transform a raw canonical block to the final AO representation:
+ * OpenQP component order, shells_pnrm2 factors, pure projection for
+ * harmonic shells with l >= 2.  dims in/out given per index.
+ */
+static void finalize_block(const int *L, const int *pure, const double *raw, double *out, double *tmp) {
+  int nc[4], no[4];
+  for (int i = 0; i < 4; ++i) {
+    nc[i] = NCART(L[i]);
+    no[i] = (pure[i] && L[i] >= 2) ? 2 * L[i] + 1 : nc[i];
+  }
+  /* reorder + norms into tmp with OpenQP order */
+  int pos[4][15];
+  double nrm[4][15];
+  for (int i = 0; i < 4; ++i)
+    for (int q = 0; q < nc[i]; ++q) {
+      pos[i][q] = out_pos(L[i], q);
+      nrm[i][q] = pnrm2(L[i], pos[i][q]);
+    }
+  for (int a = 0; a < nc[0]; ++a)
+    for (int b = 0; b < nc[1]; ++b)
+      for (int c = 0; c < nc[2]; ++c)
+        for (int d = 0; d < nc[3]; ++d) {
+          double v = raw[((a * nc[1] + b) * nc[2] + c) * nc[3] + d] * nrm[0][a] * nrm[1][b] * nrm[2][c] * nrm[3][d];
+          tmp[((pos[0][a] * nc[1] + pos[1][b]) * nc[2] + pos[2][c]) * nc[3] + pos[3][d]] = v;
+        }
+  /* successive index projections */
+  int cur[4] = {nc[0], nc[1], nc[2], nc[3]};
+  double *src = tmp, *dst = out;
+  for (int i = 0; i < 4; ++i) {
+    if (no[i] == nc[i]) continue;
+    int left = 1, right = 1;
+    for (int k = 0; k < i; ++k) left *= cur[k];
+    for (int k = i + 1; k < 4; ++k) right *= cur[k];
+    for (int lft = 0; lft < left; ++lft)
+      for (int s = 0; s < no[i]; ++s)
+        for (int r = 0; r < right; ++r) {
+          double acc = 0.0;
+          for (int q = 0; q < nc[i]; ++q) acc += c2s_mat[L[i]][q][s] * src[((size_t)lft * cur[i] + q) * right + r];
+          dst[((size_t)lft * no[i] + s) * right + r] = acc;
+        }
+    cur[i] = no[i];
+    double *t = src; src = dst; dst = t;
+  }
+  if (src != out) memcpy(out, src, sizeof(double) * cur[0] * cur[1] * cur[2] * cur[3]);
+}
+
+/* Final-representation ERI block for shells (i j | k l); returns 0 if all screened */
+static int eri_shell(const oqo_basis *bs, int i, int j, int k, int l, const oqo_cut *cut, work_t *w,
+                     ppair *bufp, ppair *bufq, double *out, double *raw, double *tmp) {
+  int np = make_pairs(bs, i, j, cut, bufp);
+  int nq = make_pairs(bs, k, l, cut, bufq);
+  if (np == 0 || nq == 0) return 0;
+  int L[4] = {bs->am[i], bs->am[j], bs->am[k], bs->am[l]};
+  int pure[4] = {bs->harm[i], bs->harm[j], bs->harm[k], bs->harm[l]};
+  const double *A = bs->xyz + 3 * bs->atom[i], *B = bs->xyz + 3 * bs->atom[j];
+  const double *C = bs->xyz + 3 * bs->atom[k], *D = bs->xyz + 3 * bs->atom[l];
+  if (!raw_block(L[0], L[1], L[2], L[3], A, B, C, D, bufp, np, bufq, nq, cut, 0, w, raw)) return 0;
+  finalize_block(L, pure, raw, out, tmp);
+  return 1;
+}
+
+static int max_prim(const oqo_basis *bs) {
+  int m = 1;
+  for (int s = 0; s < bs->nsh; ++s)
+    if (bs->nprim[s] > m) m = bs->nprim[s];
+  return m;
+}
+
+#define BLK (21 * 21 * 21 * 21)
+
+/* Schwarz table: Q_ab = sqrt(max |(ab|ab)|) with the tight cutoffs of
+ * int2.F90:1600-1604 (ic 1e-15, pair/quartet 1e-17, exponent 50). */
+void oqo_schwarz(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                 const int *nprim, const int *poff, const int *aooff, const double *ex,
+                 const double *cc, const double *xyz, double *Q) {
+  build_tables();
+  oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
+  oqo_cut cut = {1.0e-15, 1.0e-17, 1.0e-17, 50.0};
+  int mp = max_prim(&bs);
+#pragma omp parallel
+  {
+    work_t w;
+    work_init(&w);
+    ppair *bp = (ppair *)malloc(sizeof(ppair) * mp * mp), *bq = (ppair *)malloc(sizeof(ppair) * mp * mp);
+    double *out = (double *)malloc(sizeof(double) * BLK), *raw = (double *)malloc(sizeof(double) * BLK),
+           *tmp = (double *)malloc(sizeof(double) * BLK);
+#pragma omp for schedule(dynamic, 1)
+    for (int a = 0; a < nsh; ++a)
+      for (int b = 0; b <= a; ++b) {
+        double vmax = 0.0;
+        if (eri_shell(&bs, a, b, a, b, &cut, &w, bp, bq, out, raw, tmp)) {
+          int n = nao_of(&bs, a) * nao_of(&bs, b);
+          for (int x = 0; x < n * n; ++x)
+            if (fabs(out[x]) > vmax) vmax = fabs(out[x]);
+        }
+        Q[a * nsh + b] = Q[b * nsh + a] = sqrt(vmax);
+      }
+    free(bp); free(bq); free(out); free(raw); free(tmp);
+    work_free(&w);
+  }
+}
+
+/* Full final-representation ERI block for one shell quartet (tests) */
+int oqo_eri_quartet(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                    const int *nprim, const int *poff, const int *aooff, const double *ex,
+                    const double *cc, const double *xyz, int i, int j, int k, int l, double cutoff,
+                    double *out) {
+  build_tables();
+  oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
+  oqo_cut cut = make_cut(cutoff);
+  if (cutoff <= 0.0) { cut.pair = cut.quartet = 0.0; cut.expo = 1e300; }
+  int mp = max_prim(&bs);
+  work_t w;
+  work_init(&w);
+  ppair *bp = (ppair *)malloc(sizeof(ppair) * mp * mp), *bq = (ppair *)malloc(sizeof(ppair) * mp * mp);
+  double *raw = (double *)malloc(sizeof(double) * BLK), *tmp = (double *)malloc(sizeof(double) * BLK);
+  int r = eri_shell(&bs, i, j, k, l, &cut, &w, bp, bq, out, raw, tmp);
+  if (!r) {
+    int n = nao_of(&bs, i) * nao_of(&bs, j) * nao_of(&bs, k) * nao_of(&bs, l);
+    memset(out, 0, sizeof(double) * n);
+  }
+  free(bp); free(bq); free(raw); free(tmp);
+  work_free(&w);
+  return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* Fock build (int2_twoei + storeints + int2_rhf_data_t_update)        */
+/* ------------------------------------------------------------------ */
+static inline long tri(long i, long j) { return i * (i - 1) / 2 + j; } /* 1-based, i >= j */
+
+typedef struct {
+  int n, cap;
+  int *id;     /* 4 per entry, 1-based AO */
+  double *v;
+} ibuf;
+
+static void digest_rhf(const ibuf *b, const double *d, double *f, double se, double sc) {
+  double x1 = se, x4 = 4.0 * sc;
+  for (int n = 0; n < b->n; ++n) {
+    long ii = b->id[4 * n], jj = b->id[4 * n + 1], kk = b->id[4 * n + 2], ll = b->id[4 * n + 3];
+    double val = b->v[n];
+    long ii2 = ii * (ii - 1) / 2, jj2 = jj * (jj - 1) / 2, kk2 = kk * (kk - 1) / 2;
+    long ij = ii2 + jj, ik = ii2 + kk, il = ii2 + ll, jk = jj2 + kk, jl = jj2 + ll, kl = kk2 + ll;
+    if (jj < kk) jk = kk2 + jj;
+    if (jj < ll) jl = ll * (ll - 1) / 2 + jj;
+    double v1 = val * x1, v4 = val * x4;
+    /* packed arrays are 1-based in the formulas above */
+    f[ij - 1] += v4 * d[kl - 1];
+    f[kl - 1] += v4 * d[ij - 1];
+    f[ik - 1] -= v1 * d[jl - 1];
+    f[jl - 1] -= v1 * d[ik - 1];
+    f[il - 1] -= v1 * d[jk - 1];
+    f[jk - 1] -= v1 * d[il - 1];
+  }
+}
+
+/* storeints (int2.F90:1741-1865): element cutoff, canonical AO order, degeneracy halving */
+static void store_block(const oqo_basis *bs, const int *ids, const double *blk, double cutoff, ibuf *b,
+                        long *nint) {
+  int nb[4], off[4];
+  for (int i = 0; i < 4; ++i) {
+    nb[i] = nao_of(bs, ids[i]);
+    off[i] = bs->aooff[ids[i]];  /* 0-based */
+  }
+  int same = ids[0] == ids[2] && ids[1] == ids[3];
+  int iandj = ids[0] == ids[1], kandl = ids[2] == ids[3];
+  int nij = 0;
+  for (int i = 0; i < nb[0]; ++i) {
+    int maxj = iandj ? i + 1 : nb[1];
+    for (int j = 0; j < maxj; ++j) {
+      ++nij;
+      int nkl = nij, maxl = nb[3];
+      int stop = 0;
+      for (int k = 0; k < nb[2] && !stop; ++k) {
+        if (kandl) maxl = k + 1;
+        if (same) {
+          int itmp = maxl < nkl ? maxl : nkl;
+          if (itmp == 0) { stop = 1; break; }
+          maxl = itmp;
+          nkl -= itmp;
+        }
+        for (int l = 0; l < maxl; ++l) {
+          double val = blk[((i * nb[1] + j) * nb[2] + k) * nb[3] + l];
+          if (fabs(val) < cutoff) continue;
+          ++*nint;
+          int i1 = i + 1 + off[0], j1 = j + 1 + off[1], k1 = k + 1 + off[2], l1 = l + 1 + off[3];
+          if (i1 < j1) { int t = i1; i1 = j1; j1 = t; }
+          if (k1 < l1) { int t = k1; k1 = l1; l1 = t; }
+          int ii = i1, jj = j1, kk = k1, ll = l1;
+          if (ii < kk) { ii = k1; jj = l1; kk = i1; ll = j1; }
+          else if (ii == kk && jj < ll) { ii = i1; jj = l1; kk = k1; ll = j1; }
+          if (ii == jj) val *= 0.5;
+          if (kk == ll) val *= 0.5;
+          if (ii == kk && jj == ll) val *= 0.5;
+          if (b->n == b->cap) {
+            b->cap *= 2;
+            b->id = (int *)realloc(b->id, sizeof(int) * 4 * b->cap);
+            b->v = (double *)realloc(b->v, sizeof(double) * b->cap);
+          }
+          b->id[4 * b->n] = ii; b->id[4 * b->n + 1] = jj; b->id[4 * b->n + 2] = kk; b->id[4 * b->n + 3] = ll;
+          b->v[b->n++] = val;
+        }
+      }
+    }
+  }
+}
+
+/* shlden (int2.F90:999-1047) */
+static void shell_density(const oqo_basis *bs, const double *d, int nfocks, long ntri, double *dsh) {
+  int nsh = bs->nsh;
+  memset(dsh, 0, sizeof(double) * nsh * nsh);
+  for (int fk = 0; fk < nfocks; ++fk)
+    for (int si = 0; si < nsh; ++si) {
+      int mini = bs->aooff[si] + 1, maxi = mini + nao_of(bs, si) - 1;
+      for (int sj = 0; sj <= si; ++sj) {
+        int minj = bs->aooff[sj] + 1, maxj = minj + nao_of(bs, sj) - 1;
+        double dm = 0.0;
+        for (int i = mini; i <= maxi; ++i) {
+          int mj = (si == sj) ? i : maxj;
+          for (int j = minj; j <= mj; ++j) {
+            double v = fabs(d[fk * ntri + tri(i, j) - 1]);
+            if (v > dm) dm = v;
+          }
+        }
+        if (dm > dsh[si * nsh + sj]) dsh[si * nsh + sj] = dm;
+      }
+    }
+  for (int si = 0; si < nsh; ++si)
+    for (int sj = si + 1; sj < nsh; ++sj) dsh[si * nsh + sj] = dsh[sj * nsh + si];
+}
+
+static double max6(double a, double b, double c, double d, double e, double f) {
+  double m = a;
+  if (b > m) m = b;
+  if (c > m) m = c;
+  if (d > m) m = d;
+  if (e > m) m = e;
+  if (f > m) m = f;
+  return m;
+}
+
+/*
+ * fock_jk restatement for RHF (nfocks = 1): returns the packed 2e Fock
+ * contribution f = 0.5 * F_raw with the diagonal doubled, as fock_jk hands
+ * back (scf_addons.F90:1178-1185).  Q is the Schwarz table (nsh*nsh).
+ * stats[0] = nschwz (skipped quartets), stats[1] = computed quartets,
+ * stats[2] = stored integrals.
+ */
+int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                     const int *nprim, const int *poff, const int *aooff, const double *ex,
+                     const double *cc, const double *xyz, const double *Q, const double *d, double *f,
+                     double se, double sc, double cutoff, long *stats, int rank, int size);
+
+int oqo_fock_jk(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                const int *nprim, const int *poff, const int *aooff, const double *ex,
+                const double *cc, const double *xyz, const double *Q, const double *d, double *f,
+                double se, double sc, double cutoff, long *stats) {
+  return oqo_fock_jk_part(nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, d, f, se, sc,
+                          cutoff, stats, 0, 1);
+}
+
+/* Same build restricted to the shell pairs (i, j) that rank `rank` of `size`
+ * owns under OpenQP's MPI decomposition mod(ij_pair, size) == rank
+ * (int2.F90:759-761), ij_pair counting (i, j) pairs in loop order.  With
+ * size > 1 this is a representative 1/size sample of the work. */
+int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                     const int *nprim, const int *poff, const int *aooff, const double *ex,
+                     const double *cc, const double *xyz, const double *Q, const double *d, double *f,
+                     double se, double sc, double cutoff, long *stats, int rank, int size) {
+  build_tables();
+  oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
+  oqo_cut cut = make_cut(cutoff);
+  long ntri = (long)nbf * (nbf + 1) / 2;
+  double *dsh = (double *)malloc(sizeof(double) * nsh * nsh);
+  shell_density(&bs, d, 1, ntri, dsh);
+  double maxden = 0.0;
+  for (int x = 0; x < nsh * nsh; ++x)
+    if (fabs(dsh[x]) > maxden) maxden = fabs(dsh[x]);
+  int mp = max_prim(&bs);
+  long nschwz = 0, nquart = 0, nints = 0;
+  int nthr = 1;
+#ifdef _OPENMP
+  nthr = omp_get_max_threads();
+#endif
+  double *fthr = (double *)calloc((size_t)nthr * ntri, sizeof(double));
+#pragma omp parallel reduction(+ : nschwz, nquart, nints)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double *fl = fthr + (size_t)tid * ntri;
+    work_t w;
+    work_init(&w);
+    ppair *bp = (ppair *)malloc(sizeof(ppair) * mp * mp), *bq = (ppair *)malloc(sizeof(ppair) * mp * mp);
+    double *out = (double *)malloc(sizeof(double) * BLK), *raw = (double *)malloc(sizeof(double) * BLK),
+           *tmp = (double *)malloc(sizeof(double) * BLK);
+    ibuf b = {0, 4096, (int *)malloc(sizeof(int) * 4 * 4096), (double *)malloc(sizeof(double) * 4096)};
+#pragma omp for schedule(dynamic, 1)
+    for (int i = 0; i < nsh; ++i)
+      for (int j = 0; j <= i; ++j) {
+        if (size > 1 && (((long)i * (i + 1) / 2 + j + 1) % size) != rank) continue;
+        /* screen_ij (int2.F90:963-971) */
+        if (Q[i * nsh + j] * maxden < cutoff) {
+          nschwz += (long)(i + 1) * i / 2 + (j + 1);
+          continue;
+        }
+        for (int k = 0; k <= i; ++k) {
+          int jork = (k == i) ? j : k;
+          for (int l = 0; l <= jork; ++l) {
+            /* screen_ijkl (int2.F90:975-986) */
+            double t = Q[i * nsh + j] * Q[k * nsh + l] *
+                       max6(4 * dsh[i * nsh + j], 4 * dsh[k * nsh + l], dsh[j * nsh + l], dsh[j * nsh + k],
+                            dsh[i * nsh + l], dsh[i * nsh + k]);
+            if (t < cutoff) { ++nschwz; continue; }
+            ++nquart;
+            if (!eri_shell(&bs, i, j, k, l, &cut, &w, bp, bq, out, raw, tmp)) continue;
+            int ids[4] = {i, j, k, l};
+            b.n = 0;
+            store_block(&bs, ids, out, cutoff, &b, &nints);
+            digest_rhf(&b, d, fl, se, sc);
+          }
+        }
+      }
+    free(b.id); free(b.v);
+    free(bp); free(bq); free(out); free(raw); free(tmp);
+    work_free(&w);
+  }
+  for (long x = 0; x < ntri; ++x) {
+    double s = 0.0;
+    for (int t = 0; t < nthr; ++t) s += fthr[(size_t)t * ntri + x];
+    f[x] = 0.5 * s;
+  }
+  for (long i = 1; i <= nbf; ++i) f[tri(i, i) - 1] *= 2.0;
+  free(fthr);
+  free(dsh);
+  if (stats) { stats[0] = nschwz; stats[1] = nquart; stats[2] = nints; }
+  return 0;
+}
+
+/* Full unscreened ERI tensor (nbf^4, final AO representation), tests only */
+void oqo_eri_tensor(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                    const int *nprim, const int *poff, const int *aooff, const double *ex,
+                    const double *cc, const double *xyz, double *eri) {
+  build_tables();
+  oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
+  oqo_cut cut = {0.0, 0.0, 0.0, 1e300};
+  int mp = max_prim(&bs);
+  work_t w;
+  work_init(&w);
+  ppair *bp = (ppair *)malloc(sizeof(ppair) * mp * mp), *bq = (ppair *)malloc(sizeof(ppair) * mp * mp);
+  double *out = (double *)malloc(sizeof(double) * BLK), *raw = (double *)malloc(sizeof(double) * BLK),
+         *tmp = (double *)malloc(sizeof(double) * BLK);
+  long n = nbf;
+  memset(eri, 0, sizeof(double) * n * n * n * n);
+  for (int i = 0; i < nsh; ++i)
+    for (int j = 0; j < nsh; ++j)
+      for (int k = 0; k < nsh; ++k)
+        for (int l = 0; l < nsh; ++l) {
+          if (!eri_shell(&bs, i, j, k, l, &cut, &w, bp, bq, out, raw, tmp)) continue;
+          int na = nao_of(&bs, i), nb = nao_of(&bs, j), nc = nao_of(&bs, k), nd = nao_of(&bs, l);
+          for (int a = 0; a < na; ++a)
+            for (int b = 0; b < nb; ++b)
+              for (int c = 0; c < nc; ++c)
+                for (int d = 0; d < nd; ++d) {
+                  long A = aooff[i] + a, B = aooff[j] + b, C = aooff[k] + c, D = aooff[l] + d;
+                  eri[((A * n + B) * n + C) * n + D] = out[((a * nb + b) * nc + c) * nd + d];
+                }
+        }
+  free(bp); free(bq); free(out); free(raw); free(tmp);
+  work_free(&w);
+}
+
+/* ------------------------------------------------------------------ */
+/* One-electron integrals S, T, V (Obara-Saika), full nbf x nbf        */
+/* ------------------------------------------------------------------ */
+/* 1D overlap table s[i][j], i <= imax, j <= jmax (without the Gaussian prefactor) */
+static void ovl1d(int imax, int jmax, double PA, double PB, double g, double s[][LINT + 3]) {
+  double i2g = 0.5 / g;
+  s[0][0] = 1.0;
+  for (int i = 0; i <= imax; ++i)
+    for (int j = 0; j <= jmax; ++j) {
+      if (i == 0 && j == 0) continue;
+      double v;
+      if (i > 0) {
+        v = PA * s[i - 1][j];
+        if (i > 1) v += (i - 1) * i2g * s[i - 2][j];
+        if (j > 0) v += j * i2g * s[i - 1][j - 1];
+      } else {
+        v = PB * s[i][j - 1];
+        if (j > 1) v += (j - 1) * i2g * s[i][j - 2];
+      }
+      s[i][j] = v;
+    }
+}
+
+static void int1e_shell(const oqo_basis *bs, int sa, int sb, const double *zn, double *S, double *T, double *V,
+                        double *tmp) {
+  int la = bs->am[sa], lb = bs->am[sb];
+  int na = NCART(la), nb = NCART(lb);
+  const double *A = bs->xyz + 3 * bs->atom[sa], *B = bs->xyz + 3 * bs->atom[sb];
+  double AB[3], ab2 = 0.0;
+  for (int d = 0; d < 3; ++d) { AB[d] = A[d] - B[d]; ab2 += AB[d] * AB[d]; }
+  for (int x = 0; x < na * nb; ++x) S[x] = T[x] = V[x] = 0.0;
+  int L = la + lb;
+  static __thread double vr[NCUM(LINT - 1)][2 * LINT + 2];
+  for (int p = 0; p < bs->nprim[sa]; ++p)
+    for (int q = 0; q < bs->nprim[sb]; ++q) {
+      double a = bs->ex[bs->poff[sa] + p], b = bs->ex[bs->poff[sb] + q], g = a + b;
+      double cab = bs->cc[bs->poff[sa] + p] * bs->cc[bs->poff[sb] + q];
+      double E = exp(-a * b / g * ab2);
+      double P[3], PA[3], PB[3];
+      for (int d = 0; d < 3; ++d) {
+        P[d] = (a * A[d] + b * B[d]) / g;
+        PA[d] = P[d] - A[d];
+        PB[d] = P[d] - B[d];
+      }
+      double s[3][LINT + 3][LINT + 3];
+      for (int d = 0; d < 3; ++d) ovl1d(la + 1, lb + 2, PA[d], PB[d], g, s[d]);
+      double s0 = cab * E * pow(M_PI / g, 1.5);
+      int a0 = NCUM(la - 1 < 0 ? -1 : la - 1), b0 = NCUM(lb - 1 < 0 ? -1 : lb - 1);
+      for (int ia = 0; ia < na; ++ia)
+        for (int ib = 0; ib < nb; ++ib) {
+          const int *ca = comp_l[a0 + ia], *cb = comp_l[b0 + ib];
+          double sx = s[0][ca[0]][cb[0]], sy = s[1][ca[1]][cb[1]], sz = s[2][ca[2]][cb[2]];
+          S[ia * nb + ib] += s0 * sx * sy * sz;
+          double t1[3];
+          for (int d = 0; d < 3; ++d) {
+            int i = ca[d], j = cb[d];
+            double v = -2.0 * b * b * s[d][i][j + 2] + b * (2 * j + 1) * s[d][i][j];
+            if (j >= 2) v -= 0.5 * j * (j - 1) * s[d][i][j - 2];
+            t1[d] = v;
+          }
+          T[ia * nb + ib] += s0 * (t1[0] * sy * sz + sx * t1[1] * sz + sx * sy * t1[2]);
+        }
+      /* nuclear attraction */
+      double F[2 * LINT + 2];
+      for (int c = 0; c < bs->natm; ++c) {
+        if (zn[c] == 0.0) continue;
+        const double *Cn = bs->xyz + 3 * c;
+        double PC[3], pc2 = 0.0;
+        for (int d = 0; d < 3; ++d) { PC[d] = P[d] - Cn[d]; pc2 += PC[d] * PC[d]; }
+        oqo_boys(L, g * pc2, F);
+        double pre = -zn[c] * cab * E * 2.0 * M_PI / g;
+        int ne = NCUM(L);
+        for (int m = 0; m <= L; ++m) vr[0][m] = pre * F[m];
+        for (int e = 1; e < ne; ++e) {
+          int d = pick_dir(comp_l[e]);
+          int e1 = lower(e, d), ed = comp_l[e][d] - 1, e2 = ed > 0 ? lower(e1, d) : -1;
+          for (int m = 0; m <= L - lev(e); ++m) {
+            double v = PA[d] * vr[e1][m] - PC[d] * vr[e1][m + 1];
+            if (e2 >= 0) v += ed * 0.5 / g * (vr[e2][m] - vr[e2][m + 1]);
+            vr[e][m] = v;
+          }
+        }
+        /* HRR (a, b+1_i) = (a+1_i, b) + AB_i (a, b) on [e] with e in [la, L] */
+        int e0 = a0, nec = ne - e0;
+        double *cur = tmp, *nxt = tmp + 4096;
+        for (int e = 0; e < nec; ++e) cur[e] = vr[e + e0][0];
+        int nbc = 1;
+        for (int st = 1; st <= lb; ++st) {
+          int nbs = NCART(st), bo = NCUM(st - 1), bpo = NCUM(st - 2 < 0 ? -1 : st - 2);
+          int atop = NCUM(L - st) - e0;
+          for (int ar = 0; ar < atop; ++ar)
+            for (int bl = 0; bl < nbs; ++bl) {
+              int bg = bo + bl, d = pick_dir(comp_l[bg]);
+              int b1 = lower(bg, d) - bpo, au = raise_(ar + e0, d) - e0;
+              nxt[ar * nbs + bl] = cur[au * nbc + b1] + AB[d] * cur[ar * nbc + b1];
+            }
+          double *t = cur; cur = nxt; nxt = t;
+          nbc = nbs;
+        }
+        for (int ia = 0; ia < na; ++ia)
+          for (int ib = 0; ib < nb; ++ib) V[ia * nb + ib] += cur[ia * nbc + ib];
+      }
+    }
+}
+
+static void finalize_2(int la, int pa, int lb, int pb, const double *raw, double *out) {
+  int L[4] = {la, lb, 0, 0}, P[4] = {pa, pb, 0, 0};
+  double tmp[21 * 21];
+  finalize_block(L, P, raw, out, tmp);
+}
+
+/* S, T, V as full nbf x nbf matrices; zn nuclear charges per atom */
+void oqo_int1e(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+               const int *nprim, const int *poff, const int *aooff, const double *ex,
+               const double *cc, const double *xyz, const double *zn, double *S, double *T, double *V) {
+  build_tables();
+  oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
+#pragma omp parallel
+  {
+    double s[21 * 21], t[21 * 21], v[21 * 21], o[21 * 21], tmp[8192];
+#pragma omp for schedule(dynamic, 1)
+    for (int a = 0; a < nsh; ++a)
+      for (int b = 0; b < nsh; ++b) {
+        int1e_shell(&bs, a, b, zn, s, t, v, tmp);
+        int na = nao_of(&bs, a), nb = nao_of(&bs, b);
+        double *dst[3] = {S, T, V};
+        double *srcs[3] = {s, t, v};
+        for (int w = 0; w < 3; ++w) {
+          finalize_2(am[a], harm[a], am[b], harm[b], srcs[w], o);
+          for (int i = 0; i < na; ++i)
+            for (int j = 0; j < nb; ++j) dst[w][(long)(aooff[a] + i) * nbf + aooff[b] + j] = o[i * nb + j];
+        }
+      }
+  }
+}

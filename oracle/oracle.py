@@ -1,0 +1,119 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/oqo.c).
+
+Test infrastructure only: imported by tests/, by the package smoke check and
+by the CPU-baseline leg of bench.py -- always as the checker or the reported
+CPU baseline, never on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "liboqo.so")
+_lib = None
+
+_pi = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_pd = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_BAS = [C.c_int, C.c_int, C.c_int, _pi, _pi, _pi, _pi, _pi, _pi, _pd, _pd, _pd]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.oqo_boys.argtypes = [C.c_int, C.c_double, _pd]
+        L.oqo_c2s.argtypes = [C.c_int, _pd]
+        L.oqo_cart_order.argtypes = [C.c_int, _pi]
+        L.oqo_schwarz.argtypes = _BAS + [_pd]
+        L.oqo_eri_quartet.argtypes = _BAS + [C.c_int] * 4 + [C.c_double, _pd]
+        L.oqo_eri_quartet.restype = C.c_int
+        L.oqo_fock_jk.argtypes = _BAS + [_pd, _pd, _pd, C.c_double, C.c_double, C.c_double,
+                                         np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+        L.oqo_fock_jk_part.argtypes = L.oqo_fock_jk.argtypes + [C.c_int, C.c_int]
+        L.oqo_eri_tensor.argtypes = _BAS + [_pd]
+        L.oqo_int1e.argtypes = _BAS + [_pd, _pd, _pd, _pd]
+        if hasattr(L, "oqo_grad2"):
+            L.oqo_grad2.argtypes = _BAS + [_pd, _pd, _pd, C.c_double, C.c_double, C.c_double,
+                                           np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+        _lib = L
+    return _lib
+
+
+def boys(mmax: int, T: float) -> np.ndarray:
+    F = np.zeros(mmax + 1)
+    lib().oqo_boys(mmax, float(T), F)
+    return F
+
+
+def c2s(l: int) -> np.ndarray:
+    nc, ns = (l + 1) * (l + 2) // 2, 2 * l + 1
+    out = np.zeros(nc * ns)
+    lib().oqo_c2s(l, out)
+    return out.reshape(nc, ns)
+
+
+def cart_order(l: int) -> np.ndarray:
+    nc = (l + 1) * (l + 2) // 2
+    out = np.zeros(nc * 3, dtype=np.int32)
+    lib().oqo_cart_order(l, out)
+    return out.reshape(nc, 3)
+
+
+def schwarz(bas) -> np.ndarray:
+    Q = np.zeros(bas.nsh * bas.nsh)
+    lib().oqo_schwarz(*bas.carray(), Q)
+    return Q.reshape(bas.nsh, bas.nsh)
+
+
+def eri_quartet(bas, i, j, k, l, cutoff=0.0) -> np.ndarray:
+    n = bas.naos()
+    out = np.zeros(int(n[i] * n[j] * n[k] * n[l]))
+    lib().oqo_eri_quartet(*bas.carray(), i, j, k, l, cutoff, out)
+    return out.reshape(n[i], n[j], n[k], n[l])
+
+
+def eri_tensor(bas) -> np.ndarray:
+    nbf = bas.nbf
+    out = np.zeros(nbf ** 4)
+    lib().oqo_eri_tensor(*bas.carray(), out)
+    return out.reshape(nbf, nbf, nbf, nbf)
+
+
+def int1e(bas):
+    nbf = bas.nbf
+    S, T, V = (np.zeros(nbf * nbf) for _ in range(3))
+    lib().oqo_int1e(*bas.carray(), np.ascontiguousarray(bas.zn, np.float64), S, T, V)
+    return S.reshape(nbf, nbf), T.reshape(nbf, nbf), V.reshape(nbf, nbf)
+
+
+def fock_jk(bas, d_packed, se=1.0, sc=1.0, cutoff=5e-11, Q=None, rank=0, size=1):
+    """Packed RHF 2e Fock f (scaled like fock_jk) and stats (nschwz, quartets, ints).
+    rank/size restrict the build to OpenQP's round-robin share of shell pairs."""
+    if Q is None:
+        Q = schwarz(bas)
+    f = np.zeros(bas.ntri)
+    stats = np.zeros(3, dtype=np.int64)
+    lib().oqo_fock_jk_part(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)),
+                           np.ascontiguousarray(d_packed, np.float64), f, se, sc, cutoff, stats, rank, size)
+    return f, stats
+
+
+def grad2(bas, d_packed, hfscale=1.0, coulscale=1.0, cutoff=1e-10, Q=None):
+    """dE2/dR (natm, 3) at fixed density, Coulomb + hfscale * exchange."""
+    if Q is None:
+        Q = schwarz(bas)
+    de = np.zeros(bas.natm * 3)
+    stats = np.zeros(3, dtype=np.int64)
+    lib().oqo_grad2(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)),
+                    np.ascontiguousarray(d_packed, np.float64), de, hfscale, coulscale, cutoff, stats)
+    return de.reshape(bas.natm, 3), stats

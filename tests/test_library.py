@@ -1,0 +1,42 @@
+"""The C-ABI library loads and exports every entry point include/oqph.h
+declares; the Route-C names match OpenQP's seam.  CPU only: no HIP calls."""
+import ctypes
+import os
+
+import pytest
+
+from helpers import header_functions, ROOT
+
+
+def _lib_path():
+    return os.path.join(ROOT, "openqp_amd", "_lib", "liboqph.so")
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "openqp_amd", "_lib", "liboqph.so")),
+                    reason="liboqph not built")
+def test_exports_match_header():
+    lib = ctypes.CDLL(_lib_path())
+    names = header_functions()
+    assert "routec_fock_jk" in names and "routec_grad2" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    from openqp_amd import lib as L
+    assert sorted(L.EXPORTS) == names
+
+
+def test_routec_signature_matches_reference_seam():
+    """routec_bridge.F90:33-61 binds (d,f,nbf,nfocks,se,sc,info) and
+    (d,xyz,de,nbf,natm,hfscale,coulscale,info), all by reference."""
+    txt = open(os.path.join(ROOT, "include", "oqph.h")).read()
+    assert ("void routec_fock_jk(const double *d, double *f, const int *nbf, const int *nfocks, "
+            "const double *se,") in txt
+    assert ("void routec_grad2(const double *d, const double *xyz, double *de, const int *nbf, "
+            "const int *natm,") in txt
+
+
+def test_product_does_not_import_oracle():
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r); import openqp_amd, openqp_amd.lib, openqp_amd.molecules; "
+            "assert not any(m == 'oracle' or m.startswith('oracle.') for m in sys.modules)" % ROOT)
+    subprocess.check_call([sys.executable, "-c", code])
