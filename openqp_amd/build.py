@@ -40,9 +40,10 @@ def _cost(c):
 # headers each translation unit depends on (cache key = unit + these + flags)
 DEPS = {
     "jk_class.hip": ["jk_device.h", "eri_device.h", "oqph_common.h", "rys_table.inc"],
-    "dispatch.hip": ["eri_device.h", "oqph_common.h", "rys_table.inc"],
-    "session.hip": ["oqph_common.h", "../../include/oqph.h"],
-    "int1e.hip": ["../../include/oqph.h"],
+    "dispatch.hip": ["eri_device.h", "grad_device.h", "grad_params.h", "oqph_common.h", "rys_table.inc"],
+    "session.hip": ["oqph_common.h", "grad_params.h", "ao_xform.h", "../../include/oqph.h"],
+    "grad_class.hip": ["grad_device.h", "grad_params.h", "eri_device.h", "oqph_common.h", "rys_table.inc"],
+    "int1e.hip": ["../../include/oqph.h", "ao_xform.h", "rys_table.inc"],
 }
 
 
@@ -60,11 +61,13 @@ def _key(unit: str, defs) -> str:
 
 def _jobs():
     units = [("session.hip", [], "session.o"), ("dispatch.hip", [], "dispatch.o"), ("int1e.hip", [], "int1e.o")]
+    heavy, light = [], []
     for c in sorted(CLASSES, key=_cost, reverse=True):
         a, b, cc, d = c
-        units.insert(0 if _cost(c) > 1000 else len(units),
-                     ("jk_class.hip", [f"-DXA={a}", f"-DXB={b}", f"-DXC={cc}", f"-DXD={d}"], f"jk_{a}{b}{cc}{d}.o"))
-    return units
+        defs = [f"-DXA={a}", f"-DXB={b}", f"-DXC={cc}", f"-DXD={d}"]
+        for unit, tag in (("jk_class.hip", "jk"), ("grad_class.hip", "gr")):
+            (heavy if _cost(c) > 1000 else light).append((unit, defs, f"{tag}_{a}{b}{cc}{d}.o"))
+    return heavy + units + light
 
 
 def _compile(job, verbose):

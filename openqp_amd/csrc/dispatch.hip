@@ -2,6 +2,7 @@
 // per-build kernels, and the host copies of the Rys tables.
 #define OQPH_WANT_RYS_DATA
 #include "eri_device.h"
+#include "grad_device.h"
 
 namespace oqph {
 
@@ -32,6 +33,11 @@ void launch_schwarz_11(bool, const ShellPair*, const PrimPair*, const int*, int,
 void launch_schwarz_20(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
 void launch_schwarz_21(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
 void launch_schwarz_22(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
+
+__device__ __forceinline__ long long tri_idx_(int i, int j) {
+  int a = i > j ? i : j, b = i > j ? j : i;
+  return (long long)a * (a + 1) / 2 + b;
+}
 
 bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const ClassWork& cw, hipStream_t s) {
   if (cw.nwork == 0) return true;
@@ -75,12 +81,92 @@ bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const Pri
   }
 }
 
+void launch_grad_0000(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_1000(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_1010(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_1100(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_1110(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_1111(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2000(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2010(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2011(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2020(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2100(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2110(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2111(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2120(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2121(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2200(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2210(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2211(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2220(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2221(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_2222(const GradParams&, const ClassWork&, hipStream_t);
+
+bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const ClassWork& cw, hipStream_t s) {
+  if (cw.nwork == 0) return true;
+  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
+    case 0: launch_grad_0000(p, cw, s); return true;
+    case 1000: launch_grad_1000(p, cw, s); return true;
+    case 1010: launch_grad_1010(p, cw, s); return true;
+    case 1100: launch_grad_1100(p, cw, s); return true;
+    case 1110: launch_grad_1110(p, cw, s); return true;
+    case 1111: launch_grad_1111(p, cw, s); return true;
+    case 2000: launch_grad_2000(p, cw, s); return true;
+    case 2010: launch_grad_2010(p, cw, s); return true;
+    case 2011: launch_grad_2011(p, cw, s); return true;
+    case 2020: launch_grad_2020(p, cw, s); return true;
+    case 2100: launch_grad_2100(p, cw, s); return true;
+    case 2110: launch_grad_2110(p, cw, s); return true;
+    case 2111: launch_grad_2111(p, cw, s); return true;
+    case 2120: launch_grad_2120(p, cw, s); return true;
+    case 2121: launch_grad_2121(p, cw, s); return true;
+    case 2200: launch_grad_2200(p, cw, s); return true;
+    case 2210: launch_grad_2210(p, cw, s); return true;
+    case 2211: launch_grad_2211(p, cw, s); return true;
+    case 2220: launch_grad_2220(p, cw, s); return true;
+    case 2221: launch_grad_2221(p, cw, s); return true;
+    case 2222: launch_grad_2222(p, cw, s); return true;
+    default: return false;
+  }
+}
+
+// Cartesian-effective density Dc = X^T D X, one thread per shell pair block
+// (build_cart_density, source/basis_tools.F90:407-452, with the raw x^l
+// normalised Cartesians of the derivative kernels)
+__global__ void k_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off,
+                               const int* coff, const double* xf, int ncf, double* Dc) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nsh * nsh) return;
+  const int a = (int)(t / nsh), b = (int)(t % nsh);
+  const int la = am[a], lb = am[b];
+  const int ca = (la + 1) * (la + 2) / 2, cb = (lb + 1) * (lb + 2) / 2;
+  const int na = (pure[a] && la >= 2) ? 2 * la + 1 : ca, nb = (pure[b] && lb >= 2) ? 2 * lb + 1 : cb;
+  for (int p = 0; p < ca; ++p)
+    for (int q = 0; q < cb; ++q) {
+      double s = 0.0;
+      for (int i = 0; i < na; ++i) {
+        const double xa = la >= 2 ? xf[(((la - 2) * 2 + (pure[a] ? 1 : 0)) * 10 + i) * 10 + p] : (i == p ? 1.0 : 0.0);
+        if (xa == 0.0) continue;
+        for (int j = 0; j < nb; ++j) {
+          const double xb = lb >= 2 ? xf[(((lb - 2) * 2 + (pure[b] ? 1 : 0)) * 10 + j) * 10 + q] : (j == q ? 1.0 : 0.0);
+          if (xb == 0.0) continue;
+          s += xa * D[tri_idx_(ao_off[a] + i, ao_off[b] + j)] * xb;
+        }
+      }
+      Dc[(long long)(coff[a] + p) * ncf + coff[b] + q] = s;
+    }
+}
+
+void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
+                         const double* xf, int ncf, double* Dc, hipStream_t s) {
+  const long long n = (long long)nsh * nsh;
+  hipLaunchKernelGGL(k_cart_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, nsh, am, pure, ao_off,
+                     coff, xf, ncf, Dc);
+}
+
 bool kernels_support(int lmax) { return lmax <= 2; }
 
-__device__ __forceinline__ long long tri_idx_(int i, int j) {
-  int a = i > j ? i : j, b = i > j ? j : i;
-  return (long long)a * (a + 1) / 2 + b;
-}
 
 // shell-pair density maxima (shlden, source/integrals/int2.F90:999-1047)
 __global__ void k_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh) {

@@ -1,0 +1,16 @@
+// grad_class.hip -- instantiates the 2e-gradient kernel of one quartet class
+// (XA XB | XC XD); built once per class with -DXA..-DXD.
+#include "grad_device.h"
+
+namespace oqph {
+
+#define OQPH_GR_LAUNCHER_(a, b, c, d) launch_grad_##a##b##c##d
+#define OQPH_GR_LAUNCHER(a, b, c, d) OQPH_GR_LAUNCHER_(a, b, c, d)
+
+void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const ClassWork& cw, hipStream_t s) {
+  const int bs = 256;
+  const long long nb = (cw.nwork + bs - 1) / bs;
+  hipLaunchKernelGGL((k_grad<XA, XB, XC, XD>), dim3((unsigned)nb), dim3(bs), sizeof(double) * 3 * p.natm, s, p, cw);
+}
+
+}  // namespace oqph

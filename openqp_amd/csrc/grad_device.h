@@ -1,0 +1,230 @@
+// grad_device.h -- the adjoint of the J/K build: derivative ERIs contracted
+// with the RHF two-particle density, giving dE2/dR (source/integrals/grd2.F90:
+// 145-391 loops and screens, source/integrals/grd2_rys.F90:303-937 integral
+// derivatives, source/modules/hf_gradient.F90:428-495 density).
+//
+// One thread evaluates one canonical shell quartet.  Gamma is formed in raw
+// (x^l-normalised, canonical-order) Cartesians from the Cartesian-effective
+// density Dc = X^T D X,
+//   Gamma_abcd = 4 cs Dc_ab Dc_cd - hs (Dc_ac Dc_bd + Dc_ad Dc_bc),
+// and contracted per primitive quartet and Rys root with the derivative 2D
+// integrals d/dA_x G = 2 alpha G(a+1) - a G(a-1) (likewise B, C); the D
+// centre follows from translational invariance.  Per-atom sums are staged in
+// LDS and flushed with one FP64 atomic per block and coordinate.
+#pragma once
+#include "eri_device.h"
+#include "grad_params.h"
+
+namespace oqph {
+
+// wave-aggregated counters (every lane of the wave calls them)
+__device__ __forceinline__ void sum_wave_g(unsigned long long* ctr, unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (v != 0ull && __lane_id() == 0) atomicAdd(ctr, v);
+}
+__device__ __forceinline__ void count_wave_g(unsigned long long* ctr, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (m != 0ull && __lane_id() == 0) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+
+
+
+// 2D integral with bra/ket horizontal transfer, all indices compile-time:
+//   I(a,b,c,d) = sum_j sum_m C(b,j) AB^(b-j) C(d,m) CD^(d-m) G(a+j, c+m)
+template <int NI, int NK, typename R>
+__device__ __forceinline__ R i2d(const R (&G)[NI][NK], const R* pab, const R* pcd, int a, int b, int c, int d) {
+  R s = 0;
+#pragma unroll
+  for (int j = 0; j <= b; ++j)
+#pragma unroll
+    for (int m = 0; m <= d; ++m)
+      s += (R)(binom(b, j) * binom(d, m)) * pab[b - j] * pcd[d - m] * G[a + j][c + m];
+  return s;
+}
+
+template <int LA, int LB, int LC, int LD>
+__host__ __device__ constexpr long long flops_grad_prim() {
+  constexpr int LAB = LA + LB + 1, LCD = LC + LD + 1, nr = (LAB + LCD - 1) / 2 + 1;
+  constexpr int nc = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+  return (long long)nr * (56 + 12 * (LAB + 1) * (LCD + 1) + 6 * (LA + 2) * (LB + 2) * (LCD + 1) +
+                          6 * (LA + 2) * (LB + 2) * (LC + 2) * (LD + 1) + 3 * 4 * nc);
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
+  extern __shared__ double s_de[];
+  for (int i = threadIdx.x; i < 3 * prm.natm; i += blockDim.x) s_de[i] = 0.0;
+  __syncthreads();
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NG = NA * NB * NC * ND;
+  constexpr int LAB = LA + LB + 1, LCD = LC + LD + 1;
+  constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
+  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = w < cw.nwork;
+  bool skip = true;
+  ShellPair P{}, Q{};
+  int ip = 0, iq = 0;
+  double Gam[NG];
+  double gm = 0.0;
+  if (active) {
+    int lo = 0, hi = cw.nbra;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (cw.off[mid] <= w) lo = mid; else hi = mid;
+    }
+    ip = cw.bra[lo];
+    iq = cw.ket[w - cw.off[lo]];
+    P = prm.pairs[ip];
+    Q = prm.pairs[iq];
+    const int oa = prm.coff[P.sa], ob = prm.coff[P.sb], oc = prm.coff[Q.sa], od = prm.coff[Q.sb];
+    const int n = prm.ncf;
+    const double* Dc = prm.Dc;
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            const double g = 4.0 * prm.cs * Dc[(oa + a) * n + ob + b] * Dc[(oc + c) * n + od + d] -
+                             prm.hs * (Dc[(oa + a) * n + oc + c] * Dc[(ob + b) * n + od + d] +
+                                       Dc[(oa + a) * n + od + d] * Dc[(ob + b) * n + oc + c]);
+            Gam[((a * NB + b) * NC + c) * ND + d] = g;
+            gm = fmax(gm, fabs(g));
+          }
+    // fine screen on the weighted contribution (grd2.F90:348)
+    skip = gm * P.Q * Q.Q < 0.5 * prm.gcut;
+  }
+  unsigned long long* ctr = prm.counters;
+  count_wave_g(ctr + 1, active && skip);
+  count_wave_g(ctr + 0, !skip);
+  double fd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  int nprim = 0;
+  if (!skip) {
+    const double dab2 = gm * gm;
+    for (int iqq = 0; iqq < Q.nprim; ++iqq) {
+      const PrimPair qq = prm.prims[Q.prim0 + iqq];
+      for (int ipp = 0; ipp < P.nprim; ++ipp) {
+        const PrimPair pp = prm.prims[P.prim0 + ipp];
+        const double zeta = pp.g, eta = qq.g, ze = zeta + eta;
+        const double fqz = pp.Kg * qq.Kg;
+        // primitive screens (grd2_rys.F90:375-376), prefactor cutoff (1e-20)^2
+        if (fqz * fqz < 1.0e-40 * ze) continue;
+        if (fqz * fqz * dab2 < prm.dabcut2 * ze) continue;
+        ++nprim;
+        const double ize = 1.0 / ze, rho = zeta * eta * ize;
+        double PQ[3], PA[3], QC[3], pq2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          PQ[d] = pp.P[d] - qq.P[d];
+          PA[d] = pp.P[d] - P.A[d];
+          QC[d] = qq.P[d] - Q.A[d];
+          pq2 += PQ[d] * PQ[d];
+        }
+        double rx[NR], rw[NR];
+        rys_roots<NR, double>(rho * pq2, prm.rys, prm.herm, rx, rw);
+        const double pref = fqz * sqrt(ize);
+        const double rz = rho / zeta, re = rho / eta;
+        const double h2z = 0.5 / zeta, h2e = 0.5 / eta, h2ze = 0.5 * ize;
+        const double a2 = 2.0 * pp.a, b2 = 2.0 * pp.b, c2 = 2.0 * qq.a;
+        constexpr int UR = (NG > 36) ? 1 : NR;
+#pragma unroll UR
+        for (int r = 0; r < NR; ++r) {
+          const double t2 = rx[r];
+          const double B00 = h2ze * t2, B10 = h2z - h2z * rz * t2, B01 = h2e - h2e * re * t2;
+          double G[3][LAB + 1][LCD + 1];
+          double pab[3][LB + 2], pcd[3][LD + 1];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double C00 = PA[d] - rz * t2 * PQ[d];
+            const double D00 = QC[d] + re * t2 * PQ[d];
+            G[d][0][0] = (d == 0) ? pref * rw[r] : 1.0;
+#pragma unroll
+            for (int i = 1; i <= LAB; ++i) {
+              double v = C00 * G[d][i - 1][0];
+              if (i > 1) v += (double)(i - 1) * B10 * G[d][i - 2][0];
+              G[d][i][0] = v;
+            }
+#pragma unroll
+            for (int k = 1; k <= LCD; ++k)
+#pragma unroll
+              for (int i = 0; i <= LAB; ++i) {
+                double v = D00 * G[d][i][k - 1];
+                if (k > 1) v += (double)(k - 1) * B01 * G[d][i][k - 2];
+                if (i > 0) v += (double)i * B00 * G[d][i - 1][k - 1];
+                G[d][i][k] = v;
+              }
+            const double ab = P.A[d] - P.B[d], cd = Q.A[d] - Q.B[d];
+            pab[d][0] = 1.0;
+#pragma unroll
+            for (int k = 1; k <= LB + 1; ++k) pab[d][k] = pab[d][k - 1] * ab;
+            pcd[d][0] = 1.0;
+#pragma unroll
+            for (int k = 1; k <= LD; ++k) pcd[d][k] = pcd[d][k - 1] * cd;
+          }
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+              for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int dd = 0; dd < ND; ++dd) {
+                  const double g = Gam[((a * NB + b) * NC + c) * ND + dd];
+                  const int ia[3] = {c_lx(LA, a), c_ly(LA, a), c_lz(LA, a)};
+                  const int ib[3] = {c_lx(LB, b), c_ly(LB, b), c_lz(LB, b)};
+                  const int ic[3] = {c_lx(LC, c), c_ly(LC, c), c_lz(LC, c)};
+                  const int id[3] = {c_lx(LD, dd), c_ly(LD, dd), c_lz(LD, dd)};
+                  double I0[3], dA[3], dB[3], dC[3];
+#pragma unroll
+                  for (int d = 0; d < 3; ++d) {
+                    I0[d] = i2d(G[d], pab[d], pcd[d], ia[d], ib[d], ic[d], id[d]);
+                    double v = a2 * i2d(G[d], pab[d], pcd[d], ia[d] + 1, ib[d], ic[d], id[d]);
+                    if (ia[d] > 0) v -= (double)ia[d] * i2d(G[d], pab[d], pcd[d], ia[d] - 1, ib[d], ic[d], id[d]);
+                    dA[d] = v;
+                    v = b2 * i2d(G[d], pab[d], pcd[d], ia[d], ib[d] + 1, ic[d], id[d]);
+                    if (ib[d] > 0) v -= (double)ib[d] * i2d(G[d], pab[d], pcd[d], ia[d], ib[d] - 1, ic[d], id[d]);
+                    dB[d] = v;
+                    v = c2 * i2d(G[d], pab[d], pcd[d], ia[d], ib[d], ic[d] + 1, id[d]);
+                    if (ic[d] > 0) v -= (double)ic[d] * i2d(G[d], pab[d], pcd[d], ia[d], ib[d], ic[d] - 1, id[d]);
+                    dC[d] = v;
+                  }
+                  fd[0][0] += g * dA[0] * I0[1] * I0[2];
+                  fd[0][1] += g * I0[0] * dA[1] * I0[2];
+                  fd[0][2] += g * I0[0] * I0[1] * dA[2];
+                  fd[1][0] += g * dB[0] * I0[1] * I0[2];
+                  fd[1][1] += g * I0[0] * dB[1] * I0[2];
+                  fd[1][2] += g * I0[0] * I0[1] * dB[2];
+                  fd[2][0] += g * dC[0] * I0[1] * I0[2];
+                  fd[2][1] += g * I0[0] * dC[1] * I0[2];
+                  fd[2][2] += g * I0[0] * I0[1] * dC[2];
+                }
+        }
+      }
+    }
+  }
+  sum_wave_g(ctr + 3, (unsigned long long)nprim * flops_grad_prim<LA, LB, LC, LD>());
+  sum_wave_g(ctr + 4, (unsigned long long)nprim);
+  if (!skip && nprim > 0) {
+    double deg = 1.0;
+    if (P.sa == P.sb) deg *= 0.5;
+    if (Q.sa == Q.sb) deg *= 0.5;
+    if (ip == iq) deg *= 0.5;
+    const int at[4] = {prm.shell_atom[P.sa], prm.shell_atom[P.sb], prm.shell_atom[Q.sa], prm.shell_atom[Q.sb]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double fD = -(fd[0][d] + fd[1][d] + fd[2][d]);
+      atomicAdd(&s_de[3 * at[0] + d], deg * fd[0][d]);
+      atomicAdd(&s_de[3 * at[1] + d], deg * fd[1][d]);
+      atomicAdd(&s_de[3 * at[2] + d], deg * fd[2][d]);
+      atomicAdd(&s_de[3 * at[3] + d], deg * fD);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * prm.natm; i += blockDim.x)
+    if (s_de[i] != 0.0) unsafeAtomicAdd(prm.de + i, s_de[i]);
+}
+
+}  // namespace oqph

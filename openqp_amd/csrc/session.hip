@@ -23,6 +23,7 @@
 
 #include "../../include/oqph.h"
 #include "oqph_common.h"
+#include "grad_params.h"
 
 namespace oqph {
 bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const ClassWork& cw, hipStream_t s);
@@ -31,6 +32,10 @@ bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const Pri
 void launch_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh, hipStream_t s);
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
 bool kernels_support(int lmax);
+struct GradParams;
+bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const ClassWork& cw, hipStream_t s);
+void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
+                         const double* xf, int ncf, double* Dc, hipStream_t s);
 const double* rys_table_host(size_t* n);
 const double* rys_herm_host(size_t* n);
 }  // namespace oqph
@@ -131,6 +136,15 @@ struct oqph_session {
     ClassWork cw;
   };
   std::vector<Launch> launches;
+  // gradient state (pair data with the grd2 cutoffs, coarse-screened lists)
+  DBuf<ShellPair> d_gpairs;
+  DBuf<PrimPair> d_gprims;
+  DBuf<int> d_gcls, d_coff, d_satom, d_am, d_pure;
+  DBuf<long long> d_goff;
+  DBuf<double> d_Dc, d_de;
+  int ncf = 0;
+  double gcut_ready = -1.0;
+  std::vector<Launch> glaunches;
   // optional per-launch profile of the last build (HIP events on the build stream)
   bool profile = false;
   std::vector<hipEvent_t> pev;
@@ -295,7 +309,100 @@ static int prepare_geometry(oqph_session* s) {
     return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   s->cached_maxden = -1.0;
+  s->gcut_ready = -1.0;
   s->ready = true;
+  return 0;
+}
+
+// gradient pair data and coarse-screened quartet lists (grd2.F90:219-328):
+// pair data with exponent cutoff 20 ln 10 and prefactor cutoff 1e-20; a
+// quartet survives the coarse screen when Q_ij Q_kl >= grad_cutoff.
+static int prepare_grad(oqph_session* s) {
+  int rc = prepare_geometry(s);
+  if (rc) return rc;
+  if (s->gcut_ready == s->grad_cutoff) return 0;
+  std::vector<ShellPair> gp;
+  std::vector<PrimPair> gpr;
+  build_pairs(*s, Cut{1.0e-20, 20.0 * std::log(10.0)}, gp, gpr);
+  const size_t np = gp.size();
+  for (size_t pid = 0; pid < np; ++pid) gp[pid].Q = s->Q[pid];
+  if ((rc = s->d_gpairs.ensure(np)) || (rc = s->d_gprims.ensure(std::max<size_t>(gpr.size(), 1)))) return rc;
+  if ((rc = upload(s->d_gpairs.p, gp.data(), np * sizeof(ShellPair), s->stream))) return rc;
+  if (!gpr.empty() && (rc = upload(s->d_gprims.p, gpr.data(), gpr.size() * sizeof(PrimPair), s->stream))) return rc;
+  std::vector<std::vector<int>> cls(10);
+  std::vector<std::vector<double>> clsQ(10);
+  for (size_t pid = 0; pid < np; ++pid)
+    if (gp[pid].nprim > 0 && gp[pid].Q > 0.0) cls[class_of(gp[pid].la, gp[pid].lb)].push_back((int)pid);
+  std::vector<int> all;
+  std::vector<size_t> coff(11, 0);
+  for (int c = 0; c < 10; ++c) {
+    std::stable_sort(cls[c].begin(), cls[c].end(), [&](int x, int y) { return s->Q[x] > s->Q[y]; });
+    for (int x : cls[c]) clsQ[c].push_back(s->Q[x]);
+    coff[c] = all.size();
+    all.insert(all.end(), cls[c].begin(), cls[c].end());
+  }
+  if ((rc = s->d_gcls.ensure(std::max<size_t>(all.size(), 1)))) return rc;
+  if (!all.empty() && (rc = upload(s->d_gcls.p, all.data(), all.size() * sizeof(int), s->stream))) return rc;
+  std::vector<long long> off;
+  struct Pend { int la, lb, lc, ld; size_t off0; int cb, ck, nbra; long long nwork; };
+  std::vector<Pend> pend;
+  const int ncls = class_of(s->lmax, s->lmax) + 1;
+  for (int cb = 0; cb < ncls; ++cb)
+    for (int ck = 0; ck <= cb; ++ck) {
+      const auto& QB = clsQ[cb];
+      const auto& QK = clsQ[ck];
+      if (QB.empty() || QK.empty()) continue;
+      const size_t off0 = off.size();
+      off.push_back(0);
+      long long tot = 0;
+      for (size_t p = 0; p < QB.size(); ++p) {
+        const double thr = s->grad_cutoff / QB[p];
+        size_t k = std::lower_bound(QK.begin(), QK.end(), thr, [](double a, double t) { return a >= t; }) - QK.begin();
+        if (cb == ck) k = std::min(k, p + 1);
+        tot += (long long)k;
+        off.push_back(tot);
+      }
+      int la = 0, lb = 0, lc = 0, ld = 0;
+      for (int x = 0; x <= 3; ++x)
+        for (int y = 0; y <= x; ++y) {
+          if (class_of(x, y) == cb) { la = x; lb = y; }
+          if (class_of(x, y) == ck) { lc = x; ld = y; }
+        }
+      pend.push_back({la, lb, lc, ld, off0, cb, ck, (int)QB.size(), tot});
+    }
+  if ((rc = s->d_goff.ensure(std::max<size_t>(off.size(), 1)))) return rc;
+  if (!off.empty() && (rc = upload(s->d_goff.p, off.data(), off.size() * sizeof(long long), s->stream))) return rc;
+  s->glaunches.clear();
+  for (auto& p : pend) {
+    oqph_session::Launch L;
+    L.la = p.la; L.lb = p.lb; L.lc = p.lc; L.ld = p.ld;
+    L.cw.bra = s->d_gcls.p + coff[p.cb];
+    L.cw.ket = s->d_gcls.p + coff[p.ck];
+    L.cw.off = s->d_goff.p + p.off0;
+    L.cw.nbra = p.nbra;
+    L.cw.nwork = p.nwork;
+    s->glaunches.push_back(L);
+  }
+  // Cartesian offsets, shell atoms and flags
+  std::vector<int> cof(s->nsh), pu(s->nsh);
+  int ncf = 0;
+  for (int i = 0; i < s->nsh; ++i) {
+    cof[i] = ncf;
+    ncf += ncart(s->am[i]);
+    pu[i] = (s->harm[i] && s->am[i] >= 2) ? 1 : 0;
+  }
+  s->ncf = ncf;
+  if ((rc = s->d_coff.ensure(s->nsh)) || (rc = s->d_satom.ensure(s->nsh)) || (rc = s->d_am.ensure(s->nsh)) ||
+      (rc = s->d_pure.ensure(s->nsh)) || (rc = s->d_Dc.ensure((size_t)ncf * ncf)) ||
+      (rc = s->d_de.ensure(3 * (size_t)s->natm)))
+    return rc;
+  if ((rc = upload(s->d_coff.p, cof.data(), s->nsh * sizeof(int), s->stream)) ||
+      (rc = upload(s->d_satom.p, s->atom.data(), s->nsh * sizeof(int), s->stream)) ||
+      (rc = upload(s->d_am.p, s->am.data(), s->nsh * sizeof(int), s->stream)) ||
+      (rc = upload(s->d_pure.p, pu.data(), s->nsh * sizeof(int), s->stream)))
+    return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->gcut_ready = s->grad_cutoff;
   return 0;
 }
 
@@ -480,6 +587,8 @@ int oqph_session_destroy(oqph_session* s) {
   hipSetDevice(s->device);
   s->d_pairs.release(); s->d_prims.release(); s->d_rys.release(); s->d_herm.release(); s->d_cls.release();
   s->d_xform.release();
+  s->d_gpairs.release(); s->d_gprims.release(); s->d_gcls.release(); s->d_coff.release(); s->d_satom.release();
+  s->d_am.release(); s->d_pure.release(); s->d_goff.release(); s->d_Dc.release(); s->d_de.release();
   s->d_aooff.release(); s->d_nao.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
   s->d_tmp.release(); s->d_off.release(); s->d_ctr.release();
   for (auto e : s->pev) hipEventDestroy(e);
@@ -673,10 +782,72 @@ int oqph_fock_jk_inc(oqph_session* s, const double* d, double* d_old, double* f,
 }
 
 int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, double coulscale, oqph_stats* st) {
-  (void)d; (void)de; (void)hfscale; (void)coulscale; (void)st;
-  if (!s) return 4;
-  set_err("grad2: not available in this build");
-  return 1;
+  if (!s || !d || !de) return 4;
+  if (int fc = fault_code()) {
+    set_err("OQPH_FAULT injected decline");
+    return fc;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  int rc = prepare_grad(s);
+  if (rc) return rc;
+  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+  hipStream_t stream = s->stream;
+  if ((rc = upload(s->d_D.p, d, ntri * sizeof(double), stream))) return rc;
+  launch_cart_density(s->d_D.p, s->nsh, s->d_am.p, s->d_pure.p, s->d_aooff.p, s->d_coff.p, s->d_xform.p, s->ncf,
+                      s->d_Dc.p, stream);
+  HIPCHK(hipMemsetAsync(s->d_de.p, 0, 3 * (size_t)s->natm * sizeof(double), stream));
+  const size_t nl = s->glaunches.size();
+  if ((rc = s->d_ctr.ensure(8 + 8 * std::max(nl, s->launches.size())))) return rc;
+  HIPCHK(hipMemsetAsync(s->d_ctr.p + 8, 0, 8 * nl * sizeof(unsigned long long), stream));
+  double zbig = 0.0;
+  for (double e : s->ex) zbig = std::max(zbig, e);
+  double dabcut = 1.0e-11;                 // grd2.F90:224-226
+  if (zbig > 1.0e6) dabcut /= 10;
+  if (zbig > 1.0e7) dabcut /= 10;
+  GradParams p{};
+  p.pairs = s->d_gpairs.p;
+  p.prims = s->d_gprims.p;
+  p.rys = s->d_rys.p;
+  p.herm = s->d_herm.p;
+  p.Dc = s->d_Dc.p;
+  p.coff = s->d_coff.p;
+  p.shell_atom = s->d_satom.p;
+  p.de = s->d_de.p;
+  p.ncf = s->ncf;
+  p.natm = s->natm;
+  p.gcut = s->grad_cutoff;
+  p.dabcut2 = dabcut * dabcut;
+  p.cs = coulscale;
+  p.hs = hfscale;
+  HIPCHK(hipEventRecord(s->ev0, stream));
+  for (size_t i = 0; i < nl; ++i) {
+    auto& L = s->glaunches[i];
+    p.counters = s->d_ctr.p + 8 + 8 * i;
+    if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.cw, stream)) {
+      set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
+      return 1;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(s->ev1, stream));
+  HIPCHK(hipMemcpyAsync(de, s->d_de.p, 3 * (size_t)s->natm * sizeof(double), hipMemcpyDeviceToHost, stream));
+  std::vector<unsigned long long> ctr(8 * nl);
+  if (nl) HIPCHK(hipMemcpyAsync(ctr.data(), s->d_ctr.p + 8, 8 * nl * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    for (size_t i = 0; i < nl; ++i) {
+      st->quartets_computed += (long long)ctr[8 * i];
+      st->quartets_screened += (long long)ctr[8 * i + 1];
+      st->model_flops += (double)ctr[8 * i + 3];
+      st->prim_quartets += (long long)ctr[8 * i + 4];
+    }
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    st->kernel_ms = ms;
+  }
+  return 0;
 }
 
 void routec_fock_jk(const double* d, double* f, const int* nbf, const int* nfocks, const double* se,

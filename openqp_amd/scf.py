@@ -1,0 +1,122 @@
+"""Host RHF SCF loop around the device Fock builder.
+
+Restates the semantics of OpenQP's scf_driver for closed shells
+(source/scf.F90:46-1364, source/scf_addons.F90:1972-2145):
+
+* Fock: F = H + G[D], G from the device build (packed, fock_jk semantics);
+* energy: E = 0.5 (Tr D F + Tr D H) + E_nuc (scf_addons.F90:2059-2065);
+* incremental builds: F = F_old + G[D - D_old]; the history is reset every
+  10th iteration and whenever the DIIS error is below 1e-4
+  (scf.F90:768-773);
+* convergence: DIIS error max|X^T (F D S - S D F) X| below ``conv`` and the
+  energy change below ``econv``;
+* Pulay DIIS on the orthonormal-basis commutator.
+
+The initial guess is the core Hamiltonian (converged energies do not depend
+on it for the closed-shell systems used here).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sl
+
+from . import basis as B
+from .lib import FockBuilder, int1e
+
+
+@dataclass
+class SCFResult:
+    energy: float
+    converged: bool
+    iterations: int
+    density: np.ndarray          # full total density
+    fock: np.ndarray             # full Fock matrix
+    orbitals: np.ndarray
+    orbital_energies: np.ndarray
+    fock_ms: list = field(default_factory=list)
+    history: list = field(default_factory=list)
+
+
+class RHF:
+    def __init__(self, bas: B.Basis, builder: FockBuilder | None = None, nocc: int | None = None,
+                 conv: float = 1e-6, econv: float = 1e-8, max_iter: int = 100, incremental: bool = True,
+                 diis_size: int = 8):
+        self.bas = bas
+        self.builder = builder or FockBuilder(bas)
+        self.nocc = nocc if nocc is not None else int(round(bas.zn.sum())) // 2
+        self.conv, self.econv, self.max_iter = conv, econv, max_iter
+        self.incremental = incremental
+        self.diis_size = diis_size
+        S, T, V = int1e(bas)
+        self.S, self.H = S, T + V
+        self.enuc = B.nuclear_repulsion(bas.zn, bas.xyz)
+        e, U = np.linalg.eigh(S)
+        keep = e > 1e-8
+        self.X = U[:, keep] / np.sqrt(e[keep])
+
+    def _density(self, F):
+        Fo = self.X.T @ F @ self.X
+        e, Co = np.linalg.eigh(Fo)
+        C = self.X @ Co
+        Cocc = C[:, :self.nocc]
+        return 2.0 * Cocc @ Cocc.T, C, e
+
+    def run(self, D0: np.ndarray | None = None) -> SCFResult:
+        nbf = self.bas.nbf
+        D = self._density(self.H)[0] if D0 is None else D0
+        dp = B.pack_tri(D)
+        d_old = np.zeros_like(dp)
+        f_old = np.zeros_like(dp)
+        errs, focks = [], []
+        e_prev, diis_err = 0.0, 1.0
+        res = SCFResult(0.0, False, 0, D, self.H, None, None)
+        for it in range(1, self.max_iter + 1):
+            t0 = time.perf_counter()
+            if self.incremental:
+                if it > 1 and (it % 10 == 0 or diis_err < 1.0e-4):
+                    d_old[:] = 0.0
+                    f_old[:] = 0.0
+                g = self.builder.fock_jk_inc(dp, d_old, f_old)
+            else:
+                g = self.builder.fock_jk(dp)
+            res.fock_ms.append((time.perf_counter() - t0) * 1e3)
+            F = self.H + B.unpack_tri(g, nbf)
+            E = 0.5 * (np.sum(D * F) + np.sum(D * self.H)) + self.enuc
+            err = self.X.T @ (F @ D @ self.S - self.S @ D @ F) @ self.X
+            diis_err = float(np.abs(err).max())
+            res.history.append((it, E, diis_err))
+            if it > 1 and diis_err < self.conv and abs(E - e_prev) < self.econv:
+                res.energy, res.converged, res.iterations = E, True, it
+                res.density, res.fock = D, F
+                _, res.orbitals, res.orbital_energies = self._density(F)
+                return res
+            e_prev = E
+            focks.append(F)
+            errs.append(err)
+            if len(focks) > self.diis_size:
+                focks.pop(0)
+                errs.pop(0)
+            n = len(focks)
+            if n > 1:
+                Bm = -np.ones((n + 1, n + 1))
+                Bm[n, n] = 0.0
+                for i in range(n):
+                    for j in range(n):
+                        Bm[i, j] = np.sum(errs[i] * errs[j])
+                rhs = np.zeros(n + 1)
+                rhs[n] = -1.0
+                try:
+                    c = sl.solve(Bm, rhs)[:n]
+                    Fx = sum(ci * Fi for ci, Fi in zip(c, focks))
+                except sl.LinAlgError:
+                    Fx = F
+            else:
+                Fx = F
+            D, C, eps = self._density(Fx)
+            dp = B.pack_tri(D)
+        res.energy, res.iterations = E, self.max_iter
+        res.density, res.fock = D, F
+        return res

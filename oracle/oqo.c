@@ -236,10 +236,11 @@ static int nao_of(const oqo_basis *b, int s) {
 
 typedef struct {
   double integral, pair, quartet, expo;
+  double dab2, dabcut;   /* gradient: skip primitive quartets with fqz^2 dabmax^2 < dabcut ze */
 } oqo_cut;
 
 static oqo_cut make_cut(double c) { /* int2.F90:266-272 */
-  oqo_cut r = {c, 1.0e-2 * c, 1.0e-4 * c, 25.0 * log(10.0)};
+  oqo_cut r = {c, 1.0e-2 * c, 1.0e-4 * c, 25.0 * log(10.0), 0.0, 0.0};
   return r;
 }
 
@@ -346,6 +347,7 @@ static int raw_block(int La, int Lb, int Lc, int Ld, const double *A, const doub
       if (fabs(sp) < cut->quartet) continue;             /* int_rotaxis.F90:295 */
       double fqz = sp * sq, zeta = p->g, eta = q->g, ze = zeta + eta;
       if (fqz * fqz < qc2 * ze) continue;                /* int_rotaxis.F90:629 */
+      if (cut->dabcut > 0.0 && fqz * fqz * cut->dab2 < cut->dabcut * ze) continue; /* grd2_rys.F90:376 */
       any = 1;
       double rho = zeta * eta / ze;
       double Wc[3], PA[3], WP[3], QC[3], WQ[3], pq2 = 0.0;
@@ -546,7 +548,7 @@ void oqo_schwarz(int nsh, int nbf, int natm, const int *am, const int *harm, con
                  const double *cc, const double *xyz, double *Q) {
   build_tables();
   oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
-  oqo_cut cut = {1.0e-15, 1.0e-17, 1.0e-17, 50.0};
+  oqo_cut cut = {1.0e-15, 1.0e-17, 1.0e-17, 50.0, 0.0, 0.0};
   int mp = max_prim(&bs);
 #pragma omp parallel
   {
@@ -816,7 +818,7 @@ void oqo_eri_tensor(int nsh, int nbf, int natm, const int *am, const int *harm, 
                     const double *cc, const double *xyz, double *eri) {
   build_tables();
   oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
-  oqo_cut cut = {0.0, 0.0, 0.0, 1e300};
+  oqo_cut cut = {0.0, 0.0, 0.0, 1e300, 0.0, 0.0};
   int mp = max_prim(&bs);
   work_t w;
   work_init(&w);
@@ -977,4 +979,188 @@ void oqo_int1e(int nsh, int nbf, int natm, const int *am, const int *harm, const
         }
       }
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* Two-electron nuclear gradient (grd2_driver_gen + hf_2e_grad, RHF)   */
+/* ------------------------------------------------------------------ */
+/* AO <- raw canonical Cartesian map of one shell: X[i][p] with
+ * chi_i = sum_p X[i][p] phi_p, phi_p the x^l-normalised Cartesians. */
+static void shell_xmap(int l, int pure, double *X /* [nao][ncart] */) {
+  int nc = NCART(l), no = (pure && l >= 2) ? 2 * l + 1 : nc;
+  memset(X, 0, sizeof(double) * no * nc);
+  for (int p = 0; p < nc; ++p) {      /* p: canonical local index */
+    int q = out_pos(l, p);            /* OpenQP position of that component */
+    double nrm = pnrm2(l, q);
+    if (no == nc) X[q * nc + p] = nrm;
+    else
+      for (int m = 0; m < no; ++m) X[m * nc + p] = c2s_mat[l][q][m] * nrm;
+  }
+}
+
+/*
+ * dE2/dR at fixed density, E2 = 0.5 sum (ab|cd)[cs D_ab D_cd - hs/2 D_ac D_bd]
+ * (Coulomb x coulscale + exchange x hfscale), as hf_2e_grad accumulates it
+ * (hf_gradient.F90:428-495 for Gamma, grd2.F90:145-391 for loops/screens):
+ *   Gamma_abcd = 4 cs Dc_ab Dc_cd - hs (Dc_ac Dc_bd + Dc_ad Dc_bc)
+ * on the Cartesian-effective density Dc = X^T D X; coarse screen
+ * Q_ij Q_kl < grad_cutoff; fine screen max|Gamma| Q_ij Q_kl < grad_cutoff/2;
+ * pair data with exponent cutoff 20 ln 10 and prefactor cutoff 1e-20,
+ * primitive screen fqz^2 max|Gamma|^2 < (1e-11)^2 ze (grd2.F90:219-241,
+ * grd2_rys.F90:375-376).  Derivatives: d/dA (a| = 2 alpha (a+1| - a (a-1|
+ * per primitive, D by translational invariance.  de: [natm][3].
+ */
+int oqo_grad2(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+              const int *nprim, const int *poff, const int *aooff, const double *ex,
+              const double *cc, const double *xyz, const double *Q, const double *d, double *de,
+              double hfscale, double coulscale, double cutoff, long *stats) {
+  build_tables();
+  oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
+  /* Cartesian offsets and Dc = X^T D X */
+  int *coff = (int *)malloc(sizeof(int) * nsh);
+  int ncf = 0;
+  for (int s = 0; s < nsh; ++s) { coff[s] = ncf; ncf += NCART(am[s]); }
+  double *Dc = (double *)calloc((size_t)ncf * ncf, sizeof(double));
+  double Xa[15 * 15], Xb[15 * 15];
+  for (int a = 0; a < nsh; ++a) {
+    int na = nao_of(&bs, a), ca = NCART(am[a]);
+    shell_xmap(am[a], harm[a], Xa);
+    for (int b = 0; b < nsh; ++b) {
+      int nb = nao_of(&bs, b), cb = NCART(am[b]);
+      shell_xmap(am[b], harm[b], Xb);
+      for (int p = 0; p < ca; ++p)
+        for (int q = 0; q < cb; ++q) {
+          double s = 0.0;
+          for (int i = 0; i < na; ++i)
+            for (int j = 0; j < nb; ++j) {
+              long I = aooff[a] + i, J = aooff[b] + j;
+              long ij = I > J ? I * (I + 1) / 2 + J : J * (J + 1) / 2 + I;
+              s += Xa[i * ca + p] * d[ij] * Xb[j * cb + q];
+            }
+          Dc[(size_t)(coff[a] + p) * ncf + coff[b] + q] = s;
+        }
+    }
+  }
+  double zbig = 0.0;
+  int ntot = 0;
+  for (int s = 0; s < nsh; ++s) ntot += nprim[s];
+  for (int x = 0; x < ntot; ++x) if (ex[x] > zbig) zbig = ex[x];
+  double dabcut = 1.0e-11;
+  if (zbig > 1.0e6) dabcut /= 10;
+  if (zbig > 1.0e7) dabcut /= 10;
+  /* gdat%dabcut holds dabcut squared (grd2_rys.F90:146) */
+  oqo_cut base = {dabcut, 1.0e-20, 1.0e-20, 20.0 * log(10.0), 0.0, dabcut * dabcut};
+  int mp = max_prim(&bs);
+  long skip1 = 0, skip2 = 0, nint = 0;
+  int nthr = 1;
+#ifdef _OPENMP
+  nthr = omp_get_max_threads();
+#endif
+  double *dethr = (double *)calloc((size_t)nthr * natm * 3, sizeof(double));
+#pragma omp parallel reduction(+ : skip1, skip2, nint)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double *del = dethr + (size_t)tid * natm * 3;
+    work_t w;
+    work_init(&w);
+    ppair *bp = (ppair *)malloc(sizeof(ppair) * mp * mp), *bq = (ppair *)malloc(sizeof(ppair) * mp * mp);
+    double *G = (double *)malloc(sizeof(double) * 10000);
+    double *Rp = (double *)malloc(sizeof(double) * BLK), *Rm = (double *)malloc(sizeof(double) * BLK);
+#pragma omp for schedule(dynamic, 1)
+    for (int i = 0; i < nsh; ++i)
+      for (int j = 0; j <= i; ++j) {
+        int np = make_pairs(&bs, i, j, &base, bp);
+        if (np == 0) continue;
+        for (int k = 0; k <= i; ++k) {
+          int lmx = (k == i) ? j : k;
+          for (int l = 0; l <= lmx; ++l) {
+            int nq = make_pairs(&bs, k, l, &base, bq);
+            if (nq == 0) continue;
+            double gmax = Q[i * nsh + j] * Q[k * nsh + l];
+            if (gmax < cutoff) { ++skip1; continue; }
+            int sh[4] = {i, j, k, l};
+            int L[4] = {am[i], am[j], am[k], am[l]};
+            int nc[4] = {NCART(L[0]), NCART(L[1]), NCART(L[2]), NCART(L[3])};
+            /* Gamma in raw canonical Cartesians */
+            double gm = 0.0;
+            for (int a = 0; a < nc[0]; ++a)
+              for (int b = 0; b < nc[1]; ++b)
+                for (int c = 0; c < nc[2]; ++c)
+                  for (int e = 0; e < nc[3]; ++e) {
+                    size_t A = coff[i] + a, B_ = coff[j] + b, C = coff[k] + c, E = coff[l] + e;
+                    double g = 4.0 * coulscale * Dc[A * ncf + B_] * Dc[C * ncf + E] -
+                               hfscale * (Dc[A * ncf + C] * Dc[B_ * ncf + E] + Dc[A * ncf + E] * Dc[B_ * ncf + C]);
+                    G[((a * nc[1] + b) * nc[2] + c) * nc[3] + e] = g;
+                    if (fabs(g) > gm) gm = fabs(g);
+                  }
+            if (gm * gmax < 0.5 * cutoff) { ++skip2; continue; }
+            ++nint;
+            oqo_cut cut = base;
+            cut.dab2 = gm * gm;
+            double deg = 1.0;
+            if (i == j) deg *= 0.5;
+            if (k == l) deg *= 0.5;
+            if (i == k && j == l) deg *= 0.5;
+            const double *cen[4] = {xyz + 3 * atom[i], xyz + 3 * atom[j], xyz + 3 * atom[k], xyz + 3 * atom[l]};
+            double fd[4][3] = {{0}};
+            for (int ctr = 0; ctr < 3; ++ctr) {   /* derivative centre A, B, C */
+              int Lp[4] = {L[0], L[1], L[2], L[3]}, Lm[4] = {L[0], L[1], L[2], L[3]};
+              Lp[ctr] += 1;
+              Lm[ctr] -= 1;
+              int okp = raw_block(Lp[0], Lp[1], Lp[2], Lp[3], cen[0], cen[1], cen[2], cen[3], bp, np, bq, nq,
+                                  &cut, ctr + 1, &w, Rp);
+              int okm = (Lm[ctr] >= 0) ? raw_block(Lm[0], Lm[1], Lm[2], Lm[3], cen[0], cen[1], cen[2], cen[3], bp,
+                                                   np, bq, nq, &cut, 0, &w, Rm) : 0;
+              if (!okp && !okm) continue;
+              int ncp[4] = {NCART(Lp[0]), NCART(Lp[1]), NCART(Lp[2]), NCART(Lp[3])};
+              int ncm[4] = {Lm[0] >= 0 ? NCART(Lm[0]) : 0, Lm[1] >= 0 ? NCART(Lm[1]) : 0,
+                            Lm[2] >= 0 ? NCART(Lm[2]) : 0, Lm[3] >= 0 ? NCART(Lm[3]) : 0};
+              int lev0 = NCUM(L[ctr] - 1 < 0 ? -1 : L[ctr] - 1);
+              int levp = NCUM(L[ctr]);
+              int levm = NCUM(L[ctr] - 2 < 0 ? -1 : L[ctr] - 2);
+              for (int a = 0; a < nc[0]; ++a)
+                for (int b = 0; b < nc[1]; ++b)
+                  for (int c = 0; c < nc[2]; ++c)
+                    for (int e = 0; e < nc[3]; ++e) {
+                      double g = G[((a * nc[1] + b) * nc[2] + c) * nc[3] + e];
+                      if (g == 0.0) continue;
+                      int idx[4] = {a, b, c, e};
+                      int gcomp = lev0 + idx[ctr];
+                      for (int dd = 0; dd < 3; ++dd) {
+                        int ip[4] = {a, b, c, e};
+                        double v = 0.0;
+                        if (okp) {
+                          ip[ctr] = raise_(gcomp, dd) - levp;
+                          v += Rp[((ip[0] * ncp[1] + ip[1]) * ncp[2] + ip[2]) * ncp[3] + ip[3]];
+                        }
+                        int nlow = comp_l[gcomp][dd];
+                        if (okm && nlow > 0) {
+                          int im[4] = {a, b, c, e};
+                          im[ctr] = lower(gcomp, dd) - levm;
+                          v -= nlow * Rm[((im[0] * ncm[1] + im[1]) * ncm[2] + im[2]) * ncm[3] + im[3]];
+                        }
+                        fd[ctr][dd] += g * v;
+                      }
+                    }
+            }
+            for (int dd = 0; dd < 3; ++dd) fd[3][dd] = -(fd[0][dd] + fd[1][dd] + fd[2][dd]);
+            for (int ctr = 0; ctr < 4; ++ctr)
+              for (int dd = 0; dd < 3; ++dd) del[3 * atom[sh[ctr]] + dd] += deg * fd[ctr][dd];
+          }
+        }
+      }
+    free(bp); free(bq); free(G); free(Rp); free(Rm);
+    work_free(&w);
+  }
+  for (int x = 0; x < natm * 3; ++x) {
+    double s = 0.0;
+    for (int t = 0; t < nthr; ++t) s += dethr[(size_t)t * natm * 3 + x];
+    de[x] = s;
+  }
+  free(dethr); free(Dc); free(coff);
+  if (stats) { stats[0] = skip1; stats[1] = skip2; stats[2] = nint; }
+  return 0;
 }

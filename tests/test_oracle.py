@@ -198,3 +198,37 @@ def test_golden_fixtures_reproduce(oracle):
         g = np.load(os.path.join(ROOT, "tests", "golden", name + ".npz"))
         f, st = oracle.fock_jk(bas, g["d"], Q=g["Q"])
         assert np.abs(f - g["f"]).max() < 1e-12
+
+
+def _e2(oracle, bas, dp, hs=1.0, cs=1.0):
+    f, _ = oracle.fock_jk(bas, dp, se=hs, sc=cs, cutoff=1e-30)
+    idx = np.arange(1, bas.nbf + 1)
+    dg = idx * (idx + 1) // 2 - 1
+    return 0.5 * (2 * np.dot(dp, f) - np.dot(dp[dg], f[dg]))
+
+
+@pytest.mark.parametrize("name,nw,nocc", [("6-31g(d)", 1, 5), ("cc-pvdz", 2, 10)])
+def test_grad2_matches_finite_differences(oracle, name, nw, nocc):
+    """dE2/dR at fixed density vs central differences (1e-4 bohr) of
+    E2 = 0.5 Tr D G[D]; translational invariance to rounding."""
+    bas = small_system(name, nw=nw, seed=1)
+    D, dp = density_for(oracle, bas, nocc, seed=3)
+    g, st = oracle.grad2(bas, dp, cutoff=1e-30)
+    assert np.abs(g.sum(axis=0)).max() < 1e-12
+    h = 1e-4
+    rng = np.random.default_rng(0)
+    for a, d in [(int(rng.integers(bas.natm)), int(rng.integers(3))) for _ in range(4)]:
+        xp = bas.xyz.copy(); xp[a, d] += h
+        xm = bas.xyz.copy(); xm[a, d] -= h
+        fd = (_e2(oracle, bas.with_coords(xp), dp) - _e2(oracle, bas.with_coords(xm), dp)) / (2 * h)
+        assert abs(g[a, d] - fd) < 5e-8, (a, d, g[a, d], fd)
+
+
+def test_grad2_coulomb_exchange_split(oracle):
+    bas = small_system("6-31g(d)")
+    D, dp = density_for(oracle, bas, 5, seed=4)
+    gj, _ = oracle.grad2(bas, dp, hfscale=0.0, coulscale=1.0, cutoff=1e-30)
+    gk, _ = oracle.grad2(bas, dp, hfscale=1.0, coulscale=0.0, cutoff=1e-30)
+    g, _ = oracle.grad2(bas, dp, hfscale=1.0, coulscale=1.0, cutoff=1e-30)
+    # linear up to the max|Gamma|-dependent primitive screen (grd2_rys.F90:376)
+    assert np.abs(gj + gk - g).max() < 5e-9
