@@ -210,6 +210,10 @@ const double* rys_table_host(size_t* n) {
   *n = RYS_TABLE_SIZE;
   return RYS_TABLE;
 }
+const float* rys32_table_host(size_t* n) {
+  *n = RYS32_TABLE_SIZE;
+  return RYS32_TABLE;
+}
 const double* rys_herm_host(size_t* n) {
   *n = sizeof(RYS_HERM) / sizeof(double);
   return RYS_HERM;

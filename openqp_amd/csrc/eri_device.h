@@ -42,6 +42,7 @@ __host__ __device__ constexpr int binom(int n, int k) {
 
 // ------------------------------------------------------------------ Rys roots
 #include "rys_table.inc"
+#include "rys_table_f32.inc"
 
 template <int N, typename R>
 __device__ __forceinline__ void rys_roots(R T, const double* __restrict__ tab, const double* __restrict__ herm,
@@ -63,6 +64,31 @@ __device__ __forceinline__ void rys_roots(R T, const double* __restrict__ tab, c
     R v = (R)c[f * (RYS_DEG + 1) + RYS_DEG];
 #pragma unroll
     for (int j = RYS_DEG - 1; j >= 0; --j) v = v * s + (R)c[f * (RYS_DEG + 1) + j];
+    if (f < N) x[f] = v; else w[f - N] = v;
+  }
+}
+
+// FP32 roots: degree-6 fits, hardware reciprocal / reciprocal square root
+template <int N>
+__device__ __forceinline__ void rys_roots32(float T, const float* __restrict__ tab, const double* __restrict__ herm,
+                                            float* x, float* w) {
+  if (T >= (float)RYS_TASYM[N]) {
+    const float it = __builtin_amdgcn_rcpf(T), rs = __builtin_amdgcn_rsqf(T);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      x[i] = (float)herm[RYS_HOFF[N] + i] * it;
+      w[i] = (float)herm[RYS_HOFF[N] + N + i] * rs;
+    }
+    return;
+  }
+  const int k = (int)T;
+  const float s = 2.0f * (T - (float)k) - 1.0f;
+  const float* c = tab + RYS32_OFF[N] + k * (2 * N * (RYS32_DEG + 1));
+#pragma unroll
+  for (int f = 0; f < 2 * N; ++f) {
+    float v = c[f * (RYS32_DEG + 1) + RYS32_DEG];
+#pragma unroll
+    for (int j = RYS32_DEG - 1; j >= 0; --j) v = v * s + c[f * (RYS32_DEG + 1) + j];
     if (f < N) x[f] = v; else w[f - N] = v;
   }
 }
@@ -109,15 +135,24 @@ __host__ __device__ constexpr long long flops_prim() {
 template <int LA, int LB, int LC, int LD, typename R>
 __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, const double* __restrict__ Cc,
                                                const PrimPair& pp, const PrimPair& qq, R qcut2,
-                                               const double* __restrict__ tab, const double* __restrict__ herm,
+                                               const double* __restrict__ tab, const float* __restrict__ tab32,
+                                               const double* __restrict__ herm,
                                                R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]) {
   using Dm = Dims<LA, LB, LC, LD>;
   constexpr int LAB = Dm::LAB, LCD = Dm::LCD, NR = Dm::NROOT;
+  constexpr bool F32 = sizeof(R) == 4;
   const R zeta = (R)pp.g, eta = (R)qq.g;
   const R fqz = (R)pp.Kg * (R)qq.Kg;
   const R ze = zeta + eta;
   if (fqz * fqz < qcut2 * ze) return false;
-  const R ize = (R)1 / ze;
+  R ize, sq;
+  if constexpr (F32) {
+    ize = __builtin_amdgcn_rcpf(ze);
+    sq = __builtin_amdgcn_rsqf(ze);
+  } else {
+    ize = (R)1 / ze;
+    sq = sqrt(ize);
+  }
   const R rho = zeta * eta * ize;
   R PQ[3], PA[3], QC[3];
   R pq2 = 0;
@@ -129,11 +164,13 @@ __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, co
     pq2 += PQ[d] * PQ[d];
   }
   const R T = rho * pq2;
-  const R pref = fqz * sqrt(ize);
+  const R pref = fqz * sq;
   R rx[NR], rw[NR];
-  rys_roots<NR, R>(T, tab, herm, rx, rw);
-  const R rz = rho / zeta, re = rho / eta;
-  const R h2z = (R)0.5 / zeta, h2e = (R)0.5 / eta, h2ze = (R)0.5 * ize;
+  if constexpr (F32) rys_roots32<NR>(T, tab32, herm, rx, rw);
+  else rys_roots<NR, R>(T, tab, herm, rx, rw);
+  // rho/zeta = eta/(zeta+eta), rho/eta = zeta/(zeta+eta): no further divisions
+  const R rz = eta * ize, re = zeta * ize;
+  const R h2z = (R)pp.h, h2e = (R)qq.h, h2ze = (R)0.5 * ize;
   // large classes: keep the root loop rolled so the unrolled body stays compilable
   constexpr int UR = (Dm::NE * Dm::NF > 256) ? 1 : NR;
 #pragma unroll UR

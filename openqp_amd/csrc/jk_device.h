@@ -61,7 +61,8 @@ __device__ __forceinline__ void xform_index(const double* __restrict__ xf, const
 // Final-representation block of a quartet: V[a][b][c][d] in AO order.
 template <int LA, int LB, int LC, int LD, bool PURE, typename R>
 __device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
-                                              const double* __restrict__ tab, const double* __restrict__ herm,
+                                              const double* __restrict__ tab, const float* __restrict__ tab32,
+                                              const double* __restrict__ herm,
                                               const double* __restrict__ xf, R qcut, R (&Vout)[AO<LA, PURE>::N * AO<LB, PURE>::N *
                                                                 AO<LC, PURE>::N * AO<LD, PURE>::N],
                                               int& nprim_used) {
@@ -79,7 +80,7 @@ __device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPai
     for (int ip = 0; ip < P.nprim; ++ip) {
       const PrimPair pp = prims[P.prim0 + ip];
       if (fabs(pp.Kg) < (double)qcut) continue;
-      nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tab, herm, acc) ? 1 : 0;
+      nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tab, tab32, herm, acc) ? 1 : 0;
     }
   }
   R AB[3], CD[3];
@@ -135,6 +136,13 @@ __device__ __forceinline__ void sum_wave(unsigned long long* ctr, unsigned long 
   if (v != 0ull && __lane_id() == (unsigned)(__ffsll((long long)act) - 1)) atomicAdd(ctr, v);
 }
 
+// sum over all 64 lanes (call with the whole wavefront active)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 __device__ __forceinline__ void count_wave(unsigned long long* ctr, bool pred) {
   const unsigned long long m = __ballot(pred);
   const unsigned long long act = __ballot(1);
@@ -146,18 +154,19 @@ template <int LA, int LB, int LC, int LD, bool PURE>
 __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
   const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = w < cw.nwork;
-  int ip = 0, iq = 0;
+  int ip = -1, iq = 0;
   ShellPair P{}, Q{};
   bool skip = true;
   double bound = 0.0;
   if (active) {
-    int lo = 0, hi = cw.nbra;
+    // segment = bra * nbkt + ket bucket; the segment's kets are a prefix of the bucket
+    int lo = 0, hi = cw.nbra * cw.nbkt;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
       if (cw.off[mid] <= w) lo = mid; else hi = mid;
     }
-    ip = cw.bra[lo];
-    iq = cw.ket[w - cw.off[lo]];
+    ip = cw.bra[lo / cw.nbkt];
+    iq = cw.ket[cw.kbs[lo % cw.nbkt] + (int)(w - cw.off[lo])];
     P = prm.pairs[ip];
     Q = prm.pairs[iq];
     // ---- OpenQP screening (int2.F90:763-800)
@@ -178,6 +187,8 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
   constexpr int OA = AO<LA, PURE>::N, OB = AO<LB, PURE>::N, OC = AO<LC, PURE>::N, OD = AO<LD, PURE>::N;
   constexpr int NOUT = OA * OB * OC * OD;
   double V[NOUT];
+#pragma unroll
+  for (int i = 0; i < NOUT; ++i) V[i] = 0.0;
   int nprim = 0;
   bool lowp = false;
   if (!skip) {
@@ -187,25 +198,25 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
       lowp = prm.mixed && bound < prm.fp32_cut;
       if (lowp) {
         float Vf[NOUT];
-        quartet_block<LA, LB, LC, LD, PURE, float>(P, Q, prm.prims, prm.rys, prm.herm, prm.xform,
+        quartet_block<LA, LB, LC, LD, PURE, float>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, prm.xform,
                                                    (float)(1.0e-4 * prm.cutoff), Vf, nprim);
 #pragma unroll
         for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
       }
     }
     if (!lowp)
-      quartet_block<LA, LB, LC, LD, PURE, double>(P, Q, prm.prims, prm.rys, prm.herm, prm.xform,
+      quartet_block<LA, LB, LC, LD, PURE, double>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, prm.xform,
                                                   1.0e-4 * prm.cutoff, V, nprim);
   }
   count_wave(ctr + 2, lowp);
   sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() + (nprim ? 12ull * NOUT : 0ull));
   sum_wave(ctr + 4, (unsigned long long)nprim);
-  if (skip || nprim == 0) return;
+  const bool valid = !skip && nprim > 0;
+  // all lanes stay converged through the digestion; invalid lanes carry V = 0
   const double c = prm.cutoff;
 #pragma unroll
-  for (int i = 0; i < OA * OB * OC * OD; ++i)
-    if (fabs(V[i]) < c) V[i] = 0.0;
-  // degeneracy of the canonical quartet
+  for (int i = 0; i < NOUT; ++i)
+    if (!valid || fabs(V[i]) < c) V[i] = 0.0;
   double deg = 1.0;
   if (P.sa == P.sb) deg *= 0.5;
   if (Q.sa == Q.sb) deg *= 0.5;
@@ -214,6 +225,12 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
   const int oa = P.ao_a, ob = P.ao_b, oc = Q.ao_a, od = Q.ao_b;
   const double* Dp = prm.D;
   double* F = prm.F;
+  // wave-uniform bra: the J_ab block is reduced across the wavefront first
+  const unsigned long long vmask = __ballot(valid);
+  if (vmask == 0ull) return;  // wave-uniform exit: no lane has anything to digest
+  const int lead = __ffsll((long long)vmask) - 1;  // first lane with a quartet to digest
+  const int ip0 = __builtin_amdgcn_readlane(ip, lead);
+  const bool uni = __ballot(ip == ip0 || !valid) == __ballot(1);
   // J blocks
   {
     double Dcd[OC * OD], Dab[OA * OB];
@@ -233,7 +250,14 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
 #pragma unroll
         for (int cd = 0; cd < OC * OD; ++cd) s += V[(a * OB + b) * OC * OD + cd] * Dcd[cd];
         const int x = oa + a, y = ob + b;
-        fatomic(F + tri_idx(x, y), cj * s * (x == y ? 2.0 : 1.0));
+        const double v = cj * s * (x == y ? 2.0 : 1.0);
+        if (uni) {
+          const double t = wave_sum(v);
+          // the leading valid lane owns the bra's AO offsets; invalid lanes may not
+          if ((int)__lane_id() == lead && t != 0.0) fatomic(F + tri_idx(x, y), t);
+        } else if (v != 0.0) {
+          fatomic(F + tri_idx(x, y), v);
+        }
       }
 #pragma unroll
     for (int cc = 0; cc < OC; ++cc)
@@ -243,7 +267,8 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
 #pragma unroll
         for (int ab = 0; ab < OA * OB; ++ab) s += V[ab * OC * OD + cc * OD + d] * Dab[ab];
         const int x = oc + cc, y = od + d;
-        fatomic(F + tri_idx(x, y), cj * s * (x == y ? 2.0 : 1.0));
+        const double v = cj * s * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
       }
   }
   // K blocks: K_ac += V D_bd, K_ad += V D_bc, K_bc += V D_ad, K_bd += V D_ac
@@ -274,7 +299,8 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
 #pragma unroll
           for (int d = 0; d < OD; ++d) s += VV(a, b, cc, d) * Dbd[b * OD + d];
         const int x = oa + a, y = oc + cc;
-        fatomic(F + tri_idx(x, y), ck * s * (x == y ? 2.0 : 1.0));
+        const double v = ck * s * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
       }
 #pragma unroll
     for (int a = 0; a < OA; ++a)
@@ -286,7 +312,8 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
 #pragma unroll
           for (int cc = 0; cc < OC; ++cc) s += VV(a, b, cc, d) * Dbc[b * OC + cc];
         const int x = oa + a, y = od + d;
-        fatomic(F + tri_idx(x, y), ck * s * (x == y ? 2.0 : 1.0));
+        const double v = ck * s * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
       }
 #pragma unroll
     for (int b = 0; b < OB; ++b)
@@ -298,7 +325,8 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
 #pragma unroll
           for (int d = 0; d < OD; ++d) s += VV(a, b, cc, d) * Dad[a * OD + d];
         const int x = ob + b, y = oc + cc;
-        fatomic(F + tri_idx(x, y), ck * s * (x == y ? 2.0 : 1.0));
+        const double v = ck * s * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
       }
 #pragma unroll
     for (int b = 0; b < OB; ++b)
@@ -310,7 +338,8 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
 #pragma unroll
           for (int cc = 0; cc < OC; ++cc) s += VV(a, b, cc, d) * Dac[a * OC + cc];
         const int x = ob + b, y = od + d;
-        fatomic(F + tri_idx(x, y), ck * s * (x == y ? 2.0 : 1.0));
+        const double v = ck * s * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
       }
 #undef VV
   }
@@ -330,7 +359,7 @@ __global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const P
   constexpr int N = AO<LA, PURE>::N * AO<LB, PURE>::N * AO<LA, PURE>::N * AO<LB, PURE>::N;
   double V[N];
   int nprim;
-  quartet_block<LA, LB, LA, LB, PURE, double>(P, P, prims, tab, herm, xf, qcut, V, nprim);
+  quartet_block<LA, LB, LA, LB, PURE, double>(P, P, prims, tab, nullptr, herm, xf, qcut, V, nprim);
   double m = 0.0;
   if (nprim) {
 #pragma unroll

@@ -33,15 +33,19 @@ struct PrimPair {              // 64 bytes: one cache line per primitive pair
   double Kg;                   // K / zeta, K = sqrt(2) pi^(5/4) c_a c_b exp(-alpha beta/zeta AB^2)
   double P[3];                 // Gaussian product centre
   double a, b;                 // exponents of shell a and shell b
-  double pad;
+  double h;                    // 1 / (2 zeta)
 };
 
 // Per-class-pair work description for one launch.
+// The class lists are ordered by primitive-pair-count bucket, then by Q
+// descending; work segment s = bra * nbkt + bucket covers a prefix of that
+// ket bucket, so a wavefront sees one bra and kets of similar contraction.
 struct ClassWork {
-  const int* bra;              // pair indices of the bra class list, sorted by Q descending
-  const int* ket;              // pair indices of the ket class list, sorted by Q descending
-  const long long* off;        // prefix offsets of the ket counts per bra (nbra+1)
-  int nbra;
+  const int* bra;              // pair indices of the bra class list
+  const int* ket;              // pair indices of the ket class list
+  const long long* off;        // prefix offsets over segments (nbra*nbkt + 1)
+  const int* kbs;              // ket bucket starts in `ket` (nbkt + 1)
+  int nbra, nbkt;
   long long nwork;
 };
 
@@ -51,6 +55,7 @@ struct JKParams {
   const PrimPair* prims;
   const double* rys;
   const double* herm;
+  const float* rys32;   // FP32 root fits (degree 6)
   const double* xform;  // [2][2][10][10] canonical Cartesian -> AO transforms (L = 2, 3)
   const double* D;      // packed density (ntri)
   double* F;            // packed G (ntri), accumulated with atomics

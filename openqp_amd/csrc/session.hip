@@ -38,6 +38,7 @@ void launch_cart_density(const double* D, int nsh, const int* am, const int* pur
                          const double* xf, int ncf, double* Dc, hipStream_t s);
 const double* rys_table_host(size_t* n);
 const double* rys_herm_host(size_t* n);
+const float* rys32_table_host(size_t* n);
 }  // namespace oqph
 
 using namespace oqph;
@@ -100,6 +101,11 @@ struct DBuf {
 
 static inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 static inline int class_of(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+// primitive-pair-count buckets of the class lists
+static constexpr int NBKT = 6;
+static inline int bucket_of(int nprim) {
+  return nprim <= 1 ? 0 : nprim <= 4 ? 1 : nprim <= 9 ? 2 : nprim <= 20 ? 3 : nprim <= 40 ? 4 : 5;
+}
 
 struct oqph_session {
   int device = 0;
@@ -122,7 +128,10 @@ struct oqph_session {
   DBuf<ShellPair> d_pairs;
   DBuf<PrimPair> d_prims;
   DBuf<double> d_rys, d_herm, d_xform;
+  DBuf<float> d_rys32;
   DBuf<int> d_cls;                           // concatenated class lists
+  DBuf<int> d_kbs;                           // per class: bucket starts (NBKT+1)
+  std::vector<std::vector<int>> kbs;
   std::vector<size_t> cls_off;
   DBuf<int> d_aooff, d_nao;
   // per build
@@ -203,6 +212,7 @@ static void build_pairs(const oqph_session& s, const Cut& cut, std::vector<Shell
           for (int d = 0; d < 3; ++d) pp.P[d] = (al * A[d] + be * B[d]) / g;
           pp.a = al;
           pp.b = be;
+          pp.h = 0.5 / g;
           prims.push_back(pp);
         }
       sp.nprim = (int)prims.size() - sp.prim0;
@@ -230,6 +240,10 @@ static int prepare_geometry(oqph_session* s) {
   if ((rc = s->d_rys.ensure(nt)) || (rc = s->d_herm.ensure(nh))) return rc;
   if ((rc = upload(s->d_rys.p, tab, nt * sizeof(double), s->stream))) return rc;
   if ((rc = upload(s->d_herm.p, herm, nh * sizeof(double), s->stream))) return rc;
+  size_t n32;
+  const float* tab32 = rys32_table_host(&n32);
+  if ((rc = s->d_rys32.ensure(n32))) return rc;
+  if ((rc = upload(s->d_rys32.p, tab32, n32 * sizeof(float), s->stream))) return rc;
   double xf[2][2][10][10];
   build_xform(xf);
   if ((rc = s->d_xform.ensure(400))) return rc;
@@ -288,18 +302,31 @@ static int prepare_geometry(oqph_session* s) {
     if (sp.nprim == 0 || sp.Q <= 0.0) continue;
     s->cls[class_of(sp.la, sp.lb)].push_back((int)pid);
   }
-  std::vector<int> all;
+  std::vector<int> all, kbs_all;
   s->cls_off.assign(11, 0);
+  s->kbs.assign(10, std::vector<int>(NBKT + 1, 0));
   for (int c = 0; c < 10; ++c) {
     auto& v = s->cls[c];
-    std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return s->Q[x] > s->Q[y]; });
+    // primitive-count bucket first (uniform wavefronts), then Q descending
+    std::stable_sort(v.begin(), v.end(), [&](int x, int y) {
+      const int bx = bucket_of(s->pairs[x].nprim), by = bucket_of(s->pairs[y].nprim);
+      return bx != by ? bx < by : s->Q[x] > s->Q[y];
+    });
     for (int x : v) s->clsQ[c].push_back(s->Q[x]);
+    for (int b = 0; b <= NBKT; ++b) {
+      int k = 0;
+      while (k < (int)v.size() && bucket_of(s->pairs[v[k]].nprim) < b) ++k;
+      s->kbs[c][b] = (b == NBKT) ? (int)v.size() : k;
+    }
+    kbs_all.insert(kbs_all.end(), s->kbs[c].begin(), s->kbs[c].end());
     s->cls_off[c] = all.size();
     all.insert(all.end(), v.begin(), v.end());
   }
   s->cls_off[10] = all.size();
   if ((rc = s->d_cls.ensure(all.size()))) return rc;
   if (!all.empty() && (rc = upload(s->d_cls.p, all.data(), all.size() * sizeof(int), s->stream))) return rc;
+  if ((rc = s->d_kbs.ensure(kbs_all.size() + 2))) return rc;
+  if ((rc = upload(s->d_kbs.p, kbs_all.data(), kbs_all.size() * sizeof(int), s->stream))) return rc;
   if ((rc = s->d_aooff.ensure(s->nsh)) || (rc = s->d_nao.ensure(s->nsh))) return rc;
   if ((rc = upload(s->d_aooff.p, s->aooff.data(), s->nsh * sizeof(int), s->stream))) return rc;
   if ((rc = upload(s->d_nao.p, s->nao.data(), s->nsh * sizeof(int), s->stream))) return rc;
@@ -379,7 +406,9 @@ static int prepare_grad(oqph_session* s) {
     L.cw.bra = s->d_gcls.p + coff[p.cb];
     L.cw.ket = s->d_gcls.p + coff[p.ck];
     L.cw.off = s->d_goff.p + p.off0;
+    L.cw.kbs = nullptr;
     L.cw.nbra = p.nbra;
+    L.cw.nbkt = 1;
     L.cw.nwork = p.nwork;
     s->glaunches.push_back(L);
   }
@@ -406,7 +435,10 @@ static int prepare_grad(oqph_session* s) {
   return 0;
 }
 
-// per-bra ket counts and launch list for a given maxden
+// per-(bra, ket bucket) counts and launch list for a given maxden: the kets
+// of a bucket that survive Q_bra Q_ket 4 maxD >= c form a prefix of it; for a
+// class with itself each unordered pair is taken once, the bra being the later
+// entry of the (bucket, Q) order.
 static int prepare_lists(oqph_session* s, double maxden) {
   if (maxden == s->cached_maxden) return 0;
   s->launches.clear();
@@ -425,16 +457,28 @@ static int prepare_lists(oqph_session* s, double maxden) {
       const auto& QB = s->clsQ[cb];
       const auto& QK = s->clsQ[ck];
       if (QB.empty() || QK.empty()) continue;
+      const auto& kb = s->kbs[ck];
+      const auto& bb = s->kbs[cb];
       const size_t off0 = s->h_off.size();
       s->h_off.push_back(0);
       long long tot = 0;
+      int bp = 0;
       for (size_t p = 0; p < QB.size(); ++p) {
+        while (bp + 1 < NBKT && (int)p >= bb[bp + 1]) ++bp;
         const double thr = c / (4.0 * maxden * QB[p]);
-        // number of leading ket entries with Q >= thr
-        size_t k = std::lower_bound(QK.begin(), QK.end(), thr, [](double a, double t) { return a >= t; }) - QK.begin();
-        if (cb == ck) k = std::min(k, p + 1);
-        tot += (long long)k;
-        s->h_off.push_back(tot);
+        for (int b = 0; b < NBKT; ++b) {
+          long long k = 0;
+          if (kb[b + 1] > kb[b]) {
+            k = std::lower_bound(QK.begin() + kb[b], QK.begin() + kb[b + 1], thr,
+                                 [](double a, double t) { return a >= t; }) - (QK.begin() + kb[b]);
+            if (cb == ck) {
+              if (b > bp) k = 0;
+              else if (b == bp) k = std::min<long long>(k, (long long)p - bb[bp] + 1);
+            }
+          }
+          tot += k;
+          s->h_off.push_back(tot);
+        }
       }
       int la = 0, lb = 0, lc = 0, ld = 0;
       for (int x = 0; x <= 3; ++x)
@@ -453,7 +497,9 @@ static int prepare_lists(oqph_session* s, double maxden) {
     L.cw.bra = s->d_cls.p + s->cls_off[p.cb];
     L.cw.ket = s->d_cls.p + s->cls_off[p.ck];
     L.cw.off = s->d_off.p + p.off0;
+    L.cw.kbs = s->d_kbs.p + p.ck * (NBKT + 1);
     L.cw.nbra = p.nbra;
+    L.cw.nbkt = NBKT;
     L.cw.nwork = p.nwork;
     s->launches.push_back(L);
   }
@@ -495,6 +541,7 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   p.prims = s->d_prims.p;
   p.rys = s->d_rys.p;
   p.herm = s->d_herm.p;
+  p.rys32 = s->d_rys32.p;
   p.xform = s->d_xform.p;
   p.D = dD;
   p.F = dF;
@@ -586,7 +633,7 @@ int oqph_session_destroy(oqph_session* s) {
   }
   hipSetDevice(s->device);
   s->d_pairs.release(); s->d_prims.release(); s->d_rys.release(); s->d_herm.release(); s->d_cls.release();
-  s->d_xform.release();
+  s->d_xform.release(); s->d_rys32.release(); s->d_kbs.release();
   s->d_gpairs.release(); s->d_gprims.release(); s->d_gcls.release(); s->d_coff.release(); s->d_satom.release();
   s->d_am.release(); s->d_pure.release(); s->d_goff.release(); s->d_Dc.release(); s->d_de.release();
   s->d_aooff.release(); s->d_nao.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();

