@@ -85,6 +85,38 @@ def cpu_baseline(bas, dp, gpu_quartets, target_s=15.0):
             "quartets_per_s": round(int(st[1]) / dt, 1)}
 
 
+def gradient_leg(steps, warmup):
+    """2e nuclear gradient evals/s on C20H42 / 6-31G(d) (384 bf, Cartesian d) at
+    the converged FP64 density of an SCF run on the device Fock builder."""
+    from openqp_amd import basis as B, molecules as M
+    from openqp_amd.lib import FockBuilder
+    from openqp_amd.scf import RHF
+    bld, bname, nocc = M.WORKLOADS["c20h42_631gd"]
+    Z, xyz = bld(0)
+    bas = B.build_basis(bname, Z, xyz)
+    fb = FockBuilder(bas)
+    t0 = time.perf_counter()
+    res = RHF(bas, fb, conv=1e-6, econv=1e-8).run()
+    t_scf = time.perf_counter() - t0
+    dp = B.pack_tri(res.density)
+    for _ in range(warmup):
+        fb.grad2(dp)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = fb.grad2(dp)
+    dt = (time.perf_counter() - t0) / steps
+    st = fb.last_stats
+    out = {"workload": "c20h42_631gd", "nbf": bas.nbf, "natm": bas.natm, "scf_energy": res.energy,
+           "scf_iterations": res.iterations, "scf_converged": res.converged, "scf_wall_s": round(t_scf, 3),
+           "scf_fock_ms_mean": round(float(np.mean(res.fock_ms)), 3),
+           "evals_per_s": round(1.0 / dt, 4), "ms_per_eval": round(dt * 1e3, 3),
+           "kernel_ms": round(st.kernel_ms, 3), "quartets": st.quartets_computed,
+           "model_tflops": round(st.model_flops / (st.kernel_ms * 1e-3) / 1e12, 3) if st.kernel_ms else None,
+           "max_abs_grad": float(np.abs(g).max()), "translational_sum": float(np.abs(g.sum(axis=0)).max())}
+    fb.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +125,7 @@ def main():
     ap.add_argument("--workload", default="h2o64_ccpvdz")
     ap.add_argument("--fp32-cutoff", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -178,6 +211,11 @@ def main():
             "launch_ms": round(dom_ms, 3), "flops_per_launch": dom_flops,
             "build_model_tflops": round(build_tflops, 3)}
 
+    grad = None
+    if not a.no_grad:
+        log("[bench] gradient: RHF on the C20H42/6-31G(d) workload, then the 2e gradient")
+        grad = gradient_leg(a.steps, a.warmup)
+
     cpu = None
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (oracle restatement, sampled)")
@@ -199,7 +237,8 @@ def main():
         "quartets_per_build": st.quartets_computed, "quartets_per_s": round(st.quartets_computed / dt, 1),
         "prim_quartets_per_build": st.prim_quartets, "fp32_quartets": stmx.quartets_fp32,
         "E2_fp64_hartree": e64,
-        "gradient_evals_per_s": None,
+        "gradient_evals_per_s": grad["evals_per_s"] if grad else None,
+        "gradient": grad,
         "roofline": roof, "cpu_baseline": cpu,
         "per_class_ms": {str(k): round(v["ms"] / v["n"], 3) for k, v in sorted(prof.items())},
     }

@@ -6,27 +6,27 @@
 
 namespace oqph {
 
-void launch_jk_0000(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_1000(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_1010(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_1100(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_1110(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_1111(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2000(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2010(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2011(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2020(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2100(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2110(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2111(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2120(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2121(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2200(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2210(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2211(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2220(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2221(bool, const JKParams&, const ClassWork&, hipStream_t);
-void launch_jk_2222(bool, const JKParams&, const ClassWork&, hipStream_t);
+void launch_jk_0000(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_1000(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_1010(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_1100(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_1110(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_1111(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2000(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2010(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2011(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2020(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2100(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2110(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2111(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2120(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2121(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2200(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2210(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2211(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2220(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2221(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_2222(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_schwarz_00(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
 void launch_schwarz_10(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
 void launch_schwarz_11(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
@@ -39,8 +39,8 @@ __device__ __forceinline__ long long tri_idx_(int i, int j) {
   return (long long)a * (a + 1) / 2 + b;
 }
 
-bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const ClassWork& cw, hipStream_t s) {
-  if (cw.nwork == 0) return true;
+bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const WaveWork& cw, hipStream_t s) {
+  if (cw.nwaves == 0) return true;
   switch (la * 1000 + lb * 100 + lc * 10 + ld) {
     case 0: launch_jk_0000(pure, p, cw, s); return true;
     case 1000: launch_jk_1000(pure, p, cw, s); return true;

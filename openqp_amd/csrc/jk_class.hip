@@ -14,13 +14,13 @@ namespace oqph {
 #define OQPH_SCH_LAUNCHER_(a, b) launch_schwarz_##a##b
 #define OQPH_SCH_LAUNCHER(a, b) OQPH_SCH_LAUNCHER_(a, b)
 
-void OQPH_JK_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const ClassWork& cw, hipStream_t s) {
+void OQPH_JK_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const WaveWork& ww, hipStream_t s) {
   const int bs = 256;
-  const long long nb = (cw.nwork + bs - 1) / bs;
+  const long long nb = (ww.nwaves * 64 + bs - 1) / bs;
   if (pure)
-    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, true>), dim3((unsigned)nb), dim3(bs), 0, s, p, cw);
+    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, true>), dim3((unsigned)nb), dim3(bs), 0, s, p, ww);
   else
-    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, false>), dim3((unsigned)nb), dim3(bs), 0, s, p, cw);
+    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, false>), dim3((unsigned)nb), dim3(bs), 0, s, p, ww);
 }
 
 #if XA == XC && XB == XD

@@ -38,51 +38,178 @@ struct AO {
   static constexpr int N = (PURE && L >= 2) ? 2 * L + 1 : ncart(L);
 };
 
-// apply the AO transform on index position IDX of a 4-index block
+// AO transform of one index of a 4-index block, in place: V is viewed as
+// [LEFT][NI][RIGHT] on input and [LEFT][NO][RIGHT] on output (NO <= NI).
+// Each (l, r) line is read completely before it is written, in increasing l,
+// so no write lands on an unread input: within one l the lines differ in r,
+// and the outputs of line l end below the first input of line l+1.
 template <int L, bool PURE, int LEFT, int RIGHT, typename R>
-__device__ __forceinline__ void xform_index(const double* __restrict__ xf, const R* in, R* out) {
+__device__ __forceinline__ void xform_inplace(const double* __restrict__ xf, R* V) {
   constexpr int NI = ncart(L), NO = AO<L, PURE>::N;
 #pragma unroll
   for (int l = 0; l < LEFT; ++l)
 #pragma unroll
-    for (int o = 0; o < NO; ++o)
+    for (int r = 0; r < RIGHT; ++r) {
+      R t[NI];
 #pragma unroll
-      for (int r = 0; r < RIGHT; ++r) {
+      for (int i = 0; i < NI; ++i) t[i] = V[(l * NI + i) * RIGHT + r];
+#pragma unroll
+      for (int o = 0; o < NO; ++o) {
         R s = 0;
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
-          const double t = xf[(((L - 2) * 2 + (PURE ? 1 : 0)) * 10 + o) * 10 + i];
-          s += (R)t * in[(l * NI + i) * RIGHT + r];
+          const double c = xf[(((L - 2) * 2 + (PURE ? 1 : 0)) * 10 + o) * 10 + i];
+          s += (R)c * t[i];
         }
-        out[(l * NO + o) * RIGHT + r] = s;
+        V[(l * NO + o) * RIGHT + r] = s;
       }
+    }
 }
 
-// Final-representation block of a quartet: V[a][b][c][d] in AO order.
-template <int LA, int LB, int LC, int LD, bool PURE, typename R>
-__device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
-                                              const double* __restrict__ tab, const float* __restrict__ tab32,
-                                              const double* __restrict__ herm,
-                                              const double* __restrict__ xf, R qcut, R (&Vout)[AO<LA, PURE>::N * AO<LB, PURE>::N *
-                                                                AO<LC, PURE>::N * AO<LD, PURE>::N],
-                                              int& nprim_used) {
+// Load a primitive pair whose index is uniform across the wavefront through
+// the constant address space, so it is served by the scalar cache.
+__device__ __forceinline__ PrimPair load_uniform(const PrimPair* p, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(4))) double* q =
+      (const __attribute__((address_space(4))) double*)(p + i);
+  PrimPair r;
+  r.g = q[0]; r.Kg = q[1]; r.P[0] = q[2]; r.P[1] = q[3]; r.P[2] = q[4]; r.a = q[5]; r.b = q[6]; r.h = q[7];
+  return r;
+#else
+  return p[i];
+#endif
+}
+
+// Contracted (e0|f0) of a quartet: Rys quadrature over every surviving
+// primitive quartet (bra primitives uniform across the wavefront).
+template <int LA, int LB, int LC, int LD, typename R, bool UNIFORM_BRA = false>
+__device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
+                                         const double* __restrict__ tab, const float* __restrict__ tab32,
+                                         const double* __restrict__ herm, R qcut,
+                                         R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF],
+                                         int& nprim_used) {
   using Dm = Dims<LA, LB, LC, LD>;
-  R acc[Dm::NE][Dm::NF];
 #pragma unroll
   for (int e = 0; e < Dm::NE; ++e)
 #pragma unroll
     for (int f = 0; f < Dm::NF; ++f) acc[e][f] = 0;
   const R qcut2 = qcut * qcut;
   nprim_used = 0;
+  // the bra is uniform across the wavefront: read its primitive pairs through
+  // the constant address space so they come from the scalar cache
+  // (only when the caller guarantees one bra per wavefront, as k_jk does)
+  const int pp0 = UNIFORM_BRA ? __builtin_amdgcn_readfirstlane(P.prim0) : P.prim0;
+  const int npp = UNIFORM_BRA ? __builtin_amdgcn_readfirstlane(P.nprim) : P.nprim;
   for (int iq = 0; iq < Q.nprim; ++iq) {
     const PrimPair qq = prims[Q.prim0 + iq];
     if (fabs(qq.Kg) < (double)qcut) continue;
-    for (int ip = 0; ip < P.nprim; ++ip) {
-      const PrimPair pp = prims[P.prim0 + ip];
+    for (int ip = 0; ip < npp; ++ip) {
+      const PrimPair pp = UNIFORM_BRA ? load_uniform(prims, pp0 + ip) : prims[pp0 + ip];
       if (fabs(pp.Kg) < (double)qcut) continue;
       nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tab, tab32, herm, acc) ? 1 : 0;
     }
   }
+}
+
+// Large classes: the AO block is produced one bra AO pair (a', b') at a time.
+// The bra HRR and bra AO transform are evaluated on the fly from the
+// contracted set, then the ket HRR and ket AO transform give the slice
+// S[c'][d'], handed to vis(a', b', S).  Only acc and small slices stay live.
+template <int LA, int LB, int LC, int LD, bool PURE, typename Vis>
+__device__ __forceinline__ void stream_slices(const ShellPair& P, const ShellPair& Q, const double* __restrict__ xf,
+                                              const double (&acc)[Dims<LA, LB, LC, LD>::NE]
+                                                                 [Dims<LA, LB, LC, LD>::NF],
+                                              Vis&& vis) {
+  using Dm = Dims<LA, LB, LC, LD>;
+  constexpr int NA = Dm::NA, NB = Dm::NB, NC = Dm::NC, ND = Dm::ND, NF = Dm::NF;
+  constexpr int OA = AO<LA, PURE>::N, OB = AO<LB, PURE>::N;
+  double pa[3][LB + 1], pc[3][LD + 1];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double ab = P.A[d] - P.B[d], cd = Q.A[d] - Q.B[d];
+    pa[d][0] = 1.0;
+    pc[d][0] = 1.0;
+#pragma unroll
+    for (int k = 1; k <= LB; ++k) pa[d][k] = pa[d][k - 1] * ab;
+#pragma unroll
+    for (int k = 1; k <= LD; ++k) pc[d][k] = pc[d][k - 1] * cd;
+  }
+  auto xa = [&](int L, int o, int i) -> double {  // AO row o, canonical col i
+    if (L < 2) return o == i ? 1.0 : 0.0;
+    return xf[(((L - 2) * 2 + (PURE ? 1 : 0)) * 10 + o) * 10 + i];
+  };
+#pragma unroll 1
+  for (int ao = 0; ao < OA; ++ao)
+#pragma unroll 1
+    for (int bo = 0; bo < OB; ++bo) {
+      double y[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) y[f] = 0.0;
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const double wa = xa(LA, ao, a);
+        if (wa == 0.0) continue;
+        const int ax = c_lx(LA, a), ay = c_ly(LA, a), az = c_lz(LA, a);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const double wab = wa * xa(LB, bo, b);
+          if (wab == 0.0) continue;
+          const int bx = c_lx(LB, b), by = c_ly(LB, b), bz = c_lz(LB, b);
+#pragma unroll
+          for (int jx = 0; jx <= bx; ++jx)
+#pragma unroll
+            for (int jy = 0; jy <= by; ++jy)
+#pragma unroll
+              for (int jz = 0; jz <= bz; ++jz) {
+                const int e = range_index<LA>(LA + jx + jy + jz, c_idx(ax + jx, ay + jy, az + jz));
+                const double cf = wab * (double)(binom(bx, jx) * binom(by, jy) * binom(bz, jz)) *
+                                  pa[0][bx - jx] * pa[1][by - jy] * pa[2][bz - jz];
+#pragma unroll
+                for (int f = 0; f < NF; ++f) y[f] += cf * acc[e][f];
+              }
+        }
+      }
+      double sl[NC * ND];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int dd = 0; dd < ND; ++dd) {
+          const int cx = c_lx(LC, c), cy = c_ly(LC, c), cz = c_lz(LC, c);
+          const int dx = c_lx(LD, dd), dy = c_ly(LD, dd), dz = c_lz(LD, dd);
+          double v = 0.0;
+#pragma unroll
+          for (int jx = 0; jx <= dx; ++jx)
+#pragma unroll
+            for (int jy = 0; jy <= dy; ++jy)
+#pragma unroll
+              for (int jz = 0; jz <= dz; ++jz) {
+                const int f = range_index<LC>(LC + jx + jy + jz, c_idx(cx + jx, cy + jy, cz + jz));
+                v += (double)(binom(dx, jx) * binom(dy, jy) * binom(dz, jz)) * pc[0][dx - jx] * pc[1][dy - jy] *
+                     pc[2][dz - jz] * y[f];
+              }
+          sl[c * ND + dd] = v;
+        }
+      if constexpr (LC >= 2) xform_inplace<LC, PURE, 1, ND, double>(xf, sl);
+      if constexpr (LD >= 2) xform_inplace<LD, PURE, AO<LC, PURE>::N, 1, double>(xf, sl);
+      vis(ao, bo, sl);
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+struct Big {
+  static constexpr bool value = Dims<LA, LB, LC, LD>::NBLK > 256;
+};
+
+// Final-representation block of a quartet: V[a][b][c][d] in AO order.
+template <int LA, int LB, int LC, int LD, bool PURE, typename R, bool UNIFORM_BRA = false>
+__device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
+                                              const double* __restrict__ tab, const float* __restrict__ tab32,
+                                              const double* __restrict__ herm,
+                                              const double* __restrict__ xf, R qcut,
+                                              R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used) {
+  using Dm = Dims<LA, LB, LC, LD>;
+  R acc[Dm::NE][Dm::NF];
+  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, tab, tab32, herm, qcut, acc, nprim_used);
   R AB[3], CD[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -91,39 +218,15 @@ __device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPai
   }
   R Y[Dm::NA * Dm::NB][Dm::NF];
   hrr_bra<LA, LB, Dm::NF, R>(acc, AB, Y);
-  R V[Dm::NBLK];
   hrr_ket<LC, LD, Dm::NA * Dm::NB, R>(Y, CD, V);
-  // AO transform index by index (only L >= 2 indices change)
-  constexpr int NA = Dm::NA, NB = Dm::NB, NC = Dm::NC, ND = Dm::ND;
-  constexpr int OA = AO<LA, PURE>::N, OB = AO<LB, PURE>::N, OC = AO<LC, PURE>::N, OD = AO<LD, PURE>::N;
-  if constexpr (LA < 2 && LB < 2 && LC < 2 && LD < 2) {
-#pragma unroll
-    for (int i = 0; i < Dm::NBLK; ++i) Vout[i] = V[i];
-  } else {
-    R W1[OA * NB * NC * ND];
-    if constexpr (LA >= 2) xform_index<LA, PURE, 1, NB * NC * ND, R>(xf, V, W1);
-    else {
-#pragma unroll
-      for (int i = 0; i < NA * NB * NC * ND; ++i) W1[i] = V[i];
-    }
-    R W2[OA * OB * NC * ND];
-    if constexpr (LB >= 2) xform_index<LB, PURE, OA, NC * ND, R>(xf, W1, W2);
-    else {
-#pragma unroll
-      for (int i = 0; i < OA * NB * NC * ND; ++i) W2[i] = W1[i];
-    }
-    R W3[OA * OB * OC * ND];
-    if constexpr (LC >= 2) xform_index<LC, PURE, OA * OB, ND, R>(xf, W2, W3);
-    else {
-#pragma unroll
-      for (int i = 0; i < OA * OB * NC * ND; ++i) W3[i] = W2[i];
-    }
-    if constexpr (LD >= 2) xform_index<LD, PURE, OA * OB * OC, 1, R>(xf, W3, Vout);
-    else {
-#pragma unroll
-      for (int i = 0; i < OA * OB * OC * OD; ++i) Vout[i] = W3[i];
-    }
-  }
+  // AO transform index by index, in place (only L >= 2 indices change); the
+  // AO block ends up in the first OA*OB*OC*OD entries of V
+  constexpr int NB = Dm::NB, NC = Dm::NC, ND = Dm::ND;
+  constexpr int OA = AO<LA, PURE>::N, OB = AO<LB, PURE>::N, OC = AO<LC, PURE>::N;
+  if constexpr (LA >= 2) xform_inplace<LA, PURE, 1, NB * NC * ND, R>(xf, V);
+  if constexpr (LB >= 2) xform_inplace<LB, PURE, OA, NC * ND, R>(xf, V);
+  if constexpr (LC >= 2) xform_inplace<LC, PURE, OA * OB, ND, R>(xf, V);
+  if constexpr (LD >= 2) xform_inplace<LD, PURE, OA * OB * OC, 1, R>(xf, V);
 }
 
 __device__ __forceinline__ void fatomic(double* p, double v) { unsafeAtomicAdd(p, v); }
@@ -151,23 +254,27 @@ __device__ __forceinline__ void count_wave(unsigned long long* ctr, bool pred) {
 }
 
 template <int LA, int LB, int LC, int LD, bool PURE>
-__global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
-  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = w < cw.nwork;
-  int ip = -1, iq = 0;
-  ShellPair P{}, Q{};
+__global__ __launch_bounds__(256) void k_jk(JKParams prm, WaveWork ww) {
+  // one wavefront = one chunk of 64 kets of one segment; the bra is uniform
+  const long long wave = __builtin_amdgcn_readfirstlane(
+      (int)((blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6));
+  if (wave >= ww.nwaves) return;  // wave-uniform
+  int lo = 0, hi = ww.nseg;
+  while (hi - lo > 1) {  // scalar binary search over the chunk prefix
+    const int mid = (lo + hi) >> 1;
+    if (ww.choff[mid] <= wave) lo = mid; else hi = mid;
+  }
+  const int seg = __builtin_amdgcn_readfirstlane(lo);
+  const int k = (int)(wave - ww.choff[seg]) * 64 + (int)__lane_id();
+  const bool active = k < ww.seg_n[seg];
+  const int ip = __builtin_amdgcn_readfirstlane(ww.seg_bra[seg]);
+  int iq = 0;
+  const ShellPair P = prm.pairs[ip];
+  ShellPair Q{};
   bool skip = true;
   double bound = 0.0;
   if (active) {
-    // segment = bra * nbkt + ket bucket; the segment's kets are a prefix of the bucket
-    int lo = 0, hi = cw.nbra * cw.nbkt;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (cw.off[mid] <= w) lo = mid; else hi = mid;
-    }
-    ip = cw.bra[lo / cw.nbkt];
-    iq = cw.ket[cw.kbs[lo % cw.nbkt] + (int)(w - cw.off[lo])];
-    P = prm.pairs[ip];
+    iq = ww.ket[ww.seg_k0[seg] + k];
     Q = prm.pairs[iq];
     // ---- OpenQP screening (int2.F90:763-800)
     const bool p_first = (P.si > Q.si) || (P.si == Q.si && P.sj >= Q.sj);
@@ -185,8 +292,122 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
   count_wave(ctr + 1, active && skip);
   count_wave(ctr + 0, !skip);
   constexpr int OA = AO<LA, PURE>::N, OB = AO<LB, PURE>::N, OC = AO<LC, PURE>::N, OD = AO<LD, PURE>::N;
+  if constexpr (Big<LA, LB, LC, LD>::value) {
+    // ---- large classes: FP64, AO slices streamed into the digestion
+    using Dm = Dims<LA, LB, LC, LD>;
+    double acc[Dm::NE][Dm::NF];
+    int nprim = 0;
+    if (!skip)
+      contract<LA, LB, LC, LD, double, true>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, 1.0e-4 * prm.cutoff,
+                                             acc, nprim);
+    count_wave(ctr + 2, false);
+    sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() +
+                          (nprim ? 12ull * OA * OB * OC * OD : 0ull));
+    sum_wave(ctr + 4, (unsigned long long)nprim);
+    if (skip || nprim == 0) return;
+    double deg = 1.0;
+    if (P.sa == P.sb) deg *= 0.5;
+    if (Q.sa == Q.sb) deg *= 0.5;
+    if (ip == iq) deg *= 0.5;
+    const double cj = 2.0 * deg * prm.sc, ck = -0.5 * deg * prm.se, cut = prm.cutoff;
+    const int oa = P.ao_a, ob = P.ao_b, oc = Q.ao_a, od = Q.ao_b;
+    const double* Dp = prm.D;
+    double* F = prm.F;
+    double Dab[OA * OB], Dcd[OC * OD], Dac[OA * OC], Dad[OA * OD], Dbc[OB * OC], Dbd[OB * OD];
+#pragma unroll
+    for (int a = 0; a < OA; ++a) {
+#pragma unroll
+      for (int b = 0; b < OB; ++b) Dab[a * OB + b] = Dp[tri_idx(oa + a, ob + b)];
+#pragma unroll
+      for (int c = 0; c < OC; ++c) Dac[a * OC + c] = Dp[tri_idx(oa + a, oc + c)];
+#pragma unroll
+      for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dp[tri_idx(oa + a, od + d)];
+    }
+#pragma unroll
+    for (int b = 0; b < OB; ++b) {
+#pragma unroll
+      for (int c = 0; c < OC; ++c) Dbc[b * OC + c] = Dp[tri_idx(ob + b, oc + c)];
+#pragma unroll
+      for (int d = 0; d < OD; ++d) Dbd[b * OD + d] = Dp[tri_idx(ob + b, od + d)];
+    }
+#pragma unroll
+    for (int c = 0; c < OC; ++c)
+#pragma unroll
+      for (int d = 0; d < OD; ++d) Dcd[c * OD + d] = Dp[tri_idx(oc + c, od + d)];
+    double Jcd[OC * OD], Kac[OA * OC], Kad[OA * OD], Kbc[OB * OC], Kbd[OB * OD];
+#pragma unroll
+    for (int i = 0; i < OC * OD; ++i) Jcd[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < OA * OC; ++i) Kac[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < OA * OD; ++i) Kad[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < OB * OC; ++i) Kbc[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < OB * OD; ++i) Kbd[i] = 0.0;
+    stream_slices<LA, LB, LC, LD, PURE>(P, Q, prm.xform, acc, [&](int a, int b, double* S) {
+      double jab = 0.0;
+      const double dab = Dab[a * OB + b];
+#pragma unroll
+      for (int c = 0; c < OC; ++c)
+#pragma unroll
+        for (int d = 0; d < OD; ++d) {
+          double v = S[c * OD + d];
+          if (fabs(v) < cut) v = 0.0;  // OpenQP element cutoff on the AO value
+          jab += v * Dcd[c * OD + d];
+          Jcd[c * OD + d] += v * dab;
+          Kac[a * OC + c] += v * Dbd[b * OD + d];
+          Kad[a * OD + d] += v * Dbc[b * OC + c];
+          Kbc[b * OC + c] += v * Dad[a * OD + d];
+          Kbd[b * OD + d] += v * Dac[a * OC + c];
+        }
+      const int x = oa + a, y = ob + b;
+      const double v = cj * jab * (x == y ? 2.0 : 1.0);
+      if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+    });
+#pragma unroll
+    for (int c = 0; c < OC; ++c)
+#pragma unroll
+      for (int d = 0; d < OD; ++d) {
+        const int x = oc + c, y = od + d;
+        const double v = cj * Jcd[c * OD + d] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+      }
+#pragma unroll
+    for (int a = 0; a < OA; ++a) {
+#pragma unroll
+      for (int c = 0; c < OC; ++c) {
+        const int x = oa + a, y = oc + c;
+        const double v = ck * Kac[a * OC + c] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+      }
+#pragma unroll
+      for (int d = 0; d < OD; ++d) {
+        const int x = oa + a, y = od + d;
+        const double v = ck * Kad[a * OD + d] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < OB; ++b) {
+#pragma unroll
+      for (int c = 0; c < OC; ++c) {
+        const int x = ob + b, y = oc + c;
+        const double v = ck * Kbc[b * OC + c] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+      }
+#pragma unroll
+      for (int d = 0; d < OD; ++d) {
+        const int x = ob + b, y = od + d;
+        const double v = ck * Kbd[b * OD + d] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+      }
+    }
+    return;
+  } else {
   constexpr int NOUT = OA * OB * OC * OD;
-  double V[NOUT];
+  constexpr int NBLK = Dims<LA, LB, LC, LD>::NBLK;
+  double V[NBLK];
 #pragma unroll
   for (int i = 0; i < NOUT; ++i) V[i] = 0.0;
   int nprim = 0;
@@ -197,15 +418,15 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
     if constexpr (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF <= 256) {
       lowp = prm.mixed && bound < prm.fp32_cut;
       if (lowp) {
-        float Vf[NOUT];
-        quartet_block<LA, LB, LC, LD, PURE, float>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, prm.xform,
+        float Vf[NBLK];
+        quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, prm.xform,
                                                    (float)(1.0e-4 * prm.cutoff), Vf, nprim);
 #pragma unroll
         for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
       }
     }
     if (!lowp)
-      quartet_block<LA, LB, LC, LD, PURE, double>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, prm.xform,
+      quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, prm.xform,
                                                   1.0e-4 * prm.cutoff, V, nprim);
   }
   count_wave(ctr + 2, lowp);
@@ -229,8 +450,7 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
   const unsigned long long vmask = __ballot(valid);
   if (vmask == 0ull) return;  // wave-uniform exit: no lane has anything to digest
   const int lead = __ffsll((long long)vmask) - 1;  // first lane with a quartet to digest
-  const int ip0 = __builtin_amdgcn_readlane(ip, lead);
-  const bool uni = __ballot(ip == ip0 || !valid) == __ballot(1);
+  constexpr bool uni = true;  // the bra is uniform across the wavefront
   // J blocks
   {
     double Dcd[OC * OD], Dab[OA * OB];
@@ -343,6 +563,7 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, ClassWork cw) {
       }
 #undef VV
   }
+  }
 }
 
 // Schwarz bounds Q_ab = sqrt(max |(ab|ab)|) for every pair of one class
@@ -357,13 +578,23 @@ __global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const P
   const int ip = list[t];
   const ShellPair P = pairs[ip];
   constexpr int N = AO<LA, PURE>::N * AO<LB, PURE>::N * AO<LA, PURE>::N * AO<LB, PURE>::N;
-  double V[N];
-  int nprim;
-  quartet_block<LA, LB, LA, LB, PURE, double>(P, P, prims, tab, nullptr, herm, xf, qcut, V, nprim);
   double m = 0.0;
-  if (nprim) {
+  int nprim;
+  if constexpr (Big<LA, LB, LA, LB>::value) {
+    double acc[Dims<LA, LB, LA, LB>::NE][Dims<LA, LB, LA, LB>::NF];
+    contract<LA, LB, LA, LB, double>(P, P, prims, tab, nullptr, herm, qcut, acc, nprim);
+    if (nprim)
+      stream_slices<LA, LB, LA, LB, PURE>(P, P, xf, acc, [&](int, int, double* S) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) m = fmax(m, fabs(V[i]));
+        for (int i = 0; i < AO<LA, PURE>::N * AO<LB, PURE>::N; ++i) m = fmax(m, fabs(S[i]));
+      });
+  } else {
+    double V[Dims<LA, LB, LA, LB>::NBLK];
+    quartet_block<LA, LB, LA, LB, PURE, double>(P, P, prims, tab, nullptr, herm, xf, qcut, V, nprim);
+    if (nprim) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) m = fmax(m, fabs(V[i]));
+    }
   }
   Qout[ip] = sqrt(m);
 }

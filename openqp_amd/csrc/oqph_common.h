@@ -49,6 +49,20 @@ struct ClassWork {
   long long nwork;
 };
 
+// Wavefront tasks of one J/K class-pair launch: segment s pairs bra pair
+// seg_bra[s] with the kets ket[seg_k0[s] .. seg_k0[s] + seg_n[s]); it is
+// split into ceil(seg_n/64) chunks of 64 kets, one per wavefront, counted by
+// the prefix choff (nseg + 1).  Empty segments are not stored.
+struct WaveWork {
+  const int* seg_bra;
+  const int* seg_k0;
+  const int* seg_n;
+  const long long* choff;
+  const int* ket;
+  int nseg;
+  long long nwaves;
+};
+
 // Arguments of one J/K build launch (shared by host and device code).
 struct JKParams {
   const ShellPair* pairs;

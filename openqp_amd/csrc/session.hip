@@ -26,7 +26,7 @@
 #include "grad_params.h"
 
 namespace oqph {
-bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const ClassWork& cw, hipStream_t s);
+bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const WaveWork& cw, hipStream_t s);
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
                     const double* tab, const double* herm, const double* xf, double qcut, double* Q, hipStream_t s);
 void launch_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh, hipStream_t s);
@@ -144,7 +144,15 @@ struct oqph_session {
     int la, lb, lc, ld;
     ClassWork cw;
   };
-  std::vector<Launch> launches;
+  struct WLaunch {
+    int la, lb, lc, ld;
+    WaveWork ww;
+  };
+  std::vector<WLaunch> launches;
+  DBuf<int> d_seg;                 // seg_bra | seg_k0 | seg_n of all launches
+  DBuf<long long> d_choff;
+  std::vector<int> h_seg_bra, h_seg_k0, h_seg_n;
+  std::vector<long long> h_choff;
   // gradient state (pair data with the grd2 cutoffs, coarse-screened lists)
   DBuf<ShellPair> d_gpairs;
   DBuf<PrimPair> d_gprims;
@@ -153,7 +161,7 @@ struct oqph_session {
   DBuf<double> d_Dc, d_de;
   int ncf = 0;
   double gcut_ready = -1.0;
-  std::vector<Launch> glaunches;
+  std::vector<Launch> glaunches;  // gradient: ClassWork flat index space
   // optional per-launch profile of the last build (HIP events on the build stream)
   bool profile = false;
   std::vector<hipEvent_t> pev;
@@ -435,19 +443,23 @@ static int prepare_grad(oqph_session* s) {
   return 0;
 }
 
-// per-(bra, ket bucket) counts and launch list for a given maxden: the kets
-// of a bucket that survive Q_bra Q_ket 4 maxD >= c form a prefix of it; for a
-// class with itself each unordered pair is taken once, the bra being the later
-// entry of the (bucket, Q) order.
+// Wavefront tasks for a given maxden.  For bra pair p and ket bucket b the
+// kets that survive Q_bra Q_ket 4 maxD >= c form a prefix of the bucket; for a
+// class with itself each unordered pair is taken once (the bra is the later
+// entry of the (bucket, Q) order).  Non-empty (p, b) segments are stored with
+// a prefix over their 64-ket chunks.
 static int prepare_lists(oqph_session* s, double maxden) {
   if (maxden == s->cached_maxden) return 0;
   s->launches.clear();
-  s->h_off.clear();
+  s->h_seg_bra.clear();
+  s->h_seg_k0.clear();
+  s->h_seg_n.clear();
+  s->h_choff.clear();
   struct Pend {
-    int la, lb, lc, ld;
-    size_t off0;
-    int cb, ck, nbra;
-    long long nwork;
+    int la, lb, lc, ld, ck;
+    size_t seg0, ch0;
+    int nseg;
+    long long nwaves;
   };
   std::vector<Pend> pend;
   const double c = s->cutoff;
@@ -459,26 +471,33 @@ static int prepare_lists(oqph_session* s, double maxden) {
       if (QB.empty() || QK.empty()) continue;
       const auto& kb = s->kbs[ck];
       const auto& bb = s->kbs[cb];
-      const size_t off0 = s->h_off.size();
-      s->h_off.push_back(0);
-      long long tot = 0;
+      const size_t seg0 = s->h_seg_bra.size(), ch0 = s->h_choff.size();
+      long long waves = 0;
       int bp = 0;
       for (size_t p = 0; p < QB.size(); ++p) {
         while (bp + 1 < NBKT && (int)p >= bb[bp + 1]) ++bp;
         const double thr = c / (4.0 * maxden * QB[p]);
         for (int b = 0; b < NBKT; ++b) {
-          long long k = 0;
-          if (kb[b + 1] > kb[b]) {
-            k = std::lower_bound(QK.begin() + kb[b], QK.begin() + kb[b + 1], thr,
-                                 [](double a, double t) { return a >= t; }) - (QK.begin() + kb[b]);
-            if (cb == ck) {
-              if (b > bp) k = 0;
-              else if (b == bp) k = std::min<long long>(k, (long long)p - bb[bp] + 1);
-            }
+          if (kb[b + 1] <= kb[b]) continue;
+          long long k = std::lower_bound(QK.begin() + kb[b], QK.begin() + kb[b + 1], thr,
+                                         [](double a, double t) { return a >= t; }) - (QK.begin() + kb[b]);
+          if (cb == ck) {
+            if (b > bp) k = 0;
+            else if (b == bp) k = std::min<long long>(k, (long long)p - bb[bp] + 1);
           }
-          tot += k;
-          s->h_off.push_back(tot);
+          if (k == 0) continue;
+          s->h_seg_bra.push_back(s->cls[cb][p]);
+          s->h_seg_k0.push_back(kb[b]);
+          s->h_seg_n.push_back((int)k);
+          s->h_choff.push_back(waves);
+          waves += (k + 63) / 64;
         }
+      }
+      const int nseg = (int)(s->h_seg_bra.size() - seg0);
+      s->h_choff.push_back(waves);  // terminator of this launch's prefix
+      if (nseg == 0) {
+        s->h_choff.pop_back();
+        continue;
       }
       int la = 0, lb = 0, lc = 0, ld = 0;
       for (int x = 0; x <= 3; ++x)
@@ -486,23 +505,31 @@ static int prepare_lists(oqph_session* s, double maxden) {
           if (class_of(x, y) == cb) { la = x; lb = y; }
           if (class_of(x, y) == ck) { lc = x; ld = y; }
         }
-      pend.push_back({la, lb, lc, ld, off0, cb, ck, (int)QB.size(), tot});
+      pend.push_back({la, lb, lc, ld, ck, seg0, ch0, nseg, waves});
     }
   int rc;
-  if ((rc = s->d_off.ensure(s->h_off.size()))) return rc;
-  if ((rc = upload(s->d_off.p, s->h_off.data(), s->h_off.size() * sizeof(long long), s->stream))) return rc;
+  const size_t nseg = s->h_seg_bra.size();
+  if ((rc = s->d_seg.ensure(3 * nseg + 1)) || (rc = s->d_choff.ensure(s->h_choff.size() + 1))) return rc;
+  if (nseg) {
+    if ((rc = upload(s->d_seg.p, s->h_seg_bra.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload(s->d_seg.p + nseg, s->h_seg_k0.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload(s->d_seg.p + 2 * nseg, s->h_seg_n.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload(s->d_choff.p, s->h_choff.data(), s->h_choff.size() * sizeof(long long), s->stream)))
+      return rc;
+  }
   for (auto& p : pend) {
-    oqph_session::Launch L;
+    oqph_session::WLaunch L;
     L.la = p.la; L.lb = p.lb; L.lc = p.lc; L.ld = p.ld;
-    L.cw.bra = s->d_cls.p + s->cls_off[p.cb];
-    L.cw.ket = s->d_cls.p + s->cls_off[p.ck];
-    L.cw.off = s->d_off.p + p.off0;
-    L.cw.kbs = s->d_kbs.p + p.ck * (NBKT + 1);
-    L.cw.nbra = p.nbra;
-    L.cw.nbkt = NBKT;
-    L.cw.nwork = p.nwork;
+    L.ww.seg_bra = s->d_seg.p + p.seg0;
+    L.ww.seg_k0 = s->d_seg.p + nseg + p.seg0;
+    L.ww.seg_n = s->d_seg.p + 2 * nseg + p.seg0;
+    L.ww.choff = s->d_choff.p + p.ch0;
+    L.ww.ket = s->d_cls.p + s->cls_off[p.ck];
+    L.ww.nseg = p.nseg;
+    L.ww.nwaves = p.nwaves;
     s->launches.push_back(L);
   }
+  HIPCHK(hipStreamSynchronize(s->stream));  // host vectors stay alive; keep uploads ordered
   s->cached_maxden = maxden;
   return 0;
 }
@@ -559,7 +586,7 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     auto& L = s->launches[i];
     p.counters = s->d_ctr.p + 8 + 8 * i;
     if (s->profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
-    if (!launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.cw, stream)) {
+    if (!launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream)) {
       set_err("no J/K kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }
@@ -633,7 +660,7 @@ int oqph_session_destroy(oqph_session* s) {
   }
   hipSetDevice(s->device);
   s->d_pairs.release(); s->d_prims.release(); s->d_rys.release(); s->d_herm.release(); s->d_cls.release();
-  s->d_xform.release(); s->d_rys32.release(); s->d_kbs.release();
+  s->d_xform.release(); s->d_rys32.release(); s->d_kbs.release(); s->d_seg.release(); s->d_choff.release();
   s->d_gpairs.release(); s->d_gprims.release(); s->d_gcls.release(); s->d_coff.release(); s->d_satom.release();
   s->d_am.release(); s->d_pure.release(); s->d_goff.release(); s->d_Dc.release(); s->d_de.release();
   s->d_aooff.release(); s->d_nao.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
