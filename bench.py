@@ -240,7 +240,10 @@ def main():
         "gradient_evals_per_s": grad["evals_per_s"] if grad else None,
         "gradient": grad,
         "roofline": roof, "cpu_baseline": cpu,
-        "per_class_ms": {str(k): round(v["ms"] / v["n"], 3) for k, v in sorted(prof.items())},
+        "per_class": {str(k): {"ms_fp64": round(v["ms"] / v["n"], 3),
+                               "ms_mixed": round(profmx[k]["ms"] / profmx[k]["n"], 3) if k in profmx else None,
+                               "quartets": v["quartets"] // v["n"], "prim_quartets": v["prims"] // v["n"]}
+                      for k, v in sorted(prof64.items())},
     }
     print(json.dumps(out), flush=True)
 

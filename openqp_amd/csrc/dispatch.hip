@@ -27,12 +27,12 @@ void launch_jk_2211(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2220(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2221(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2222(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_schwarz_00(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_10(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_11(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_20(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_21(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_22(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_00(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_10(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_11(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_20(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_21(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_22(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
 
 __device__ __forceinline__ long long tri_idx_(int i, int j) {
   int a = i > j ? i : j, b = i > j ? j : i;
@@ -68,15 +68,15 @@ bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, con
 }
 
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
-                    const double* tab, const double* herm, const double* xf, double qcut, double* Q, hipStream_t s) {
+                    const double* tab, const double* xf, double qcut, double* Q, hipStream_t s) {
   if (n == 0) return true;
   switch (la * 10 + lb) {
-    case 0: launch_schwarz_00(pure, pairs, prims, list, n, tab, herm, xf, qcut, Q, s); return true;
-    case 10: launch_schwarz_10(pure, pairs, prims, list, n, tab, herm, xf, qcut, Q, s); return true;
-    case 11: launch_schwarz_11(pure, pairs, prims, list, n, tab, herm, xf, qcut, Q, s); return true;
-    case 20: launch_schwarz_20(pure, pairs, prims, list, n, tab, herm, xf, qcut, Q, s); return true;
-    case 21: launch_schwarz_21(pure, pairs, prims, list, n, tab, herm, xf, qcut, Q, s); return true;
-    case 22: launch_schwarz_22(pure, pairs, prims, list, n, tab, herm, xf, qcut, Q, s); return true;
+    case 0: launch_schwarz_00(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
+    case 10: launch_schwarz_10(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
+    case 11: launch_schwarz_11(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
+    case 20: launch_schwarz_20(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
+    case 21: launch_schwarz_21(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
+    case 22: launch_schwarz_22(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
     default: return false;
   }
 }

@@ -11,6 +11,8 @@
 // so the loops unroll into straight-line register code for each class.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
+#include <utility>
 #include "oqph_common.h"
 
 namespace oqph {
@@ -40,56 +42,144 @@ __host__ __device__ constexpr int binom(int n, int k) {
   return r;
 }
 
+// ------------------------------------------------------ compile-time loops
+// sfor<N>(f) calls f(std::integral_constant<int, i>{}) for i = 0..N-1, so the
+// body sees every index as a constant expression: Cartesian exponents, HRR
+// binomials and register-array offsets are all resolved at compile time.
+template <typename F, int... I>
+__device__ __forceinline__ void sfor_impl(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (N > 0) sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+#define OQPH_CI(x) decltype(x)::value
+
+// ------------------------------------------------------------------ FP64 rcp
+// Hardware reciprocal / reciprocal square root seeds refined by two Newton
+// steps (quadratic convergence from the seed's ~2^-22 to full precision);
+// arguments are positive normal numbers (exponent sums, Boys arguments).
+__device__ __forceinline__ double rcp64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double rsq64(double x) {
+  const double h = 0.5 * x;
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  return fma(y, e, y);
+}
+
 // ------------------------------------------------------------------ Rys roots
 #include "rys_table.inc"
 #include "rys_table_f32.inc"
 
-template <int N, typename R>
-__device__ __forceinline__ void rys_roots(R T, const double* __restrict__ tab, const double* __restrict__ herm,
-                                          R* x, R* w) {
-  if (T >= (R)RYS_TASYM[N]) {
-    R it = (R)1 / T, rs = (R)1 / sqrt(T);
+// length of the N-root block of each table (entries); the blocks start at
+// RYS_OFF[N] / RYS32_OFF[N]
+template <int N>
+__host__ __device__ constexpr int rys_len() { return RYS_TASYM[N] * 2 * N * (RYS_DEG + 1); }
+template <int N>
+__host__ __device__ constexpr int rys32_len() { return RYS_TASYM[N] * 2 * N * (RYS32_DEG + 1); }
+
+// Nodes t^2 and weights of the N-root Rys quadrature at argument T.  tabN is
+// the N-root block of the FP64 table (global memory or a block-shared copy):
+// per unit interval k = floor(T), 2N degree-12 polynomials in s = 2(T-k)-1.
+// Beyond RYS_TASYM[N] the Hermite limit t^2 = r_i / T, w = h_i / sqrt(T).
+template <int N>
+__device__ __forceinline__ void rys_roots(double T, const double* __restrict__ tabN, double* x, double* w) {
+  if (T >= (double)RYS_TASYM[N]) {
+    const double rs = rsq64(T), it = rs * rs;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      x[i] = (R)herm[RYS_HOFF[N] + i] * it;
-      w[i] = (R)herm[RYS_HOFF[N] + N + i] * rs;
+      x[i] = RYS_HERM[RYS_HOFF[N] + i] * it;
+      w[i] = RYS_HERM[RYS_HOFF[N] + N + i] * rs;
     }
     return;
   }
-  int k = (int)T;
-  R s = (R)2 * (T - (R)k) - (R)1;
-  const double* c = tab + RYS_OFF[N] + k * (2 * N * (RYS_DEG + 1));
+  const int k = (int)T;
+  const double s = 2.0 * (T - (double)k) - 1.0;
+  const double* c = tabN + k * (2 * N * (RYS_DEG + 1));
 #pragma unroll
   for (int f = 0; f < 2 * N; ++f) {
-    R v = (R)c[f * (RYS_DEG + 1) + RYS_DEG];
+    double v = c[f * (RYS_DEG + 1) + RYS_DEG];
 #pragma unroll
-    for (int j = RYS_DEG - 1; j >= 0; --j) v = v * s + (R)c[f * (RYS_DEG + 1) + j];
+    for (int j = RYS_DEG - 1; j >= 0; --j) v = fma(v, s, c[f * (RYS_DEG + 1) + j]);
     if (f < N) x[f] = v; else w[f - N] = v;
   }
 }
 
-// FP32 roots: degree-6 fits, hardware reciprocal / reciprocal square root
+// FP32 roots: degree-6 fits on the same intervals, hardware rcp / rsqrt
 template <int N>
-__device__ __forceinline__ void rys_roots32(float T, const float* __restrict__ tab, const double* __restrict__ herm,
-                                            float* x, float* w) {
+__device__ __forceinline__ void rys_roots32(float T, const float* __restrict__ tabN, float* x, float* w) {
   if (T >= (float)RYS_TASYM[N]) {
     const float it = __builtin_amdgcn_rcpf(T), rs = __builtin_amdgcn_rsqf(T);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      x[i] = (float)herm[RYS_HOFF[N] + i] * it;
-      w[i] = (float)herm[RYS_HOFF[N] + N + i] * rs;
+      x[i] = (float)RYS_HERM[RYS_HOFF[N] + i] * it;
+      w[i] = (float)RYS_HERM[RYS_HOFF[N] + N + i] * rs;
     }
     return;
   }
   const int k = (int)T;
   const float s = 2.0f * (T - (float)k) - 1.0f;
-  const float* c = tab + RYS32_OFF[N] + k * (2 * N * (RYS32_DEG + 1));
+  const float* c = tabN + k * (2 * N * (RYS32_DEG + 1));
 #pragma unroll
   for (int f = 0; f < 2 * N; ++f) {
     float v = c[f * (RYS32_DEG + 1) + RYS32_DEG];
 #pragma unroll
-    for (int j = RYS32_DEG - 1; j >= 0; --j) v = v * s + c[f * (RYS32_DEG + 1) + j];
+    for (int j = RYS32_DEG - 1; j >= 0; --j) v = fmaf(v, s, c[f * (RYS32_DEG + 1) + j]);
     if (f < N) x[f] = v; else w[f - N] = v;
+  }
+}
+
+// One node/weight pair (root r, runtime index) of the N-root quadrature, for
+// root loops kept rolled to bound register pressure.
+template <int N, typename R>
+__device__ __forceinline__ void rys_root_one(R T, const double* __restrict__ tabN, const float* __restrict__ tab32N,
+                                             int r, R& x, R& w) {
+  if constexpr (sizeof(R) == 4) {
+    if (T >= (float)RYS_TASYM[N]) {
+      x = (float)RYS_HERM[RYS_HOFF[N] + r] * __builtin_amdgcn_rcpf(T);
+      w = (float)RYS_HERM[RYS_HOFF[N] + N + r] * __builtin_amdgcn_rsqf(T);
+      return;
+    }
+    const int k = (int)T;
+    const float s = 2.0f * (T - (float)k) - 1.0f;
+    const float* cx = tab32N + k * (2 * N * (RYS32_DEG + 1)) + r * (RYS32_DEG + 1);
+    const float* cw = cx + N * (RYS32_DEG + 1);
+    float vx = cx[RYS32_DEG], vw = cw[RYS32_DEG];
+#pragma unroll
+    for (int j = RYS32_DEG - 1; j >= 0; --j) {
+      vx = fmaf(vx, s, cx[j]);
+      vw = fmaf(vw, s, cw[j]);
+    }
+    x = vx;
+    w = vw;
+  } else {
+    if (T >= (double)RYS_TASYM[N]) {
+      const double rs = rsq64(T);
+      x = RYS_HERM[RYS_HOFF[N] + r] * rs * rs;
+      w = RYS_HERM[RYS_HOFF[N] + N + r] * rs;
+      return;
+    }
+    const int k = (int)T;
+    const double s = 2.0 * (T - (double)k) - 1.0;
+    const double* cx = tabN + k * (2 * N * (RYS_DEG + 1)) + r * (RYS_DEG + 1);
+    const double* cw = cx + N * (RYS_DEG + 1);
+    double vx = cx[RYS_DEG], vw = cw[RYS_DEG];
+#pragma unroll
+    for (int j = RYS_DEG - 1; j >= 0; --j) {
+      vx = fma(vx, s, cx[j]);
+      vw = fma(vw, s, cw[j]);
+    }
+    x = vx;
+    w = vw;
   }
 }
 
@@ -132,11 +222,24 @@ __host__ __device__ constexpr long long flops_prim() {
                           6 * (LA + 1) * (LB + 1) * (LC + 1) * (LD + 1) + 3 * nc);
 }
 
-template <int LA, int LB, int LC, int LD, typename R>
+// The fields of a primitive pair that the quadrature reads, in one precision.
+template <typename R>
+struct PrimR {
+  R g, Kg, P[3], h;
+};
+template <typename R>
+__device__ __forceinline__ PrimR<R> prim_r(const PrimPair& p) {
+  return {(R)p.g, (R)p.Kg, {(R)p.P[0], (R)p.P[1], (R)p.P[2]}, (R)p.h};
+}
+
+// Adds one primitive quartet's (e0|f0) to acc; returns false when the
+// primitive prefactor screen |K_ab K_cd| / sqrt(zeta+eta) < qcut drops it.
+// tabN / tab32N: the NROOT-root blocks of the FP64 / FP32 root tables.
+// PB, PK: PrimPair or PrimR (fields g, Kg, P, h).
+template <int LA, int LB, int LC, int LD, typename R, typename PB, typename PK>
 __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, const double* __restrict__ Cc,
-                                               const PrimPair& pp, const PrimPair& qq, R qcut2,
-                                               const double* __restrict__ tab, const float* __restrict__ tab32,
-                                               const double* __restrict__ herm,
+                                               const PB& pp, const PK& qq, R qcut2,
+                                               const double* __restrict__ tabN, const float* __restrict__ tab32N,
                                                R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]) {
   using Dm = Dims<LA, LB, LC, LD>;
   constexpr int LAB = Dm::LAB, LCD = Dm::LCD, NR = Dm::NROOT;
@@ -147,11 +250,11 @@ __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, co
   if (fqz * fqz < qcut2 * ze) return false;
   R ize, sq;
   if constexpr (F32) {
-    ize = __builtin_amdgcn_rcpf(ze);
     sq = __builtin_amdgcn_rsqf(ze);
+    ize = __builtin_amdgcn_rcpf(ze);
   } else {
-    ize = (R)1 / ze;
-    sq = sqrt(ize);
+    sq = rsq64(ze);
+    ize = sq * sq;
   }
   const R rho = zeta * eta * ize;
   R PQ[3], PA[3], QC[3];
@@ -165,17 +268,12 @@ __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, co
   }
   const R T = rho * pq2;
   const R pref = fqz * sq;
-  R rx[NR], rw[NR];
-  if constexpr (F32) rys_roots32<NR>(T, tab32, herm, rx, rw);
-  else rys_roots<NR, R>(T, tab, herm, rx, rw);
   // rho/zeta = eta/(zeta+eta), rho/eta = zeta/(zeta+eta): no further divisions
   const R rz = eta * ize, re = zeta * ize;
   const R h2z = (R)pp.h, h2e = (R)qq.h, h2ze = (R)0.5 * ize;
-  // large classes: keep the root loop rolled so the unrolled body stays compilable
-  constexpr int UR = (Dm::NE * Dm::NF > 256) ? 1 : NR;
-#pragma unroll UR
-  for (int r = 0; r < NR; ++r) {
-    const R t2 = rx[r];
+  // one root: 2D integrals G_d(i, k) by the vertical recursion, then the
+  // (e0|f0) products
+  auto root = [&](const R t2, const R wr) {
     const R B00 = h2ze * t2;
     const R B10 = h2z - h2z * rz * t2;
     const R B01 = h2e - h2e * re * t2;
@@ -184,7 +282,7 @@ __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, co
     for (int d = 0; d < 3; ++d) {
       const R C00 = PA[d] - rz * t2 * PQ[d];
       const R D00 = QC[d] + re * t2 * PQ[d];
-      G[d][0][0] = (d == 0) ? pref * rw[r] : (R)1;
+      G[d][0][0] = (d == 0) ? pref * wr : (R)1;
 #pragma unroll
       for (int i = 1; i <= LAB; ++i) {
         R v = C00 * G[d][i - 1][0];
@@ -202,16 +300,33 @@ __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, co
         }
       }
     }
-#pragma unroll
-    for (int e = 0; e < Dm::NE; ++e) {
-      const int Le = range_level<LA>(e), ce = range_local<LA>(e);
-      const int ex = c_lx(Le, ce), ey = c_ly(Le, ce), ez = c_lz(Le, ce);
-#pragma unroll
-      for (int f = 0; f < Dm::NF; ++f) {
-        const int Lf = range_level<LC>(f), cf = range_local<LC>(f);
-        const int fx = c_lx(Lf, cf), fy = c_ly(Lf, cf), fz = c_lz(Lf, cf);
+    sfor<Dm::NE>([&](auto E) {
+      constexpr int e = OQPH_CI(E);
+      constexpr int Le = range_level<LA>(e), ce = range_local<LA>(e);
+      constexpr int ex = c_lx(Le, ce), ey = c_ly(Le, ce), ez = c_lz(Le, ce);
+      sfor<Dm::NF>([&](auto F) {
+        constexpr int f = OQPH_CI(F);
+        constexpr int Lf = range_level<LC>(f), cf = range_local<LC>(f);
+        constexpr int fx = c_lx(Lf, cf), fy = c_ly(Lf, cf), fz = c_lz(Lf, cf);
         acc[e][f] += G[0][ex][fx] * G[1][ey][fy] * G[2][ez][fz];
-      }
+      });
+    });
+  };
+  if constexpr (Dm::NE * Dm::NF <= 64) {
+    // small classes: all roots at once, root loop unrolled
+    R rx[NR], rw[NR];
+    if constexpr (F32) rys_roots32<NR>(T, tab32N, rx, rw);
+    else rys_roots<NR>(T, tabN, rx, rw);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) root(rx[r], rw[r]);
+  } else {
+    // larger classes: one root per iteration of a rolled loop, so only one
+    // root's 2D integrals are live next to the accumulators
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) {
+      R t2, wr;
+      rys_root_one<NR, R>(T, tabN, tab32N, r, t2, wr);
+      root(t2, wr);
     }
   }
   return true;
@@ -219,6 +334,7 @@ __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, co
 
 // ---------------------------------------------------------------- HRR
 // Y[a][b][f] = sum_j prod_d C(b_d,j_d) AB_d^(b_d-j_d) acc[(a+j)][f]
+// f runs outermost so each column of acc dies as its Y column is formed.
 template <int LA, int LB, int NF, typename R>
 __device__ __forceinline__ void hrr_bra(const R (&acc)[ncum(LA + LB) - ncum(LA - 1)][NF], const R* AB,
                                         R (&Y)[ncart(LA) * ncart(LB)][NF]) {
@@ -229,32 +345,34 @@ __device__ __forceinline__ void hrr_bra(const R (&acc)[ncum(LA + LB) - ncum(LA -
 #pragma unroll
     for (int k = 1; k <= LB; ++k) pw[d][k] = pw[d][k - 1] * AB[d];
   }
-#pragma unroll
-  for (int a = 0; a < ncart(LA); ++a)
-#pragma unroll
-    for (int b = 0; b < ncart(LB); ++b) {
-      const int ax = c_lx(LA, a), ay = c_ly(LA, a), az = c_lz(LA, a);
-      const int bx = c_lx(LB, b), by = c_ly(LB, b), bz = c_lz(LB, b);
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
+  sfor<NF>([&](auto Fi) {
+    constexpr int f = OQPH_CI(Fi);
+    sfor<ncart(LA)>([&](auto Ai) {
+      constexpr int a = OQPH_CI(Ai);
+      constexpr int ax = c_lx(LA, a), ay = c_ly(LA, a), az = c_lz(LA, a);
+      sfor<ncart(LB)>([&](auto Bi) {
+        constexpr int b = OQPH_CI(Bi);
+        constexpr int bx = c_lx(LB, b), by = c_ly(LB, b), bz = c_lz(LB, b);
         R s = 0;
-#pragma unroll
-        for (int jx = 0; jx <= bx; ++jx)
-#pragma unroll
-          for (int jy = 0; jy <= by; ++jy)
-#pragma unroll
-            for (int jz = 0; jz <= bz; ++jz) {
-              const int L = LA + jx + jy + jz;
-              const int e = range_index<LA>(L, c_idx(ax + jx, ay + jy, az + jz));
-              const R cf = (R)(binom(bx, jx) * binom(by, jy) * binom(bz, jz)) * pw[0][bx - jx] *
-                           pw[1][by - jy] * pw[2][bz - jz];
-              s += cf * acc[e][f];
-            }
+        sfor<bx + 1>([&](auto JX) {
+          constexpr int jx = OQPH_CI(JX);
+          sfor<by + 1>([&](auto JY) {
+            constexpr int jy = OQPH_CI(JY);
+            sfor<bz + 1>([&](auto JZ) {
+              constexpr int jz = OQPH_CI(JZ);
+              constexpr int e = range_index<LA>(LA + jx + jy + jz, c_idx(ax + jx, ay + jy, az + jz));
+              constexpr int cb = binom(bx, jx) * binom(by, jy) * binom(bz, jz);
+              s += (R)cb * pw[0][bx - jx] * pw[1][by - jy] * pw[2][bz - jz] * acc[e][f];
+            });
+          });
+        });
         Y[a * ncart(LB) + b][f] = s;
-      }
-    }
+      });
+    });
+  });
 }
 
+// V[ab][c][d] = sum_j prod_d C(d_d,j_d) CD_d^(d_d-j_d) Y[ab][(c+j)]
 template <int LC, int LD, int NAB, typename R>
 __device__ __forceinline__ void hrr_ket(const R (&Y)[NAB][ncum(LC + LD) - ncum(LC - 1)], const R* CD,
                                         R (&V)[NAB * ncart(LC) * ncart(LD)]) {
@@ -265,29 +383,31 @@ __device__ __forceinline__ void hrr_ket(const R (&Y)[NAB][ncum(LC + LD) - ncum(L
 #pragma unroll
     for (int k = 1; k <= LD; ++k) pw[d][k] = pw[d][k - 1] * CD[d];
   }
-#pragma unroll
-  for (int ab = 0; ab < NAB; ++ab)
-#pragma unroll
-    for (int c = 0; c < ncart(LC); ++c)
-#pragma unroll
-      for (int dd = 0; dd < ncart(LD); ++dd) {
-        const int cx = c_lx(LC, c), cy = c_ly(LC, c), cz = c_lz(LC, c);
-        const int dx = c_lx(LD, dd), dy = c_ly(LD, dd), dz = c_lz(LD, dd);
+  sfor<NAB>([&](auto ABi) {
+    constexpr int ab = OQPH_CI(ABi);
+    sfor<ncart(LC)>([&](auto Ci) {
+      constexpr int c = OQPH_CI(Ci);
+      constexpr int cx = c_lx(LC, c), cy = c_ly(LC, c), cz = c_lz(LC, c);
+      sfor<ncart(LD)>([&](auto Di) {
+        constexpr int dd = OQPH_CI(Di);
+        constexpr int dx = c_lx(LD, dd), dy = c_ly(LD, dd), dz = c_lz(LD, dd);
         R s = 0;
-#pragma unroll
-        for (int jx = 0; jx <= dx; ++jx)
-#pragma unroll
-          for (int jy = 0; jy <= dy; ++jy)
-#pragma unroll
-            for (int jz = 0; jz <= dz; ++jz) {
-              const int L = LC + jx + jy + jz;
-              const int f = range_index<LC>(L, c_idx(cx + jx, cy + jy, cz + jz));
-              const R cf = (R)(binom(dx, jx) * binom(dy, jy) * binom(dz, jz)) * pw[0][dx - jx] *
-                           pw[1][dy - jy] * pw[2][dz - jz];
-              s += cf * Y[ab][f];
-            }
+        sfor<dx + 1>([&](auto JX) {
+          constexpr int jx = OQPH_CI(JX);
+          sfor<dy + 1>([&](auto JY) {
+            constexpr int jy = OQPH_CI(JY);
+            sfor<dz + 1>([&](auto JZ) {
+              constexpr int jz = OQPH_CI(JZ);
+              constexpr int f = range_index<LC>(LC + jx + jy + jz, c_idx(cx + jx, cy + jy, cz + jz));
+              constexpr int cb = binom(dx, jx) * binom(dy, jy) * binom(dz, jz);
+              s += (R)cb * pw[0][dx - jx] * pw[1][dy - jy] * pw[2][dz - jz] * Y[ab][f];
+            });
+          });
+        });
         V[(ab * ncart(LC) + c) * ncart(LD) + dd] = s;
-      }
+      });
+    });
+  });
 }
 
 }  // namespace oqph

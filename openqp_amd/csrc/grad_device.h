@@ -32,14 +32,17 @@ __device__ __forceinline__ void count_wave_g(unsigned long long* ctr, bool pred)
 
 // 2D integral with bra/ket horizontal transfer, all indices compile-time:
 //   I(a,b,c,d) = sum_j sum_m C(b,j) AB^(b-j) C(d,m) CD^(d-m) G(a+j, c+m)
-template <int NI, int NK, typename R>
-__device__ __forceinline__ R i2d(const R (&G)[NI][NK], const R* pab, const R* pcd, int a, int b, int c, int d) {
+template <int A, int B, int C, int D, int NI, int NK, typename R>
+__device__ __forceinline__ R i2d(const R (&G)[NI][NK], const R* pab, const R* pcd) {
   R s = 0;
-#pragma unroll
-  for (int j = 0; j <= b; ++j)
-#pragma unroll
-    for (int m = 0; m <= d; ++m)
-      s += (R)(binom(b, j) * binom(d, m)) * pab[b - j] * pcd[d - m] * G[a + j][c + m];
+  sfor<B + 1>([&](auto Ji) {
+    constexpr int j = OQPH_CI(Ji);
+    sfor<D + 1>([&](auto Mi) {
+      constexpr int m = OQPH_CI(Mi);
+      constexpr int cb = binom(B, j) * binom(D, m);
+      s += (R)cb * pab[B - j] * pcd[D - m] * G[A + j][C + m];
+    });
+  });
   return s;
 }
 
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
     // fine screen on the weighted contribution (grd2.F90:348)
     skip = gm * P.Q * Q.Q < 0.5 * prm.gcut;
   }
-  unsigned long long* ctr = prm.counters;
+  unsigned long long* ctr = prm.counters + 8 * (int)((w >> 6) & (OQPH_CTR_SLOTS - 1));
   count_wave_g(ctr + 1, active && skip);
   count_wave_g(ctr + 0, !skip);
   double fd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
         if (fqz * fqz < 1.0e-40 * ze) continue;
         if (fqz * fqz * dab2 < prm.dabcut2 * ze) continue;
         ++nprim;
-        const double ize = 1.0 / ze, rho = zeta * eta * ize;
+        const double sq = rsq64(ze), ize = sq * sq, rho = zeta * eta * ize;
         double PQ[3], PA[3], QC[3], pq2 = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -124,10 +127,10 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
           pq2 += PQ[d] * PQ[d];
         }
         double rx[NR], rw[NR];
-        rys_roots<NR, double>(rho * pq2, prm.rys, prm.herm, rx, rw);
-        const double pref = fqz * sqrt(ize);
-        const double rz = rho / zeta, re = rho / eta;
-        const double h2z = 0.5 / zeta, h2e = 0.5 / eta, h2ze = 0.5 * ize;
+        rys_roots<NR>(rho * pq2, prm.rys + RYS_OFF[NR], rx, rw);
+        const double pref = fqz * sq;
+        const double rz = eta * ize, re = zeta * ize;
+        const double h2z = pp.h, h2e = qq.h, h2ze = 0.5 * ize;
         const double a2 = 2.0 * pp.a, b2 = 2.0 * pp.b, c2 = 2.0 * qq.a;
         constexpr int UR = (NG > 36) ? 1 : NR;
 #pragma unroll UR
@@ -164,33 +167,32 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
 #pragma unroll
             for (int k = 1; k <= LD; ++k) pcd[d][k] = pcd[d][k - 1] * cd;
           }
-#pragma unroll
-          for (int a = 0; a < NA; ++a)
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-              for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int dd = 0; dd < ND; ++dd) {
+          sfor<NA>([&](auto Ai) {
+            constexpr int a = OQPH_CI(Ai);
+            sfor<NB>([&](auto Bi) {
+              constexpr int b = OQPH_CI(Bi);
+              sfor<NC>([&](auto Ci) {
+                constexpr int c = OQPH_CI(Ci);
+                sfor<ND>([&](auto Di) {
+                  constexpr int dd = OQPH_CI(Di);
                   const double g = Gam[((a * NB + b) * NC + c) * ND + dd];
-                  const int ia[3] = {c_lx(LA, a), c_ly(LA, a), c_lz(LA, a)};
-                  const int ib[3] = {c_lx(LB, b), c_ly(LB, b), c_lz(LB, b)};
-                  const int ic[3] = {c_lx(LC, c), c_ly(LC, c), c_lz(LC, c)};
-                  const int id[3] = {c_lx(LD, dd), c_ly(LD, dd), c_lz(LD, dd)};
                   double I0[3], dA[3], dB[3], dC[3];
-#pragma unroll
-                  for (int d = 0; d < 3; ++d) {
-                    I0[d] = i2d(G[d], pab[d], pcd[d], ia[d], ib[d], ic[d], id[d]);
-                    double v = a2 * i2d(G[d], pab[d], pcd[d], ia[d] + 1, ib[d], ic[d], id[d]);
-                    if (ia[d] > 0) v -= (double)ia[d] * i2d(G[d], pab[d], pcd[d], ia[d] - 1, ib[d], ic[d], id[d]);
-                    dA[d] = v;
-                    v = b2 * i2d(G[d], pab[d], pcd[d], ia[d], ib[d] + 1, ic[d], id[d]);
-                    if (ib[d] > 0) v -= (double)ib[d] * i2d(G[d], pab[d], pcd[d], ia[d], ib[d] - 1, ic[d], id[d]);
-                    dB[d] = v;
-                    v = c2 * i2d(G[d], pab[d], pcd[d], ia[d], ib[d], ic[d] + 1, id[d]);
-                    if (ic[d] > 0) v -= (double)ic[d] * i2d(G[d], pab[d], pcd[d], ia[d], ib[d], ic[d] - 1, id[d]);
-                    dC[d] = v;
-                  }
+                  sfor<3>([&](auto Xi) {
+                    constexpr int x = OQPH_CI(Xi);
+                    constexpr int ia = c_l(LA, a, x), ib = c_l(LB, b, x), ic = c_l(LC, c, x), id = c_l(LD, dd, x);
+                    const double* pa = pab[x];
+                    const double* pc = pcd[x];
+                    I0[x] = i2d<ia, ib, ic, id>(G[x], pa, pc);
+                    double v = a2 * i2d<ia + 1, ib, ic, id>(G[x], pa, pc);
+                    if constexpr (ia > 0) v -= (double)ia * i2d<ia - 1, ib, ic, id>(G[x], pa, pc);
+                    dA[x] = v;
+                    v = b2 * i2d<ia, ib + 1, ic, id>(G[x], pa, pc);
+                    if constexpr (ib > 0) v -= (double)ib * i2d<ia, ib - 1, ic, id>(G[x], pa, pc);
+                    dB[x] = v;
+                    v = c2 * i2d<ia, ib, ic + 1, id>(G[x], pa, pc);
+                    if constexpr (ic > 0) v -= (double)ic * i2d<ia, ib, ic - 1, id>(G[x], pa, pc);
+                    dC[x] = v;
+                  });
                   fd[0][0] += g * dA[0] * I0[1] * I0[2];
                   fd[0][1] += g * I0[0] * dA[1] * I0[2];
                   fd[0][2] += g * I0[0] * I0[1] * dA[2];
@@ -200,7 +202,10 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
                   fd[2][0] += g * dC[0] * I0[1] * I0[2];
                   fd[2][1] += g * I0[0] * dC[1] * I0[2];
                   fd[2][2] += g * I0[0] * I0[1] * dC[2];
-                }
+                });
+              });
+            });
+          });
         }
       }
     }
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 3 * prm.natm; i += blockDim.x)
-    if (s_de[i] != 0.0) unsafeAtomicAdd(prm.de + i, s_de[i]);
+    if (s_de[i] != 0.0) unsafeAtomicAdd(prm.de + (size_t)(blockIdx.x & (OQPH_DE_SLOTS - 1)) * 3 * prm.natm + i, s_de[i]);
 }
 
 }  // namespace oqph

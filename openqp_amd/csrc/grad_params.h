@@ -4,15 +4,19 @@
 
 namespace oqph {
 
+// The per-block atomic flush of the atomic gradient goes to one of
+// OQPH_DE_SLOTS copies of the natm x 3 array (block index mod slots), summed
+// by the host, so thousands of blocks do not serialise on the same lines.
+#define OQPH_DE_SLOTS 32
+
 struct GradParams {
   const ShellPair* pairs;
   const PrimPair* prims;
   const double* rys;
-  const double* herm;
   const double* Dc;       // ncf x ncf raw Cartesian effective density
   const int* coff;        // Cartesian offset per shell
   const int* shell_atom;  // atom of each shell
-  double* de;             // natm*3
+  double* de;             // OQPH_DE_SLOTS x natm*3
   unsigned long long* counters;  // [0] computed, [1] fine-screened, [3] model flops, [4] prim quartets
   int ncf, natm;
   double gcut;            // grad_cutoff

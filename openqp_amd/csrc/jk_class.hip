@@ -15,7 +15,7 @@ namespace oqph {
 #define OQPH_SCH_LAUNCHER(a, b) OQPH_SCH_LAUNCHER_(a, b)
 
 void OQPH_JK_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const WaveWork& ww, hipStream_t s) {
-  const int bs = 256;
+  const int bs = JK_BLOCK;
   const long long nb = (ww.nwaves * 64 + bs - 1) / bs;
   if (pure)
     hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, true>), dim3((unsigned)nb), dim3(bs), 0, s, p, ww);
@@ -25,15 +25,15 @@ void OQPH_JK_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const WaveWo
 
 #if XA == XC && XB == XD
 void OQPH_SCH_LAUNCHER(XA, XB)(bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
-                               const double* tab, const double* herm, const double* xf, double qcut, double* Q,
+                               const double* tab, const double* xf, double qcut, double* Q,
                                hipStream_t s) {
   const int bs = 64;
   if (pure)
     hipLaunchKernelGGL((k_schwarz<XA, XB, true>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, pairs, prims, list, n, tab,
-                       herm, xf, qcut, Q);
+                       xf, qcut, Q);
   else
     hipLaunchKernelGGL((k_schwarz<XA, XB, false>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, pairs, prims, list, n,
-                       tab, herm, xf, qcut, Q);
+                       tab, xf, qcut, Q);
 }
 #endif
 

@@ -21,6 +21,9 @@
 
 namespace oqph {
 
+// threads per J/K block: 8 wavefronts share one LDS copy of the root tables
+#define JK_BLOCK 512
+
 // Canonical-Cartesian -> AO transforms for L = 2, 3 (Cartesian and pure) live
 // in device memory as xform[2][2][10][10]: rows = AO component, cols =
 // canonical Cartesian component; built by the host from the solid-harmonic
@@ -80,14 +83,48 @@ __device__ __forceinline__ PrimPair load_uniform(const PrimPair* p, int i) {
 #endif
 }
 
+// Bra primitive pairs of a wave-uniform bra held in registers: lane l keeps
+// pairs l and 64 + l (loaded while the wavefront is converged), and the
+// quartet loops broadcast pair i to every lane with v_readlane -- no memory
+// access in the inner loop.  Used for bras of up to 128 primitive pairs.
+struct BraRegs {
+  PrimR<double> lo, hi;
+  int pp0, npp;
+  bool ok;
+};
+__device__ __forceinline__ double bcast(double v, int lane) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename R>
+__device__ __forceinline__ PrimR<R> bcast_prim(const PrimR<double>& p, int lane) {
+  return {(R)bcast(p.g, lane), (R)bcast(p.Kg, lane),
+          {(R)bcast(p.P[0], lane), (R)bcast(p.P[1], lane), (R)bcast(p.P[2], lane)}, (R)bcast(p.h, lane)};
+}
+// every lane of the wavefront must call this (converged)
+__device__ __forceinline__ BraRegs load_bra(const PrimPair* __restrict__ prims, const ShellPair& P) {
+  BraRegs br;
+  br.pp0 = __builtin_amdgcn_readfirstlane(P.prim0);
+  br.npp = __builtin_amdgcn_readfirstlane(P.nprim);
+  br.ok = br.npp <= 128;
+  const int l = (int)__lane_id();
+  const PrimR<double> z{0.0, 0.0, {0.0, 0.0, 0.0}, 0.0};
+  br.lo = (l < br.npp) ? prim_r<double>(prims[br.pp0 + l]) : z;
+  br.hi = (l + 64 < br.npp) ? prim_r<double>(prims[br.pp0 + 64 + l]) : z;
+  return br;
+}
+
 // Contracted (e0|f0) of a quartet: Rys quadrature over every surviving
-// primitive quartet (bra primitives uniform across the wavefront).
+// primitive quartet.  UNIFORM_BRA: the caller guarantees one bra per
+// wavefront; its primitive pairs come from the registers of br when given
+// (br->ok), else through the constant address space (scalar cache).
 template <int LA, int LB, int LC, int LD, typename R, bool UNIFORM_BRA = false>
 __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
-                                         const double* __restrict__ tab, const float* __restrict__ tab32,
-                                         const double* __restrict__ herm, R qcut,
+                                         const double* __restrict__ tabN, const float* __restrict__ tab32N, R qcut,
                                          R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF],
-                                         int& nprim_used) {
+                                         int& nprim_used, const BraRegs* br = nullptr) {
   using Dm = Dims<LA, LB, LC, LD>;
 #pragma unroll
   for (int e = 0; e < Dm::NE; ++e)
@@ -95,9 +132,25 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
     for (int f = 0; f < Dm::NF; ++f) acc[e][f] = 0;
   const R qcut2 = qcut * qcut;
   nprim_used = 0;
-  // the bra is uniform across the wavefront: read its primitive pairs through
-  // the constant address space so they come from the scalar cache
-  // (only when the caller guarantees one bra per wavefront, as k_jk does)
+  if (UNIFORM_BRA && br && br->ok) {
+    const int npp = br->npp;
+    const int n0 = npp < 64 ? npp : 64;
+    for (int iq = 0; iq < Q.nprim; ++iq) {
+      const PrimR<R> qq = prim_r<R>(prims[Q.prim0 + iq]);
+      if (fabs(qq.Kg) < qcut) continue;
+      for (int ip = 0; ip < n0; ++ip) {
+        const PrimR<R> pp = bcast_prim<R>(br->lo, ip);
+        if (fabs(pp.Kg) < qcut) continue;
+        nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tabN, tab32N, acc) ? 1 : 0;
+      }
+      for (int ip = 64; ip < npp; ++ip) {
+        const PrimR<R> pp = bcast_prim<R>(br->hi, ip - 64);
+        if (fabs(pp.Kg) < qcut) continue;
+        nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tabN, tab32N, acc) ? 1 : 0;
+      }
+    }
+    return;
+  }
   const int pp0 = UNIFORM_BRA ? __builtin_amdgcn_readfirstlane(P.prim0) : P.prim0;
   const int npp = UNIFORM_BRA ? __builtin_amdgcn_readfirstlane(P.nprim) : P.nprim;
   for (int iq = 0; iq < Q.nprim; ++iq) {
@@ -106,7 +159,7 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
     for (int ip = 0; ip < npp; ++ip) {
       const PrimPair pp = UNIFORM_BRA ? load_uniform(prims, pp0 + ip) : prims[pp0 + ip];
       if (fabs(pp.Kg) < (double)qcut) continue;
-      nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tab, tab32, herm, acc) ? 1 : 0;
+      nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tabN, tab32N, acc) ? 1 : 0;
     }
   }
 }
@@ -145,50 +198,55 @@ __device__ __forceinline__ void stream_slices(const ShellPair& P, const ShellPai
       double y[NF];
 #pragma unroll
       for (int f = 0; f < NF; ++f) y[f] = 0.0;
-#pragma unroll
-      for (int a = 0; a < NA; ++a) {
+      sfor<NA>([&](auto Ai) {
+        constexpr int a = OQPH_CI(Ai);
         const double wa = xa(LA, ao, a);
-        if (wa == 0.0) continue;
-        const int ax = c_lx(LA, a), ay = c_ly(LA, a), az = c_lz(LA, a);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
+        if (wa == 0.0) return;
+        constexpr int ax = c_lx(LA, a), ay = c_ly(LA, a), az = c_lz(LA, a);
+        sfor<NB>([&](auto Bi) {
+          constexpr int b = OQPH_CI(Bi);
           const double wab = wa * xa(LB, bo, b);
-          if (wab == 0.0) continue;
-          const int bx = c_lx(LB, b), by = c_ly(LB, b), bz = c_lz(LB, b);
-#pragma unroll
-          for (int jx = 0; jx <= bx; ++jx)
-#pragma unroll
-            for (int jy = 0; jy <= by; ++jy)
-#pragma unroll
-              for (int jz = 0; jz <= bz; ++jz) {
-                const int e = range_index<LA>(LA + jx + jy + jz, c_idx(ax + jx, ay + jy, az + jz));
-                const double cf = wab * (double)(binom(bx, jx) * binom(by, jy) * binom(bz, jz)) *
-                                  pa[0][bx - jx] * pa[1][by - jy] * pa[2][bz - jz];
+          if (wab == 0.0) return;
+          constexpr int bx = c_lx(LB, b), by = c_ly(LB, b), bz = c_lz(LB, b);
+          sfor<bx + 1>([&](auto JX) {
+            constexpr int jx = OQPH_CI(JX);
+            sfor<by + 1>([&](auto JY) {
+              constexpr int jy = OQPH_CI(JY);
+              sfor<bz + 1>([&](auto JZ) {
+                constexpr int jz = OQPH_CI(JZ);
+                constexpr int e = range_index<LA>(LA + jx + jy + jz, c_idx(ax + jx, ay + jy, az + jz));
+                constexpr int cb = binom(bx, jx) * binom(by, jy) * binom(bz, jz);
+                const double cf = wab * (double)cb * pa[0][bx - jx] * pa[1][by - jy] * pa[2][bz - jz];
 #pragma unroll
                 for (int f = 0; f < NF; ++f) y[f] += cf * acc[e][f];
-              }
-        }
-      }
+              });
+            });
+          });
+        });
+      });
       double sl[NC * ND];
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int dd = 0; dd < ND; ++dd) {
-          const int cx = c_lx(LC, c), cy = c_ly(LC, c), cz = c_lz(LC, c);
-          const int dx = c_lx(LD, dd), dy = c_ly(LD, dd), dz = c_lz(LD, dd);
+      sfor<NC>([&](auto Ci) {
+        constexpr int c = OQPH_CI(Ci);
+        constexpr int cx = c_lx(LC, c), cy = c_ly(LC, c), cz = c_lz(LC, c);
+        sfor<ND>([&](auto Di) {
+          constexpr int dd = OQPH_CI(Di);
+          constexpr int dx = c_lx(LD, dd), dy = c_ly(LD, dd), dz = c_lz(LD, dd);
           double v = 0.0;
-#pragma unroll
-          for (int jx = 0; jx <= dx; ++jx)
-#pragma unroll
-            for (int jy = 0; jy <= dy; ++jy)
-#pragma unroll
-              for (int jz = 0; jz <= dz; ++jz) {
-                const int f = range_index<LC>(LC + jx + jy + jz, c_idx(cx + jx, cy + jy, cz + jz));
-                v += (double)(binom(dx, jx) * binom(dy, jy) * binom(dz, jz)) * pc[0][dx - jx] * pc[1][dy - jy] *
-                     pc[2][dz - jz] * y[f];
-              }
+          sfor<dx + 1>([&](auto JX) {
+            constexpr int jx = OQPH_CI(JX);
+            sfor<dy + 1>([&](auto JY) {
+              constexpr int jy = OQPH_CI(JY);
+              sfor<dz + 1>([&](auto JZ) {
+                constexpr int jz = OQPH_CI(JZ);
+                constexpr int f = range_index<LC>(LC + jx + jy + jz, c_idx(cx + jx, cy + jy, cz + jz));
+                constexpr int cb = binom(dx, jx) * binom(dy, jy) * binom(dz, jz);
+                v += (double)cb * pc[0][dx - jx] * pc[1][dy - jy] * pc[2][dz - jz] * y[f];
+              });
+            });
+          });
           sl[c * ND + dd] = v;
-        }
+        });
+      });
       if constexpr (LC >= 2) xform_inplace<LC, PURE, 1, ND, double>(xf, sl);
       if constexpr (LD >= 2) xform_inplace<LD, PURE, AO<LC, PURE>::N, 1, double>(xf, sl);
       vis(ao, bo, sl);
@@ -203,13 +261,13 @@ struct Big {
 // Final-representation block of a quartet: V[a][b][c][d] in AO order.
 template <int LA, int LB, int LC, int LD, bool PURE, typename R, bool UNIFORM_BRA = false>
 __device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
-                                              const double* __restrict__ tab, const float* __restrict__ tab32,
-                                              const double* __restrict__ herm,
+                                              const double* __restrict__ tabN, const float* __restrict__ tab32N,
                                               const double* __restrict__ xf, R qcut,
-                                              R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used) {
+                                              R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used,
+                                              const BraRegs* br = nullptr) {
   using Dm = Dims<LA, LB, LC, LD>;
   R acc[Dm::NE][Dm::NF];
-  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, tab, tab32, herm, qcut, acc, nprim_used);
+  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, tabN, tab32N, qcut, acc, nprim_used, br);
   R AB[3], CD[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -253,8 +311,32 @@ __device__ __forceinline__ void count_wave(unsigned long long* ctr, bool pred) {
     atomicAdd(ctr, (unsigned long long)__popcll(m));
 }
 
+// Root tables of the class's root count are copied into LDS once per block
+// when they fit (FP64 for up to 3 roots, FP32 for every class that has an
+// FP32 path): the table lookups are per-lane gathers at random intervals,
+// which LDS serves at a fraction of the vector-memory cost.
+template <int NR>
+struct RysLds {
+  static constexpr bool f64 = rys_len<NR>() * 8 <= 36 * 1024;
+};
+
 template <int LA, int LB, int LC, int LD, bool PURE>
-__global__ __launch_bounds__(256) void k_jk(JKParams prm, WaveWork ww) {
+__global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
+  using Dm = Dims<LA, LB, LC, LD>;
+  constexpr int NR = Dm::NROOT;
+  constexpr bool T64 = RysLds<NR>::f64 && !Big<LA, LB, LC, LD>::value;
+  constexpr bool T32 = Dm::NE * Dm::NF <= 256 && !Big<LA, LB, LC, LD>::value;  // classes with an FP32 path
+  __shared__ double s_t64[T64 ? rys_len<NR>() : 1];
+  __shared__ float s_t32[T32 ? rys32_len<NR>() : 1];
+  if constexpr (T64)
+    for (int i = threadIdx.x; i < rys_len<NR>(); i += blockDim.x) s_t64[i] = prm.rys[RYS_OFF[NR] + i];
+  if constexpr (T32)
+    if (prm.mixed)
+      for (int i = threadIdx.x; i < rys32_len<NR>(); i += blockDim.x) s_t32[i] = prm.rys32[RYS32_OFF[NR] + i];
+  __syncthreads();
+  const double* tabN = T64 ? (const double*)s_t64 : prm.rys + RYS_OFF[NR];
+  const float* tab32N = T32 ? (const float*)s_t32 : prm.rys32 + RYS32_OFF[NR];
+
   // one wavefront = one chunk of 64 kets of one segment; the bra is uniform
   const long long wave = __builtin_amdgcn_readfirstlane(
       (int)((blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6));
@@ -287,7 +369,7 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, WaveWork ww) {
     bound = P.Q * Q.Q * dm;
     skip = (qbra * prm.maxden < prm.cutoff) || (bound < prm.cutoff);
   }
-  unsigned long long* ctr = prm.counters;
+  unsigned long long* ctr = prm.counters + 8 * (int)(wave & (OQPH_CTR_SLOTS - 1));
   // every lane of the wave reaches the counters (no early exit before them)
   count_wave(ctr + 1, active && skip);
   count_wave(ctr + 0, !skip);
@@ -298,7 +380,7 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, WaveWork ww) {
     double acc[Dm::NE][Dm::NF];
     int nprim = 0;
     if (!skip)
-      contract<LA, LB, LC, LD, double, true>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, 1.0e-4 * prm.cutoff,
+      contract<LA, LB, LC, LD, double, true>(P, Q, prm.prims, tabN, tab32N, 1.0e-4 * prm.cutoff,
                                              acc, nprim);
     count_wave(ctr + 2, false);
     sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() +
@@ -411,25 +493,31 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, WaveWork ww) {
 #pragma unroll
   for (int i = 0; i < NOUT; ++i) V[i] = 0.0;
   int nprim = 0;
+  // small classes keep the bra's primitive pairs in registers
+  constexpr bool BR = Dm::NE * Dm::NF <= 64;
+  BraRegs br{};
+  if constexpr (BR) br = load_bra(prm.prims, P);
+  // FP32 evaluation when every quartet of the wavefront has its bound below
+  // the mixed-precision cutoff (a per-wavefront choice: no divergence
+  // between the two paths); classes small enough to carry both paths
   bool lowp = false;
+  if constexpr (T32) lowp = prm.mixed && __ballot(!skip && bound >= prm.fp32_cut) == 0ull;
   if (!skip) {
-    // FP32 evaluation for quartets whose bound is below the mixed-precision
-    // cutoff (classes small enough to carry both instantiations)
-    if constexpr (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF <= 256) {
-      lowp = prm.mixed && bound < prm.fp32_cut;
+    if constexpr (T32) {
       if (lowp) {
         float Vf[NBLK];
-        quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, prm.xform,
-                                                   (float)(1.0e-4 * prm.cutoff), Vf, nprim);
+        quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
+                                                         (float)(1.0e-4 * prm.cutoff), Vf, nprim,
+                                                         BR ? &br : nullptr);
 #pragma unroll
         for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
       }
     }
     if (!lowp)
-      quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, prm.rys, prm.rys32, prm.herm, prm.xform,
-                                                  1.0e-4 * prm.cutoff, V, nprim);
+      quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
+                                                        1.0e-4 * prm.cutoff, V, nprim, BR ? &br : nullptr);
   }
-  count_wave(ctr + 2, lowp);
+  count_wave(ctr + 2, lowp && !skip);
   sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() + (nprim ? 12ull * NOUT : 0ull));
   sum_wave(ctr + 4, (unsigned long long)nprim);
   const bool valid = !skip && nprim > 0;
@@ -571,7 +659,7 @@ __global__ __launch_bounds__(256) void k_jk(JKParams prm, WaveWork ww) {
 // primitive pairs passed here are built with the tight cutoffs of :1600-1604.
 template <int LA, int LB, bool PURE>
 __global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const PrimPair* prims, const int* list,
-                                                 int n, const double* tab, const double* herm, const double* xf,
+                                                 int n, const double* tab, const double* xf,
                                                  double qcut, double* Qout) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -582,7 +670,8 @@ __global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const P
   int nprim;
   if constexpr (Big<LA, LB, LA, LB>::value) {
     double acc[Dims<LA, LB, LA, LB>::NE][Dims<LA, LB, LA, LB>::NF];
-    contract<LA, LB, LA, LB, double>(P, P, prims, tab, nullptr, herm, qcut, acc, nprim);
+    contract<LA, LB, LA, LB, double>(P, P, prims, tab + RYS_OFF[Dims<LA, LB, LA, LB>::NROOT], nullptr, qcut, acc,
+                                     nprim);
     if (nprim)
       stream_slices<LA, LB, LA, LB, PURE>(P, P, xf, acc, [&](int, int, double* S) {
 #pragma unroll
@@ -590,7 +679,8 @@ __global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const P
       });
   } else {
     double V[Dims<LA, LB, LA, LB>::NBLK];
-    quartet_block<LA, LB, LA, LB, PURE, double>(P, P, prims, tab, nullptr, herm, xf, qcut, V, nprim);
+    quartet_block<LA, LB, LA, LB, PURE, double>(P, P, prims, tab + RYS_OFF[Dims<LA, LB, LA, LB>::NROOT], nullptr,
+                                                xf, qcut, V, nprim);
     if (nprim) {
 #pragma unroll
       for (int i = 0; i < N; ++i) m = fmax(m, fabs(V[i]));

@@ -63,12 +63,17 @@ struct WaveWork {
   long long nwaves;
 };
 
+// Statistics counters of one launch: OQPH_CTR_SLOTS slots of 8 counters, one
+// 64-byte line each.  A wavefront adds its totals to slot (wavefront index mod
+// OQPH_CTR_SLOTS), so the atomics of millions of short wavefronts spread over
+// many L2 lines instead of serialising on one; the host sums the slots.
+#define OQPH_CTR_SLOTS 128
+
 // Arguments of one J/K build launch (shared by host and device code).
 struct JKParams {
   const ShellPair* pairs;
   const PrimPair* prims;
   const double* rys;
-  const double* herm;
   const float* rys32;   // FP32 root fits (degree 6)
   const double* xform;  // [2][2][10][10] canonical Cartesian -> AO transforms (L = 2, 3)
   const double* D;      // packed density (ntri)

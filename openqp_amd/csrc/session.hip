@@ -28,7 +28,7 @@
 namespace oqph {
 bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const WaveWork& cw, hipStream_t s);
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
-                    const double* tab, const double* herm, const double* xf, double qcut, double* Q, hipStream_t s);
+                    const double* tab, const double* xf, double qcut, double* Q, hipStream_t s);
 void launch_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh, hipStream_t s);
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
 bool kernels_support(int lmax);
@@ -37,7 +37,6 @@ bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const Clas
 void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
                          const double* xf, int ncf, double* Dc, hipStream_t s);
 const double* rys_table_host(size_t* n);
-const double* rys_herm_host(size_t* n);
 const float* rys32_table_host(size_t* n);
 }  // namespace oqph
 
@@ -127,7 +126,7 @@ struct oqph_session {
   std::vector<std::vector<double>> clsQ;     // matching Q values
   DBuf<ShellPair> d_pairs;
   DBuf<PrimPair> d_prims;
-  DBuf<double> d_rys, d_herm, d_xform;
+  DBuf<double> d_rys, d_xform;
   DBuf<float> d_rys32;
   DBuf<int> d_cls;                           // concatenated class lists
   DBuf<int> d_kbs;                           // per class: bucket starts (NBKT+1)
@@ -242,12 +241,10 @@ static int prepare_geometry(oqph_session* s) {
   HIPCHK(hipSetDevice(s->device));
   int rc;
   // Rys tables and AO transforms
-  size_t nt, nh;
+  size_t nt;
   const double* tab = rys_table_host(&nt);
-  const double* herm = rys_herm_host(&nh);
-  if ((rc = s->d_rys.ensure(nt)) || (rc = s->d_herm.ensure(nh))) return rc;
+  if ((rc = s->d_rys.ensure(nt))) return rc;
   if ((rc = upload(s->d_rys.p, tab, nt * sizeof(double), s->stream))) return rc;
-  if ((rc = upload(s->d_herm.p, herm, nh * sizeof(double), s->stream))) return rc;
   size_t n32;
   const float* tab32 = rys32_table_host(&n32);
   if ((rc = s->d_rys32.ensure(n32))) return rc;
@@ -284,7 +281,7 @@ static int prepare_geometry(oqph_session* s) {
     for (int lb = 0; lb <= la; ++lb) {
       const int c = class_of(la, lb);
       if (!launch_schwarz(la, lb, s->pure, dtp.p, dtpr.p, dlist.p + foff[c], (int)byc[c].size(), s->d_rys.p,
-                          s->d_herm.p, s->d_xform.p, 1.0e-17, dQ.p, s->stream)) {
+                          s->d_xform.p, 1.0e-17, dQ.p, s->stream)) {
         set_err("no Schwarz kernel for class (%d %d)", la, lb);
         return 1;
       }
@@ -431,7 +428,7 @@ static int prepare_grad(oqph_session* s) {
   s->ncf = ncf;
   if ((rc = s->d_coff.ensure(s->nsh)) || (rc = s->d_satom.ensure(s->nsh)) || (rc = s->d_am.ensure(s->nsh)) ||
       (rc = s->d_pure.ensure(s->nsh)) || (rc = s->d_Dc.ensure((size_t)ncf * ncf)) ||
-      (rc = s->d_de.ensure(3 * (size_t)s->natm)))
+      (rc = s->d_de.ensure(3 * (size_t)s->natm * OQPH_DE_SLOTS)))
     return rc;
   if ((rc = upload(s->d_coff.p, cof.data(), s->nsh * sizeof(int), s->stream)) ||
       (rc = upload(s->d_satom.p, s->atom.data(), s->nsh * sizeof(int), s->stream)) ||
@@ -472,11 +469,20 @@ static int prepare_lists(oqph_session* s, double maxden) {
       const auto& kb = s->kbs[ck];
       const auto& bb = s->kbs[cb];
       const size_t seg0 = s->h_seg_bra.size(), ch0 = s->h_choff.size();
-      long long waves = 0;
+      // largest ket primitive-pair count of each bucket
+      int kmax[NBKT] = {0};
+      for (int b = 0; b < NBKT; ++b)
+        for (int k = kb[b]; k < kb[b + 1]; ++k) kmax[b] = std::max(kmax[b], s->pairs[s->cls[ck][k]].nprim);
+      struct Seg {
+        int bra, k0, n;
+        long long cost;  // primitive quartets one wavefront of the segment loops over
+      };
+      std::vector<Seg> segs;
       int bp = 0;
       for (size_t p = 0; p < QB.size(); ++p) {
         while (bp + 1 < NBKT && (int)p >= bb[bp + 1]) ++bp;
         const double thr = c / (4.0 * maxden * QB[p]);
+        const int bra = s->cls[cb][p];
         for (int b = 0; b < NBKT; ++b) {
           if (kb[b + 1] <= kb[b]) continue;
           long long k = std::lower_bound(QK.begin() + kb[b], QK.begin() + kb[b + 1], thr,
@@ -486,12 +492,19 @@ static int prepare_lists(oqph_session* s, double maxden) {
             else if (b == bp) k = std::min<long long>(k, (long long)p - bb[bp] + 1);
           }
           if (k == 0) continue;
-          s->h_seg_bra.push_back(s->cls[cb][p]);
-          s->h_seg_k0.push_back(kb[b]);
-          s->h_seg_n.push_back((int)k);
-          s->h_choff.push_back(waves);
-          waves += (k + 63) / 64;
+          segs.push_back({bra, kb[b], (int)k, (long long)s->pairs[bra].nprim * kmax[b]});
         }
+      }
+      // longest wavefronts first, so the launch does not end on a tail of
+      // heavily contracted quartets
+      std::stable_sort(segs.begin(), segs.end(), [](const Seg& x, const Seg& y) { return x.cost > y.cost; });
+      long long waves = 0;
+      for (const Seg& g : segs) {
+        s->h_seg_bra.push_back(g.bra);
+        s->h_seg_k0.push_back(g.k0);
+        s->h_seg_n.push_back(g.n);
+        s->h_choff.push_back(waves);
+        waves += (g.n + 63) / 64;
       }
       const int nseg = (int)(s->h_seg_bra.size() - seg0);
       s->h_choff.push_back(waves);  // terminator of this launch's prefix
@@ -554,8 +567,9 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   }
   if ((rc = prepare_lists(s, maxden))) return rc;
   const size_t nl = s->launches.size();
-  if ((rc = s->d_ctr.ensure(8 + 8 * nl))) return rc;
-  HIPCHK(hipMemsetAsync(s->d_ctr.p + 8, 0, 8 * nl * sizeof(unsigned long long), stream));
+  const size_t nc = 8 * (size_t)OQPH_CTR_SLOTS;  // counters per launch
+  if ((rc = s->d_ctr.ensure(8 + nc * nl))) return rc;
+  HIPCHK(hipMemsetAsync(s->d_ctr.p + 8, 0, nc * nl * sizeof(unsigned long long), stream));
   if (s->profile && s->pev.size() < 2 * nl) {
     for (size_t i = s->pev.size(); i < 2 * nl; ++i) {
       hipEvent_t e;
@@ -567,7 +581,6 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   p.pairs = s->d_pairs.p;
   p.prims = s->d_prims.p;
   p.rys = s->d_rys.p;
-  p.herm = s->d_herm.p;
   p.rys32 = s->d_rys32.p;
   p.xform = s->d_xform.p;
   p.D = dD;
@@ -584,7 +597,7 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   HIPCHK(hipEventRecord(s->ev0, stream));
   for (size_t i = 0; i < nl; ++i) {
     auto& L = s->launches[i];
-    p.counters = s->d_ctr.p + 8 + 8 * i;
+    p.counters = s->d_ctr.p + 8 + nc * i;
     if (s->profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
     if (!launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream)) {
       set_err("no J/K kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
@@ -595,10 +608,13 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, stream));
   if (st || s->profile) {
-    s->prof_ctr.assign(8 * nl, 0ull);
-    if (nl) HIPCHK(hipMemcpyAsync(s->prof_ctr.data(), s->d_ctr.p + 8, 8 * nl * sizeof(unsigned long long),
+    std::vector<unsigned long long> raw(nc * nl);
+    if (nl) HIPCHK(hipMemcpyAsync(raw.data(), s->d_ctr.p + 8, nc * nl * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    s->prof_ctr.assign(8 * nl, 0ull);
+    for (size_t i = 0; i < nl; ++i)
+      for (size_t k = 0; k < nc; ++k) s->prof_ctr[8 * i + k % 8] += raw[nc * i + k];
     s->prof_ms.assign(nl, 0.f);
     if (s->profile)
       for (size_t i = 0; i < nl; ++i) HIPCHK(hipEventElapsedTime(&s->prof_ms[i], s->pev[2 * i], s->pev[2 * i + 1]));
@@ -659,7 +675,7 @@ int oqph_session_destroy(oqph_session* s) {
     if (g_default == s) g_default = nullptr;
   }
   hipSetDevice(s->device);
-  s->d_pairs.release(); s->d_prims.release(); s->d_rys.release(); s->d_herm.release(); s->d_cls.release();
+  s->d_pairs.release(); s->d_prims.release(); s->d_rys.release(); s->d_cls.release();
   s->d_xform.release(); s->d_rys32.release(); s->d_kbs.release(); s->d_seg.release(); s->d_choff.release();
   s->d_gpairs.release(); s->d_gprims.release(); s->d_gcls.release(); s->d_coff.release(); s->d_satom.release();
   s->d_am.release(); s->d_pure.release(); s->d_goff.release(); s->d_Dc.release(); s->d_de.release();
@@ -869,10 +885,12 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   if ((rc = upload(s->d_D.p, d, ntri * sizeof(double), stream))) return rc;
   launch_cart_density(s->d_D.p, s->nsh, s->d_am.p, s->d_pure.p, s->d_aooff.p, s->d_coff.p, s->d_xform.p, s->ncf,
                       s->d_Dc.p, stream);
-  HIPCHK(hipMemsetAsync(s->d_de.p, 0, 3 * (size_t)s->natm * sizeof(double), stream));
+  const size_t nde = 3 * (size_t)s->natm;
+  HIPCHK(hipMemsetAsync(s->d_de.p, 0, nde * OQPH_DE_SLOTS * sizeof(double), stream));
   const size_t nl = s->glaunches.size();
-  if ((rc = s->d_ctr.ensure(8 + 8 * std::max(nl, s->launches.size())))) return rc;
-  HIPCHK(hipMemsetAsync(s->d_ctr.p + 8, 0, 8 * nl * sizeof(unsigned long long), stream));
+  const size_t nc = 8 * (size_t)OQPH_CTR_SLOTS;  // counters per launch
+  if ((rc = s->d_ctr.ensure(8 + nc * std::max(nl, s->launches.size())))) return rc;
+  HIPCHK(hipMemsetAsync(s->d_ctr.p + 8, 0, nc * nl * sizeof(unsigned long long), stream));
   double zbig = 0.0;
   for (double e : s->ex) zbig = std::max(zbig, e);
   double dabcut = 1.0e-11;                 // grd2.F90:224-226
@@ -882,7 +900,6 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   p.pairs = s->d_gpairs.p;
   p.prims = s->d_gprims.p;
   p.rys = s->d_rys.p;
-  p.herm = s->d_herm.p;
   p.Dc = s->d_Dc.p;
   p.coff = s->d_coff.p;
   p.shell_atom = s->d_satom.p;
@@ -896,7 +913,7 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   HIPCHK(hipEventRecord(s->ev0, stream));
   for (size_t i = 0; i < nl; ++i) {
     auto& L = s->glaunches[i];
-    p.counters = s->d_ctr.p + 8 + 8 * i;
+    p.counters = s->d_ctr.p + 8 + nc * i;
     if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.cw, stream)) {
       set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
@@ -904,11 +921,20 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, stream));
-  HIPCHK(hipMemcpyAsync(de, s->d_de.p, 3 * (size_t)s->natm * sizeof(double), hipMemcpyDeviceToHost, stream));
-  std::vector<unsigned long long> ctr(8 * nl);
-  if (nl) HIPCHK(hipMemcpyAsync(ctr.data(), s->d_ctr.p + 8, 8 * nl * sizeof(unsigned long long),
+  std::vector<double> deraw(nde * OQPH_DE_SLOTS);
+  HIPCHK(hipMemcpyAsync(deraw.data(), s->d_de.p, deraw.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  std::vector<unsigned long long> raw(nc * nl);
+  if (nl) HIPCHK(hipMemcpyAsync(raw.data(), s->d_ctr.p + 8, nc * nl * sizeof(unsigned long long),
                                 hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
+  for (size_t i = 0; i < nde; ++i) {
+    double v = 0.0;
+    for (int k = 0; k < OQPH_DE_SLOTS; ++k) v += deraw[k * nde + i];
+    de[i] = v;
+  }
+  std::vector<unsigned long long> ctr(8 * nl, 0ull);
+  for (size_t i = 0; i < nl; ++i)
+    for (size_t k = 0; k < nc; ++k) ctr[8 * i + k % 8] += raw[nc * i + k];
   if (st) {
     memset(st, 0, sizeof(*st));
     for (size_t i = 0; i < nl; ++i) {

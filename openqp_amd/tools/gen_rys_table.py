@@ -168,9 +168,8 @@ def main():
         ho += 2 * n
         hvals.extend([float(v) for v in r] + [float(v) for v in h])
     lines.append("static constexpr int RYS_HOFF[RYS_NMAX + 1] = {0, " + ", ".join(str(v) for v in hoffs) + "};")
-    lines.append("#ifdef OQPH_WANT_RYS_DATA")
-    lines.append(f"static const double RYS_HERM[{ho}] = {{" + ", ".join(float.hex(v) for v in hvals) + "};")
-    lines.append("#endif")
+    lines.append("// small and indexed by compile-time constants in device code")
+    lines.append(f"static constexpr double RYS_HERM[{ho}] = {{" + ", ".join(float.hex(v) for v in hvals) + "};")
     with open(a.out, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     print("wrote", a.out, "doubles", o)
