@@ -27,6 +27,48 @@ void launch_jk_2211(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2220(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2221(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2222(bool, const JKParams&, const WaveWork&, hipStream_t);
+bool launch_jkb_0000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_0000(bool, int*, size_t*, size_t);
+bool launch_jkb_1000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_1000(bool, int*, size_t*, size_t);
+bool launch_jkb_1010(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_1010(bool, int*, size_t*, size_t);
+bool launch_jkb_1100(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_1100(bool, int*, size_t*, size_t);
+bool launch_jkb_1110(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_1110(bool, int*, size_t*, size_t);
+bool launch_jkb_1111(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_1111(bool, int*, size_t*, size_t);
+bool launch_jkb_2000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2000(bool, int*, size_t*, size_t);
+bool launch_jkb_2010(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2010(bool, int*, size_t*, size_t);
+bool launch_jkb_2011(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2011(bool, int*, size_t*, size_t);
+bool launch_jkb_2020(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2020(bool, int*, size_t*, size_t);
+bool launch_jkb_2100(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2100(bool, int*, size_t*, size_t);
+bool launch_jkb_2110(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2110(bool, int*, size_t*, size_t);
+bool launch_jkb_2111(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2111(bool, int*, size_t*, size_t);
+bool launch_jkb_2120(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2120(bool, int*, size_t*, size_t);
+bool launch_jkb_2121(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2121(bool, int*, size_t*, size_t);
+bool launch_jkb_2200(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2200(bool, int*, size_t*, size_t);
+bool launch_jkb_2210(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2210(bool, int*, size_t*, size_t);
+bool launch_jkb_2211(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2211(bool, int*, size_t*, size_t);
+bool launch_jkb_2220(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2220(bool, int*, size_t*, size_t);
+bool launch_jkb_2221(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2221(bool, int*, size_t*, size_t);
+bool launch_jkb_2222(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool jkb_info_2222(bool, int*, size_t*, size_t);
 void launch_schwarz_00(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
 void launch_schwarz_10(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
 void launch_schwarz_11(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
@@ -63,6 +105,63 @@ bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, con
     case 2220: launch_jk_2220(pure, p, cw, s); return true;
     case 2221: launch_jk_2221(pure, p, cw, s); return true;
     case 2222: launch_jk_2222(pure, p, cw, s); return true;
+    default: return false;
+  }
+}
+
+bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
+                size_t lds, hipStream_t s) {
+  if (bw.nitem == 0) return true;
+  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
+    case 0: return launch_jkb_0000(pure, p, bw, cm, lds, s);
+    case 1000: return launch_jkb_1000(pure, p, bw, cm, lds, s);
+    case 1010: return launch_jkb_1010(pure, p, bw, cm, lds, s);
+    case 1100: return launch_jkb_1100(pure, p, bw, cm, lds, s);
+    case 1110: return launch_jkb_1110(pure, p, bw, cm, lds, s);
+    case 1111: return launch_jkb_1111(pure, p, bw, cm, lds, s);
+    case 2000: return launch_jkb_2000(pure, p, bw, cm, lds, s);
+    case 2010: return launch_jkb_2010(pure, p, bw, cm, lds, s);
+    case 2011: return launch_jkb_2011(pure, p, bw, cm, lds, s);
+    case 2020: return launch_jkb_2020(pure, p, bw, cm, lds, s);
+    case 2100: return launch_jkb_2100(pure, p, bw, cm, lds, s);
+    case 2110: return launch_jkb_2110(pure, p, bw, cm, lds, s);
+    case 2111: return launch_jkb_2111(pure, p, bw, cm, lds, s);
+    case 2120: return launch_jkb_2120(pure, p, bw, cm, lds, s);
+    case 2121: return launch_jkb_2121(pure, p, bw, cm, lds, s);
+    case 2200: return launch_jkb_2200(pure, p, bw, cm, lds, s);
+    case 2210: return launch_jkb_2210(pure, p, bw, cm, lds, s);
+    case 2211: return launch_jkb_2211(pure, p, bw, cm, lds, s);
+    case 2220: return launch_jkb_2220(pure, p, bw, cm, lds, s);
+    case 2221: return launch_jkb_2221(pure, p, bw, cm, lds, s);
+    case 2222: return launch_jkb_2222(pure, p, bw, cm, lds, s);
+    default: return false;
+  }
+}
+
+// false when the class has no bra-row kernel
+bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* static_lds, size_t lds_max) {
+  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
+    case 0: return jkb_info_0000(pure, nrow, static_lds, lds_max);
+    case 1000: return jkb_info_1000(pure, nrow, static_lds, lds_max);
+    case 1010: return jkb_info_1010(pure, nrow, static_lds, lds_max);
+    case 1100: return jkb_info_1100(pure, nrow, static_lds, lds_max);
+    case 1110: return jkb_info_1110(pure, nrow, static_lds, lds_max);
+    case 1111: return jkb_info_1111(pure, nrow, static_lds, lds_max);
+    case 2000: return jkb_info_2000(pure, nrow, static_lds, lds_max);
+    case 2010: return jkb_info_2010(pure, nrow, static_lds, lds_max);
+    case 2011: return jkb_info_2011(pure, nrow, static_lds, lds_max);
+    case 2020: return jkb_info_2020(pure, nrow, static_lds, lds_max);
+    case 2100: return jkb_info_2100(pure, nrow, static_lds, lds_max);
+    case 2110: return jkb_info_2110(pure, nrow, static_lds, lds_max);
+    case 2111: return jkb_info_2111(pure, nrow, static_lds, lds_max);
+    case 2120: return jkb_info_2120(pure, nrow, static_lds, lds_max);
+    case 2121: return jkb_info_2121(pure, nrow, static_lds, lds_max);
+    case 2200: return jkb_info_2200(pure, nrow, static_lds, lds_max);
+    case 2210: return jkb_info_2210(pure, nrow, static_lds, lds_max);
+    case 2211: return jkb_info_2211(pure, nrow, static_lds, lds_max);
+    case 2220: return jkb_info_2220(pure, nrow, static_lds, lds_max);
+    case 2221: return jkb_info_2221(pure, nrow, static_lds, lds_max);
+    case 2222: return jkb_info_2222(pure, nrow, static_lds, lds_max);
     default: return false;
   }
 }

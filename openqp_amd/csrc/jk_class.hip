@@ -23,6 +23,44 @@ void OQPH_JK_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const WaveWo
     hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, false>), dim3((unsigned)nb), dim3(bs), 0, s, p, ww);
 }
 
+#define OQPH_JKB_LAUNCHER_(a, b, c, d) launch_jkb_##a##b##c##d
+#define OQPH_JKB_LAUNCHER(a, b, c, d) OQPH_JKB_LAUNCHER_(a, b, c, d)
+#define OQPH_JKB_INFO_(a, b, c, d) jkb_info_##a##b##c##d
+#define OQPH_JKB_INFO(a, b, c, d) OQPH_JKB_INFO_(a, b, c, d)
+
+// Bra-row kernel of this class (not for the streamed large classes):
+// K-row count and static LDS bytes, and the launch with `lds` bytes of K rows.
+bool OQPH_JKB_INFO(XA, XB, XC, XD)(bool pure, int* nrow, size_t* static_lds, size_t lds_max) {
+  if constexpr (Big<XA, XB, XC, XD>::value) {
+    return false;
+  } else {
+    const void* f = pure ? (const void*)k_jkb<XA, XB, XC, XD, true> : (const void*)k_jkb<XA, XB, XC, XD, false>;
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, f) != hipSuccess) return false;
+    if (lds_max > a.sharedSizeBytes &&
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max - a.sharedSizeBytes)) !=
+            hipSuccess)
+      return false;
+    *nrow = pure ? AO<XA, true>::N + AO<XB, true>::N : AO<XA, false>::N + AO<XB, false>::N;
+    *static_lds = a.sharedSizeBytes;
+    return true;
+  }
+}
+
+bool OQPH_JKB_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm, size_t lds,
+                                       hipStream_t s) {
+  if constexpr (Big<XA, XB, XC, XD>::value) {
+    return false;
+  } else {
+    const int bs = JKB<XA, XB, XC, XD>::threads;
+    if (pure)
+      hipLaunchKernelGGL((k_jkb<XA, XB, XC, XD, true>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm);
+    else
+      hipLaunchKernelGGL((k_jkb<XA, XB, XC, XD, false>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm);
+    return true;
+  }
+}
+
 #if XA == XC && XB == XD
 void OQPH_SCH_LAUNCHER(XA, XB)(bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
                                const double* tab, const double* xf, double qcut, double* Q,

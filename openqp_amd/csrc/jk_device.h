@@ -83,48 +83,20 @@ __device__ __forceinline__ PrimPair load_uniform(const PrimPair* p, int i) {
 #endif
 }
 
-// Bra primitive pairs of a wave-uniform bra held in registers: lane l keeps
-// pairs l and 64 + l (loaded while the wavefront is converged), and the
-// quartet loops broadcast pair i to every lane with v_readlane -- no memory
-// access in the inner loop.  Used for bras of up to 128 primitive pairs.
-struct BraRegs {
-  PrimR<double> lo, hi;
-  int pp0, npp;
-  bool ok;
-};
-__device__ __forceinline__ double bcast(double v, int lane) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-template <typename R>
-__device__ __forceinline__ PrimR<R> bcast_prim(const PrimR<double>& p, int lane) {
-  return {(R)bcast(p.g, lane), (R)bcast(p.Kg, lane),
-          {(R)bcast(p.P[0], lane), (R)bcast(p.P[1], lane), (R)bcast(p.P[2], lane)}, (R)bcast(p.h, lane)};
-}
-// every lane of the wavefront must call this (converged)
-__device__ __forceinline__ BraRegs load_bra(const PrimPair* __restrict__ prims, const ShellPair& P) {
-  BraRegs br;
-  br.pp0 = __builtin_amdgcn_readfirstlane(P.prim0);
-  br.npp = __builtin_amdgcn_readfirstlane(P.nprim);
-  br.ok = br.npp <= 128;
-  const int l = (int)__lane_id();
-  const PrimR<double> z{0.0, 0.0, {0.0, 0.0, 0.0}, 0.0};
-  br.lo = (l < br.npp) ? prim_r<double>(prims[br.pp0 + l]) : z;
-  br.hi = (l + 64 < br.npp) ? prim_r<double>(prims[br.pp0 + 64 + l]) : z;
-  return br;
-}
+// Bra primitive pairs staged in LDS by a block whose bra is fixed (k_jkb):
+// every lane reads pair i from the same LDS address (a broadcast), with no
+// vector-memory access in the quadrature loop.  Up to JKB_MAX_BRA_PRIMS pairs.
+#define JKB_MAX_BRA_PRIMS 128
 
 // Contracted (e0|f0) of a quartet: Rys quadrature over every surviving
 // primitive quartet.  UNIFORM_BRA: the caller guarantees one bra per
-// wavefront; its primitive pairs come from the registers of br when given
-// (br->ok), else through the constant address space (scalar cache).
+// wavefront; its primitive pairs come from the LDS copy sbra when given, else
+// through the constant address space (scalar cache).
 template <int LA, int LB, int LC, int LD, typename R, bool UNIFORM_BRA = false>
 __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
                                          const double* __restrict__ tabN, const float* __restrict__ tab32N, R qcut,
                                          R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF],
-                                         int& nprim_used, const BraRegs* br = nullptr) {
+                                         int& nprim_used, const PrimR<double>* sbra = nullptr) {
   using Dm = Dims<LA, LB, LC, LD>;
 #pragma unroll
   for (int e = 0; e < Dm::NE; ++e)
@@ -132,19 +104,14 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
     for (int f = 0; f < Dm::NF; ++f) acc[e][f] = 0;
   const R qcut2 = qcut * qcut;
   nprim_used = 0;
-  if (UNIFORM_BRA && br && br->ok) {
-    const int npp = br->npp;
-    const int n0 = npp < 64 ? npp : 64;
+  if (UNIFORM_BRA && sbra) {
+    const int npp = __builtin_amdgcn_readfirstlane(P.nprim);
     for (int iq = 0; iq < Q.nprim; ++iq) {
       const PrimR<R> qq = prim_r<R>(prims[Q.prim0 + iq]);
       if (fabs(qq.Kg) < qcut) continue;
-      for (int ip = 0; ip < n0; ++ip) {
-        const PrimR<R> pp = bcast_prim<R>(br->lo, ip);
-        if (fabs(pp.Kg) < qcut) continue;
-        nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tabN, tab32N, acc) ? 1 : 0;
-      }
-      for (int ip = 64; ip < npp; ++ip) {
-        const PrimR<R> pp = bcast_prim<R>(br->hi, ip - 64);
+      for (int ip = 0; ip < npp; ++ip) {
+        const PrimR<double>& s = sbra[ip];
+        const PrimR<R> pp{(R)s.g, (R)s.Kg, {(R)s.P[0], (R)s.P[1], (R)s.P[2]}, (R)s.h};
         if (fabs(pp.Kg) < qcut) continue;
         nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tabN, tab32N, acc) ? 1 : 0;
       }
@@ -264,10 +231,10 @@ __device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPai
                                               const double* __restrict__ tabN, const float* __restrict__ tab32N,
                                               const double* __restrict__ xf, R qcut,
                                               R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used,
-                                              const BraRegs* br = nullptr) {
+                                              const PrimR<double>* sbra = nullptr) {
   using Dm = Dims<LA, LB, LC, LD>;
   R acc[Dm::NE][Dm::NF];
-  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, tabN, tab32N, qcut, acc, nprim_used, br);
+  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, tabN, tab32N, qcut, acc, nprim_used, sbra);
   R AB[3], CD[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -493,10 +460,6 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
 #pragma unroll
   for (int i = 0; i < NOUT; ++i) V[i] = 0.0;
   int nprim = 0;
-  // small classes keep the bra's primitive pairs in registers
-  constexpr bool BR = Dm::NE * Dm::NF <= 64;
-  BraRegs br{};
-  if constexpr (BR) br = load_bra(prm.prims, P);
   // FP32 evaluation when every quartet of the wavefront has its bound below
   // the mixed-precision cutoff (a per-wavefront choice: no divergence
   // between the two paths); classes small enough to carry both paths
@@ -507,15 +470,14 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
       if (lowp) {
         float Vf[NBLK];
         quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
-                                                         (float)(1.0e-4 * prm.cutoff), Vf, nprim,
-                                                         BR ? &br : nullptr);
+                                                         (float)(1.0e-4 * prm.cutoff), Vf, nprim);
 #pragma unroll
         for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
       }
     }
     if (!lowp)
       quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
-                                                        1.0e-4 * prm.cutoff, V, nprim, BR ? &br : nullptr);
+                                                        1.0e-4 * prm.cutoff, V, nprim);
   }
   count_wave(ctr + 2, lowp && !skip);
   sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() + (nprim ? 12ull * NOUT : 0ull));
@@ -651,6 +613,252 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
       }
 #undef VV
   }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bra-row kernel.  One block takes one bra pair (a b) with all of its
+// surviving kets.  Every K update of a quartet -- K_ac, K_ad, K_bc, K_bd --
+// lies in the AO rows of the bra shells a and b, and over the ~1e4 kets of one
+// bra these rows are hit only at the AOs of the ket classes' shells.  The
+// block therefore accumulates the rows in LDS (ds_add_f64) and flushes every
+// nonzero entry with one global atomic at the end; J_ab stays in registers
+// and is reduced once per wavefront; only J_cd goes to memory per quartet.
+// Global FP64 atomics (about 2.4e10 lane-updates/s on the whole device) are
+// what bounds the per-quartet digestion of k_jk; this cuts them 5-13x for the
+// s/p/d classes.  Screening, quadrature and element cutoff are those of k_jk.
+template <int LA, int LB, int LC, int LD>
+struct JKB {
+  static constexpr int threads = (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF <= 4) ? 1024 : 512;
+};
+
+template <int LA, int LB, int LC, int LD, bool PURE>
+__global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams prm, BraWork bw, ColMap cm) {
+  using Dm = Dims<LA, LB, LC, LD>;
+  constexpr int NR = Dm::NROOT;
+  constexpr bool T64 = RysLds<NR>::f64;
+  constexpr bool T32 = Dm::NE * Dm::NF <= 256;
+  constexpr int OA = AO<LA, PURE>::N, OB = AO<LB, PURE>::N, OC = AO<LC, PURE>::N, OD = AO<LD, PURE>::N;
+  constexpr int NROW = OA + OB, NOUT = OA * OB * OC * OD, NBLK = Dm::NBLK;
+  __shared__ double s_t64[T64 ? rys_len<NR>() : 1];
+  __shared__ PrimR<double> s_bra[JKB_MAX_BRA_PRIMS];
+  __shared__ float s_t32[T32 ? rys32_len<NR>() : 1];
+  __shared__ double s_jab[OA * OB];
+  extern __shared__ double s_k[];  // K rows: [NROW][cm.ncol]
+  const int ncol = cm.ncol, nk = NROW * ncol;
+  for (int i = threadIdx.x; i < nk; i += blockDim.x) s_k[i] = 0.0;
+  if (threadIdx.x < OA * OB) s_jab[threadIdx.x] = 0.0;
+  if constexpr (T64)
+    for (int i = threadIdx.x; i < rys_len<NR>(); i += blockDim.x) s_t64[i] = prm.rys[RYS_OFF[NR] + i];
+  if constexpr (T32)
+    if (prm.mixed)
+      for (int i = threadIdx.x; i < rys32_len<NR>(); i += blockDim.x) s_t32[i] = prm.rys32[RYS32_OFF[NR] + i];
+  const int* item = bw.item + (size_t)blockIdx.x * (1 + OQPH_NBKT);
+  const int ip = __builtin_amdgcn_readfirstlane(item[0]);
+  const ShellPair P = prm.pairs[ip];
+  const bool bra_lds = P.nprim <= JKB_MAX_BRA_PRIMS;  // block-uniform
+  if (bra_lds)
+    for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra[i] = prim_r<double>(prm.prims[P.prim0 + i]);
+  __syncthreads();
+  const double* tabN = T64 ? (const double*)s_t64 : prm.rys + RYS_OFF[NR];
+  const float* tab32N = T32 ? (const float*)s_t32 : prm.rys32 + RYS32_OFF[NR];
+  const PrimR<double>* sbra = bra_lds ? (const PrimR<double>*)s_bra : nullptr;
+  const int lane = (int)__lane_id(), wv = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+  const int oa = P.ao_a, ob = P.ao_b;
+  const double* Dp = prm.D;
+  double* F = prm.F;
+  double Dab[OA * OB];
+#pragma unroll
+  for (int x = 0; x < OA; ++x)
+#pragma unroll
+    for (int y = 0; y < OB; ++y) Dab[x * OB + y] = Dp[tri_idx(oa + x, ob + y)];
+  double jab[OA * OB];
+#pragma unroll
+  for (int i = 0; i < OA * OB; ++i) jab[i] = 0.0;
+  unsigned long long n_comp = 0, n_skip = 0, n_f32 = 0, n_flops = 0, n_prim = 0;
+
+  for (int b = 0; b < OQPH_NBKT; ++b) {
+    const int n = __builtin_amdgcn_readfirstlane(item[1 + b]);
+    const int kbase = bw.kb[b];
+    for (int k0 = wv * 64; k0 < n; k0 += nw * 64) {
+      const int k = k0 + lane;
+      const bool active = k < n;
+      int iq = 0;
+      ShellPair Q{};
+      bool skip = true;
+      double bound = 0.0;
+      if (active) {
+        iq = bw.ket[kbase + k];
+        Q = prm.pairs[iq];
+        // ---- OpenQP screening (int2.F90:763-800)
+        const bool p_first = (P.si > Q.si) || (P.si == Q.si && P.sj >= Q.sj);
+        const double qbra = p_first ? P.Q : Q.Q;
+        const int ns = prm.nsh;
+        const double* ds = prm.dsh;
+        double dm = fmax(4.0 * ds[P.si * ns + P.sj], 4.0 * ds[Q.si * ns + Q.sj]);
+        dm = fmax(dm, fmax(ds[P.sj * ns + Q.sj], ds[P.sj * ns + Q.si]));
+        dm = fmax(dm, fmax(ds[P.si * ns + Q.sj], ds[P.si * ns + Q.si]));
+        bound = P.Q * Q.Q * dm;
+        skip = (qbra * prm.maxden < prm.cutoff) || (bound < prm.cutoff);
+      }
+      n_skip += (active && skip) ? 1 : 0;
+      n_comp += skip ? 0 : 1;
+      double V[NBLK];
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) V[i] = 0.0;
+      int nprim = 0;
+      bool lowp = false;
+      if constexpr (T32) lowp = prm.mixed && __ballot(!skip && bound >= prm.fp32_cut) == 0ull;
+      if (!skip) {
+        if constexpr (T32) {
+          if (lowp) {
+            float Vf[NBLK];
+            quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
+                                                             (float)(1.0e-4 * prm.cutoff), Vf, nprim, sbra);
+#pragma unroll
+            for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
+          }
+        }
+        if (!lowp)
+          quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
+                                                            1.0e-4 * prm.cutoff, V, nprim, sbra);
+      }
+      n_f32 += (lowp && !skip) ? 1 : 0;
+      n_flops += (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() + (nprim ? 12ull * NOUT : 0ull);
+      n_prim += (unsigned long long)nprim;
+      const bool valid = !skip && nprim > 0;
+      if (__ballot(valid) == 0ull) continue;  // wave-uniform
+      const double c = prm.cutoff;
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i)
+        if (!valid || fabs(V[i]) < c) V[i] = 0.0;
+      double deg = 1.0;
+      if (P.sa == P.sb) deg *= 0.5;
+      if (Q.sa == Q.sb) deg *= 0.5;
+      if (ip == iq) deg *= 0.5;
+      const double cj = 2.0 * deg * prm.sc, ck = -0.5 * deg * prm.se;
+      const int oc = Q.ao_a, od = Q.ao_b;
+#define VV(a, b, cc, d) V[(((a) * OB + (b)) * OC + (cc)) * OD + (d)]
+      // J_ab in registers; J_cd to memory
+      {
+        double Dcd[OC * OD];
+#pragma unroll
+        for (int x = 0; x < OC; ++x)
+#pragma unroll
+          for (int y = 0; y < OD; ++y) Dcd[x * OD + y] = valid ? Dp[tri_idx(oc + x, od + y)] : 0.0;
+#pragma unroll
+        for (int ab = 0; ab < OA * OB; ++ab) {
+          double t = 0.0;
+#pragma unroll
+          for (int cd = 0; cd < OC * OD; ++cd) t += V[ab * OC * OD + cd] * Dcd[cd];
+          jab[ab] += cj * t;
+        }
+#pragma unroll
+        for (int cc = 0; cc < OC; ++cc)
+#pragma unroll
+          for (int d = 0; d < OD; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int ab = 0; ab < OA * OB; ++ab) t += V[ab * OC * OD + cc * OD + d] * Dab[ab];
+            const int x = oc + cc, y = od + d;
+            const double v = cj * t * (x == y ? 2.0 : 1.0);
+            if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+          }
+      }
+      // K rows a (0..OA-1) and b (OA..OA+OB-1) in LDS
+      if (valid) {
+        double Dbd[OB * OD], Dbc[OB * OC], Dad[OA * OD], Dac[OA * OC];
+#pragma unroll
+        for (int bb = 0; bb < OB; ++bb) {
+#pragma unroll
+          for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = Dp[tri_idx(ob + bb, od + d)];
+#pragma unroll
+          for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = Dp[tri_idx(ob + bb, oc + cc)];
+        }
+#pragma unroll
+        for (int a = 0; a < OA; ++a) {
+#pragma unroll
+          for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dp[tri_idx(oa + a, od + d)];
+#pragma unroll
+          for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dp[tri_idx(oa + a, oc + cc)];
+        }
+        const int colC = cm.colidx[Q.sa], colD = cm.dofs + cm.colidx[Q.sb];
+#pragma unroll
+        for (int a = 0; a < OA; ++a) {
+#pragma unroll
+          for (int cc = 0; cc < OC; ++cc) {
+            double t = 0.0;
+#pragma unroll
+            for (int bb = 0; bb < OB; ++bb)
+#pragma unroll
+              for (int d = 0; d < OD; ++d) t += VV(a, bb, cc, d) * Dbd[bb * OD + d];
+            if (t != 0.0) atomicAdd(&s_k[a * ncol + colC + cc], ck * t);
+          }
+#pragma unroll
+          for (int d = 0; d < OD; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int bb = 0; bb < OB; ++bb)
+#pragma unroll
+              for (int cc = 0; cc < OC; ++cc) t += VV(a, bb, cc, d) * Dbc[bb * OC + cc];
+            if (t != 0.0) atomicAdd(&s_k[a * ncol + colD + d], ck * t);
+          }
+        }
+#pragma unroll
+        for (int bb = 0; bb < OB; ++bb) {
+#pragma unroll
+          for (int cc = 0; cc < OC; ++cc) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < OA; ++a)
+#pragma unroll
+              for (int d = 0; d < OD; ++d) t += VV(a, bb, cc, d) * Dad[a * OD + d];
+            if (t != 0.0) atomicAdd(&s_k[(OA + bb) * ncol + colC + cc], ck * t);
+          }
+#pragma unroll
+          for (int d = 0; d < OD; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < OA; ++a)
+#pragma unroll
+              for (int cc = 0; cc < OC; ++cc) t += VV(a, bb, cc, d) * Dac[a * OC + cc];
+            if (t != 0.0) atomicAdd(&s_k[(OA + bb) * ncol + colD + d], ck * t);
+          }
+        }
+      }
+#undef VV
+    }
+  }
+  // J_ab: one LDS add per wavefront; statistics: one slot update per wavefront
+#pragma unroll
+  for (int ab = 0; ab < OA * OB; ++ab) {
+    const double t = wave_sum(jab[ab]);
+    if (lane == 0 && t != 0.0) atomicAdd(&s_jab[ab], t);
+  }
+  {
+    unsigned long long* ctr = prm.counters + 8 * (int)((blockIdx.x * nw + wv) & (OQPH_CTR_SLOTS - 1));
+    unsigned long long v[5] = {n_comp, n_skip, n_f32, n_flops, n_prim};
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      unsigned long long t = v[i];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+      if (lane == 0 && t != 0ull) atomicAdd(ctr + i, t);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+    const double v = s_k[i];
+    if (v == 0.0) continue;
+    const int r = i / ncol, col = i - r * ncol;
+    const int x = r < OA ? oa + r : ob + (r - OA);
+    const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
+    fatomic(F + tri_idx(x, y), v * (x == y ? 2.0 : 1.0));
+  }
+  if (threadIdx.x < OA * OB) {
+    const double v = s_jab[threadIdx.x];
+    const int x = oa + (int)threadIdx.x / OB, y = ob + (int)threadIdx.x % OB;
+    if (v != 0.0) fatomic(F + tri_idx(x, y), v * (x == y ? 2.0 : 1.0));
   }
 }
 

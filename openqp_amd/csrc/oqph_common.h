@@ -63,6 +63,30 @@ struct WaveWork {
   long long nwaves;
 };
 
+// Primitive-pair-count buckets of the class lists.
+#define OQPH_NBKT 6
+
+// Bra-row work of one J/K class-pair launch: block i takes bra pair item[i*7]
+// and, for each ket bucket b, the first item[i*7 + 1 + b] kets of that bucket
+// (ket[kb[b] ...]); blocks are ordered by decreasing cost.
+struct BraWork {
+  const int* item;
+  const int* ket;
+  int kb[OQPH_NBKT + 1];
+  int nitem;
+};
+
+// LDS column space of the K rows of one launch: the AOs of the shells of
+// angular momentum lc (C space) and, when ld != lc, of ld (D space, after C).
+struct ColMap {
+  const int* colidx;  // per shell: AO offset among the shells of its angular momentum
+  const int* aoC;     // global AO index of each C-space column
+  const int* aoD;     // global AO index of each D-space column
+  int ncolC;          // C-space columns
+  int ncol;           // all columns (ncolC, or ncolC + D-space columns)
+  int dofs;           // first D-space column (0 when lc == ld)
+};
+
 // Statistics counters of one launch: OQPH_CTR_SLOTS slots of 8 counters, one
 // 64-byte line each.  A wavefront adds its totals to slot (wavefront index mod
 // OQPH_CTR_SLOTS), so the atomics of millions of short wavefronts spread over

@@ -27,6 +27,9 @@
 
 namespace oqph {
 bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const WaveWork& cw, hipStream_t s);
+bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
+                size_t lds, hipStream_t s);
+bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* static_lds, size_t lds_max);
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
                     const double* tab, const double* xf, double qcut, double* Q, hipStream_t s);
 void launch_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh, hipStream_t s);
@@ -100,8 +103,9 @@ struct DBuf {
 
 static inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 static inline int class_of(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+static inline int nao_of(int l, bool pure) { return (pure && l >= 2) ? 2 * l + 1 : (l + 1) * (l + 2) / 2; }
 // primitive-pair-count buckets of the class lists
-static constexpr int NBKT = 6;
+static constexpr int NBKT = OQPH_NBKT;
 static inline int bucket_of(int nprim) {
   return nprim <= 1 ? 0 : nprim <= 4 ? 1 : nprim <= 9 ? 2 : nprim <= 20 ? 3 : nprim <= 40 ? 4 : 5;
 }
@@ -145,8 +149,20 @@ struct oqph_session {
   };
   struct WLaunch {
     int la, lb, lc, ld;
+    bool rows;   // bra-row kernel (bw, cm, lds) instead of wavefront tasks (ww)
     WaveWork ww;
+    BraWork bw;
+    ColMap cm;
+    size_t lds;
   };
+  // bra-row kernel: LDS column spaces per angular momentum and its work items
+  std::vector<int> h_colidx, h_colao, colbase, ncolL;
+  DBuf<int> d_colidx, d_colao, d_items;
+  std::vector<int> h_items;
+  size_t lds_max = 0;
+  // OQPH_JK_ROWS: 0 = wavefront-task kernel only, 2 = bra-row kernel wherever its
+  // K rows fit in LDS, otherwise (default) where it also saves atomics
+  int rows_mode = 1;
   std::vector<WLaunch> launches;
   DBuf<int> d_seg;                 // seg_bra | seg_k0 | seg_n of all launches
   DBuf<long long> d_choff;
@@ -335,6 +351,34 @@ static int prepare_geometry(oqph_session* s) {
   if ((rc = s->d_aooff.ensure(s->nsh)) || (rc = s->d_nao.ensure(s->nsh))) return rc;
   if ((rc = upload(s->d_aooff.p, s->aooff.data(), s->nsh * sizeof(int), s->stream))) return rc;
   if ((rc = upload(s->d_nao.p, s->nao.data(), s->nsh * sizeof(int), s->stream))) return rc;
+  // LDS column spaces of the bra-row kernel: the AOs of the shells of each
+  // angular momentum, in shell order
+  s->h_colidx.assign(s->nsh, 0);
+  s->h_colao.clear();
+  s->colbase.assign(s->lmax + 1, 0);
+  s->ncolL.assign(s->lmax + 1, 0);
+  for (int L = 0; L <= s->lmax; ++L) {
+    s->colbase[L] = (int)s->h_colao.size();
+    int off = 0;
+    for (int i = 0; i < s->nsh; ++i)
+      if (s->am[i] == L) {
+        s->h_colidx[i] = off;
+        for (int k = 0; k < s->nao[i]; ++k) s->h_colao.push_back(s->aooff[i] + k);
+        off += s->nao[i];
+      }
+    s->ncolL[L] = off;
+  }
+  if ((rc = s->d_colidx.ensure(s->nsh)) || (rc = s->d_colao.ensure(s->h_colao.size()))) return rc;
+  if ((rc = upload(s->d_colidx.p, s->h_colidx.data(), s->nsh * sizeof(int), s->stream)) ||
+      (rc = upload(s->d_colao.p, s->h_colao.data(), s->h_colao.size() * sizeof(int), s->stream)))
+    return rc;
+  {
+    int v = 0;
+    HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
+    s->lds_max = (size_t)v;
+    const char* e = getenv("OQPH_JK_ROWS");
+    s->rows_mode = (e && (e[0] == '0' || e[0] == '2')) ? e[0] - '0' : 1;
+  }
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
   if ((rc = s->d_D.ensure(ntri)) || (rc = s->d_F.ensure(ntri)) || (rc = s->d_dsh.ensure((size_t)s->nsh * s->nsh)) ||
       (rc = s->d_ctr.ensure(8)))
@@ -457,7 +501,12 @@ static int prepare_lists(oqph_session* s, double maxden) {
     size_t seg0, ch0;
     int nseg;
     long long nwaves;
+    bool rows = false;
+    size_t item0 = 0;
+    int nitem = 0, ncolC = 0, ncol = 0;
+    size_t lds = 0;
   };
+  s->h_items.clear();
   std::vector<Pend> pend;
   const double c = s->cutoff;
   const int ncls = class_of(s->lmax, s->lmax) + 1;
@@ -468,21 +517,25 @@ static int prepare_lists(oqph_session* s, double maxden) {
       if (QB.empty() || QK.empty()) continue;
       const auto& kb = s->kbs[ck];
       const auto& bb = s->kbs[cb];
-      const size_t seg0 = s->h_seg_bra.size(), ch0 = s->h_choff.size();
+      int la = 0, lb = 0, lc = 0, ld = 0;
+      for (int x = 0; x <= 3; ++x)
+        for (int y = 0; y <= x; ++y) {
+          if (class_of(x, y) == cb) { la = x; lb = y; }
+          if (class_of(x, y) == ck) { lc = x; ld = y; }
+        }
       // largest ket primitive-pair count of each bucket
       int kmax[NBKT] = {0};
       for (int b = 0; b < NBKT; ++b)
         for (int k = kb[b]; k < kb[b + 1]; ++k) kmax[b] = std::max(kmax[b], s->pairs[s->cls[ck][k]].nprim);
-      struct Seg {
-        int bra, k0, n;
-        long long cost;  // primitive quartets one wavefront of the segment loops over
-      };
-      std::vector<Seg> segs;
-      int bp = 0;
-      for (size_t p = 0; p < QB.size(); ++p) {
-        while (bp + 1 < NBKT && (int)p >= bb[bp + 1]) ++bp;
+      // surviving kets of every bra: a prefix of each ket bucket
+      const int nb = (int)QB.size();
+      std::vector<int> cnt((size_t)nb * NBKT, 0);
+      long long nket = 0;
+      int nbra = 0, bp = 0;
+      for (int p = 0; p < nb; ++p) {
+        while (bp + 1 < NBKT && p >= bb[bp + 1]) ++bp;
         const double thr = c / (4.0 * maxden * QB[p]);
-        const int bra = s->cls[cb][p];
+        long long any = 0;
         for (int b = 0; b < NBKT; ++b) {
           if (kb[b + 1] <= kb[b]) continue;
           long long k = std::lower_bound(QK.begin() + kb[b], QK.begin() + kb[b + 1], thr,
@@ -491,8 +544,68 @@ static int prepare_lists(oqph_session* s, double maxden) {
             if (b > bp) k = 0;
             else if (b == bp) k = std::min<long long>(k, (long long)p - bb[bp] + 1);
           }
+          cnt[(size_t)p * NBKT + b] = (int)k;
+          any += k;
+        }
+        nket += any;
+        nbra += any > 0;
+      }
+      if (nket == 0) continue;
+      // bra-row kernel when its K rows fit in LDS and it saves global atomics:
+      // per quartet it leaves OC*OD (J_cd) of OC*OD + (OA+OB)(OC+OD), plus one
+      // flush of the K rows per bra
+      const int oa = nao_of(la, s->pure), obn = nao_of(lb, s->pure), oc = nao_of(lc, s->pure),
+                od = nao_of(ld, s->pure);
+      const int ncC = s->ncolL[lc], ncol = (lc == ld) ? ncC : ncC + s->ncolL[ld];
+      int nrow = 0;
+      size_t slds = 0;
+      bool rows = s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max) &&
+                  slds + (size_t)nrow * ncol * sizeof(double) <= s->lds_max;
+      if (rows && s->rows_mode == 1) {
+        const double old_at = (double)nket * (oc * od + (oa + obn) * (oc + od));
+        const double new_at = (double)nket * oc * od + (double)nbra * nrow * ncol;
+        rows = new_at < 0.5 * old_at;
+      }
+      if (rows) {
+        struct It {
+          int p;
+          long long cost;
+        };
+        std::vector<It> its;
+        for (int p = 0; p < nb; ++p) {
+          long long cost = 0;
+          for (int b = 0; b < NBKT; ++b) cost += (long long)cnt[(size_t)p * NBKT + b] * kmax[b];
+          if (cost > 0) its.push_back({p, cost * s->pairs[s->cls[cb][p]].nprim});
+        }
+        // longest blocks first
+        std::stable_sort(its.begin(), its.end(), [](const It& x, const It& y) { return x.cost > y.cost; });
+        const size_t i0 = s->h_items.size();
+        for (const It& t : its) {
+          s->h_items.push_back(s->cls[cb][t.p]);
+          for (int b = 0; b < NBKT; ++b) s->h_items.push_back(cnt[(size_t)t.p * NBKT + b]);
+        }
+        Pend q{la, lb, lc, ld, ck, 0, 0, 0, 0};
+        q.rows = true;
+        q.item0 = i0;
+        q.nitem = (int)its.size();
+        q.ncolC = ncC;
+        q.ncol = ncol;
+        q.lds = (size_t)nrow * ncol * sizeof(double);
+        pend.push_back(q);
+        continue;
+      }
+      const size_t seg0 = s->h_seg_bra.size(), ch0 = s->h_choff.size();
+      struct Seg {
+        int bra, k0, n;
+        long long cost;  // primitive quartets one wavefront of the segment loops over
+      };
+      std::vector<Seg> segs;
+      for (int p = 0; p < nb; ++p) {
+        const int bra = s->cls[cb][p];
+        for (int b = 0; b < NBKT; ++b) {
+          const int k = cnt[(size_t)p * NBKT + b];
           if (k == 0) continue;
-          segs.push_back({bra, kb[b], (int)k, (long long)s->pairs[bra].nprim * kmax[b]});
+          segs.push_back({bra, kb[b], k, (long long)s->pairs[bra].nprim * kmax[b]});
         }
       }
       // longest wavefronts first, so the launch does not end on a tail of
@@ -508,16 +621,6 @@ static int prepare_lists(oqph_session* s, double maxden) {
       }
       const int nseg = (int)(s->h_seg_bra.size() - seg0);
       s->h_choff.push_back(waves);  // terminator of this launch's prefix
-      if (nseg == 0) {
-        s->h_choff.pop_back();
-        continue;
-      }
-      int la = 0, lb = 0, lc = 0, ld = 0;
-      for (int x = 0; x <= 3; ++x)
-        for (int y = 0; y <= x; ++y) {
-          if (class_of(x, y) == cb) { la = x; lb = y; }
-          if (class_of(x, y) == ck) { lc = x; ld = y; }
-        }
       pend.push_back({la, lb, lc, ld, ck, seg0, ch0, nseg, waves});
     }
   int rc;
@@ -530,9 +633,29 @@ static int prepare_lists(oqph_session* s, double maxden) {
         (rc = upload(s->d_choff.p, s->h_choff.data(), s->h_choff.size() * sizeof(long long), s->stream)))
       return rc;
   }
+  if (!s->h_items.empty()) {
+    if ((rc = s->d_items.ensure(s->h_items.size()))) return rc;
+    if ((rc = upload(s->d_items.p, s->h_items.data(), s->h_items.size() * sizeof(int), s->stream))) return rc;
+  }
   for (auto& p : pend) {
-    oqph_session::WLaunch L;
+    oqph_session::WLaunch L{};
     L.la = p.la; L.lb = p.lb; L.lc = p.lc; L.ld = p.ld;
+    L.rows = p.rows;
+    if (p.rows) {
+      L.bw.item = s->d_items.p + p.item0;
+      L.bw.ket = s->d_cls.p + s->cls_off[p.ck];
+      for (int b = 0; b <= NBKT; ++b) L.bw.kb[b] = s->kbs[p.ck][b];
+      L.bw.nitem = p.nitem;
+      L.cm.colidx = s->d_colidx.p;
+      L.cm.aoC = s->d_colao.p + s->colbase[p.lc];
+      L.cm.aoD = s->d_colao.p + s->colbase[p.ld];
+      L.cm.ncolC = p.ncolC;
+      L.cm.ncol = p.ncol;
+      L.cm.dofs = (p.lc == p.ld) ? 0 : p.ncolC;
+      L.lds = p.lds;
+      s->launches.push_back(L);
+      continue;
+    }
     L.ww.seg_bra = s->d_seg.p + p.seg0;
     L.ww.seg_k0 = s->d_seg.p + nseg + p.seg0;
     L.ww.seg_n = s->d_seg.p + 2 * nseg + p.seg0;
@@ -599,7 +722,9 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     auto& L = s->launches[i];
     p.counters = s->d_ctr.p + 8 + nc * i;
     if (s->profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
-    if (!launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream)) {
+    const bool ok = L.rows ? launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, stream)
+                           : launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream);
+    if (!ok) {
       set_err("no J/K kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }
@@ -679,7 +804,7 @@ int oqph_session_destroy(oqph_session* s) {
   s->d_xform.release(); s->d_rys32.release(); s->d_kbs.release(); s->d_seg.release(); s->d_choff.release();
   s->d_gpairs.release(); s->d_gprims.release(); s->d_gcls.release(); s->d_coff.release(); s->d_satom.release();
   s->d_am.release(); s->d_pure.release(); s->d_goff.release(); s->d_Dc.release(); s->d_de.release();
-  s->d_aooff.release(); s->d_nao.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
+  s->d_aooff.release(); s->d_nao.release(); s->d_colidx.release(); s->d_colao.release(); s->d_items.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
   s->d_tmp.release(); s->d_off.release(); s->d_ctr.release();
   for (auto e : s->pev) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
