@@ -644,8 +644,14 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
   __shared__ PrimR<double> s_bra[JKB_MAX_BRA_PRIMS];
   __shared__ float s_t32[T32 ? rys32_len<NR>() : 1];
   __shared__ double s_jab[OA * OB];
-  extern __shared__ double s_k[];  // K rows: [NROW][cm.ncol]
-  const int ncol = cm.ncol, nk = NROW * ncol;
+  // dynamic LDS: K rows [NROW][ncol] (accumulated), D rows [NROW][ncol] (the
+  // density at the same positions), shell-pair density maxima of the bra
+  // shells' rows [2][nsh]
+  extern __shared__ double s_dyn[];
+  const int ncol = cm.ncol, nk = NROW * ncol, nsh = prm.nsh;
+  double* s_k = s_dyn;
+  double* s_d = s_dyn + nk;
+  double* s_ds = s_dyn + 2 * nk;
   for (int i = threadIdx.x; i < nk; i += blockDim.x) s_k[i] = 0.0;
   if (threadIdx.x < OA * OB) s_jab[threadIdx.x] = 0.0;
   if constexpr (T64)
@@ -659,6 +665,18 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
   const bool bra_lds = P.nprim <= JKB_MAX_BRA_PRIMS;  // block-uniform
   if (bra_lds)
     for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra[i] = prim_r<double>(prm.prims[P.prim0 + i]);
+  // the density entries the digestion reads in the bra rows a and b, gathered
+  // once per block instead of once per quartet
+  for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+    const int r = i / ncol, col = i - r * ncol;
+    const int x = r < OA ? P.ao_a + r : P.ao_b + (r - OA);
+    const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
+    s_d[i] = prm.D[tri_idx(x, y)];
+  }
+  for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
+    s_ds[i] = prm.dsh[P.si * nsh + i];
+    s_ds[nsh + i] = prm.dsh[P.sj * nsh + i];
+  }
   __syncthreads();
   const double* tabN = T64 ? (const double*)s_t64 : prm.rys + RYS_OFF[NR];
   const float* tab32N = T32 ? (const float*)s_t32 : prm.rys32 + RYS32_OFF[NR];
@@ -690,14 +708,13 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
       if (active) {
         iq = bw.ket[kbase + k];
         Q = prm.pairs[iq];
-        // ---- OpenQP screening (int2.F90:763-800)
+        // ---- OpenQP screening (int2.F90:763-800); the bra shells' rows of
+        // the shell-pair maxima come from LDS
         const bool p_first = (P.si > Q.si) || (P.si == Q.si && P.sj >= Q.sj);
         const double qbra = p_first ? P.Q : Q.Q;
-        const int ns = prm.nsh;
-        const double* ds = prm.dsh;
-        double dm = fmax(4.0 * ds[P.si * ns + P.sj], 4.0 * ds[Q.si * ns + Q.sj]);
-        dm = fmax(dm, fmax(ds[P.sj * ns + Q.sj], ds[P.sj * ns + Q.si]));
-        dm = fmax(dm, fmax(ds[P.si * ns + Q.sj], ds[P.si * ns + Q.si]));
+        double dm = fmax(4.0 * prm.dsh[P.si * nsh + P.sj], 4.0 * prm.dsh[Q.si * nsh + Q.sj]);
+        dm = fmax(dm, fmax(s_ds[nsh + Q.sj], s_ds[nsh + Q.si]));
+        dm = fmax(dm, fmax(s_ds[Q.sj], s_ds[Q.si]));
         bound = P.Q * Q.Q * dm;
         skip = (qbra * prm.maxden < prm.cutoff) || (bound < prm.cutoff);
       }
@@ -765,24 +782,25 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
             if (v != 0.0) fatomic(F + tri_idx(x, y), v);
           }
       }
-      // K rows a (0..OA-1) and b (OA..OA+OB-1) in LDS
+      // K rows a (0..OA-1) and b (OA..OA+OB-1) in LDS; the density in those
+      // rows comes from the LDS copy at the same positions
       if (valid) {
+        const int colC = cm.colidx[Q.sa], colD = cm.dofs + cm.colidx[Q.sb];
         double Dbd[OB * OD], Dbc[OB * OC], Dad[OA * OD], Dac[OA * OC];
 #pragma unroll
         for (int bb = 0; bb < OB; ++bb) {
 #pragma unroll
-          for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = Dp[tri_idx(ob + bb, od + d)];
+          for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = s_d[(OA + bb) * ncol + colD + d];
 #pragma unroll
-          for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = Dp[tri_idx(ob + bb, oc + cc)];
+          for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = s_d[(OA + bb) * ncol + colC + cc];
         }
 #pragma unroll
         for (int a = 0; a < OA; ++a) {
 #pragma unroll
-          for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dp[tri_idx(oa + a, od + d)];
+          for (int d = 0; d < OD; ++d) Dad[a * OD + d] = s_d[a * ncol + colD + d];
 #pragma unroll
-          for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dp[tri_idx(oa + a, oc + cc)];
+          for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = s_d[a * ncol + colC + cc];
         }
-        const int colC = cm.colidx[Q.sa], colD = cm.dofs + cm.colidx[Q.sb];
 #pragma unroll
         for (int a = 0; a < OA; ++a) {
 #pragma unroll

@@ -559,8 +559,10 @@ static int prepare_lists(oqph_session* s, double maxden) {
       const int ncC = s->ncolL[lc], ncol = (lc == ld) ? ncC : ncC + s->ncolL[ld];
       int nrow = 0;
       size_t slds = 0;
-      bool rows = s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max) &&
-                  slds + (size_t)nrow * ncol * sizeof(double) <= s->lds_max;
+      bool rows = s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max);
+      // dynamic LDS: K rows and D rows (nrow x ncol each), bra rows of dsh (2 x nsh)
+      const size_t dynb = (2 * (size_t)nrow * ncol + 2 * (size_t)s->nsh) * sizeof(double);
+      rows = rows && slds + dynb <= s->lds_max;
       if (rows && s->rows_mode == 1) {
         const double old_at = (double)nket * (oc * od + (oa + obn) * (oc + od));
         const double new_at = (double)nket * oc * od + (double)nbra * nrow * ncol;
@@ -590,7 +592,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
         q.nitem = (int)its.size();
         q.ncolC = ncC;
         q.ncol = ncol;
-        q.lds = (size_t)nrow * ncol * sizeof(double);
+        q.lds = dynb;
         pend.push_back(q);
         continue;
       }
@@ -923,7 +925,7 @@ int oqph_last_profile(oqph_session* s, int maxn, int* cls, double* ms, double* f
   for (int i = 0; i < n; ++i) {
     const auto& L = s->launches[i];
     const unsigned long long* c = i * 8 + 8 <= (int)s->prof_ctr.size() ? &s->prof_ctr[8 * i] : nullptr;
-    if (cls) cls[i] = L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
+    if (cls) cls[i] = (L.rows ? 10000 : 0) + L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
     if (ms) ms[i] = i < (int)s->prof_ms.size() ? s->prof_ms[i] : 0.0;
     if (flops) flops[i] = c ? (double)c[3] : 0.0;
     if (quartets) quartets[i] = c ? (long long)c[0] : 0;
