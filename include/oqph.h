@@ -117,6 +117,18 @@ int oqph_schwarz(oqph_session *s, double *q);
 int oqph_int1e(int nsh, const int *am, const int *harmonic, const int *atom, const int *nprim, const double *ex,
                const double *cc, int natm, const double *xyz, const double *zn, double *S, double *T, double *V);
 
+/* One-electron part of the RHF nuclear gradient, added to grad[3*natm]
+ * (Eh/bohr): sum D dH/dR - sum W dS/dR with D the total density and W the
+ * energy-weighted density 2 C_occ diag(eps_occ) C_occ^T, both full
+ * nbf x nbf in the AO order above; H = T + V (kinetic, Hellmann-Feynman and
+ * basis-function terms of V).  Replaces grad_ee_overlap, grad_ee_kinetic,
+ * grad_en_hellman_feynman and grad_en_pulay called from hf_1e_grad
+ * (source/modules/hf_gradient.F90:143-224; source/integrals/grd1.F90).
+ * Host code (OpenMP), l <= 3; the nuclear repulsion term is the caller's. */
+int oqph_grad1e(int nsh, const int *am, const int *harmonic, const int *atom, const int *nprim, const double *ex,
+                const double *cc, int natm, const double *xyz, const double *zn, const double *D, const double *W,
+                double *grad);
+
 /* Basis dimensions of the session. */
 int oqph_session_nbf(oqph_session *s);
 

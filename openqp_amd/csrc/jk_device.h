@@ -644,9 +644,9 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
   __shared__ PrimR<double> s_bra[JKB_MAX_BRA_PRIMS];
   __shared__ float s_t32[T32 ? rys32_len<NR>() : 1];
   __shared__ double s_jab[OA * OB];
-  // dynamic LDS: K rows [NROW][ncol] (accumulated), D rows [NROW][ncol] (the
-  // density at the same positions), shell-pair density maxima of the bra
-  // shells' rows [2][nsh]
+  // dynamic LDS: K rows [NROW][ncol] (accumulated); when cm.drows, also D rows
+  // [NROW][ncol] (the density at the same positions) and the bra shells' rows
+  // of the shell-pair density maxima [2][nsh]
   extern __shared__ double s_dyn[];
   const int ncol = cm.ncol, nk = NROW * ncol, nsh = prm.nsh;
   double* s_k = s_dyn;
@@ -667,15 +667,18 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
     for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra[i] = prim_r<double>(prm.prims[P.prim0 + i]);
   // the density entries the digestion reads in the bra rows a and b, gathered
   // once per block instead of once per quartet
-  for (int i = threadIdx.x; i < nk; i += blockDim.x) {
-    const int r = i / ncol, col = i - r * ncol;
-    const int x = r < OA ? P.ao_a + r : P.ao_b + (r - OA);
-    const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
-    s_d[i] = prm.D[tri_idx(x, y)];
-  }
-  for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
-    s_ds[i] = prm.dsh[P.si * nsh + i];
-    s_ds[nsh + i] = prm.dsh[P.sj * nsh + i];
+  const bool drows = cm.drows != 0;  // launch-uniform
+  if (drows) {
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+      const int r = i / ncol, col = i - r * ncol;
+      const int x = r < OA ? P.ao_a + r : P.ao_b + (r - OA);
+      const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
+      s_d[i] = prm.D[tri_idx(x, y)];
+    }
+    for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
+      s_ds[i] = prm.dsh[P.si * nsh + i];
+      s_ds[nsh + i] = prm.dsh[P.sj * nsh + i];
+    }
   }
   __syncthreads();
   const double* tabN = T64 ? (const double*)s_t64 : prm.rys + RYS_OFF[NR];
@@ -713,8 +716,14 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
         const bool p_first = (P.si > Q.si) || (P.si == Q.si && P.sj >= Q.sj);
         const double qbra = p_first ? P.Q : Q.Q;
         double dm = fmax(4.0 * prm.dsh[P.si * nsh + P.sj], 4.0 * prm.dsh[Q.si * nsh + Q.sj]);
-        dm = fmax(dm, fmax(s_ds[nsh + Q.sj], s_ds[nsh + Q.si]));
-        dm = fmax(dm, fmax(s_ds[Q.sj], s_ds[Q.si]));
+        if (drows) {
+          dm = fmax(dm, fmax(s_ds[nsh + Q.sj], s_ds[nsh + Q.si]));
+          dm = fmax(dm, fmax(s_ds[Q.sj], s_ds[Q.si]));
+        } else {
+          const double* ds = prm.dsh;
+          dm = fmax(dm, fmax(ds[P.sj * nsh + Q.sj], ds[P.sj * nsh + Q.si]));
+          dm = fmax(dm, fmax(ds[P.si * nsh + Q.sj], ds[P.si * nsh + Q.si]));
+        }
         bound = P.Q * Q.Q * dm;
         skip = (qbra * prm.maxden < prm.cutoff) || (bound < prm.cutoff);
       }
@@ -787,19 +796,36 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
       if (valid) {
         const int colC = cm.colidx[Q.sa], colD = cm.dofs + cm.colidx[Q.sb];
         double Dbd[OB * OD], Dbc[OB * OC], Dad[OA * OD], Dac[OA * OC];
+        if (drows) {
 #pragma unroll
-        for (int bb = 0; bb < OB; ++bb) {
+          for (int bb = 0; bb < OB; ++bb) {
 #pragma unroll
-          for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = s_d[(OA + bb) * ncol + colD + d];
+            for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = s_d[(OA + bb) * ncol + colD + d];
 #pragma unroll
-          for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = s_d[(OA + bb) * ncol + colC + cc];
-        }
+            for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = s_d[(OA + bb) * ncol + colC + cc];
+          }
 #pragma unroll
-        for (int a = 0; a < OA; ++a) {
+          for (int a = 0; a < OA; ++a) {
 #pragma unroll
-          for (int d = 0; d < OD; ++d) Dad[a * OD + d] = s_d[a * ncol + colD + d];
+            for (int d = 0; d < OD; ++d) Dad[a * OD + d] = s_d[a * ncol + colD + d];
 #pragma unroll
-          for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = s_d[a * ncol + colC + cc];
+            for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = s_d[a * ncol + colC + cc];
+          }
+        } else {
+#pragma unroll
+          for (int bb = 0; bb < OB; ++bb) {
+#pragma unroll
+            for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = Dp[tri_idx(ob + bb, od + d)];
+#pragma unroll
+            for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = Dp[tri_idx(ob + bb, oc + cc)];
+          }
+#pragma unroll
+          for (int a = 0; a < OA; ++a) {
+#pragma unroll
+            for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dp[tri_idx(oa + a, od + d)];
+#pragma unroll
+            for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dp[tri_idx(oa + a, oc + cc)];
+          }
         }
 #pragma unroll
         for (int a = 0; a < OA; ++a) {

@@ -85,6 +85,8 @@ struct ColMap {
   int ncolC;          // C-space columns
   int ncol;           // all columns (ncolC, or ncolC + D-space columns)
   int dofs;           // first D-space column (0 when lc == ld)
+  int drows;          // 1: the density in the K rows and the bra rows of the
+                      // shell-pair maxima are staged in LDS as well
 };
 
 // Statistics counters of one launch: OQPH_CTR_SLOTS slots of 8 counters, one

@@ -161,7 +161,8 @@ struct oqph_session {
   std::vector<int> h_items;
   size_t lds_max = 0;
   // OQPH_JK_ROWS: 0 = wavefront-task kernel only, 2 = bra-row kernel wherever its
-  // K rows fit in LDS, otherwise (default) where it also saves atomics
+  // K rows fit in LDS, 3 = as 2 without the LDS density rows, otherwise
+  // (default) the bra-row kernel where it fits and also saves atomics
   int rows_mode = 1;
   std::vector<WLaunch> launches;
   DBuf<int> d_seg;                 // seg_bra | seg_k0 | seg_n of all launches
@@ -377,7 +378,7 @@ static int prepare_geometry(oqph_session* s) {
     HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
     s->lds_max = (size_t)v;
     const char* e = getenv("OQPH_JK_ROWS");
-    s->rows_mode = (e && (e[0] == '0' || e[0] == '2')) ? e[0] - '0' : 1;
+    s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
   }
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
   if ((rc = s->d_D.ensure(ntri)) || (rc = s->d_F.ensure(ntri)) || (rc = s->d_dsh.ensure((size_t)s->nsh * s->nsh)) ||
@@ -505,6 +506,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
     size_t item0 = 0;
     int nitem = 0, ncolC = 0, ncol = 0;
     size_t lds = 0;
+    bool drows = false;
   };
   s->h_items.clear();
   std::vector<Pend> pend;
@@ -560,8 +562,12 @@ static int prepare_lists(oqph_session* s, double maxden) {
       int nrow = 0;
       size_t slds = 0;
       bool rows = s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max);
-      // dynamic LDS: K rows and D rows (nrow x ncol each), bra rows of dsh (2 x nsh)
-      const size_t dynb = (2 * (size_t)nrow * ncol + 2 * (size_t)s->nsh) * sizeof(double);
+      // dynamic LDS: K rows (nrow x ncol), plus D rows (nrow x ncol) and the
+      // bra rows of dsh (2 x nsh) when those fit as well
+      const size_t dynk = (size_t)nrow * ncol * sizeof(double);
+      const size_t dynd = (2 * (size_t)nrow * ncol + 2 * (size_t)s->nsh) * sizeof(double);
+      const bool drows = rows && s->rows_mode != 3 && slds + dynd <= s->lds_max;
+      const size_t dynb = drows ? dynd : dynk;
       rows = rows && slds + dynb <= s->lds_max;
       if (rows && s->rows_mode == 1) {
         const double old_at = (double)nket * (oc * od + (oa + obn) * (oc + od));
@@ -593,6 +599,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
         q.ncolC = ncC;
         q.ncol = ncol;
         q.lds = dynb;
+        q.drows = drows;
         pend.push_back(q);
         continue;
       }
@@ -654,6 +661,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
       L.cm.ncolC = p.ncolC;
       L.cm.ncol = p.ncol;
       L.cm.dofs = (p.lc == p.ld) ? 0 : p.ncolC;
+      L.cm.drows = p.drows ? 1 : 0;
       L.lds = p.lds;
       s->launches.push_back(L);
       continue;

@@ -28,7 +28,7 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_session_set_basis", "oqph_session_set_geometry", "oqph_session_set_thresholds",
            "oqph_session_set_precision", "oqph_set_default_session", "oqph_fock_jk", "oqph_fock_jk_inc",
            "oqph_grad2", "oqph_fock_jk_device", "oqph_schwarz", "oqph_session_nbf", "oqph_last_error",
-           "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e"]
+           "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e"]
 
 
 class OqphError(RuntimeError):
@@ -71,6 +71,8 @@ def load(path: str | None = None):
     L.oqph_last_profile.argtypes = [vp, C.c_int, _pi, _pd, _pd, _pl, _pl]
     L.oqph_int1e.argtypes = [C.c_int, _pi, _pi, _pi, _pi, _pd, _pd, C.c_int, _pd, _pd, C.c_void_p, C.c_void_p,
                              C.c_void_p]
+    L.oqph_grad1e.argtypes = [C.c_int, _pi, _pi, _pi, _pi, _pd, _pd, C.c_int, _pd, _pd, C.c_void_p, C.c_void_p,
+                              C.c_void_p]
     ip = C.POINTER(C.c_int)
     dp = C.POINTER(C.c_double)
     L.routec_fock_jk.argtypes = [dp, dp, ip, ip, dp, dp, ip]
@@ -187,6 +189,21 @@ def int1e(bas: Basis):
                       S.ctypes.data, T.ctypes.data, V.ctypes.data)
     _check(rc, "int1e")
     return S, T, V
+
+
+def grad1e(bas: Basis, D: np.ndarray, W: np.ndarray) -> np.ndarray:
+    """One-electron gradient terms (natm x 3): sum D dH/dR - sum W dS/dR (host code in liboqph)."""
+    L = load()
+    g = np.zeros(3 * bas.natm)
+    Dm = np.ascontiguousarray(D, np.float64)
+    Wm = np.ascontiguousarray(W, np.float64)
+    rc = L.oqph_grad1e(bas.nsh, np.ascontiguousarray(bas.am, np.int32), np.ascontiguousarray(bas.harmonic, np.int32),
+                       np.ascontiguousarray(bas.atom, np.int32), np.ascontiguousarray(bas.nprim, np.int32),
+                       np.ascontiguousarray(bas.ex, np.float64), np.ascontiguousarray(bas.cc, np.float64), bas.natm,
+                       np.ascontiguousarray(bas.xyz.reshape(-1)), np.ascontiguousarray(bas.zn, np.float64),
+                       Dm.ctypes.data, Wm.ctypes.data, g.ctypes.data)
+    _check(rc, "grad1e")
+    return g.reshape(bas.natm, 3)
 
 
 def routec_fock_jk(d_packed, nbf: int, se: float = 1.0, sc: float = 1.0):

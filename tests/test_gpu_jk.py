@@ -61,14 +61,15 @@ def test_screen_counts_match_oracle(L, oracle):
 def test_bra_row_kernel_matches_wave_kernel(L, name):
     """The bra-row kernel (K rows staged in LDS) and the wavefront-task kernel
     give the same G and the same screening statistics (OQPH_JK_ROWS=2 forces
-    the bra-row kernel wherever its rows fit in LDS, =0 disables it)."""
+    the bra-row kernel wherever its rows fit in LDS, =3 the same without the
+    LDS copy of the density rows, =0 disables it)."""
     _say("bra-row vs wavefront kernel", name)
     bas = workload("h2o8_631g")[0] if name == "h2o8_631g" else small_system("cc-pvdz", nw=2)
     g = _golden(name)
     out = {}
     old = os.environ.get("OQPH_JK_ROWS")
     try:
-        for mode in ("0", "2"):
+        for mode in ("0", "2", "3"):
             os.environ["OQPH_JK_ROWS"] = mode
             fb = L.FockBuilder(bas)
             fb.schwarz()
@@ -79,12 +80,13 @@ def test_bra_row_kernel_matches_wave_kernel(L, name):
             os.environ.pop("OQPH_JK_ROWS", None)
         else:
             os.environ["OQPH_JK_ROWS"] = old
-    for mode in ("0", "2"):
+    for mode in ("0", "2", "3"):
         err = np.abs(out[mode][0] - g["f"]).max()
         _say("  mode", mode, "max |f - f_oracle| =", err)
         assert err <= TOL
-    assert np.abs(out["0"][0] - out["2"][0]).max() <= 1e-12
-    assert out["0"][1:] == out["2"][1:]
+    for mode in ("2", "3"):
+        assert np.abs(out["0"][0] - out[mode][0]).max() <= 1e-12
+        assert out["0"][1:] == out[mode][1:]
 
 
 @pytest.mark.parametrize("name,nw,nocc", [("cc-pvdz", 3, 15), ("6-31g(d)", 4, 20)])
