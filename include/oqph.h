@@ -36,7 +36,9 @@ extern "C" {
  * caller source/scf_addons.F90:1138-1146).  d, f: packed (ntri, nfocks); f is
  * returned ready to use, i.e. f = sc*J[D] - 0.5*se*K[D] exactly as fock_jk's
  * native path leaves it after its scaling (source/scf_addons.F90:1178-1185).
- * nfocks must be 1 (RHF); other values decline with info = 1. */
+ * nfocks = 1 (RHF, d the total density) or 2 (UHF/ROHF, d and f hold the
+ * alpha then the beta matrices, int2_urohf_data_t semantics, int2.F90:1488-1578);
+ * other values decline with info = 1. */
 void routec_fock_jk(const double *d, double *f, const int *nbf, const int *nfocks, const double *se,
                     const double *sc, int *info);
 

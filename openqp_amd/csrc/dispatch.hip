@@ -268,7 +268,10 @@ bool kernels_support(int lmax) { return lmax <= 2; }
 
 
 // shell-pair density maxima (shlden, source/integrals/int2.F90:999-1047)
-__global__ void k_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh) {
+// shell-pair max |D| (shlden, int2.F90:999-1047); with D2 the max over both
+// densities (nfocks = 2)
+__global__ void k_shell_density(const double* D, const double* D2, const int* ao_off, const int* nao, int nsh,
+                                double* dsh) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const long long np = (long long)nsh * (nsh + 1) / 2;
   if (t >= np) return;
@@ -280,7 +283,11 @@ __global__ void k_shell_density(const double* D, const int* ao_off, const int* n
   for (int a = 0; a < nao[i]; ++a) {
     const int x = ao_off[i] + a;
     const int jmax = (i == j) ? a + 1 : nao[j];
-    for (int b = 0; b < jmax; ++b) m = fmax(m, fabs(D[tri_idx_(x, ao_off[j] + b)]));
+    for (int b = 0; b < jmax; ++b) {
+      const long long xy = tri_idx_(x, ao_off[j] + b);
+      m = fmax(m, fabs(D[xy]));
+      if (D2) m = fmax(m, fabs(D2[xy]));
+    }
   }
   dsh[i * nsh + j] = m;
   dsh[j * nsh + i] = m;
@@ -294,10 +301,22 @@ __global__ void k_max_reduce(const double* x, long long n, unsigned long long* o
   if (__lane_id() == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
-void launch_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh, hipStream_t s) {
+void launch_shell_density(const double* D, const double* D2, const int* ao_off, const int* nao, int nsh, double* dsh,
+                          hipStream_t s) {
   const long long np = (long long)nsh * (nsh + 1) / 2;
   const int bs = 256;
-  hipLaunchKernelGGL(k_shell_density, dim3((unsigned)((np + bs - 1) / bs)), dim3(bs), 0, s, D, ao_off, nao, nsh, dsh);
+  hipLaunchKernelGGL(k_shell_density, dim3((unsigned)((np + bs - 1) / bs)), dim3(bs), 0, s, D, D2, ao_off, nao, nsh,
+                     dsh);
+}
+
+__global__ void k_add(const double* x, const double* y, double* z, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    z[i] = x[i] + y[i];
+}
+
+// z = x + y (n doubles)
+void launch_add(const double* x, const double* y, double* z, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_add, dim3(1024), dim3(256), 0, s, x, y, z, n);
 }
 
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s) {

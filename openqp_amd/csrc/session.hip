@@ -32,7 +32,9 @@ bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, co
 bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* static_lds, size_t lds_max);
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
                     const double* tab, const double* xf, double qcut, double* Q, hipStream_t s);
-void launch_shell_density(const double* D, const int* ao_off, const int* nao, int nsh, double* dsh, hipStream_t s);
+void launch_shell_density(const double* D, const double* D2, const int* ao_off, const int* nao, int nsh, double* dsh,
+                          hipStream_t s);
+void launch_add(const double* x, const double* y, double* z, long long n, hipStream_t s);
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
 bool kernels_support(int lmax);
 struct GradParams;
@@ -175,6 +177,7 @@ struct oqph_session {
   DBuf<int> d_gcls, d_coff, d_satom, d_am, d_pure;
   DBuf<long long> d_goff;
   DBuf<double> d_Dc, d_de;
+  DBuf<double> d_Dt;  // total density of a UHF build
   int ncf = 0;
   double gcut_ready = -1.0;
   std::vector<Launch> glaunches;  // gradient: ClassWork flat index space
@@ -680,12 +683,14 @@ static int prepare_lists(oqph_session* s, double maxden) {
   return 0;
 }
 
-// G[D] on device buffers; D already on the device
+// G[D] on device buffers; D already on the device.  The screening densities
+// scr_a (and scr_b) default to D; zero_f = false accumulates into dF.
 static int build_device(oqph_session* s, const double* dD, double* dF, double se, double sc, oqph_stats* st,
-                        hipStream_t stream) {
+                        hipStream_t stream, const double* scr_a = nullptr, const double* scr_b = nullptr,
+                        bool zero_f = true) {
   int rc;
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
-  launch_shell_density(dD, s->d_aooff.p, s->d_nao.p, s->nsh, s->d_dsh.p, stream);
+  launch_shell_density(scr_a ? scr_a : dD, scr_b, s->d_aooff.p, s->d_nao.p, s->nsh, s->d_dsh.p, stream);
   HIPCHK(hipMemsetAsync(s->d_ctr.p, 0, 8 * sizeof(unsigned long long), stream));
   launch_max_reduce(s->d_dsh.p, (long long)s->nsh * s->nsh, s->d_ctr.p + 4, stream);
   unsigned long long mbits = 0;
@@ -693,7 +698,7 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   HIPCHK(hipStreamSynchronize(stream));
   double maxden;
   memcpy(&maxden, &mbits, sizeof(double));
-  HIPCHK(hipMemsetAsync(dF, 0, ntri * sizeof(double), stream));
+  if (zero_f) HIPCHK(hipMemsetAsync(dF, 0, ntri * sizeof(double), stream));
   if (maxden <= 0.0) {
     if (st) memset(st, 0, sizeof(*st));
     return 0;
@@ -814,7 +819,7 @@ int oqph_session_destroy(oqph_session* s) {
   s->d_xform.release(); s->d_rys32.release(); s->d_kbs.release(); s->d_seg.release(); s->d_choff.release();
   s->d_gpairs.release(); s->d_gprims.release(); s->d_gcls.release(); s->d_coff.release(); s->d_satom.release();
   s->d_am.release(); s->d_pure.release(); s->d_goff.release(); s->d_Dc.release(); s->d_de.release();
-  s->d_aooff.release(); s->d_nao.release(); s->d_colidx.release(); s->d_colao.release(); s->d_items.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
+  s->d_aooff.release(); s->d_nao.release(); s->d_colidx.release(); s->d_colao.release(); s->d_items.release(); s->d_Dt.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
   s->d_tmp.release(); s->d_off.release(); s->d_ctr.release();
   for (auto e : s->pev) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
@@ -951,6 +956,32 @@ int oqph_schwarz(oqph_session* s, double* q) {
   return 0;
 }
 
+// UHF/ROHF (nfocks = 2, int2_urohf_data_t): with G(X; se, sc) = sc J[X] -
+// se K[X] / 2 (the RHF build), F_s = G(Da + Db; 0, sc) + G(Ds; 2 se, 0).  All
+// three builds screen with the shell maxima over both spin densities, as
+// shlden does for nfocks = 2 (int2.F90:999-1047), so they keep the same
+// quartets as the reference's single pass.  d and f hold alpha then beta.
+static int build_uhf(oqph_session* s, const double* d_dev, double* f_dev, double se, double sc, oqph_stats* st,
+                     hipStream_t stream) {
+  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+  int rc;
+  if ((rc = s->d_Dt.ensure(ntri))) return rc;
+  const double *da = d_dev, *db = d_dev + ntri;
+  double *fa = f_dev, *fb = f_dev + ntri;
+  launch_add(da, db, s->d_Dt.p, (long long)ntri, stream);
+  oqph_stats s1{}, s2{}, s3{};
+  if ((rc = build_device(s, s->d_Dt.p, fa, 0.0, sc, &s1, stream, da, db, true))) return rc;
+  HIPCHK(hipMemcpyAsync(fb, fa, ntri * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  if ((rc = build_device(s, da, fa, 2.0 * se, 0.0, &s2, stream, da, db, false))) return rc;
+  if ((rc = build_device(s, db, fb, 2.0 * se, 0.0, &s3, stream, da, db, false))) return rc;
+  if (st) {
+    *st = s1;
+    st->kernel_ms = s1.kernel_ms + s2.kernel_ms + s3.kernel_ms;
+    st->model_flops = s1.model_flops + s2.model_flops + s3.model_flops;
+  }
+  return 0;
+}
+
 int oqph_fock_jk_device(oqph_session* s, const double* d_dev, double* f_dev, int nfocks, double se, double sc,
                         oqph_stats* st, void* stream) {
   if (!s || !d_dev || !f_dev) return 4;
@@ -958,14 +989,15 @@ int oqph_fock_jk_device(oqph_session* s, const double* d_dev, double* f_dev, int
     set_err("OQPH_FAULT injected decline");
     return fc;
   }
-  if (nfocks != 1) {
-    set_err("nfocks=%d: only RHF (nfocks=1) is built on the device", nfocks);
+  if (nfocks != 1 && nfocks != 2) {
+    set_err("nfocks=%d: only nfocks = 1 (RHF) and 2 (UHF/ROHF) are supported", nfocks);
     return 1;
   }
   HIPCHK(hipSetDevice(s->device));
   int rc = prepare_geometry(s);
   if (rc) return rc;
   hipStream_t st_ = stream ? (hipStream_t)stream : s->stream;
+  if (nfocks == 2) return build_uhf(s, d_dev, f_dev, se, sc, st, st_);
   return build_device(s, d_dev, f_dev, se, sc, st, st_);
 }
 
@@ -975,17 +1007,21 @@ int oqph_fock_jk(oqph_session* s, const double* d, double* f, int nfocks, double
     set_err("OQPH_FAULT injected decline");
     return fc;
   }
-  if (nfocks != 1) {
-    set_err("nfocks=%d: only RHF (nfocks=1) is built on the device", nfocks);
+  if (nfocks != 1 && nfocks != 2) {
+    set_err("nfocks=%d: only nfocks = 1 (RHF) and 2 (UHF/ROHF) are supported", nfocks);
     return 1;
   }
   HIPCHK(hipSetDevice(s->device));
   int rc = prepare_geometry(s);
   if (rc) return rc;
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
-  if ((rc = upload(s->d_D.p, d, ntri * sizeof(double), s->stream))) return rc;
-  if ((rc = build_device(s, s->d_D.p, s->d_F.p, se, sc, st, s->stream))) return rc;
-  HIPCHK(hipMemcpyAsync(f, s->d_F.p, ntri * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  const size_t nt = ntri * (size_t)nfocks;
+  if ((rc = s->d_D.ensure(nt)) || (rc = s->d_F.ensure(nt))) return rc;
+  if ((rc = upload(s->d_D.p, d, nt * sizeof(double), s->stream))) return rc;
+  if (nfocks == 2) rc = build_uhf(s, s->d_D.p, s->d_F.p, se, sc, st, s->stream);
+  else rc = build_device(s, s->d_D.p, s->d_F.p, se, sc, st, s->stream);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(f, s->d_F.p, nt * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }

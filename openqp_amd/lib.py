@@ -144,6 +144,14 @@ class FockBuilder:
         _check(load().oqph_fock_jk(self._s, d, f, 1, se, sc, C.byref(self.last_stats)), "fock_jk")
         return f
 
+    def fock_jk_uhf(self, d_alpha, d_beta, se: float = 1.0, sc: float = 1.0):
+        """Packed UHF/ROHF (F_alpha, F_beta) = sc J[Da + Db] - se K[Ds] (fock_jk with nfocks = 2)."""
+        d = np.ascontiguousarray(np.concatenate([np.asarray(d_alpha, np.float64), np.asarray(d_beta, np.float64)]))
+        f = np.zeros_like(d)
+        _check(load().oqph_fock_jk(self._s, d, f, 2, se, sc, C.byref(self.last_stats)), "fock_jk (nfocks=2)")
+        n = d.size // 2
+        return f[:n], f[n:]
+
     def fock_jk_inc(self, d, d_old, f_old, se: float = 1.0, sc: float = 1.0):
         """Incremental build: returns f = G[d - d_old] + f_old and updates d_old, f_old in place."""
         d = np.ascontiguousarray(d, np.float64)

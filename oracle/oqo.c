@@ -628,6 +628,31 @@ static void digest_rhf(const ibuf *b, const double *d, double *f, double se, dou
   }
 }
 
+/* UHF/ROHF digestion (int2_urohf_data_t_update, int2.F90:1488-1578): both
+ * Fock matrices get the Coulomb term of the total density Da + Db, and each
+ * the exchange term of its own spin density with scale 2 se. */
+static void digest_urohf(const ibuf *b, const double *d, double *f, long ntri, double se, double sc) {
+  double x2 = 2.0 * se, x4 = 4.0 * sc;
+  const double *da = d, *db = d + ntri;
+  double *fa = f, *fb = f + ntri;
+  for (int n = 0; n < b->n; ++n) {
+    long ii = b->id[4 * n], jj = b->id[4 * n + 1], kk = b->id[4 * n + 2], ll = b->id[4 * n + 3];
+    double val = b->v[n];
+    long ii2 = ii * (ii - 1) / 2, jj2 = jj * (jj - 1) / 2, kk2 = kk * (kk - 1) / 2;
+    long ij = ii2 + jj, ik = ii2 + kk, il = ii2 + ll, jk = jj2 + kk, jl = jj2 + ll, kl = kk2 + ll;
+    if (jj < kk) jk = kk2 + jj;
+    if (jj < ll) jl = ll * (ll - 1) / 2 + jj;
+    double v1 = val * x2, v4 = val * x4;
+    double cij = v4 * (da[ij - 1] + db[ij - 1]), ckl = v4 * (da[kl - 1] + db[kl - 1]);
+    fa[ij - 1] += ckl; fa[kl - 1] += cij;
+    fa[ik - 1] -= v1 * da[jl - 1]; fa[jl - 1] -= v1 * da[ik - 1];
+    fa[il - 1] -= v1 * da[jk - 1]; fa[jk - 1] -= v1 * da[il - 1];
+    fb[ij - 1] += ckl; fb[kl - 1] += cij;
+    fb[ik - 1] -= v1 * db[jl - 1]; fb[jl - 1] -= v1 * db[ik - 1];
+    fb[il - 1] -= v1 * db[jk - 1]; fb[jk - 1] -= v1 * db[il - 1];
+  }
+}
+
 /* storeints (int2.F90:1741-1865): element cutoff, canonical AO order, degeneracy halving */
 static void store_block(const oqo_basis *bs, const int *ids, const double *blk, double cutoff, ibuf *b,
                         long *nint) {
@@ -737,16 +762,41 @@ int oqo_fock_jk(int nsh, int nbf, int natm, const int *am, const int *harm, cons
  * owns under OpenQP's MPI decomposition mod(ij_pair, size) == rank
  * (int2.F90:759-761), ij_pair counting (i, j) pairs in loop order.  With
  * size > 1 this is a representative 1/size sample of the work. */
+static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                      const int *nprim, const int *poff, const int *aooff, const double *ex,
+                      const double *cc, const double *xyz, const double *Q, const double *d, double *f,
+                      double se, double sc, double cutoff, long *stats, int rank, int size);
+
 int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                      const int *nprim, const int *poff, const int *aooff, const double *ex,
                      const double *cc, const double *xyz, const double *Q, const double *d, double *f,
                      double se, double sc, double cutoff, long *stats, int rank, int size) {
+  return fock_jk_nf(1, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, d, f, se, sc, cutoff,
+                    stats, rank, size);
+}
+
+/* UHF/ROHF build (nfocks = 2): d and f hold the alpha then the beta packed
+ * matrices; screening uses the shell maxima over both densities (shlden with
+ * nfocks = 2, int2.F90:999-1047), digestion int2_urohf_data_t_update, and the
+ * same final 0.5 scaling and diagonal doubling (scf_addons.F90:1178-1185). */
+int oqo_fock_jk_uhf(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                    const int *nprim, const int *poff, const int *aooff, const double *ex,
+                    const double *cc, const double *xyz, const double *Q, const double *d, double *f,
+                    double se, double sc, double cutoff, long *stats) {
+  return fock_jk_nf(2, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, d, f, se, sc, cutoff,
+                    stats, 0, 1);
+}
+
+static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                      const int *nprim, const int *poff, const int *aooff, const double *ex,
+                      const double *cc, const double *xyz, const double *Q, const double *d, double *f,
+                      double se, double sc, double cutoff, long *stats, int rank, int size) {
   build_tables();
   oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
   oqo_cut cut = make_cut(cutoff);
   long ntri = (long)nbf * (nbf + 1) / 2;
   double *dsh = (double *)malloc(sizeof(double) * nsh * nsh);
-  shell_density(&bs, d, 1, ntri, dsh);
+  shell_density(&bs, d, nfocks, ntri, dsh);
   double maxden = 0.0;
   for (int x = 0; x < nsh * nsh; ++x)
     if (fabs(dsh[x]) > maxden) maxden = fabs(dsh[x]);
@@ -756,14 +806,15 @@ int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm,
 #ifdef _OPENMP
   nthr = omp_get_max_threads();
 #endif
-  double *fthr = (double *)calloc((size_t)nthr * ntri, sizeof(double));
+  const long nf = (long)nfocks * ntri;
+  double *fthr = (double *)calloc((size_t)nthr * nf, sizeof(double));
 #pragma omp parallel reduction(+ : nschwz, nquart, nints)
   {
     int tid = 0;
 #ifdef _OPENMP
     tid = omp_get_thread_num();
 #endif
-    double *fl = fthr + (size_t)tid * ntri;
+    double *fl = fthr + (size_t)tid * nf;
     work_t w;
     work_init(&w);
     ppair *bp = (ppair *)malloc(sizeof(ppair) * mp * mp), *bq = (ppair *)malloc(sizeof(ppair) * mp * mp);
@@ -792,7 +843,8 @@ int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm,
             int ids[4] = {i, j, k, l};
             b.n = 0;
             store_block(&bs, ids, out, cutoff, &b, &nints);
-            digest_rhf(&b, d, fl, se, sc);
+            if (nfocks == 1) digest_rhf(&b, d, fl, se, sc);
+            else digest_urohf(&b, d, fl, ntri, se, sc);
           }
         }
       }
@@ -800,12 +852,13 @@ int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm,
     free(bp); free(bq); free(out); free(raw); free(tmp);
     work_free(&w);
   }
-  for (long x = 0; x < ntri; ++x) {
+  for (long x = 0; x < nf; ++x) {
     double s = 0.0;
-    for (int t = 0; t < nthr; ++t) s += fthr[(size_t)t * ntri + x];
+    for (int t = 0; t < nthr; ++t) s += fthr[(size_t)t * nf + x];
     f[x] = 0.5 * s;
   }
-  for (long i = 1; i <= nbf; ++i) f[tri(i, i) - 1] *= 2.0;
+  for (int fk = 0; fk < nfocks; ++fk)
+    for (long i = 1; i <= nbf; ++i) f[fk * ntri + tri(i, i) - 1] *= 2.0;
   free(fthr);
   free(dsh);
   if (stats) { stats[0] = nschwz; stats[1] = nquart; stats[2] = nints; }

@@ -40,6 +40,7 @@ def lib():
         L.oqo_fock_jk.argtypes = _BAS + [_pd, _pd, _pd, C.c_double, C.c_double, C.c_double,
                                          np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.oqo_fock_jk_part.argtypes = L.oqo_fock_jk.argtypes + [C.c_int, C.c_int]
+        L.oqo_fock_jk_uhf.argtypes = L.oqo_fock_jk.argtypes
         L.oqo_eri_tensor.argtypes = _BAS + [_pd]
         L.oqo_int1e.argtypes = _BAS + [_pd, _pd, _pd, _pd]
         if hasattr(L, "oqo_grad2"):
@@ -106,6 +107,18 @@ def fock_jk(bas, d_packed, se=1.0, sc=1.0, cutoff=5e-11, Q=None, rank=0, size=1)
     lib().oqo_fock_jk_part(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)),
                            np.ascontiguousarray(d_packed, np.float64), f, se, sc, cutoff, stats, rank, size)
     return f, stats
+
+
+def fock_jk_uhf(bas, d_alpha, d_beta, se=1.0, sc=1.0, cutoff=5e-11, Q=None):
+    """Packed UHF/ROHF 2e Fock matrices (f_alpha, f_beta) for spin densities
+    (int2_urohf_data_t_update semantics) and stats."""
+    if Q is None:
+        Q = schwarz(bas)
+    d = np.concatenate([np.asarray(d_alpha, np.float64), np.asarray(d_beta, np.float64)])
+    f = np.zeros(2 * bas.ntri)
+    stats = np.zeros(3, dtype=np.int64)
+    lib().oqo_fock_jk_uhf(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)), d, f, se, sc, cutoff, stats)
+    return f[:bas.ntri], f[bas.ntri:], stats
 
 
 def grad2(bas, d_packed, hfscale=1.0, coulscale=1.0, cutoff=1e-10, Q=None):

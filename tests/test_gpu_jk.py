@@ -162,3 +162,29 @@ def test_mixed_precision_energy(L, oracle, cut):
     assert n32 > 0
     assert dE <= 1e-6
     fb.close()
+
+
+@pytest.mark.parametrize("name,nw", [("cc-pvdz", 2), ("6-31g(d)", 3)])
+def test_uhf_parity(L, oracle, name, nw):
+    """nfocks = 2 through the C ABI (and routec_fock_jk) against the oracle's
+    int2_urohf_data_t restatement: same screening, 1e-10 elementwise."""
+    from openqp_amd import basis as B
+    _say("UHF parity", name, nw)
+    bas = small_system(name, nw=nw)
+    S = oracle.int1e(bas)[0]
+    ds = []
+    for k, sd in ((5 * nw, 11), (5 * nw - 1, 12)):
+        rng = np.random.default_rng(sd)
+        C = rng.standard_normal((bas.nbf, k))
+        Lc = np.linalg.cholesky(C.T @ S @ C)
+        C = C @ np.linalg.inv(Lc).T
+        ds.append(B.pack_tri(C @ C.T))
+    Q = oracle.schwarz(bas)
+    ra, rb, st = oracle.fock_jk_uhf(bas, ds[0], ds[1], se=1.0, sc=1.0, cutoff=5e-11, Q=Q)
+    fb = L.FockBuilder(bas)
+    fa, fbb = fb.fock_jk_uhf(ds[0], ds[1])
+    err = max(np.abs(fa - ra).max(), np.abs(fbb - rb).max())
+    _say("  max |f - f_oracle| =", err, "quartets", fb.last_stats.quartets_computed, int(st[1]))
+    assert err <= TOL
+    assert fb.last_stats.quartets_computed == int(st[1])
+    fb.close()

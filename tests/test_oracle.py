@@ -232,3 +232,31 @@ def test_grad2_coulomb_exchange_split(oracle):
     g, _ = oracle.grad2(bas, dp, hfscale=1.0, coulscale=1.0, cutoff=1e-30)
     # linear up to the max|Gamma|-dependent primitive screen (grd2_rys.F90:376)
     assert np.abs(gj + gk - g).max() < 5e-9
+
+
+def _spin_densities(oracle, bas, na, nb, seed):
+    S = oracle.int1e(bas)[0]
+    out = []
+    for k, sd in ((na, seed), (nb, seed + 1)):
+        rng = np.random.default_rng(sd)
+        C = rng.standard_normal((bas.nbf, k))
+        L = np.linalg.cholesky(C.T @ S @ C)
+        C = C @ np.linalg.inv(L).T
+        out.append(C @ C.T)
+    return out
+
+
+@pytest.mark.parametrize("name", ["6-31g(d)", "cc-pvdz"])
+def test_uhf_build_matches_brute_force(oracle, name):
+    """UHF/ROHF digestion (int2_urohf_data_t_update): F_s = sc J[Da + Db] -
+    se K[Ds] against the nbf^4 tensor contraction, unscreened."""
+    bas = small_system(name, nw=1)
+    n = bas.nbf
+    Da, Db = _spin_densities(oracle, bas, 5, 4, 3)
+    se, sc = 0.7, 1.3
+    fa, fb, _ = oracle.fock_jk_uhf(bas, B.pack_tri(Da), B.pack_tri(Db), se=se, sc=sc, cutoff=0.0)
+    eri = oracle.eri_tensor(bas).reshape(n, n, n, n)
+    J = np.einsum("ijkl,kl->ij", eri, Da + Db)
+    for Ds, fs in ((Da, fa), (Db, fb)):
+        K = np.einsum("ikjl,kl->ij", eri, Ds)
+        assert np.abs(B.unpack_tri(fs, n) - (sc * J - se * K)).max() < 1e-12
