@@ -145,7 +145,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="h2o64_ccpvdz")
-    ap.add_argument("--fp32-cutoff", type=float, default=1e-5)
+    ap.add_argument("--fp32-cutoff", type=float, default=None,
+                    help="FP32 cutoff of the mixed build; default: swept over 1e-3..1e-6, the fastest "
+                         "with |dE2| <= 1 uEh is timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
@@ -181,8 +183,8 @@ def main():
     fb.schwarz()
     log(f"[bench] geometry setup (pairs + Schwarz) {time.perf_counter() - t0:.2f} s")
 
-    def run(fout, steps, warm, precision):
-        fb.set_precision(precision, a.fp32_cutoff)
+    def run(fout, steps, warm, precision, cut=0.0):
+        fb.set_precision(precision, cut)
         for _ in range(warm):
             fb.fock_jk_device(d_dev.data_ptr(), fout.data_ptr(), stream=stream)
         torch.cuda.synchronize()
@@ -207,8 +209,21 @@ def main():
     dt64, st64, prof64 = run(f64, a.steps, a.warmup, 0)
     log(f"[bench] FP64 {dt64 * 1e3:.1f} ms/build, quartets {st64.quartets_computed}, "
         f"model {st64.model_flops / 1e12:.3f} TFLOP")
-    log("[bench] mixed builds")
-    dtmx, stmx, profmx = run(fmx, a.steps, a.warmup, 1)
+    e64_ref = trace_energy(dp, f64.cpu().numpy(), bas.nbf)
+    # FP32 cutoff: the fastest of the swept values whose energy error is within 1 uEh
+    sweep = []
+    cuts = [a.fp32_cutoff] if a.fp32_cutoff is not None else [1e-3, 1e-4, 1e-5, 1e-6]
+    for cut in cuts:
+        dts, sts, _ = run(fmx, 1, 1, 1, cut)
+        de = abs(trace_energy(dp, fmx.cpu().numpy(), bas.nbf) - e64_ref)
+        sweep.append({"fp32_cutoff": cut, "ms": round(dts * 1e3, 3), "dE2_hartree": de,
+                      "fp32_quartets": sts.quartets_fp32})
+        log(f"[bench] sweep fp32 cutoff {cut:g}: {dts * 1e3:.1f} ms, |dE2| {de:.2e}")
+    ok = [x for x in sweep if x["dE2_hartree"] <= 1e-6]
+    best = min(ok, key=lambda x: x["ms"]) if ok else min(sweep, key=lambda x: x["dE2_hartree"])
+    a.fp32_cutoff = best["fp32_cutoff"]
+    log(f"[bench] mixed builds (fp32 cutoff {a.fp32_cutoff:g})")
+    dtmx, stmx, profmx = run(fmx, a.steps, a.warmup, 1, a.fp32_cutoff)
     log(f"[bench] mixed {dtmx * 1e3:.1f} ms/build, fp32 quartets {stmx.quartets_fp32}")
 
     g64 = f64.cpu().numpy()
@@ -266,6 +281,7 @@ def main():
         "quartets_per_build": st.quartets_computed, "quartets_per_s": round(st.quartets_computed / dt, 1),
         "prim_quartets_per_build": st.prim_quartets, "fp32_quartets": stmx.quartets_fp32,
         "E2_fp64_hartree": e64,
+        "fp32_cutoff_sweep": sweep,
         "gradient_evals_per_s": grad["evals_per_s"] if grad else None,
         "gradient": grad,
         "scf": scf,

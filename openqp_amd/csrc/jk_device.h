@@ -630,10 +630,13 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
 template <int LA, int LB, int LC, int LD>
 struct JKB {
   static constexpr int threads = (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF <= 4) ? 1024 : 512;
+  // (ss|ss) fits in 64 VGPRs without spilling: two 1024-thread blocks per CU
+  static constexpr int min_waves = (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF == 1) ? 8 : 1;
 };
 
 template <int LA, int LB, int LC, int LD, bool PURE>
-__global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams prm, BraWork bw, ColMap cm) {
+__global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads))
+__attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(JKParams prm, BraWork bw, ColMap cm) {
   using Dm = Dims<LA, LB, LC, LD>;
   constexpr int NR = Dm::NROOT;
   constexpr bool T64 = RysLds<NR>::f64;
@@ -696,7 +699,8 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
   double jab[OA * OB];
 #pragma unroll
   for (int i = 0; i < OA * OB; ++i) jab[i] = 0.0;
-  unsigned long long n_comp = 0, n_skip = 0, n_f32 = 0, n_flops = 0, n_prim = 0;
+  // per-lane statistics (32-bit; the model FLOPs follow from n_prim and n_valid)
+  unsigned n_comp = 0, n_skip = 0, n_f32 = 0, n_valid = 0, n_prim = 0;
 
   for (int b = 0; b < OQPH_NBKT; ++b) {
     const int n = __builtin_amdgcn_readfirstlane(item[1 + b]);
@@ -750,8 +754,8 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
                                                             1.0e-4 * prm.cutoff, V, nprim, sbra);
       }
       n_f32 += (lowp && !skip) ? 1 : 0;
-      n_flops += (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() + (nprim ? 12ull * NOUT : 0ull);
-      n_prim += (unsigned long long)nprim;
+      n_valid += nprim ? 1 : 0;
+      n_prim += (unsigned)nprim;
       const bool valid = !skip && nprim > 0;
       if (__ballot(valid) == 0ull) continue;  // wave-uniform
       const double c = prm.cutoff;
@@ -881,7 +885,10 @@ __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads)) void k_jkb(JKParams
   }
   {
     unsigned long long* ctr = prm.counters + 8 * (int)((blockIdx.x * nw + wv) & (OQPH_CTR_SLOTS - 1));
-    unsigned long long v[5] = {n_comp, n_skip, n_f32, n_flops, n_prim};
+    unsigned long long v[5] = {n_comp, n_skip, n_f32,
+                               (unsigned long long)n_prim * flops_prim<LA, LB, LC, LD>() +
+                                   (unsigned long long)n_valid * 12ull * NOUT,
+                               n_prim};
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
       unsigned long long t = v[i];
