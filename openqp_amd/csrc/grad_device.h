@@ -57,12 +57,19 @@ __host__ __device__ constexpr long long flops_grad_prim() {
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
   extern __shared__ double s_de[];
+  constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
+  // the root table block of this class in LDS when it fits (per-lane gathers
+  // at random intervals are cheaper from LDS than through vector memory)
+  constexpr bool T64 = rys_len<NR>() * 8 <= 36 * 1024;
+  __shared__ double s_t64[T64 ? rys_len<NR>() : 1];
   for (int i = threadIdx.x; i < 3 * prm.natm; i += blockDim.x) s_de[i] = 0.0;
+  if constexpr (T64)
+    for (int i = threadIdx.x; i < rys_len<NR>(); i += blockDim.x) s_t64[i] = prm.rys[RYS_OFF[NR] + i];
   __syncthreads();
+  const double* tabN = T64 ? (const double*)s_t64 : prm.rys + RYS_OFF[NR];
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NG = NA * NB * NC * ND;
   constexpr int LAB = LA + LB + 1, LCD = LC + LD + 1;
-  constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
   const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = w < cw.nwork;
   bool skip = true;
@@ -127,7 +134,7 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
           pq2 += PQ[d] * PQ[d];
         }
         double rx[NR], rw[NR];
-        rys_roots<NR>(rho * pq2, prm.rys + RYS_OFF[NR], rx, rw);
+        rys_roots<NR>(rho * pq2, tabN, rx, rw);
         const double pref = fqz * sq;
         const double rz = eta * ize, re = zeta * ize;
         const double h2z = pp.h, h2e = qq.h, h2ze = 0.5 * ize;
