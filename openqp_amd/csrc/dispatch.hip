@@ -180,52 +180,52 @@ bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const Pri
   }
 }
 
-void launch_grad_0000(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_1000(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_1010(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_1100(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_1110(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_1111(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2000(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2010(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2011(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2020(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2100(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2110(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2111(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2120(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2121(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2200(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2210(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2211(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2220(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2221(const GradParams&, const ClassWork&, hipStream_t);
-void launch_grad_2222(const GradParams&, const ClassWork&, hipStream_t);
+void launch_grad_0000(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_1000(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_1010(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_1100(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_1110(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_1111(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2000(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2010(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2011(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2020(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2100(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2110(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2111(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2120(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2121(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2200(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2210(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2211(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2220(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2221(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2222(const GradParams&, const WaveWork&, hipStream_t);
 
-bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const ClassWork& cw, hipStream_t s) {
-  if (cw.nwork == 0) return true;
+bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s) {
+  if (ww.nwaves == 0) return true;
   switch (la * 1000 + lb * 100 + lc * 10 + ld) {
-    case 0: launch_grad_0000(p, cw, s); return true;
-    case 1000: launch_grad_1000(p, cw, s); return true;
-    case 1010: launch_grad_1010(p, cw, s); return true;
-    case 1100: launch_grad_1100(p, cw, s); return true;
-    case 1110: launch_grad_1110(p, cw, s); return true;
-    case 1111: launch_grad_1111(p, cw, s); return true;
-    case 2000: launch_grad_2000(p, cw, s); return true;
-    case 2010: launch_grad_2010(p, cw, s); return true;
-    case 2011: launch_grad_2011(p, cw, s); return true;
-    case 2020: launch_grad_2020(p, cw, s); return true;
-    case 2100: launch_grad_2100(p, cw, s); return true;
-    case 2110: launch_grad_2110(p, cw, s); return true;
-    case 2111: launch_grad_2111(p, cw, s); return true;
-    case 2120: launch_grad_2120(p, cw, s); return true;
-    case 2121: launch_grad_2121(p, cw, s); return true;
-    case 2200: launch_grad_2200(p, cw, s); return true;
-    case 2210: launch_grad_2210(p, cw, s); return true;
-    case 2211: launch_grad_2211(p, cw, s); return true;
-    case 2220: launch_grad_2220(p, cw, s); return true;
-    case 2221: launch_grad_2221(p, cw, s); return true;
-    case 2222: launch_grad_2222(p, cw, s); return true;
+    case 0: launch_grad_0000(p, ww, s); return true;
+    case 1000: launch_grad_1000(p, ww, s); return true;
+    case 1010: launch_grad_1010(p, ww, s); return true;
+    case 1100: launch_grad_1100(p, ww, s); return true;
+    case 1110: launch_grad_1110(p, ww, s); return true;
+    case 1111: launch_grad_1111(p, ww, s); return true;
+    case 2000: launch_grad_2000(p, ww, s); return true;
+    case 2010: launch_grad_2010(p, ww, s); return true;
+    case 2011: launch_grad_2011(p, ww, s); return true;
+    case 2020: launch_grad_2020(p, ww, s); return true;
+    case 2100: launch_grad_2100(p, ww, s); return true;
+    case 2110: launch_grad_2110(p, ww, s); return true;
+    case 2111: launch_grad_2111(p, ww, s); return true;
+    case 2120: launch_grad_2120(p, ww, s); return true;
+    case 2121: launch_grad_2121(p, ww, s); return true;
+    case 2200: launch_grad_2200(p, ww, s); return true;
+    case 2210: launch_grad_2210(p, ww, s); return true;
+    case 2211: launch_grad_2211(p, ww, s); return true;
+    case 2220: launch_grad_2220(p, ww, s); return true;
+    case 2221: launch_grad_2221(p, ww, s); return true;
+    case 2222: launch_grad_2222(p, ww, s); return true;
     default: return false;
   }
 }

@@ -76,6 +76,20 @@ __device__ __forceinline__ double rsq64(double x) {
   return fma(y, e, y);
 }
 
+// Load a primitive pair whose index is uniform across the wavefront through
+// the constant address space, so it is served by the scalar cache.
+__device__ __forceinline__ PrimPair load_uniform(const PrimPair* p, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(4))) double* q =
+      (const __attribute__((address_space(4))) double*)(p + i);
+  PrimPair r;
+  r.g = q[0]; r.Kg = q[1]; r.P[0] = q[2]; r.P[1] = q[3]; r.P[2] = q[4]; r.a = q[5]; r.b = q[6]; r.h = q[7];
+  return r;
+#else
+  return p[i];
+#endif
+}
+
 // ------------------------------------------------------------------ Rys roots
 #include "rys_table.inc"
 #include "rys_table_f32.inc"

@@ -7,10 +7,10 @@ namespace oqph {
 #define OQPH_GR_LAUNCHER_(a, b, c, d) launch_grad_##a##b##c##d
 #define OQPH_GR_LAUNCHER(a, b, c, d) OQPH_GR_LAUNCHER_(a, b, c, d)
 
-void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const ClassWork& cw, hipStream_t s) {
-  const int bs = 256;
-  const long long nb = (cw.nwork + bs - 1) / bs;
-  hipLaunchKernelGGL((k_grad<XA, XB, XC, XD>), dim3((unsigned)nb), dim3(bs), sizeof(double) * 3 * p.natm, s, p, cw);
+void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const WaveWork& ww, hipStream_t s) {
+  const int bs = 256;  // four wavefronts
+  const long long nb = (ww.nwaves + 3) / 4;
+  hipLaunchKernelGGL((k_grad<XA, XB, XC, XD>), dim3((unsigned)nb), dim3(bs), sizeof(double) * 3 * p.natm, s, p, ww);
 }
 
 }  // namespace oqph

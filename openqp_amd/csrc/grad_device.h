@@ -3,7 +3,9 @@
 // 145-391 loops and screens, source/integrals/grd2_rys.F90:303-937 integral
 // derivatives, source/modules/hf_gradient.F90:428-495 density).
 //
-// One thread evaluates one canonical shell quartet.  Gamma is formed in raw
+// One thread evaluates one canonical shell quartet; a wavefront takes a chunk
+// of 64 kets of one bra pair (WaveWork), so the bra pair, its primitives and
+// its density block are wave-uniform scalar loads.  Gamma is formed in raw
 // (x^l-normalised, canonical-order) Cartesians from the Cartesian-effective
 // density Dc = X^T D X,
 //   Gamma_abcd = 4 cs Dc_ab Dc_cd - hs (Dc_ac Dc_bd + Dc_ad Dc_bc),
@@ -55,7 +57,7 @@ __host__ __device__ constexpr long long flops_grad_prim() {
 }
 
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
+__global__ __launch_bounds__(256) void k_grad(GradParams prm, WaveWork ww) {
   extern __shared__ double s_de[];
   constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
   // the root table block of this class in LDS when it fits (per-lane gathers
@@ -70,22 +72,31 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NG = NA * NB * NC * ND;
   constexpr int LAB = LA + LB + 1, LCD = LC + LD + 1;
-  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = w < cw.nwork;
+  // one wavefront = one chunk of 64 kets of one segment; the bra is uniform.
+  // Waves past the end still reach the block barrier below.
+  const long long wave = __builtin_amdgcn_readfirstlane(
+      (int)((blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6));
+  const bool wave_ok = wave < ww.nwaves;
+  bool active = false;
   bool skip = true;
   ShellPair P{}, Q{};
   int ip = 0, iq = 0;
   double Gam[NG];
   double gm = 0.0;
-  if (active) {
-    int lo = 0, hi = cw.nbra;
-    while (hi - lo > 1) {
+  if (wave_ok) {
+    int lo = 0, hi = ww.nseg;
+    while (hi - lo > 1) {  // scalar binary search over the chunk prefix
       const int mid = (lo + hi) >> 1;
-      if (cw.off[mid] <= w) lo = mid; else hi = mid;
+      if (ww.choff[mid] <= wave) lo = mid; else hi = mid;
     }
-    ip = cw.bra[lo];
-    iq = cw.ket[w - cw.off[lo]];
+    const int seg = __builtin_amdgcn_readfirstlane(lo);
+    const int k = (int)(wave - ww.choff[seg]) * 64 + (int)__lane_id();
+    active = k < ww.seg_n[seg];
+    ip = __builtin_amdgcn_readfirstlane(ww.seg_bra[seg]);
     P = prm.pairs[ip];
+    if (active) iq = ww.ket[ww.seg_k0[seg] + k];
+  }
+  if (active) {
     Q = prm.pairs[iq];
     const int oa = prm.coff[P.sa], ob = prm.coff[P.sb], oc = prm.coff[Q.sa], od = prm.coff[Q.sb];
     const int n = prm.ncf;
@@ -107,7 +118,7 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
     // fine screen on the weighted contribution (grd2.F90:348)
     skip = gm * P.Q * Q.Q < 0.5 * prm.gcut;
   }
-  unsigned long long* ctr = prm.counters + 8 * (int)((w >> 6) & (OQPH_CTR_SLOTS - 1));
+  unsigned long long* ctr = prm.counters + 8 * (int)(wave & (OQPH_CTR_SLOTS - 1));
   count_wave_g(ctr + 1, active && skip);
   count_wave_g(ctr + 0, !skip);
   double fd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -117,7 +128,7 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, ClassWork cw) {
     for (int iqq = 0; iqq < Q.nprim; ++iqq) {
       const PrimPair qq = prm.prims[Q.prim0 + iqq];
       for (int ipp = 0; ipp < P.nprim; ++ipp) {
-        const PrimPair pp = prm.prims[P.prim0 + ipp];
+        const PrimPair pp = load_uniform(prm.prims, P.prim0 + ipp);
         const double zeta = pp.g, eta = qq.g, ze = zeta + eta;
         const double fqz = pp.Kg * qq.Kg;
         // primitive screens (grd2_rys.F90:375-376), prefactor cutoff (1e-20)^2

@@ -69,20 +69,6 @@ __device__ __forceinline__ void xform_inplace(const double* __restrict__ xf, R* 
     }
 }
 
-// Load a primitive pair whose index is uniform across the wavefront through
-// the constant address space, so it is served by the scalar cache.
-__device__ __forceinline__ PrimPair load_uniform(const PrimPair* p, int i) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const __attribute__((address_space(4))) double* q =
-      (const __attribute__((address_space(4))) double*)(p + i);
-  PrimPair r;
-  r.g = q[0]; r.Kg = q[1]; r.P[0] = q[2]; r.P[1] = q[3]; r.P[2] = q[4]; r.a = q[5]; r.b = q[6]; r.h = q[7];
-  return r;
-#else
-  return p[i];
-#endif
-}
-
 // Bra primitive pairs staged in LDS by a block whose bra is fixed (k_jkb):
 // every lane reads pair i from the same LDS address (a broadcast), with no
 // vector-memory access in the quadrature loop.  Up to JKB_MAX_BRA_PRIMS pairs.

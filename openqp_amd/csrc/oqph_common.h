@@ -36,22 +36,9 @@ struct PrimPair {              // 64 bytes: one cache line per primitive pair
   double h;                    // 1 / (2 zeta)
 };
 
-// Per-class-pair work description for one launch.
-// The class lists are ordered by primitive-pair-count bucket, then by Q
-// descending; work segment s = bra * nbkt + bucket covers a prefix of that
-// ket bucket, so a wavefront sees one bra and kets of similar contraction.
-struct ClassWork {
-  const int* bra;              // pair indices of the bra class list
-  const int* ket;              // pair indices of the ket class list
-  const long long* off;        // prefix offsets over segments (nbra*nbkt + 1)
-  const int* kbs;              // ket bucket starts in `ket` (nbkt + 1)
-  int nbra, nbkt;
-  long long nwork;
-};
-
-// Wavefront tasks of one J/K class-pair launch: segment s pairs bra pair
-// seg_bra[s] with the kets ket[seg_k0[s] .. seg_k0[s] + seg_n[s]); it is
-// split into ceil(seg_n/64) chunks of 64 kets, one per wavefront, counted by
+// Wavefront tasks of one class-pair launch (J/K build or gradient): segment
+// s pairs bra pair seg_bra[s] with the kets ket[seg_k0[s] .. seg_k0[s] +
+// seg_n[s]); it is split into ceil(seg_n/64) chunks of 64 kets, one per wavefront, counted by
 // the prefix choff (nseg + 1).  Empty segments are not stored.
 struct WaveWork {
   const int* seg_bra;

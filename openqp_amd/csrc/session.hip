@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -38,7 +39,7 @@ void launch_add(const double* x, const double* y, double* z, long long n, hipStr
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
 bool kernels_support(int lmax);
 struct GradParams;
-bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const ClassWork& cw, hipStream_t s);
+bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s);
 void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
                          const double* xf, int ncf, double* Dc, hipStream_t s);
 const double* rys_table_host(size_t* n);
@@ -145,9 +146,9 @@ struct oqph_session {
   DBuf<unsigned long long> d_ctr;
   double cached_maxden = -1.0;
   std::vector<long long> h_off;
-  struct Launch {
+  struct GLaunch {
     int la, lb, lc, ld;
-    ClassWork cw;
+    WaveWork ww;
   };
   struct WLaunch {
     int la, lb, lc, ld;
@@ -175,12 +176,13 @@ struct oqph_session {
   DBuf<ShellPair> d_gpairs;
   DBuf<PrimPair> d_gprims;
   DBuf<int> d_gcls, d_coff, d_satom, d_am, d_pure;
-  DBuf<long long> d_goff;
+  DBuf<int> d_gseg;                // gradient seg_bra | seg_k0 | seg_n
+  DBuf<long long> d_gchoff;
   DBuf<double> d_Dc, d_de;
   DBuf<double> d_Dt;  // total density of a UHF build
   int ncf = 0;
   double gcut_ready = -1.0;
-  std::vector<Launch> glaunches;  // gradient: ClassWork flat index space
+  std::vector<GLaunch> glaunches;  // gradient wavefront tasks
   // optional per-launch profile of the last build (HIP events on the build stream)
   bool profile = false;
   std::vector<hipEvent_t> pev;
@@ -409,22 +411,38 @@ static int prepare_grad(oqph_session* s) {
   if ((rc = s->d_gpairs.ensure(np)) || (rc = s->d_gprims.ensure(std::max<size_t>(gpr.size(), 1)))) return rc;
   if ((rc = upload(s->d_gpairs.p, gp.data(), np * sizeof(ShellPair), s->stream))) return rc;
   if (!gpr.empty() && (rc = upload(s->d_gprims.p, gpr.data(), gpr.size() * sizeof(PrimPair), s->stream))) return rc;
+  // class lists ordered by primitive-pair bucket, then Q descending (as the
+  // J/K lists), so a wavefront's kets have similar contraction lengths
   std::vector<std::vector<int>> cls(10);
   std::vector<std::vector<double>> clsQ(10);
+  std::vector<std::array<int, NBKT + 1>> kbs(10);
   for (size_t pid = 0; pid < np; ++pid)
     if (gp[pid].nprim > 0 && gp[pid].Q > 0.0) cls[class_of(gp[pid].la, gp[pid].lb)].push_back((int)pid);
   std::vector<int> all;
   std::vector<size_t> coff(11, 0);
   for (int c = 0; c < 10; ++c) {
-    std::stable_sort(cls[c].begin(), cls[c].end(), [&](int x, int y) { return s->Q[x] > s->Q[y]; });
-    for (int x : cls[c]) clsQ[c].push_back(s->Q[x]);
+    auto& v = cls[c];
+    std::stable_sort(v.begin(), v.end(), [&](int x, int y) {
+      const int bx = bucket_of(gp[x].nprim), by = bucket_of(gp[y].nprim);
+      return bx != by ? bx < by : s->Q[x] > s->Q[y];
+    });
+    for (int x : v) clsQ[c].push_back(s->Q[x]);
+    for (int b = 0; b <= NBKT; ++b) {
+      int k = 0;
+      while (k < (int)v.size() && bucket_of(gp[v[k]].nprim) < b) ++k;
+      kbs[c][b] = (b == NBKT) ? (int)v.size() : k;
+    }
     coff[c] = all.size();
-    all.insert(all.end(), cls[c].begin(), cls[c].end());
+    all.insert(all.end(), v.begin(), v.end());
   }
   if ((rc = s->d_gcls.ensure(std::max<size_t>(all.size(), 1)))) return rc;
   if (!all.empty() && (rc = upload(s->d_gcls.p, all.data(), all.size() * sizeof(int), s->stream))) return rc;
-  std::vector<long long> off;
-  struct Pend { int la, lb, lc, ld; size_t off0; int cb, ck, nbra; long long nwork; };
+  // segments (bra p, ket bucket b): the kets with Q_p Q_k >= grad_cutoff form
+  // a prefix of the bucket; within one class each unordered pair is taken
+  // once, with the bra the later entry of the (bucket, Q) order
+  std::vector<int> sbra, sk0, sn;
+  std::vector<long long> choff;
+  struct Pend { int la, lb, lc, ld, ck; size_t seg0, ch0; int nseg; long long nwaves; };
   std::vector<Pend> pend;
   const int ncls = class_of(s->lmax, s->lmax) + 1;
   for (int cb = 0; cb < ncls; ++cb)
@@ -432,37 +450,70 @@ static int prepare_grad(oqph_session* s) {
       const auto& QB = clsQ[cb];
       const auto& QK = clsQ[ck];
       if (QB.empty() || QK.empty()) continue;
-      const size_t off0 = off.size();
-      off.push_back(0);
-      long long tot = 0;
-      for (size_t p = 0; p < QB.size(); ++p) {
+      const auto& kb = kbs[ck];
+      const auto& bb = kbs[cb];
+      int kmax[NBKT] = {0};
+      for (int b = 0; b < NBKT; ++b)
+        for (int k = kb[b]; k < kb[b + 1]; ++k) kmax[b] = std::max(kmax[b], gp[cls[ck][k]].nprim);
+      struct Seg { int bra, k0, n; long long cost; };
+      std::vector<Seg> segs;
+      int bp = 0;
+      for (int p = 0; p < (int)QB.size(); ++p) {
+        while (bp + 1 < NBKT && p >= bb[bp + 1]) ++bp;
         const double thr = s->grad_cutoff / QB[p];
-        size_t k = std::lower_bound(QK.begin(), QK.end(), thr, [](double a, double t) { return a >= t; }) - QK.begin();
-        if (cb == ck) k = std::min(k, p + 1);
-        tot += (long long)k;
-        off.push_back(tot);
+        const int bra = cls[cb][p];
+        for (int b = 0; b < NBKT; ++b) {
+          if (kb[b + 1] <= kb[b]) continue;
+          int k = (int)(std::lower_bound(QK.begin() + kb[b], QK.begin() + kb[b + 1], thr,
+                                         [](double a, double t) { return a >= t; }) - (QK.begin() + kb[b]));
+          if (cb == ck) {
+            if (b > bp) k = 0;
+            else if (b == bp) k = std::min(k, p - bb[bp] + 1);
+          }
+          if (k > 0) segs.push_back({bra, kb[b], k, (long long)gp[bra].nprim * kmax[b]});
+        }
       }
+      if (segs.empty()) continue;
+      // longest wavefronts first
+      std::stable_sort(segs.begin(), segs.end(), [](const Seg& x, const Seg& y) { return x.cost > y.cost; });
+      const size_t seg0 = sbra.size(), ch0 = choff.size();
+      long long waves = 0;
+      for (const Seg& g : segs) {
+        sbra.push_back(g.bra);
+        sk0.push_back(g.k0);
+        sn.push_back(g.n);
+        choff.push_back(waves);
+        waves += (g.n + 63) / 64;
+      }
+      choff.push_back(waves);
       int la = 0, lb = 0, lc = 0, ld = 0;
       for (int x = 0; x <= 3; ++x)
         for (int y = 0; y <= x; ++y) {
           if (class_of(x, y) == cb) { la = x; lb = y; }
           if (class_of(x, y) == ck) { lc = x; ld = y; }
         }
-      pend.push_back({la, lb, lc, ld, off0, cb, ck, (int)QB.size(), tot});
+      pend.push_back({la, lb, lc, ld, ck, seg0, ch0, (int)segs.size(), waves});
     }
-  if ((rc = s->d_goff.ensure(std::max<size_t>(off.size(), 1)))) return rc;
-  if (!off.empty() && (rc = upload(s->d_goff.p, off.data(), off.size() * sizeof(long long), s->stream))) return rc;
+  const size_t nseg = sbra.size();
+  if ((rc = s->d_gseg.ensure(3 * nseg + 1)) || (rc = s->d_gchoff.ensure(choff.size() + 1))) return rc;
+  if (nseg) {
+    if ((rc = upload(s->d_gseg.p, sbra.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload(s->d_gseg.p + nseg, sk0.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload(s->d_gseg.p + 2 * nseg, sn.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload(s->d_gchoff.p, choff.data(), choff.size() * sizeof(long long), s->stream)))
+      return rc;
+  }
   s->glaunches.clear();
   for (auto& p : pend) {
-    oqph_session::Launch L;
+    oqph_session::GLaunch L{};
     L.la = p.la; L.lb = p.lb; L.lc = p.lc; L.ld = p.ld;
-    L.cw.bra = s->d_gcls.p + coff[p.cb];
-    L.cw.ket = s->d_gcls.p + coff[p.ck];
-    L.cw.off = s->d_goff.p + p.off0;
-    L.cw.kbs = nullptr;
-    L.cw.nbra = p.nbra;
-    L.cw.nbkt = 1;
-    L.cw.nwork = p.nwork;
+    L.ww.seg_bra = s->d_gseg.p + p.seg0;
+    L.ww.seg_k0 = s->d_gseg.p + nseg + p.seg0;
+    L.ww.seg_n = s->d_gseg.p + 2 * nseg + p.seg0;
+    L.ww.choff = s->d_gchoff.p + p.ch0;
+    L.ww.ket = s->d_gcls.p + coff[p.ck];
+    L.ww.nseg = p.nseg;
+    L.ww.nwaves = p.nwaves;
     s->glaunches.push_back(L);
   }
   // Cartesian offsets, shell atoms and flags
@@ -818,7 +869,7 @@ int oqph_session_destroy(oqph_session* s) {
   s->d_pairs.release(); s->d_prims.release(); s->d_rys.release(); s->d_cls.release();
   s->d_xform.release(); s->d_rys32.release(); s->d_kbs.release(); s->d_seg.release(); s->d_choff.release();
   s->d_gpairs.release(); s->d_gprims.release(); s->d_gcls.release(); s->d_coff.release(); s->d_satom.release();
-  s->d_am.release(); s->d_pure.release(); s->d_goff.release(); s->d_Dc.release(); s->d_de.release();
+  s->d_am.release(); s->d_pure.release(); s->d_gseg.release(); s->d_gchoff.release(); s->d_Dc.release(); s->d_de.release();
   s->d_aooff.release(); s->d_nao.release(); s->d_colidx.release(); s->d_colao.release(); s->d_items.release(); s->d_Dt.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
   s->d_tmp.release(); s->d_off.release(); s->d_ctr.release();
   for (auto e : s->pev) hipEventDestroy(e);
@@ -1085,7 +1136,7 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   for (size_t i = 0; i < nl; ++i) {
     auto& L = s->glaunches[i];
     p.counters = s->d_ctr.p + 8 + nc * i;
-    if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.cw, stream)) {
+    if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, stream)) {
       set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }
