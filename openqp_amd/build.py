@@ -44,6 +44,8 @@ DEPS = {
     "session.hip": ["oqph_common.h", "grad_params.h", "ao_xform.h", "../../include/oqph.h"],
     "grad_class.hip": ["grad_device.h", "grad_params.h", "eri_device.h", "oqph_common.h", "rys_table.inc", "rys_table_f32.inc"],
     "int1e.hip": ["../../include/oqph.h", "ao_xform.h", "rys_table.inc", "rys_table_f32.inc"],
+    "hl.hip": ["hl_device.h", "jk_device.h", "grad_params.h", "eri_device.h", "oqph_common.h", "rys_table.inc",
+               "rys_table_f32.inc"],
 }
 
 
@@ -60,7 +62,8 @@ def _key(unit: str, defs) -> str:
 
 
 def _jobs():
-    units = [("session.hip", [], "session.o"), ("dispatch.hip", [], "dispatch.o"), ("int1e.hip", [], "int1e.o")]
+    units = [("session.hip", [], "session.o"), ("dispatch.hip", [], "dispatch.o"), ("int1e.hip", [], "int1e.o"),
+             ("hl.hip", [], "hl.o")]
     heavy, light = [], []
     for c in sorted(CLASSES, key=_cost, reverse=True):
         a, b, cc, d = c

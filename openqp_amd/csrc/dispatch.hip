@@ -264,7 +264,7 @@ void launch_cart_density(const double* D, int nsh, const int* am, const int* pur
                      coff, xf, ncf, Dc);
 }
 
-bool kernels_support(int lmax) { return lmax <= 2; }
+bool kernels_support(int lmax) { return lmax <= 3; }
 
 
 // shell-pair density maxima (shlden, source/integrals/int2.F90:999-1047)

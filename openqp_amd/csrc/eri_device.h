@@ -228,12 +228,24 @@ __host__ __device__ constexpr int range_index(int L, int c) { return ncum(L - 1)
 // Algorithmic FLOP model of one surviving primitive quartet (Rys): roots,
 // recursion coefficients, 2D VRR, (e0|f0) assembly -- shared with the
 // benchmark's roofline accounting.
-template <int LA, int LB, int LC, int LD>
-__host__ __device__ constexpr long long flops_prim() {
-  constexpr int LAB = LA + LB, LCD = LC + LD, nr = (LAB + LCD) / 2 + 1;
-  constexpr int nc = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+__host__ __device__ constexpr long long flops_prim_rt(int LA, int LB, int LC, int LD) {
+  const int LAB = LA + LB, LCD = LC + LD, nr = (LAB + LCD) / 2 + 1;
+  const int nc = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
   return (long long)nr * (56 + 12 * (LAB + 1) * (LCD + 1) + 6 * (LA + 1) * (LB + 1) * (LCD + 1) +
                           6 * (LA + 1) * (LB + 1) * (LC + 1) * (LD + 1) + 3 * nc);
+}
+template <int LA, int LB, int LC, int LD>
+__host__ __device__ constexpr long long flops_prim() {
+  return flops_prim_rt(LA, LB, LC, LD);
+}
+// The same model for the derivative integrals of the gradient: the
+// differentiated centres raised by one, one more root level, and the
+// assembly term four times (value and three derivative components).
+__host__ __device__ constexpr long long flops_grad_prim_rt(int LA, int LB, int LC, int LD) {
+  const int LAB = LA + LB + 1, LCD = LC + LD + 1, nr = (LAB + LCD - 1) / 2 + 1;
+  const int nc = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+  return (long long)nr * (56 + 12 * (LAB + 1) * (LCD + 1) + 6 * (LA + 2) * (LB + 2) * (LCD + 1) +
+                          6 * (LA + 2) * (LB + 2) * (LC + 2) * (LD + 1) + 3 * 4 * nc);
 }
 
 // The fields of a primitive pair that the quadrature reads, in one precision.

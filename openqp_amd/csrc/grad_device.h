@@ -50,10 +50,7 @@ __device__ __forceinline__ R i2d(const R (&G)[NI][NK], const R* pab, const R* pc
 
 template <int LA, int LB, int LC, int LD>
 __host__ __device__ constexpr long long flops_grad_prim() {
-  constexpr int LAB = LA + LB + 1, LCD = LC + LD + 1, nr = (LAB + LCD - 1) / 2 + 1;
-  constexpr int nc = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
-  return (long long)nr * (56 + 12 * (LAB + 1) * (LCD + 1) + 6 * (LA + 2) * (LB + 2) * (LCD + 1) +
-                          6 * (LA + 2) * (LB + 2) * (LC + 2) * (LD + 1) + 3 * 4 * nc);
+  return flops_grad_prim_rt(LA, LB, LC, LD);
 }
 
 template <int LA, int LB, int LC, int LD>

@@ -1,0 +1,80 @@
+// hl.hip -- launchers of the run-time-class kernels (classes with an f
+// shell) and the host-side sizing of their LDS layout.
+#include <algorithm>
+
+#include "hl_device.h"
+
+namespace oqph {
+
+// Extents and LDS layout of class (la lb|lc ld).  grad = 1: the gradient
+// layout (2D integrals one level higher, Gamma block, 3 natm per-atom sums);
+// else the quartet-block layout (acc / HRR / AO-transform ping-pong buffers).
+HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm) {
+  HLDims h{};
+  h.la = la; h.lb = lb; h.lc = lc; h.ld = ld;
+  h.pure = pure ? 1 : 0;
+  const int L = la + lb + lc + ld + (grad ? 1 : 0);
+  h.nr = L / 2 + 1;
+  h.ni = la + lb + 1 + (grad ? 1 : 0);
+  h.nk = lc + ld + 1 + (grad ? 1 : 0);
+  h.ne = ncum(la + lb) - ncum(la - 1);
+  h.nf = ncum(lc + ld) - ncum(lc - 1);
+  h.na = ncart(la); h.nb = ncart(lb); h.nc = ncart(lc); h.nd = ncart(ld);
+  auto nao = [&](int l) { return (pure && l >= 2) ? 2 * l + 1 : ncart(l); };
+  h.oa = nao(la); h.ob = nao(lb); h.oc = nao(lc); h.od = nao(ld);
+  const int nG = 3 * h.nr * h.ni * h.nk;
+  int sA = 0, sB = 0;
+  if (grad) {
+    sA = h.na * h.nb * h.nc * h.nd;
+    sB = 3 * natm;
+  } else {
+    // the sequence of hl_block: acc in A, bra HRR into B, then every stage
+    // writes the other buffer
+    sA = h.ne * h.nf;
+    sB = h.na * h.nb * h.nf;
+    bool curB = true;
+    int oa = h.na, ob = h.nb;
+    auto write = [&](int n) {
+      if (curB) sA = std::max(sA, n); else sB = std::max(sB, n);
+      curB = !curB;
+    };
+    // (a transform that keeps the component count works in place)
+    if (lb >= 2) { ob = h.ob; if (h.ob != h.nb) write(h.na * ob * h.nf); }
+    if (la >= 2) { oa = h.oa; if (h.oa != h.na) write(oa * ob * h.nf); }
+    write(oa * ob * h.nc * h.nd);
+    if (ld >= 2 && h.od != h.nd) write(oa * ob * h.nc * h.od);
+    if (lc >= 2 && h.oc != h.nc) write(oa * ob * h.oc * h.od);
+  }
+  h.offG = 0;
+  h.offA = nG;
+  h.offB = nG + sA;
+  h.lds = (int)sizeof(double) * (nG + sA + sB);
+  return h;
+}
+
+template <typename K>
+static void allow_lds(K kern, int bytes) {
+  if (bytes > 64 * 1024) hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStream_t s) {
+  if (ww.nwaves == 0) return;
+  allow_lds(k_jkh, h.lds);
+  hipLaunchKernelGGL(k_jkh, dim3((unsigned)ww.nwaves), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+}
+
+void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+                     const double* rys, const double* xf, double qcut, double* Q, hipStream_t s) {
+  if (n == 0) return;
+  allow_lds(k_schwarzh, h.lds);
+  hipLaunchKernelGGL(k_schwarzh, dim3((unsigned)n), dim3(HL_BLOCK), h.lds, s, pairs, prims, list, n, rys, xf, qcut,
+                     Q, h);
+}
+
+void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s) {
+  if (ww.nwaves == 0) return;
+  allow_lds(k_gradh, h.lds);
+  hipLaunchKernelGGL(k_gradh, dim3((unsigned)ww.nwaves), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+}
+
+}  // namespace oqph

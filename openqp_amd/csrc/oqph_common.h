@@ -50,6 +50,19 @@ struct WaveWork {
   long long nwaves;
 };
 
+// One quartet class (la lb|lc ld), la >= lb, lc >= ld, for the kernels that
+// take the class at run time (classes with an f shell): extents of every
+// intermediate and the LDS layout (offsets in doubles), filled by hl_dims().
+struct HLDims {
+  int la, lb, lc, ld, nr, pure;
+  int ni, nk;                  // 2D-integral table extents per root and axis
+  int ne, nf;                  // contracted (e0|f0) extents
+  int na, nb, nc, nd;          // Cartesian components
+  int oa, ob, oc, od;          // AO components
+  int offG, offA, offB;        // LDS: 2D integrals, buffer A, buffer B
+  int lds;                     // bytes of dynamic LDS
+};
+
 // Primitive-pair-count buckets of the class lists.
 #define OQPH_NBKT 6
 

@@ -38,7 +38,12 @@ void launch_shell_density(const double* D, const double* D2, const int* ao_off, 
 void launch_add(const double* x, const double* y, double* z, long long n, hipStream_t s);
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
 bool kernels_support(int lmax);
+HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm);
+void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStream_t s);
+void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+                     const double* rys, const double* xf, double qcut, double* Q, hipStream_t s);
 struct GradParams;
+void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s);
 bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s);
 void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
                          const double* xf, int ncf, double* Dc, hipStream_t s);
@@ -149,10 +154,14 @@ struct oqph_session {
   struct GLaunch {
     int la, lb, lc, ld;
     WaveWork ww;
+    bool hl;     // run-time-class kernel (a class with an f shell), layout hd
+    HLDims hd;
   };
   struct WLaunch {
     int la, lb, lc, ld;
     bool rows;   // bra-row kernel (bw, cm, lds) instead of wavefront tasks (ww)
+    bool hl;     // run-time-class kernel on the wavefront tasks (f shells), layout hd
+    HLDims hd;
     WaveWork ww;
     BraWork bw;
     ColMap cm;
@@ -262,6 +271,11 @@ static int prepare_geometry(oqph_session* s) {
   }
   HIPCHK(hipSetDevice(s->device));
   int rc;
+  {
+    int v = 0;
+    HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
+    s->lds_max = (size_t)v;
+  }
   // Rys tables and AO transforms
   size_t nt;
   const double* tab = rys_table_host(&nt);
@@ -302,6 +316,16 @@ static int prepare_geometry(oqph_session* s) {
   for (int la = 0; la <= s->lmax; ++la)
     for (int lb = 0; lb <= la; ++lb) {
       const int c = class_of(la, lb);
+      if (la == 3) {
+        const HLDims h = hl_dims(la, lb, la, lb, s->pure, 0, 0);
+        if ((size_t)h.lds > s->lds_max) {
+          set_err("Schwarz class (%d %d): %d bytes of LDS exceed the device's %zu", la, lb, h.lds, s->lds_max);
+          return 1;
+        }
+        launch_schwarzh(h, dtp.p, dtpr.p, dlist.p + foff[c], (int)byc[c].size(), s->d_rys.p, s->d_xform.p, 1.0e-17,
+                        dQ.p, s->stream);
+        continue;
+      }
       if (!launch_schwarz(la, lb, s->pure, dtp.p, dtpr.p, dlist.p + foff[c], (int)byc[c].size(), s->d_rys.p,
                           s->d_xform.p, 1.0e-17, dQ.p, s->stream)) {
         set_err("no Schwarz kernel for class (%d %d)", la, lb);
@@ -379,9 +403,6 @@ static int prepare_geometry(oqph_session* s) {
       (rc = upload(s->d_colao.p, s->h_colao.data(), s->h_colao.size() * sizeof(int), s->stream)))
     return rc;
   {
-    int v = 0;
-    HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
-    s->lds_max = (size_t)v;
     const char* e = getenv("OQPH_JK_ROWS");
     s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
   }
@@ -514,6 +535,15 @@ static int prepare_grad(oqph_session* s) {
     L.ww.ket = s->d_gcls.p + coff[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
+    L.hl = p.la == 3 || p.lc == 3;
+    if (L.hl) {
+      L.hd = hl_dims(p.la, p.lb, p.lc, p.ld, s->pure, 1, s->natm);
+      if ((size_t)L.hd.lds > s->lds_max) {
+        set_err("gradient class (%d %d|%d %d): %d bytes of LDS exceed the device's %zu", p.la, p.lb, p.lc, p.ld,
+                L.hd.lds, s->lds_max);
+        return 1;
+      }
+    }
     s->glaunches.push_back(L);
   }
   // Cartesian offsets, shell atoms and flags
@@ -615,7 +645,8 @@ static int prepare_lists(oqph_session* s, double maxden) {
       const int ncC = s->ncolL[lc], ncol = (lc == ld) ? ncC : ncC + s->ncolL[ld];
       int nrow = 0;
       size_t slds = 0;
-      bool rows = s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max);
+      const bool hl = la == 3 || lc == 3;  // classes with an f shell: run-time-class kernel
+      bool rows = !hl && s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max);
       // dynamic LDS: K rows (nrow x ncol), plus D rows (nrow x ncol) and the
       // bra rows of dsh (2 x nsh) when those fit as well
       const size_t dynk = (size_t)nrow * ncol * sizeof(double);
@@ -727,6 +758,15 @@ static int prepare_lists(oqph_session* s, double maxden) {
     L.ww.ket = s->d_cls.p + s->cls_off[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
+    L.hl = p.la == 3 || p.lc == 3;
+    if (L.hl) {
+      L.hd = hl_dims(p.la, p.lb, p.lc, p.ld, s->pure, 0, 0);
+      if ((size_t)L.hd.lds > s->lds_max) {
+        set_err("class (%d %d|%d %d): %d bytes of LDS exceed the device's %zu", p.la, p.lb, p.lc, p.ld, L.hd.lds,
+                s->lds_max);
+        return 1;
+      }
+    }
     s->launches.push_back(L);
   }
   HIPCHK(hipStreamSynchronize(s->stream));  // host vectors stay alive; keep uploads ordered
@@ -788,8 +828,10 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     auto& L = s->launches[i];
     p.counters = s->d_ctr.p + 8 + nc * i;
     if (s->profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
-    const bool ok = L.rows ? launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, stream)
-                           : launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream);
+    bool ok = true;
+    if (L.hl) launch_jkh(L.hd, p, L.ww, stream);
+    else if (L.rows) ok = launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, stream);
+    else ok = launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream);
     if (!ok) {
       set_err("no J/K kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
@@ -989,7 +1031,7 @@ int oqph_last_profile(oqph_session* s, int maxn, int* cls, double* ms, double* f
   for (int i = 0; i < n; ++i) {
     const auto& L = s->launches[i];
     const unsigned long long* c = i * 8 + 8 <= (int)s->prof_ctr.size() ? &s->prof_ctr[8 * i] : nullptr;
-    if (cls) cls[i] = (L.rows ? 10000 : 0) + L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
+    if (cls) cls[i] = (L.hl ? 20000 : L.rows ? 10000 : 0) + L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
     if (ms) ms[i] = i < (int)s->prof_ms.size() ? s->prof_ms[i] : 0.0;
     if (flops) flops[i] = c ? (double)c[3] : 0.0;
     if (quartets) quartets[i] = c ? (long long)c[0] : 0;
@@ -1136,7 +1178,9 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   for (size_t i = 0; i < nl; ++i) {
     auto& L = s->glaunches[i];
     p.counters = s->d_ctr.p + 8 + nc * i;
-    if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, stream)) {
+    if (L.hl) {
+      launch_gradh(L.hd, p, L.ww, stream);
+    } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, stream)) {
       set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }

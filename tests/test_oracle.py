@@ -116,6 +116,60 @@ def test_eri_higher_l_by_centre_derivatives(oracle):
     assert vd[3] == pytest.approx(fd * Nd * 1.0, rel=1e-5)
 
 
+def test_eri_f_shell_by_centre_derivatives(oracle):
+    """The xyz component of an f shell (index 9 in OpenQP order, norm factor
+    sqrt(15) over x^3) is (1/2a)^3 d^3/dA_x dA_y dA_z of the s primitive: pins
+    the f-shell Rys path, component order and normalisation against central
+    differences of the closed-form (ss|ss), with the f shell in either the
+    bra or the ket and next to p and d partners."""
+    R = [[0.1, -0.2, 0.3], [0.3, -0.7, 1.1], [1.5, 0.2, -0.4], [-0.9, 1.3, 0.6]]
+    e = [0.9, 0.4, 1.2, 0.75]
+    a = e[0]
+    h = 2e-3
+
+    def s_un(RA):
+        return _ssss(e, [RA] + R[1:], oracle) / (2 * a / math.pi) ** 0.75
+
+    acc = 0.0
+    for sx in (1, -1):
+        for sy in (1, -1):
+            for sz in (1, -1):
+                Rq = list(R[0]); Rq[0] += sx * h; Rq[1] += sy * h; Rq[2] += sz * h
+                acc += sx * sy * sz * s_un(Rq)
+    fd = acc / (8 * h ** 3) / (2 * a) ** 3
+    Nf = (2 * a / math.pi) ** 0.75 * (4 * a) ** 1.5    # xyz: unit-normalised, no double factorial
+    bas = _prim_basis([3, 0, 0, 0], e, R)
+    vf = oracle.eri_quartet(bas, 0, 1, 2, 3)[:, 0, 0, 0]
+    assert vf[9] == pytest.approx(fd * Nf, rel=2e-5)
+    # the same integral from the ket side: (s2 s3|f0 s1) = (f0 s1|s2 s3)
+    bas2 = _prim_basis([0, 0, 3, 0], [e[2], e[3], e[0], e[1]], [R[2], R[3], R[0], R[1]])
+    v2 = oracle.eri_quartet(bas2, 0, 1, 2, 3)[0, 0, :, 0]
+    assert v2[9] == pytest.approx(vf[9], rel=1e-12)
+    # 8-fold symmetry of a quartet with f, d and p shells
+    bas3 = _prim_basis([3, 2, 1, 3], e, R)
+    I = oracle.eri_quartet(bas3, 0, 1, 2, 3)
+    J = oracle.eri_quartet(bas3, 2, 3, 0, 1)
+    assert np.abs(I - J.transpose(2, 3, 0, 1)).max() < 1e-13
+    K = oracle.eri_quartet(bas3, 1, 0, 3, 2)
+    assert np.abs(I - K.transpose(1, 0, 3, 2)).max() < 1e-13
+
+
+def test_fock_f_shells_vs_brute_force(oracle):
+    """cc-pVTZ (pure f) and its Cartesian form: screened build vs the full
+    tensor contraction."""
+    for harm in (None, 0):
+        from openqp_amd import molecules as M
+        Z, xyz = M.h2o_kat()
+        bas = B.build_basis("cc-pvtz", Z, xyz, harmonic=harm)
+        D, dp = density_for(oracle, bas, 5)
+        I = oracle.eri_tensor(bas)
+        G = np.einsum("ijkl,kl->ij", I, D) - 0.5 * np.einsum("ikjl,kl->ij", I, D)
+        f, st = oracle.fock_jk(bas, dp, cutoff=1e-30)
+        assert np.abs(B.unpack_tri(f, bas.nbf) - G).max() < 1e-12
+        for perm in [(1, 0, 2, 3), (2, 3, 0, 1)]:
+            assert np.abs(I - I.transpose(perm)).max() < 1e-13
+
+
 def test_eri_permutational_symmetry(oracle):
     bas = small_system("cc-pvdz")
     I = oracle.eri_tensor(bas)
@@ -207,7 +261,7 @@ def _e2(oracle, bas, dp, hs=1.0, cs=1.0):
     return 0.5 * (2 * np.dot(dp, f) - np.dot(dp[dg], f[dg]))
 
 
-@pytest.mark.parametrize("name,nw,nocc", [("6-31g(d)", 1, 5), ("cc-pvdz", 2, 10)])
+@pytest.mark.parametrize("name,nw,nocc", [("6-31g(d)", 1, 5), ("cc-pvdz", 2, 10), ("cc-pvtz", 1, 5)])
 def test_grad2_matches_finite_differences(oracle, name, nw, nocc):
     """dE2/dR at fixed density vs central differences (1e-4 bohr) of
     E2 = 0.5 Tr D G[D]; translational invariance to rounding."""
