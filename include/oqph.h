@@ -102,7 +102,8 @@ int oqph_fock_jk_device(oqph_session *s, const double *d_dev, double *f_dev, int
 /* Per-launch profile of the next builds (HIP events around each class
  * launch); oqph_last_profile copies up to maxn entries of the last build:
  * class code la*1000+lb*100+lc*10+ld (plus 10000 when the launch used the
- * bra-row kernel k_jkb rather than k_jk), device ms, model FLOPs, computed
+ * bra-row kernel k_jkb rather than k_jk, 20000 for the run-time-class kernel
+ * k_jkh of the classes with an f shell), device ms, model FLOPs, computed
  * quartets, primitive quartets.  Returns the number of launches.
  * No counterpart in the seam: measurement hooks for benchmarks. */
 int oqph_session_set_profiling(oqph_session *s, int on);

@@ -30,6 +30,16 @@ CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0),
            (2, 0, 0, 0), (2, 0, 1, 0), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 0, 0), (2, 1, 1, 0),
            (2, 1, 1, 1), (2, 1, 2, 0), (2, 1, 2, 1), (2, 2, 0, 0), (2, 2, 1, 0), (2, 2, 1, 1),
            (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)]
+# classes with an f shell that get class-templated kernels (the rest run on
+# the run-time-class kernels of hl.hip); the lists match jk_template() and
+# grad_template() in dispatch.hip
+F_JK = [(3, 0, 0, 0), (3, 0, 1, 0), (3, 0, 1, 1), (3, 0, 2, 0), (3, 0, 2, 1), (3, 0, 3, 0), (3, 1, 0, 0),
+        (3, 1, 1, 0), (3, 1, 2, 0), (3, 2, 0, 0), (3, 2, 1, 0), (3, 3, 0, 0)]
+# the gradient classes whose templates would spill 5-17 KB per lane run on
+# the run-time-class kernel instead
+GRAD_HL = [(2, 2, 2, 2), (2, 2, 2, 1), (2, 2, 1, 1), (2, 1, 2, 1)]
+F_GRAD = [(3, 0, 0, 0), (3, 0, 1, 0), (3, 0, 1, 1), (3, 0, 2, 0), (3, 0, 3, 0), (3, 1, 0, 0), (3, 1, 1, 0),
+          (3, 2, 0, 0), (3, 3, 0, 0)]
 
 
 def _cost(c):
@@ -65,11 +75,12 @@ def _jobs():
     units = [("session.hip", [], "session.o"), ("dispatch.hip", [], "dispatch.o"), ("int1e.hip", [], "int1e.o"),
              ("hl.hip", [], "hl.o")]
     heavy, light = [], []
-    for c in sorted(CLASSES, key=_cost, reverse=True):
+    per_class = [(c, "jk_class.hip", "jk") for c in CLASSES + F_JK] + \
+                [(c, "grad_class.hip", "gr") for c in CLASSES + F_GRAD if c not in GRAD_HL]
+    for c, unit, tag in sorted(per_class, key=lambda t: _cost(t[0]), reverse=True):
         a, b, cc, d = c
         defs = [f"-DXA={a}", f"-DXB={b}", f"-DXC={cc}", f"-DXD={d}"]
-        for unit, tag in (("jk_class.hip", "jk"), ("grad_class.hip", "gr")):
-            (heavy if _cost(c) > 1000 else light).append((unit, defs, f"{tag}_{a}{b}{cc}{d}.o"))
+        (heavy if _cost(c) > 1000 else light).append((unit, defs, f"{tag}_{a}{b}{cc}{d}.o"))
     return heavy + units + light
 
 

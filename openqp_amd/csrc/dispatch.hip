@@ -27,6 +27,18 @@ void launch_jk_2211(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2220(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2221(bool, const JKParams&, const WaveWork&, hipStream_t);
 void launch_jk_2222(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3000(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3010(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3011(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3020(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3021(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3030(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3100(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3110(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3120(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3200(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3210(bool, const JKParams&, const WaveWork&, hipStream_t);
+void launch_jk_3300(bool, const JKParams&, const WaveWork&, hipStream_t);
 bool launch_jkb_0000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
 bool jkb_info_0000(bool, int*, size_t*, size_t);
 bool launch_jkb_1000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
@@ -68,7 +80,31 @@ bool jkb_info_2220(bool, int*, size_t*, size_t);
 bool launch_jkb_2221(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
 bool jkb_info_2221(bool, int*, size_t*, size_t);
 bool launch_jkb_2222(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3010(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3011(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3020(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3021(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3030(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3100(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3110(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3120(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3200(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3210(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
+bool launch_jkb_3300(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
 bool jkb_info_2222(bool, int*, size_t*, size_t);
+bool jkb_info_3000(bool, int*, size_t*, size_t);
+bool jkb_info_3010(bool, int*, size_t*, size_t);
+bool jkb_info_3011(bool, int*, size_t*, size_t);
+bool jkb_info_3020(bool, int*, size_t*, size_t);
+bool jkb_info_3021(bool, int*, size_t*, size_t);
+bool jkb_info_3030(bool, int*, size_t*, size_t);
+bool jkb_info_3100(bool, int*, size_t*, size_t);
+bool jkb_info_3110(bool, int*, size_t*, size_t);
+bool jkb_info_3120(bool, int*, size_t*, size_t);
+bool jkb_info_3200(bool, int*, size_t*, size_t);
+bool jkb_info_3210(bool, int*, size_t*, size_t);
+bool jkb_info_3300(bool, int*, size_t*, size_t);
 void launch_schwarz_00(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
 void launch_schwarz_10(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
 void launch_schwarz_11(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
@@ -105,6 +141,18 @@ bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, con
     case 2220: launch_jk_2220(pure, p, cw, s); return true;
     case 2221: launch_jk_2221(pure, p, cw, s); return true;
     case 2222: launch_jk_2222(pure, p, cw, s); return true;
+    case 3000: launch_jk_3000(pure, p, cw, s); return true;
+    case 3010: launch_jk_3010(pure, p, cw, s); return true;
+    case 3011: launch_jk_3011(pure, p, cw, s); return true;
+    case 3020: launch_jk_3020(pure, p, cw, s); return true;
+    case 3021: launch_jk_3021(pure, p, cw, s); return true;
+    case 3030: launch_jk_3030(pure, p, cw, s); return true;
+    case 3100: launch_jk_3100(pure, p, cw, s); return true;
+    case 3110: launch_jk_3110(pure, p, cw, s); return true;
+    case 3120: launch_jk_3120(pure, p, cw, s); return true;
+    case 3200: launch_jk_3200(pure, p, cw, s); return true;
+    case 3210: launch_jk_3210(pure, p, cw, s); return true;
+    case 3300: launch_jk_3300(pure, p, cw, s); return true;
     default: return false;
   }
 }
@@ -134,6 +182,18 @@ bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, co
     case 2220: return launch_jkb_2220(pure, p, bw, cm, lds, s);
     case 2221: return launch_jkb_2221(pure, p, bw, cm, lds, s);
     case 2222: return launch_jkb_2222(pure, p, bw, cm, lds, s);
+    case 3000: return launch_jkb_3000(pure, p, bw, cm, lds, s);
+    case 3010: return launch_jkb_3010(pure, p, bw, cm, lds, s);
+    case 3011: return launch_jkb_3011(pure, p, bw, cm, lds, s);
+    case 3020: return launch_jkb_3020(pure, p, bw, cm, lds, s);
+    case 3021: return launch_jkb_3021(pure, p, bw, cm, lds, s);
+    case 3030: return launch_jkb_3030(pure, p, bw, cm, lds, s);
+    case 3100: return launch_jkb_3100(pure, p, bw, cm, lds, s);
+    case 3110: return launch_jkb_3110(pure, p, bw, cm, lds, s);
+    case 3120: return launch_jkb_3120(pure, p, bw, cm, lds, s);
+    case 3200: return launch_jkb_3200(pure, p, bw, cm, lds, s);
+    case 3210: return launch_jkb_3210(pure, p, bw, cm, lds, s);
+    case 3300: return launch_jkb_3300(pure, p, bw, cm, lds, s);
     default: return false;
   }
 }
@@ -162,6 +222,18 @@ bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* stat
     case 2220: return jkb_info_2220(pure, nrow, static_lds, lds_max);
     case 2221: return jkb_info_2221(pure, nrow, static_lds, lds_max);
     case 2222: return jkb_info_2222(pure, nrow, static_lds, lds_max);
+    case 3000: return jkb_info_3000(pure, nrow, static_lds, lds_max);
+    case 3010: return jkb_info_3010(pure, nrow, static_lds, lds_max);
+    case 3011: return jkb_info_3011(pure, nrow, static_lds, lds_max);
+    case 3020: return jkb_info_3020(pure, nrow, static_lds, lds_max);
+    case 3021: return jkb_info_3021(pure, nrow, static_lds, lds_max);
+    case 3030: return jkb_info_3030(pure, nrow, static_lds, lds_max);
+    case 3100: return jkb_info_3100(pure, nrow, static_lds, lds_max);
+    case 3110: return jkb_info_3110(pure, nrow, static_lds, lds_max);
+    case 3120: return jkb_info_3120(pure, nrow, static_lds, lds_max);
+    case 3200: return jkb_info_3200(pure, nrow, static_lds, lds_max);
+    case 3210: return jkb_info_3210(pure, nrow, static_lds, lds_max);
+    case 3300: return jkb_info_3300(pure, nrow, static_lds, lds_max);
     default: return false;
   }
 }
@@ -194,13 +266,18 @@ void launch_grad_2100(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2110(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2111(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2120(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2121(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2200(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2210(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2211(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2220(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2221(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2222(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3000(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3010(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3011(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3020(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3030(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3100(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3110(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3200(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_3300(const GradParams&, const WaveWork&, hipStream_t);
 
 bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s) {
   if (ww.nwaves == 0) return true;
@@ -219,13 +296,18 @@ bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const Wave
     case 2110: launch_grad_2110(p, ww, s); return true;
     case 2111: launch_grad_2111(p, ww, s); return true;
     case 2120: launch_grad_2120(p, ww, s); return true;
-    case 2121: launch_grad_2121(p, ww, s); return true;
     case 2200: launch_grad_2200(p, ww, s); return true;
     case 2210: launch_grad_2210(p, ww, s); return true;
-    case 2211: launch_grad_2211(p, ww, s); return true;
     case 2220: launch_grad_2220(p, ww, s); return true;
-    case 2221: launch_grad_2221(p, ww, s); return true;
-    case 2222: launch_grad_2222(p, ww, s); return true;
+    case 3000: launch_grad_3000(p, ww, s); return true;
+    case 3010: launch_grad_3010(p, ww, s); return true;
+    case 3011: launch_grad_3011(p, ww, s); return true;
+    case 3020: launch_grad_3020(p, ww, s); return true;
+    case 3030: launch_grad_3030(p, ww, s); return true;
+    case 3100: launch_grad_3100(p, ww, s); return true;
+    case 3110: launch_grad_3110(p, ww, s); return true;
+    case 3200: launch_grad_3200(p, ww, s); return true;
+    case 3300: launch_grad_3300(p, ww, s); return true;
     default: return false;
   }
 }
@@ -265,6 +347,50 @@ void launch_cart_density(const double* D, int nsh, const int* am, const int* pur
 }
 
 bool kernels_support(int lmax) { return lmax <= 3; }
+
+// classes with an f shell that have a class-templated kernel (the others run
+// on the run-time-class kernels k_jkh / k_gradh)
+bool jk_template(int la, int lb, int lc, int ld) {
+  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
+    case 3000:
+    case 3010:
+    case 3011:
+    case 3020:
+    case 3021:
+    case 3030:
+    case 3100:
+    case 3110:
+    case 3120:
+    case 3200:
+    case 3210:
+    case 3300:
+      return true;
+    default: return la < 3 && lc < 3;
+  }
+}
+// (dd|dd), (dd|dp), (dd|pp) and (dp|dp) are also left to k_gradh: their
+// templates would need 5-17 KB of spill scratch per lane, more than the
+// runtime can always provide next to the other kernels' scratch
+bool grad_template(int la, int lb, int lc, int ld) {
+  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
+    case 2222:
+    case 2221:
+    case 2211:
+    case 2121:
+      return false;
+    case 3000:
+    case 3010:
+    case 3011:
+    case 3020:
+    case 3030:
+    case 3100:
+    case 3110:
+    case 3200:
+    case 3300:
+      return true;
+    default: return la < 3 && lc < 3;
+  }
+}
 
 
 // shell-pair density maxima (shlden, source/integrals/int2.F90:999-1047)

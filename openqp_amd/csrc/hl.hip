@@ -45,10 +45,18 @@ HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm) {
     if (ld >= 2 && h.od != h.nd) write(oa * ob * h.nc * h.od);
     if (lc >= 2 && h.oc != h.nc) write(oa * ob * h.oc * h.od);
   }
+  // gradient: 2D integrals transferred to every exponent the derivatives use
+  h.nt = grad ? (la + 2) * (lb + 2) * (lc + 2) * (ld + 1) : 0;
+  const int nT = 3 * h.nt;
+  const int nC = 123;  // HRR coefficient tables
+  const int nI = h.ne + h.nf + h.na * h.nb + h.nc * h.nd + h.nt;  // int tables
   h.offG = 0;
   h.offA = nG;
-  h.offB = nG + sA;
-  h.lds = (int)sizeof(double) * (nG + sA + sB);
+  h.offB = h.offA + sA;
+  h.offT = h.offB + sB;
+  h.offC = h.offT + nT;
+  h.offI = h.offC + nC;
+  h.lds = (int)sizeof(double) * (h.offI + (nI + 1) / 2);
   return h;
 }
 

@@ -1,11 +1,13 @@
-// hl_device.h -- shell quartets of classes that contain an f shell (any
-// L <= 3), evaluated by one wavefront per quartet with every intermediate in
-// LDS, sized per class at launch (HLDims).  Covers the Fock build, the
-// Schwarz bounds and the 2e gradient for those classes; the algorithm is the
-// one of the register kernels (Rys quadrature, contracted (e0|f0), binomial
-// HRR, AO transform, OpenQP element cutoff and digestion), with loop bounds
-// read at run time instead of unrolled, so (ff|ff) needs no register file
-// larger than the hardware's.
+// hl_device.h -- shell quartets of the classes with an f shell that have no
+// class-templated kernel (the larger ones, up to (ff|ff)), evaluated by one
+// wavefront per quartet with every intermediate in LDS, sized per class at
+// launch (HLDims).  Covers the Fock build, the Schwarz bounds and the 2e
+// gradient for those classes; the algorithm is the one of the register
+// kernels (Rys quadrature, contracted (e0|f0), binomial HRR, AO transform,
+// OpenQP element cutoff and digestion), with loop bounds read at run time
+// instead of unrolled, so (ff|ff) needs no register file larger than the
+// hardware's.  Component exponents come from per-class tables and the HRR
+// binomial-power coefficients from per-quartet tables, both in LDS.
 //
 // Reference: int2_rys_compute (source/integrals/int_rys.F90:156-785) for the
 // integrals, int2.F90:763-800 / :1806-1812 for the screens and element cutoff,
@@ -21,18 +23,6 @@ namespace oqph {
 #define HL_BLOCK 64  // one wavefront per block
 
 // ---------------------------------------------------------- small helpers
-// Cartesian exponents of entry e of the level range that starts at L0
-__device__ __forceinline__ void hl_comp(int L0, int e, int& x, int& y, int& z) {
-  int L = L0;
-  while (e >= ncart(L)) { e -= ncart(L); ++L; }
-  x = c_lx(L, e);
-  y = c_ly(L, e);
-  z = c_lz(L, e);
-}
-// index of (x, y, z) inside the level range that starts at L0
-__device__ __forceinline__ int hl_eidx(int L0, int x, int y, int z) {
-  return ncum(x + y + z - 1) - ncum(L0 - 1) + c_idx(x, y, z);
-}
 __device__ __forceinline__ double hl_pow(double v, int n) {
   double r = 1.0;
   for (int i = 0; i < n; ++i) r *= v;
@@ -42,6 +32,10 @@ __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
   return v;
+}
+// index of (x, y, z) inside the level range that starts at L0
+__device__ __forceinline__ int hl_eidx(int L0, int x, int y, int z) {
+  return ncum(x + y + z - 1) - ncum(L0 - 1) + c_idx(x, y, z);
 }
 // one node/weight pair of the nr-root quadrature, nr known at run time
 __device__ __forceinline__ void hl_root(int nr, double T, const double* __restrict__ rys, int r, double& x,
@@ -58,9 +52,93 @@ __device__ __forceinline__ void hl_root(int nr, double T, const double* __restri
   }
 }
 
-// Scalars of one primitive quartet shared by every lane.
+// Two-level index (i, j) of a flat index stepped by the wavefront width:
+// one division at the start, then additions (j runs fastest, j < n2).
+struct HLStep {
+  int i, j, q, r, n2;
+  __device__ __forceinline__ HLStep(int start, int n) : n2(n) {
+    i = start / n;
+    j = start - i * n;
+    q = HL_BLOCK / n;
+    r = HL_BLOCK - q * n;
+  }
+  __device__ __forceinline__ void next() {
+    i += q;
+    j += r;
+    if (j >= n2) {
+      j -= n2;
+      ++i;
+    }
+  }
+};
+
+// Per-class tables (ints, LDS), filled once per block:
+//   eT[ne], fT[nf]     exponents of the (e0| and |f0) level ranges, x | y<<4 | z<<8
+//   abT[na nb], cdT[nc nd]  a | b<<4 | ax<<8 | ay<<12 | az<<16 | bx<<20 | by<<24 | bz<<28
+//   tT[nt]             gradient transferred-table entries ia | ib<<4 | ic<<8 | id<<12
+struct HLTabs {
+  const int *eT, *fT, *abT, *cdT, *tT;
+};
+__device__ __forceinline__ HLTabs hl_tables(const HLDims& h, double* lds) {
+  int* b = (int*)(lds + h.offI);
+  HLTabs t;
+  int* eT = b;
+  int* fT = eT + h.ne;
+  int* abT = fT + h.nf;
+  int* cdT = abT + h.na * h.nb;
+  int* tT = cdT + h.nc * h.nd;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < h.ne; e += HL_BLOCK) {
+    int L = h.la, c = e;
+    while (c >= ncart(L)) { c -= ncart(L); ++L; }
+    eT[e] = c_lx(L, c) | (c_ly(L, c) << 4) | (c_lz(L, c) << 8);
+  }
+  for (int f = lane; f < h.nf; f += HL_BLOCK) {
+    int L = h.lc, c = f;
+    while (c >= ncart(L)) { c -= ncart(L); ++L; }
+    fT[f] = c_lx(L, c) | (c_ly(L, c) << 4) | (c_lz(L, c) << 8);
+  }
+  for (int t = lane; t < h.na * h.nb; t += HL_BLOCK) {
+    const int a = t / h.nb, bb = t - a * h.nb;
+    abT[t] = a | (bb << 4) | (c_lx(h.la, a) << 8) | (c_ly(h.la, a) << 12) | (c_lz(h.la, a) << 16) |
+             (c_lx(h.lb, bb) << 20) | (c_ly(h.lb, bb) << 24) | (c_lz(h.lb, bb) << 28);
+  }
+  for (int t = lane; t < h.nc * h.nd; t += HL_BLOCK) {
+    const int c = t / h.nd, d = t - c * h.nd;
+    cdT[t] = c | (d << 4) | (c_lx(h.lc, c) << 8) | (c_ly(h.lc, c) << 12) | (c_lz(h.lc, c) << 16) |
+             (c_lx(h.ld, d) << 20) | (c_ly(h.ld, d) << 24) | (c_lz(h.ld, d) << 28);
+  }
+  const int NTb = h.lb + 2, NTc = h.lc + 2, NTd = h.ld + 1;
+  for (int t = lane; t < h.nt; t += HL_BLOCK) {
+    const int ia = t / (NTb * NTc * NTd), r1 = t - ia * NTb * NTc * NTd, ib = r1 / (NTc * NTd),
+              r2 = r1 - ib * NTc * NTd, ic = r2 / NTd, id = r2 - ic * NTd;
+    tT[t] = ia | (ib << 4) | (ic << 8) | (id << 12);
+  }
+  __syncthreads();
+  t.eT = eT; t.fT = fT; t.abT = abT; t.cdT = cdT; t.tT = tT;
+  return t;
+}
+
+// HRR coefficients of one quartet (LDS, 123 doubles):
+//   cab[x][b][j] = C(b, j) AB_x^(b-j), b <= 4;  ccd[x][d][m] = C(d, m) CD_x^(d-m), d <= 3
+__device__ __forceinline__ void hl_coef(const ShellPair& P, const ShellPair& Q, double* C) {
+  for (int t = threadIdx.x; t < 123; t += HL_BLOCK) {  // 123 entries over the 64 lanes
+    if (t < 75) {
+      const int x = t / 25, b = (t - 25 * x) / 5, j = t - 25 * x - 5 * b;
+      const double v = x == 0 ? P.A[0] - P.B[0] : (x == 1 ? P.A[1] - P.B[1] : P.A[2] - P.B[2]);
+      C[t] = j <= b ? (double)binom(b, j) * hl_pow(v, b - j) : 0.0;
+    } else {
+      const int u = t - 75, x = u / 16, d = (u - 16 * x) / 4, m = u - 16 * x - 4 * d;
+      const double v = x == 0 ? Q.A[0] - Q.B[0] : (x == 1 ? Q.A[1] - Q.B[1] : Q.A[2] - Q.B[2]);
+      C[t] = m <= d ? (double)binom(d, m) * hl_pow(v, d - m) : 0.0;
+    }
+  }
+}
+
+// Scalars of one primitive quartet shared by every lane (scalar members, so
+// no lane indexes a private array with its run-time axis).
 struct HLPrim {
-  double PQ[3], PA[3], QC[3];
+  double PQx, PQy, PQz, PAx, PAy, PAz, QCx, QCy, QCz;
   double T, pref, rz, re, h2z, h2e, h2ze;
 };
 __device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q, const PrimPair& pp,
@@ -68,15 +146,10 @@ __device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q
   HLPrim s;
   const double zeta = pp.g, eta = qq.g, ze = zeta + eta;
   const double sq = rsq64(ze), ize = sq * sq, rho = zeta * eta * ize;
-  double pq2 = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    s.PQ[d] = pp.P[d] - qq.P[d];
-    s.PA[d] = pp.P[d] - P.A[d];
-    s.QC[d] = qq.P[d] - Q.A[d];
-    pq2 += s.PQ[d] * s.PQ[d];
-  }
-  s.T = rho * pq2;
+  s.PQx = pp.P[0] - qq.P[0]; s.PQy = pp.P[1] - qq.P[1]; s.PQz = pp.P[2] - qq.P[2];
+  s.PAx = pp.P[0] - P.A[0]; s.PAy = pp.P[1] - P.A[1]; s.PAz = pp.P[2] - P.A[2];
+  s.QCx = qq.P[0] - Q.A[0]; s.QCy = qq.P[1] - Q.A[1]; s.QCz = qq.P[2] - Q.A[2];
+  s.T = rho * (s.PQx * s.PQx + s.PQy * s.PQy + s.PQz * s.PQz);
   s.pref = pp.Kg * qq.Kg * sq;
   s.rz = eta * ize;
   s.re = zeta * ize;
@@ -87,36 +160,58 @@ __device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q
 }
 
 // 2D integrals of one primitive quartet by the vertical recursion: lane
-// r < nr fills G[(r*3 + d)][i][k], i < ni, k < nk, for root r and the three
-// axes d (the x table carries the prefactor and the weight).
+// r < nr fills G[(r*3 + d)][i][k], i < ni <= 8, k < nk, for root r and the
+// three axes d (the x table carries the prefactor and the weight); the last
+// two ket columns stay in registers.  The axis loop is unrolled, so every
+// axis component is a compile-time field of HLPrim.
+template <int D>
+__device__ __forceinline__ void hl_vrr_axis(const HLDims& h, const HLPrim& s, double t2, double wr, double* g) {
+  const double B00 = s.h2ze * t2, B10 = s.h2z - s.h2z * s.rz * t2, B01 = s.h2e - s.h2e * s.re * t2;
+  const double pa = D == 0 ? s.PAx : (D == 1 ? s.PAy : s.PAz);
+  const double pq = D == 0 ? s.PQx : (D == 1 ? s.PQy : s.PQz);
+  const double qc = D == 0 ? s.QCx : (D == 1 ? s.QCy : s.QCz);
+  const double C00 = pa - s.rz * t2 * pq;
+  const double D00 = qc + s.re * t2 * pq;
+  const int ni = h.ni, nk = h.nk;
+  double c1[8], c2[8];
+  c1[0] = (D == 0) ? s.pref * wr : 1.0;
+  g[0] = c1[0];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) {
+    double v = C00 * c1[i - 1];
+    if (i > 1) v += (double)(i - 1) * B10 * c1[i - 2];
+    c1[i] = v;
+    if (i < ni) g[i * nk] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) c2[i] = 0.0;
+  for (int k = 1; k < nk; ++k) {
+    const double bk = (double)(k - 1) * B01;
+    double cn[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      double v = D00 * c1[i] + bk * c2[i];
+      if (i > 0) v += (double)i * B00 * c1[i - 1];
+      cn[i] = v;
+      if (i < ni) g[i * nk + k] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      c2[i] = c1[i];
+      c1[i] = cn[i];
+    }
+  }
+}
 __device__ __forceinline__ void hl_vrr(const HLDims& h, const HLPrim& s, const double* __restrict__ rys, double* G) {
   const int r = threadIdx.x;
   if (r >= h.nr) return;
   double t2, wr;
   hl_root(h.nr, s.T, rys, r, t2, wr);
-  const double B00 = s.h2ze * t2, B10 = s.h2z - s.h2z * s.rz * t2, B01 = s.h2e - s.h2e * s.re * t2;
-  const int ni = h.ni, nk = h.nk;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double C00 = s.PA[d] - s.rz * t2 * s.PQ[d];
-    const double D00 = s.QC[d] + s.re * t2 * s.PQ[d];
-    double* g = G + (3 * r + d) * ni * nk;
-    g[0] = (d == 0) ? s.pref * wr : 1.0;
-    double gm2 = 0.0, gm1 = g[0];
-    for (int i = 1; i < ni; ++i) {
-      const double v = C00 * gm1 + (double)(i - 1) * B10 * gm2;
-      g[i * nk] = v;
-      gm2 = gm1;
-      gm1 = v;
-    }
-    for (int k = 1; k < nk; ++k)
-      for (int i = 0; i < ni; ++i) {
-        double v = D00 * g[i * nk + k - 1];
-        if (k > 1) v += (double)(k - 1) * B01 * g[i * nk + k - 2];
-        if (i > 0) v += (double)i * B00 * g[(i - 1) * nk + k - 1];
-        g[i * nk + k] = v;
-      }
-  }
+  const int nrk = h.ni * h.nk;
+  double* g = G + 3 * r * nrk;
+  hl_vrr_axis<0>(h, s, t2, wr, g);
+  hl_vrr_axis<1>(h, s, t2, wr, g + nrk);
+  hl_vrr_axis<2>(h, s, t2, wr, g + 2 * nrk);
 }
 
 // one index of a 4-index block through the canonical-Cartesian -> AO
@@ -160,16 +255,18 @@ __device__ __forceinline__ void hl_xform_inplace(double* buf, int left, int n, i
 // pointer); nprim = surviving primitive quartets, 0 when none survived (the
 // block is then not formed).  Primitive screens as contract(): |K| < qcut
 // per pair and K_ab^2 K_cd^2 < qcut^2 (zeta + eta) per quartet.
-__device__ double* hl_block(const HLDims& h, const ShellPair& P, const ShellPair& Q, const PrimPair* prims,
-                            const double* __restrict__ rys, const double* __restrict__ xf, double qcut, double* lds,
-                            int& nprim) {
+__device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& P, const ShellPair& Q,
+                            const PrimPair* prims, const double* __restrict__ rys, const double* __restrict__ xf,
+                            double qcut, double* lds, int& nprim) {
   const int lane = threadIdx.x;
   double* G = lds + h.offG;
   double* bufA = lds + h.offA;
   double* bufB = lds + h.offB;
-  const int nef = h.ne * h.nf, nrk = h.ni * h.nk;
+  double* C = lds + h.offC;
+  const int nef = h.ne * h.nf, nrk = h.ni * h.nk, nk = h.nk, nf = h.nf;
   __syncthreads();  // the previous quartet's readers of LDS are done
   for (int i = lane; i < nef; i += HL_BLOCK) bufA[i] = 0.0;
+  hl_coef(P, Q, C);
   nprim = 0;
   const double qcut2 = qcut * qcut;
   for (int iq = 0; iq < Q.nprim; ++iq) {
@@ -185,15 +282,15 @@ __device__ double* hl_block(const HLDims& h, const ShellPair& P, const ShellPair
       __syncthreads();
       hl_vrr(h, s, rys, G);
       __syncthreads();
-      for (int idx = lane; idx < nef; idx += HL_BLOCK) {
-        const int e = idx / h.nf, f = idx - e * h.nf;
-        int ex, ey, ez, fx, fy, fz;
-        hl_comp(h.la, e, ex, ey, ez);
-        hl_comp(h.lc, f, fx, fy, fz);
+      HLStep st(lane, nf);
+      for (int idx = lane; idx < nef; idx += HL_BLOCK, st.next()) {
+        const int pe = tb.eT[st.i], pf = tb.fT[st.j];
+        const int ox = (pe & 15) * nk + (pf & 15), oy = nrk + ((pe >> 4) & 15) * nk + ((pf >> 4) & 15),
+                  oz = 2 * nrk + (pe >> 8) * nk + (pf >> 8);
         double v = 0.0;
         for (int r = 0; r < h.nr; ++r) {
           const double* g = G + 3 * r * nrk;
-          v += g[ex * h.nk + fx] * g[nrk + ey * h.nk + fy] * g[2 * nrk + ez * h.nk + fz];
+          v += g[ox] * g[oy] * g[oz];
         }
         bufA[idx] += v;
       }
@@ -201,27 +298,23 @@ __device__ double* hl_block(const HLDims& h, const ShellPair& P, const ShellPair
   }
   if (nprim == 0) return nullptr;
   __syncthreads();
-  double AB[3], CD[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    AB[d] = P.A[d] - P.B[d];
-    CD[d] = Q.A[d] - Q.B[d];
-  }
+  const double* cab = C;
+  const double* ccd = C + 75;
   // bra HRR: Y[a][b][f] = sum_j prod_d C(b_d, j_d) AB_d^(b_d - j_d) acc[a + j][f]
   {
-    const int n = h.na * h.nb * h.nf;
-    for (int idx = lane; idx < n; idx += HL_BLOCK) {
-      const int a = idx / (h.nb * h.nf), rem = idx - a * h.nb * h.nf, b = rem / h.nf, f = rem - b * h.nf;
-      const int ax = c_lx(h.la, a), ay = c_ly(h.la, a), az = c_lz(h.la, a);
-      const int bx = c_lx(h.lb, b), by = c_ly(h.lb, b), bz = c_lz(h.lb, b);
+    const int n = h.na * h.nb * nf;
+    HLStep st(lane, nf);
+    for (int idx = lane; idx < n; idx += HL_BLOCK, st.next()) {
+      const int p = tb.abT[st.i], f = st.j;
+      const int ax = (p >> 8) & 15, ay = (p >> 12) & 15, az = (p >> 16) & 15;
+      const int bx = (p >> 20) & 15, by = (p >> 24) & 15, bz = (p >> 28) & 15;
       double v = 0.0;
       for (int jx = 0; jx <= bx; ++jx)
-        for (int jy = 0; jy <= by; ++jy)
-          for (int jz = 0; jz <= bz; ++jz) {
-            const double cf = (double)(binom(bx, jx) * binom(by, jy) * binom(bz, jz)) * hl_pow(AB[0], bx - jx) *
-                              hl_pow(AB[1], by - jy) * hl_pow(AB[2], bz - jz);
-            v += cf * bufA[hl_eidx(h.la, ax + jx, ay + jy, az + jz) * h.nf + f];
-          }
+        for (int jy = 0; jy <= by; ++jy) {
+          const double cxy = cab[bx * 5 + jx] * cab[25 + by * 5 + jy];
+          for (int jz = 0; jz <= bz; ++jz)
+            v += cxy * cab[50 + bz * 5 + jz] * bufA[hl_eidx(h.la, ax + jx, ay + jy, az + jz) * nf + f];
+        }
       bufB[idx] = v;
     }
   }
@@ -236,9 +329,9 @@ __device__ double* hl_block(const HLDims& h, const ShellPair& P, const ShellPair
   if (h.lb >= 2) {
     __syncthreads();
     if (h.ob == h.nb) {
-      hl_xform_inplace(cur, h.na, h.nb, h.nf, h.lb, h.pure, xf);
+      hl_xform_inplace(cur, h.na, h.nb, nf, h.lb, h.pure, xf);
     } else {
-      hl_xform(cur, oth, h.na, h.nb, h.ob, h.nf, h.lb, h.pure, xf);
+      hl_xform(cur, oth, h.na, h.nb, h.ob, nf, h.lb, h.pure, xf);
       flip();
     }
     ob = h.ob;
@@ -246,9 +339,9 @@ __device__ double* hl_block(const HLDims& h, const ShellPair& P, const ShellPair
   if (h.la >= 2) {
     __syncthreads();
     if (h.oa == h.na) {
-      hl_xform_inplace(cur, 1, h.na, ob * h.nf, h.la, h.pure, xf);
+      hl_xform_inplace(cur, 1, h.na, ob * nf, h.la, h.pure, xf);
     } else {
-      hl_xform(cur, oth, 1, h.na, h.oa, ob * h.nf, h.la, h.pure, xf);
+      hl_xform(cur, oth, 1, h.na, h.oa, ob * nf, h.la, h.pure, xf);
       flip();
     }
     oa = h.oa;
@@ -257,18 +350,19 @@ __device__ double* hl_block(const HLDims& h, const ShellPair& P, const ShellPair
   __syncthreads();
   {
     const int ncd = h.nc * h.nd, n = oa * ob * ncd;
-    for (int idx = lane; idx < n; idx += HL_BLOCK) {
-      const int ab = idx / ncd, rem = idx - ab * ncd, c = rem / h.nd, d = rem - c * h.nd;
-      const int cx = c_lx(h.lc, c), cy = c_ly(h.lc, c), cz = c_lz(h.lc, c);
-      const int dx = c_lx(h.ld, d), dy = c_ly(h.ld, d), dz = c_lz(h.ld, d);
+    HLStep st(lane, ncd);
+    for (int idx = lane; idx < n; idx += HL_BLOCK, st.next()) {
+      const int p = tb.cdT[st.j];
+      const int cx = (p >> 8) & 15, cy = (p >> 12) & 15, cz = (p >> 16) & 15;
+      const int dx = (p >> 20) & 15, dy = (p >> 24) & 15, dz = (p >> 28) & 15;
+      const double* y = cur + st.i * nf;
       double v = 0.0;
       for (int mx = 0; mx <= dx; ++mx)
-        for (int my = 0; my <= dy; ++my)
-          for (int mz = 0; mz <= dz; ++mz) {
-            const double cf = (double)(binom(dx, mx) * binom(dy, my) * binom(dz, mz)) * hl_pow(CD[0], dx - mx) *
-                              hl_pow(CD[1], dy - my) * hl_pow(CD[2], dz - mz);
-            v += cf * cur[ab * h.nf + hl_eidx(h.lc, cx + mx, cy + my, cz + mz)];
-          }
+        for (int my = 0; my <= dy; ++my) {
+          const double cxy = ccd[dx * 4 + mx] * ccd[16 + dy * 4 + my];
+          for (int mz = 0; mz <= dz; ++mz)
+            v += cxy * ccd[32 + dz * 4 + mz] * y[hl_eidx(h.lc, cx + mx, cy + my, cz + mz)];
+        }
       oth[idx] = v;
     }
     flip();
@@ -301,6 +395,7 @@ __device__ double* hl_block(const HLDims& h, const ShellPair& P, const ShellPair
 // digests the surviving quartets one after another, all 64 lanes on each.
 __global__ __launch_bounds__(HL_BLOCK) void k_jkh(JKParams prm, WaveWork ww, HLDims h) {
   extern __shared__ double lds[];
+  const HLTabs tb = hl_tables(h, lds);
   const long long wave = blockIdx.x;
   if (wave >= ww.nwaves) return;  // block-uniform
   int lo = 0, hi = ww.nseg;
@@ -345,7 +440,7 @@ __global__ __launch_bounds__(HL_BLOCK) void k_jkh(JKParams prm, WaveWork ww, HLD
     const int jq = __builtin_amdgcn_readfirstlane(__shfl(iq, l));
     const ShellPair Q = prm.pairs[jq];
     int nprim = 0;
-    const double* V = hl_block(h, P, Q, prm.prims, prm.rys, prm.xform, 1.0e-4 * cut, lds, nprim);
+    const double* V = hl_block(h, tb, P, Q, prm.prims, prm.rys, prm.xform, 1.0e-4 * cut, lds, nprim);
     prims_used += (unsigned long long)nprim;
     if (nprim == 0) continue;
     flops += (unsigned long long)(nprim * fprim + 12 * nout);
@@ -421,11 +516,12 @@ __global__ __launch_bounds__(HL_BLOCK) void k_schwarzh(const ShellPair* pairs, c
                                                        int n, const double* rys, const double* xf, double qcut,
                                                        double* Qout, HLDims h) {
   extern __shared__ double lds[];
+  const HLTabs tb = hl_tables(h, lds);
   if ((int)blockIdx.x >= n) return;
   const int ip = list[blockIdx.x];
   const ShellPair P = pairs[ip];
   int nprim = 0;
-  const double* V = hl_block(h, P, P, prims, rys, xf, qcut, lds, nprim);
+  const double* V = hl_block(h, tb, P, P, prims, rys, xf, qcut, lds, nprim);
   double m = 0.0;
   if (nprim) {
     const int nv = h.oa * h.ob * h.oc * h.od;
@@ -436,24 +532,15 @@ __global__ __launch_bounds__(HL_BLOCK) void k_schwarzh(const ShellPair* pairs, c
 }
 
 // --------------------------------------------------------------- gradient
-// runtime form of the 2D integral with bra and ket transfer:
-//   I(a,b,c,d) = sum_j sum_m C(b,j) AB^(b-j) C(d,m) CD^(d-m) G(a+j, c+m)
-__device__ __forceinline__ double hl_i2d(const double* g, int nk, int a, int b, int c, int d, double ab, double cd) {
-  double s = 0.0;
-  for (int j = 0; j <= b; ++j) {
-    double t = 0.0;
-    for (int m = 0; m <= d; ++m) t += (double)binom(d, m) * hl_pow(cd, d - m) * g[(a + j) * nk + c + m];
-    s += (double)binom(b, j) * hl_pow(ab, b - j) * t;
-  }
-  return s;
-}
-
 // Gradient wavefront tasks (coarse-screened lists): a block takes one chunk
 // of kets of one bra; per quartet the lanes form the Cartesian Gamma block
-// in LDS, apply the fine screen, and contract it with the derivative
-// integrals of their Gamma entries; per-atom sums go through LDS.
+// in LDS and apply the fine screen; per primitive quartet and root they
+// transfer the 2D integrals to every (a, b, c, d) exponent the derivatives
+// need (T, LDS) and contract them with their Gamma entries; per-atom sums
+// go through LDS.
 __global__ __launch_bounds__(HL_BLOCK) void k_gradh(GradParams prm, WaveWork ww, HLDims h) {
   extern __shared__ double lds[];
+  const HLTabs tb = hl_tables(h, lds);
   double* s_de = lds + h.offB;  // 3 natm
   for (int i = threadIdx.x; i < 3 * prm.natm; i += HL_BLOCK) s_de[i] = 0.0;
   __syncthreads();
@@ -475,29 +562,39 @@ __global__ __launch_bounds__(HL_BLOCK) void k_gradh(GradParams prm, WaveWork ww,
   unsigned long long todo = __ballot(active);
   double* G = lds + h.offG;
   double* Gam = lds + h.offA;
-  const int NA = h.na, NB = h.nb, NC = h.nc, ND = h.nd, NG = NA * NB * NC * ND;
+  double* T = lds + h.offT;
+  double* C = lds + h.offC;
+  const double* cab = C;
+  const double* ccd = C + 75;
+  const int NCD = h.nc * h.nd, NG = h.na * h.nb * NCD;
   const int n = prm.ncf;
   const double* Dc = prm.Dc;
   const int oa = prm.coff[P.sa], ob = prm.coff[P.sb];
   const long long fprim = flops_grad_prim_rt(h.la, h.lb, h.lc, h.ld);
   unsigned long long ncomp = 0ull, nskip = 0ull, flops = 0ull, prims_used = 0ull;
-  const int nrk = h.ni * h.nk;
+  const int nrk = h.ni * h.nk, nk = h.nk, nt = h.nt;
+  const int NTb = h.lb + 2, NTc = h.lc + 2, NTd = h.ld + 1;
+  const int sa = NTb * NTc * NTd, sb = NTc * NTd, sc = NTd;  // T strides of ia, ib, ic (id: 1)
+  const int lab1 = h.la + h.lb + 1, lcd1 = h.lc + h.ld + 1;
   while (todo) {
     const int l = __ffsll((long long)todo) - 1;
     todo &= todo - 1ull;
     const int jq = __builtin_amdgcn_readfirstlane(__shfl(iq, l));
     const ShellPair Q = prm.pairs[jq];
     const int oc = prm.coff[Q.sa], od = prm.coff[Q.sb];
-    __syncthreads();  // the previous quartet's readers of Gamma are done
+    __syncthreads();  // the previous quartet's readers of Gamma, C and T are done
     double gm = 0.0;
-    for (int t = lane; t < NG; t += HL_BLOCK) {
-      const int a = t / (NB * NC * ND), r1 = t - a * NB * NC * ND, b = r1 / (NC * ND), r2 = r1 - b * NC * ND,
-                c = r2 / ND, d = r2 - c * ND;
-      const double g = 4.0 * prm.cs * Dc[(oa + a) * n + ob + b] * Dc[(oc + c) * n + od + d] -
-                       prm.hs * (Dc[(oa + a) * n + oc + c] * Dc[(ob + b) * n + od + d] +
-                                 Dc[(oa + a) * n + od + d] * Dc[(ob + b) * n + oc + c]);
-      Gam[t] = g;
-      gm = fmax(gm, fabs(g));
+    {
+      HLStep st(lane, NCD);
+      for (int t = lane; t < NG; t += HL_BLOCK, st.next()) {
+        const int pab = tb.abT[st.i], pcd = tb.cdT[st.j];
+        const int a = pab & 15, b = (pab >> 4) & 15, c = pcd & 15, d = (pcd >> 4) & 15;
+        const double g = 4.0 * prm.cs * Dc[(oa + a) * n + ob + b] * Dc[(oc + c) * n + od + d] -
+                         prm.hs * (Dc[(oa + a) * n + oc + c] * Dc[(ob + b) * n + od + d] +
+                                   Dc[(oa + a) * n + od + d] * Dc[(ob + b) * n + oc + c]);
+        Gam[t] = g;
+        gm = fmax(gm, fabs(g));
+      }
     }
     gm = wave_max(gm);
     // fine screen on the weighted contribution (grd2.F90:348)
@@ -506,15 +603,10 @@ __global__ __launch_bounds__(HL_BLOCK) void k_gradh(GradParams prm, WaveWork ww,
       continue;
     }
     ++ncomp;
+    hl_coef(P, Q, C);
     const double dab2 = gm * gm;
-    double fd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double f0x = 0, f0y = 0, f0z = 0, f1x = 0, f1y = 0, f1z = 0, f2x = 0, f2y = 0, f2z = 0;
     int nprim = 0;
-    double AB[3], CD[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      AB[d] = P.A[d] - P.B[d];
-      CD[d] = Q.A[d] - Q.B[d];
-    }
     for (int iqq = 0; iqq < Q.nprim; ++iqq) {
       const PrimPair qq = load_uniform(prm.prims, Q.prim0 + iqq);
       for (int ipp = 0; ipp < P.nprim; ++ipp) {
@@ -525,46 +617,52 @@ __global__ __launch_bounds__(HL_BLOCK) void k_gradh(GradParams prm, WaveWork ww,
         if (fqz * fqz * dab2 < prm.dabcut2 * ze) continue;
         ++nprim;
         const HLPrim s = hl_prim(P, Q, pp, qq);
-        __syncthreads();
+        __syncthreads();  // the previous primitive's readers of G and T are done
         hl_vrr(h, s, prm.rys, G);
-        __syncthreads();
         const double a2 = 2.0 * pp.a, b2 = 2.0 * pp.b, c2 = 2.0 * qq.a;
-        for (int t = lane; t < NG; t += HL_BLOCK) {
-          const int a = t / (NB * NC * ND), r1 = t - a * NB * NC * ND, b = r1 / (NC * ND), r2 = r1 - b * NC * ND,
-                    c = r2 / ND, dd = r2 - c * ND;
-          const double g = Gam[t];
-          if (g == 0.0) continue;
-          int ia[3], ib[3], ic[3], id[3];
-          ia[0] = c_lx(h.la, a); ia[1] = c_ly(h.la, a); ia[2] = c_lz(h.la, a);
-          ib[0] = c_lx(h.lb, b); ib[1] = c_ly(h.lb, b); ib[2] = c_lz(h.lb, b);
-          ic[0] = c_lx(h.lc, c); ic[1] = c_ly(h.lc, c); ic[2] = c_lz(h.lc, c);
-          id[0] = c_lx(h.ld, dd); id[1] = c_ly(h.ld, dd); id[2] = c_lz(h.ld, dd);
-          for (int r = 0; r < h.nr; ++r) {
+        for (int r = 0; r < h.nr; ++r) {
+          __syncthreads();  // G written, or the previous root's readers of T done
+          // T[x][ia][ib][ic][id] = sum_j sum_m C(ib,j) AB^(ib-j) C(id,m) CD^(id-m) G_x(ia+j, ic+m)
+          {
+            HLStep st(lane, nt);
+            for (int idx = lane; idx < 3 * nt; idx += HL_BLOCK, st.next()) {
+              const int x = st.i, pt = tb.tT[st.j];
+              const int ia = pt & 15, ib = (pt >> 4) & 15, ic = (pt >> 8) & 15, id = (pt >> 12) & 15;
+              double v = 0.0;
+              if (ia + ib <= lab1 && ic + id <= lcd1) {
+                const double* g = G + (3 * r + x) * nrk;
+                const double* cb = cab + 25 * x + 5 * ib;
+                const double* cd = ccd + 16 * x + 4 * id;
+                for (int j = 0; j <= ib; ++j) {
+                  double u = 0.0;
+                  for (int m = 0; m <= id; ++m) u += cd[m] * g[(ia + j) * nk + ic + m];
+                  v += cb[j] * u;
+                }
+              }
+              T[idx] = v;
+            }
+          }
+          __syncthreads();
+          HLStep st(lane, NCD);
+          for (int t = lane; t < NG; t += HL_BLOCK, st.next()) {
+            const double g = Gam[t];
+            if (g == 0.0) continue;
+            const int pab = tb.abT[st.i], pcd = tb.cdT[st.j];
             double I0[3], dA[3], dB[3], dC[3];
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
-              const double* gx = G + (3 * r + x) * nrk;
-              const int nk = h.nk;
-              I0[x] = hl_i2d(gx, nk, ia[x], ib[x], ic[x], id[x], AB[x], CD[x]);
-              double v = a2 * hl_i2d(gx, nk, ia[x] + 1, ib[x], ic[x], id[x], AB[x], CD[x]);
-              if (ia[x] > 0) v -= (double)ia[x] * hl_i2d(gx, nk, ia[x] - 1, ib[x], ic[x], id[x], AB[x], CD[x]);
-              dA[x] = v;
-              v = b2 * hl_i2d(gx, nk, ia[x], ib[x] + 1, ic[x], id[x], AB[x], CD[x]);
-              if (ib[x] > 0) v -= (double)ib[x] * hl_i2d(gx, nk, ia[x], ib[x] - 1, ic[x], id[x], AB[x], CD[x]);
-              dB[x] = v;
-              v = c2 * hl_i2d(gx, nk, ia[x], ib[x], ic[x] + 1, id[x], AB[x], CD[x]);
-              if (ic[x] > 0) v -= (double)ic[x] * hl_i2d(gx, nk, ia[x], ib[x], ic[x] - 1, id[x], AB[x], CD[x]);
-              dC[x] = v;
+              const int ia = (pab >> (8 + 4 * x)) & 15, ib = (pab >> (20 + 4 * x)) & 15;
+              const int ic = (pcd >> (8 + 4 * x)) & 15, id = (pcd >> (20 + 4 * x)) & 15;
+              const double* tx = T + x * nt + ia * sa + ib * sb + ic * sc + id;
+              I0[x] = tx[0];
+              dA[x] = a2 * tx[sa] - (ia > 0 ? (double)ia * tx[-sa] : 0.0);
+              dB[x] = b2 * tx[sb] - (ib > 0 ? (double)ib * tx[-sb] : 0.0);
+              dC[x] = c2 * tx[sc] - (ic > 0 ? (double)ic * tx[-sc] : 0.0);
             }
-            fd[0][0] += g * dA[0] * I0[1] * I0[2];
-            fd[0][1] += g * I0[0] * dA[1] * I0[2];
-            fd[0][2] += g * I0[0] * I0[1] * dA[2];
-            fd[1][0] += g * dB[0] * I0[1] * I0[2];
-            fd[1][1] += g * I0[0] * dB[1] * I0[2];
-            fd[1][2] += g * I0[0] * I0[1] * dB[2];
-            fd[2][0] += g * dC[0] * I0[1] * I0[2];
-            fd[2][1] += g * I0[0] * dC[1] * I0[2];
-            fd[2][2] += g * I0[0] * I0[1] * dC[2];
+            const double gyz = g * I0[1] * I0[2], gxz = g * I0[0] * I0[2], gxy = g * I0[0] * I0[1];
+            f0x += dA[0] * gyz; f0y += dA[1] * gxz; f0z += dA[2] * gxy;
+            f1x += dB[0] * gyz; f1y += dB[1] * gxz; f1z += dB[2] * gxy;
+            f2x += dC[0] * gyz; f2y += dC[1] * gxz; f2z += dC[2] * gxy;
           }
         }
       }
@@ -576,15 +674,17 @@ __global__ __launch_bounds__(HL_BLOCK) void k_gradh(GradParams prm, WaveWork ww,
     if (P.sa == P.sb) deg *= 0.5;
     if (Q.sa == Q.sb) deg *= 0.5;
     if (ip == jq) deg *= 0.5;
-    const int at[4] = {prm.shell_atom[P.sa], prm.shell_atom[P.sb], prm.shell_atom[Q.sa], prm.shell_atom[Q.sb]};
+    const int at0 = prm.shell_atom[P.sa], at1 = prm.shell_atom[P.sb], at2 = prm.shell_atom[Q.sa],
+              at3 = prm.shell_atom[Q.sb];
+    const double fd[3][3] = {{f0x, f0y, f0z}, {f1x, f1y, f1z}, {f2x, f2y, f2z}};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double fa = wave_sum(fd[0][d]), fb = wave_sum(fd[1][d]), fc = wave_sum(fd[2][d]);
       if (lane == 0) {
-        atomicAdd(&s_de[3 * at[0] + d], deg * fa);
-        atomicAdd(&s_de[3 * at[1] + d], deg * fb);
-        atomicAdd(&s_de[3 * at[2] + d], deg * fc);
-        atomicAdd(&s_de[3 * at[3] + d], -deg * (fa + fb + fc));
+        atomicAdd(&s_de[3 * at0 + d], deg * fa);
+        atomicAdd(&s_de[3 * at1 + d], deg * fb);
+        atomicAdd(&s_de[3 * at2 + d], deg * fc);
+        atomicAdd(&s_de[3 * at3 + d], -deg * (fa + fb + fc));
       }
     }
   }

@@ -59,7 +59,9 @@ struct HLDims {
   int ne, nf;                  // contracted (e0|f0) extents
   int na, nb, nc, nd;          // Cartesian components
   int oa, ob, oc, od;          // AO components
-  int offG, offA, offB;        // LDS: 2D integrals, buffer A, buffer B
+  int nt;                      // gradient: transferred 2D entries per axis, (la+2)(lb+2)(lc+2)(ld+1)
+  int offG, offA, offB;        // LDS (doubles): 2D integrals, buffer A, buffer B
+  int offT, offC, offI;        // LDS (doubles): transferred 2D table, HRR coefficients, int tables
   int lds;                     // bytes of dynamic LDS
 };
 

@@ -38,6 +38,8 @@ void launch_shell_density(const double* D, const double* D2, const int* ao_off, 
 void launch_add(const double* x, const double* y, double* z, long long n, hipStream_t s);
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
 bool kernels_support(int lmax);
+bool jk_template(int la, int lb, int lc, int ld);
+bool grad_template(int la, int lb, int lc, int ld);
 HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm);
 void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStream_t s);
 void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
@@ -176,6 +178,9 @@ struct oqph_session {
   // K rows fit in LDS, 3 = as 2 without the LDS density rows, otherwise
   // (default) the bra-row kernel where it fits and also saves atomics
   int rows_mode = 1;
+  // OQPH_F_TEMPLATES=0: every class with an f shell on the run-time-class
+  // kernels (k_jkh, k_gradh) instead of the class templates where they exist
+  bool f_templates = true;
   std::vector<WLaunch> launches;
   DBuf<int> d_seg;                 // seg_bra | seg_k0 | seg_n of all launches
   DBuf<long long> d_choff;
@@ -403,6 +408,8 @@ static int prepare_geometry(oqph_session* s) {
       (rc = upload(s->d_colao.p, s->h_colao.data(), s->h_colao.size() * sizeof(int), s->stream)))
     return rc;
   {
+    const char* ft = getenv("OQPH_F_TEMPLATES");
+    s->f_templates = !(ft && ft[0] == '0');
     const char* e = getenv("OQPH_JK_ROWS");
     s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
   }
@@ -535,7 +542,7 @@ static int prepare_grad(oqph_session* s) {
     L.ww.ket = s->d_gcls.p + coff[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
-    L.hl = p.la == 3 || p.lc == 3;
+    L.hl = !grad_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3));
     if (L.hl) {
       L.hd = hl_dims(p.la, p.lb, p.lc, p.ld, s->pure, 1, s->natm);
       if ((size_t)L.hd.lds > s->lds_max) {
@@ -645,7 +652,8 @@ static int prepare_lists(oqph_session* s, double maxden) {
       const int ncC = s->ncolL[lc], ncol = (lc == ld) ? ncC : ncC + s->ncolL[ld];
       int nrow = 0;
       size_t slds = 0;
-      const bool hl = la == 3 || lc == 3;  // classes with an f shell: run-time-class kernel
+      // run-time-class kernel for the f classes without a class template
+      const bool hl = !jk_template(la, lb, lc, ld) || (!s->f_templates && (la == 3 || lc == 3));
       bool rows = !hl && s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max);
       // dynamic LDS: K rows (nrow x ncol), plus D rows (nrow x ncol) and the
       // bra rows of dsh (2 x nsh) when those fit as well
@@ -758,7 +766,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
     L.ww.ket = s->d_cls.p + s->cls_off[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
-    L.hl = p.la == 3 || p.lc == 3;
+    L.hl = !jk_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3));
     if (L.hl) {
       L.hd = hl_dims(p.la, p.lb, p.lc, p.ld, s->pure, 0, 0);
       if ((size_t)L.hd.lds > s->lds_max) {

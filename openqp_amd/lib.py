@@ -123,12 +123,14 @@ class FockBuilder:
     def last_profile(self):
         """Per-launch (class code, ms, model FLOPs, quartets, primitive quartets) of the last build.
 
-        cls = la*1000 + lb*100 + lc*10 + ld; kernel is "k_jkb" (bra-row kernel) or "k_jk"."""
+        cls = la*1000 + lb*100 + lc*10 + ld; kernel is "k_jkb" (bra-row kernel), "k_jk"
+        (wavefront tasks) or "k_jkh" (run-time class, the classes with an f shell)."""
         n = 64
         cls = np.zeros(n, np.int32); ms = np.zeros(n); fl = np.zeros(n)
         q = np.zeros(n, np.int64); pq = np.zeros(n, np.int64)
         m = load().oqph_last_profile(self._s, n, cls, ms, fl, q, pq)
-        return [dict(cls=int(cls[i]) % 10000, kernel="k_jkb" if cls[i] >= 10000 else "k_jk", ms=float(ms[i]),
+        kern = lambda c: "k_jkh" if c >= 20000 else "k_jkb" if c >= 10000 else "k_jk"
+        return [dict(cls=int(cls[i]) % 10000, kernel=kern(int(cls[i])), ms=float(ms[i]),
                      flops=float(fl[i]), quartets=int(q[i]), prims=int(pq[i]))
                 for i in range(max(m, 0))]
 
