@@ -139,6 +139,63 @@ def scf_leg():
             "fock_ms_total": round(float(fms.sum()), 3), "incremental": True, "guess": "core Hamiltonian"}
 
 
+def fshell_leg(fp32_cutoff, steps):
+    """(H2O)16 / cc-pVTZ (928 bf, pure d and f shells): full direct RHF SCF to
+    1e-8 Eh on the device builder in FP64 and in mixed precision, the total
+    RHF gradient at each converged density, and at the FP64 density the J+K
+    build rate (FP64 and mixed) and the 2e-gradient rate."""
+    from openqp_amd import basis as B, molecules as M
+    from openqp_amd.lib import FockBuilder
+    from openqp_amd.scf import RHF
+    from openqp_amd.gradient import rhf_gradient
+    bld, bname, nocc = M.WORKLOADS["h2o16_ccpvtz"]
+    Z, xyz = bld(0)
+    bas = B.build_basis(bname, Z, xyz)
+    fb = FockBuilder(bas)
+    t0 = time.perf_counter()
+    r64 = RHF(bas, fb, nocc=nocc, conv=1e-6, econv=1e-8).run()
+    w64 = time.perf_counter() - t0
+    g64 = rhf_gradient(bas, r64.orbitals, r64.orbital_energies, nocc, fb)["total"]
+    dp = B.pack_tri(r64.density)
+    fb.grad2(dp)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fb.grad2(dp)
+    dtg = (time.perf_counter() - t0) / steps
+    # last_stats is overwritten by every call: keep the values
+    g_kernel_ms, g_quartets = fb.last_stats.kernel_ms, fb.last_stats.quartets_computed
+
+    def builds(precision):
+        fb.set_precision(precision, fp32_cutoff)
+        fb.fock_jk(dp)
+        t = time.perf_counter()
+        for _ in range(steps):
+            fb.fock_jk(dp)
+        st = fb.last_stats
+        return (time.perf_counter() - t) / steps, st.quartets_computed, st.quartets_fp32
+    dt64, q64, _ = builds(0)
+    dtmx, _, q32 = builds(1)
+    # mixed-precision SCF and gradient against the FP64 ones
+    t0 = time.perf_counter()
+    rmx = RHF(bas, fb, nocc=nocc, conv=1e-6, econv=1e-8).run()
+    wmx = time.perf_counter() - t0
+    gmx = rhf_gradient(bas, rmx.orbitals, rmx.orbital_energies, nocc, fb)["total"]
+    fb.close()
+    return {"workload": "h2o16_ccpvtz", "nbf": bas.nbf, "nshell": bas.nsh, "nocc": nocc,
+            "scf_energy_fp64": r64.energy, "scf_iterations_fp64": r64.iterations, "scf_converged": r64.converged,
+            "scf_wall_s_fp64": round(w64, 3), "scf_fock_ms_mean_fp64": round(float(np.mean(r64.fock_ms)), 3),
+            "fp64_builds_per_s": round(1.0 / dt64, 4), "mixed_builds_per_s": round(1.0 / dtmx, 4),
+            "mixed_speedup": round(dt64 / dtmx, 3), "fp32_cutoff": fp32_cutoff,
+            "quartets_per_build": q64, "fp32_quartets": q32,
+            "gradient_evals_per_s": round(1.0 / dtg, 4), "gradient_kernel_ms": round(g_kernel_ms, 3),
+            "gradient_quartets": g_quartets,
+            "scf_energy_mixed": rmx.energy, "scf_iterations_mixed": rmx.iterations,
+            "scf_wall_s_mixed": round(wmx, 3),
+            "mixed_dE_scf_hartree": abs(rmx.energy - r64.energy),
+            "mixed_max_dgrad": float(np.abs(gmx - g64).max()),
+            "max_abs_grad": float(np.abs(g64).max()), "translational_sum": float(np.abs(g64.sum(axis=0)).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +208,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-fshell", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -260,6 +318,13 @@ def main():
         log("[bench] SCF: RHF on the (H2O)32/6-31G(d) workload, incremental builds")
         scf = scf_leg()
 
+    fsh = None
+    if not a.no_fshell:
+        log("[bench] f shells: RHF SCF, builds and gradient on the (H2O)16/cc-pVTZ workload")
+        fsh = fshell_leg(a.fp32_cutoff, 2)
+        log(f"[bench] f shells: FP64 {fsh['fp64_builds_per_s']} builds/s, gradient {fsh['gradient_evals_per_s']} "
+            f"evals/s, mixed |dE_SCF| {fsh['mixed_dE_scf_hartree']:.2e}")
+
     cpu = None
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (oracle restatement, sampled)")
@@ -285,6 +350,7 @@ def main():
         "gradient_evals_per_s": grad["evals_per_s"] if grad else None,
         "gradient": grad,
         "scf": scf,
+        "fshell": fsh,
         "roofline": roof, "cpu_baseline": cpu,
         "per_class": {str(k): {"kernel": v["kernel"], "ms_fp64": round(v["ms"] / v["n"], 3),
                                "ms_mixed": round(profmx[k]["ms"] / profmx[k]["n"], 3) if k in profmx else None,

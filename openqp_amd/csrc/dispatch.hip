@@ -266,8 +266,10 @@ void launch_grad_2100(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2110(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2111(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2120(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2121(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2200(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2210(const GradParams&, const WaveWork&, hipStream_t);
+void launch_grad_2211(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_2220(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_3000(const GradParams&, const WaveWork&, hipStream_t);
 void launch_grad_3010(const GradParams&, const WaveWork&, hipStream_t);
@@ -296,8 +298,10 @@ bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const Wave
     case 2110: launch_grad_2110(p, ww, s); return true;
     case 2111: launch_grad_2111(p, ww, s); return true;
     case 2120: launch_grad_2120(p, ww, s); return true;
+    case 2121: launch_grad_2121(p, ww, s); return true;
     case 2200: launch_grad_2200(p, ww, s); return true;
     case 2210: launch_grad_2210(p, ww, s); return true;
+    case 2211: launch_grad_2211(p, ww, s); return true;
     case 2220: launch_grad_2220(p, ww, s); return true;
     case 3000: launch_grad_3000(p, ww, s); return true;
     case 3010: launch_grad_3010(p, ww, s); return true;
@@ -368,15 +372,13 @@ bool jk_template(int la, int lb, int lc, int ld) {
     default: return la < 3 && lc < 3;
   }
 }
-// (dd|dd), (dd|dp), (dd|pp) and (dp|dp) are also left to k_gradh: their
-// templates would need 5-17 KB of spill scratch per lane, more than the
-// runtime can always provide next to the other kernels' scratch
+// (dd|dd) and (dd|dp) are also left to k_gradh: their templates would need
+// 9-17 KB of spill scratch per lane, more than the runtime can always
+// provide next to the other kernels' scratch
 bool grad_template(int la, int lb, int lc, int ld) {
   switch (la * 1000 + lb * 100 + lc * 10 + ld) {
     case 2222:
     case 2221:
-    case 2211:
-    case 2121:
       return false;
     case 3000:
     case 3010:
