@@ -21,6 +21,11 @@
 namespace oqph {
 
 #define HL_BLOCK 64  // one wavefront per block
+// occupancy hint of the run-time-class kernels (waves per SIMD): 3 caps them
+// at 168 VGPRs with a few bytes of spill, 2.6 % faster than the unhinted 206
+#ifndef OQPH_HL_WAVES
+#define OQPH_HL_WAVES 3
+#endif
 
 // ---------------------------------------------------------- small helpers
 __device__ __forceinline__ double hl_pow(double v, int n) {
@@ -393,7 +398,7 @@ __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& 
 // Wavefront tasks as k_jk: a block takes one 64-ket chunk of one segment,
 // screens its kets lane-parallel with OpenQP's tests, then evaluates and
 // digests the surviving quartets one after another, all 64 lanes on each.
-__global__ __launch_bounds__(HL_BLOCK) void k_jkh(JKParams prm, WaveWork ww, HLDims h) {
+__global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_jkh(JKParams prm, WaveWork ww, HLDims h) {
   extern __shared__ double lds[];
   const HLTabs tb = hl_tables(h, lds);
   const long long wave = blockIdx.x;
@@ -538,7 +543,7 @@ __global__ __launch_bounds__(HL_BLOCK) void k_schwarzh(const ShellPair* pairs, c
 // transfer the 2D integrals to every (a, b, c, d) exponent the derivatives
 // need (T, LDS) and contract them with their Gamma entries; per-atom sums
 // go through LDS.
-__global__ __launch_bounds__(HL_BLOCK) void k_gradh(GradParams prm, WaveWork ww, HLDims h) {
+__global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_gradh(GradParams prm, WaveWork ww, HLDims h) {
   extern __shared__ double lds[];
   const HLTabs tb = hl_tables(h, lds);
   double* s_de = lds + h.offB;  // 3 natm
