@@ -346,10 +346,6 @@ static int prepare_geometry(oqph_session* s) {
   std::vector<PrimPair> prims;
   build_pairs(*s, Cut{1.0e-4 * s->cutoff, 25.0 * std::log(10.0)}, s->pairs, prims);
   for (size_t pid = 0; pid < np; ++pid) s->pairs[pid].Q = s->Q[pid];
-  if ((rc = s->d_pairs.ensure(np)) || (rc = s->d_prims.ensure(prims.size()))) return rc;
-  if ((rc = upload(s->d_pairs.p, s->pairs.data(), np * sizeof(ShellPair), s->stream))) return rc;
-  if (!prims.empty() && (rc = upload(s->d_prims.p, prims.data(), prims.size() * sizeof(PrimPair), s->stream)))
-    return rc;
   // class lists of significant pairs, Q descending
   s->cls.assign(10, {});
   s->clsQ.assign(10, {});
@@ -379,8 +375,32 @@ static int prepare_geometry(oqph_session* s) {
     all.insert(all.end(), v.begin(), v.end());
   }
   s->cls_off[10] = all.size();
-  if ((rc = s->d_cls.ensure(all.size()))) return rc;
-  if (!all.empty() && (rc = upload(s->d_cls.p, all.data(), all.size() * sizeof(int), s->stream))) return rc;
+  // Device pair and primitive-pair records in class-list order: device pair
+  // id i is s->pairs[all[i]], so the 64 kets of a wavefront (consecutive
+  // list entries) read consecutive records; the class lists on the device
+  // are then the identity.
+  {
+    std::vector<ShellPair> dp(all.size());
+    std::vector<PrimPair> dpr;
+    dpr.reserve(prims.size());
+    for (size_t i = 0; i < all.size(); ++i) {
+      dp[i] = s->pairs[all[i]];
+      dp[i].prim0 = (int)dpr.size();
+      const ShellPair& o = s->pairs[all[i]];
+      dpr.insert(dpr.end(), prims.begin() + o.prim0, prims.begin() + o.prim0 + o.nprim);
+    }
+    std::vector<int> ident(all.size());
+    for (size_t i = 0; i < all.size(); ++i) ident[i] = (int)i;
+    if ((rc = s->d_pairs.ensure(std::max<size_t>(dp.size(), 1))) ||
+        (rc = s->d_prims.ensure(std::max<size_t>(dpr.size(), 1))) ||
+        (rc = s->d_cls.ensure(std::max<size_t>(all.size(), 1))))
+      return rc;
+    if (!dp.empty() && ((rc = upload(s->d_pairs.p, dp.data(), dp.size() * sizeof(ShellPair), s->stream)) ||
+                        (rc = upload(s->d_prims.p, dpr.data(), dpr.size() * sizeof(PrimPair), s->stream)) ||
+                        (rc = upload(s->d_cls.p, ident.data(), ident.size() * sizeof(int), s->stream))))
+      return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));  // the host vectors end here
+  }
   if ((rc = s->d_kbs.ensure(kbs_all.size() + 2))) return rc;
   if ((rc = upload(s->d_kbs.p, kbs_all.data(), kbs_all.size() * sizeof(int), s->stream))) return rc;
   if ((rc = s->d_aooff.ensure(s->nsh)) || (rc = s->d_nao.ensure(s->nsh))) return rc;
@@ -682,7 +702,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
         std::stable_sort(its.begin(), its.end(), [](const It& x, const It& y) { return x.cost > y.cost; });
         const size_t i0 = s->h_items.size();
         for (const It& t : its) {
-          s->h_items.push_back(s->cls[cb][t.p]);
+          s->h_items.push_back((int)s->cls_off[cb] + t.p);  // device pair id
           for (int b = 0; b < NBKT; ++b) s->h_items.push_back(cnt[(size_t)t.p * NBKT + b]);
         }
         Pend q{la, lb, lc, ld, ck, 0, 0, 0, 0};
@@ -707,7 +727,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
         for (int b = 0; b < NBKT; ++b) {
           const int k = cnt[(size_t)p * NBKT + b];
           if (k == 0) continue;
-          segs.push_back({bra, kb[b], k, (long long)s->pairs[bra].nprim * kmax[b]});
+          segs.push_back({(int)s->cls_off[cb] + p, kb[b], k, (long long)s->pairs[bra].nprim * kmax[b]});
         }
       }
       // longest wavefronts first, so the launch does not end on a tail of
