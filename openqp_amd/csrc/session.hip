@@ -456,9 +456,6 @@ static int prepare_grad(oqph_session* s) {
   build_pairs(*s, Cut{1.0e-20, 20.0 * std::log(10.0)}, gp, gpr);
   const size_t np = gp.size();
   for (size_t pid = 0; pid < np; ++pid) gp[pid].Q = s->Q[pid];
-  if ((rc = s->d_gpairs.ensure(np)) || (rc = s->d_gprims.ensure(std::max<size_t>(gpr.size(), 1)))) return rc;
-  if ((rc = upload(s->d_gpairs.p, gp.data(), np * sizeof(ShellPair), s->stream))) return rc;
-  if (!gpr.empty() && (rc = upload(s->d_gprims.p, gpr.data(), gpr.size() * sizeof(PrimPair), s->stream))) return rc;
   // class lists ordered by primitive-pair bucket, then Q descending (as the
   // J/K lists), so a wavefront's kets have similar contraction lengths
   std::vector<std::vector<int>> cls(10);
@@ -483,8 +480,26 @@ static int prepare_grad(oqph_session* s) {
     coff[c] = all.size();
     all.insert(all.end(), v.begin(), v.end());
   }
-  if ((rc = s->d_gcls.ensure(std::max<size_t>(all.size(), 1)))) return rc;
-  if (!all.empty() && (rc = upload(s->d_gcls.p, all.data(), all.size() * sizeof(int), s->stream))) return rc;
+  // device records in class-list order (device pair id i = gp[all[i]]), as
+  // for the J/K build; the device class lists are the identity
+  std::vector<ShellPair> dgp(all.size());
+  std::vector<PrimPair> dgpr;
+  dgpr.reserve(gpr.size());
+  std::vector<int> ident(all.size());
+  for (size_t i = 0; i < all.size(); ++i) {
+    dgp[i] = gp[all[i]];
+    dgp[i].prim0 = (int)dgpr.size();
+    dgpr.insert(dgpr.end(), gpr.begin() + gp[all[i]].prim0, gpr.begin() + gp[all[i]].prim0 + gp[all[i]].nprim);
+    ident[i] = (int)i;
+  }
+  if ((rc = s->d_gpairs.ensure(std::max<size_t>(dgp.size(), 1))) ||
+      (rc = s->d_gprims.ensure(std::max<size_t>(dgpr.size(), 1))) ||
+      (rc = s->d_gcls.ensure(std::max<size_t>(all.size(), 1))))
+    return rc;
+  if (!all.empty() && ((rc = upload(s->d_gpairs.p, dgp.data(), dgp.size() * sizeof(ShellPair), s->stream)) ||
+                       (rc = upload(s->d_gprims.p, dgpr.data(), dgpr.size() * sizeof(PrimPair), s->stream)) ||
+                       (rc = upload(s->d_gcls.p, ident.data(), ident.size() * sizeof(int), s->stream))))
+    return rc;
   // segments (bra p, ket bucket b): the kets with Q_p Q_k >= grad_cutoff form
   // a prefix of the bucket; within one class each unordered pair is taken
   // once, with the bra the later entry of the (bucket, Q) order
@@ -518,7 +533,7 @@ static int prepare_grad(oqph_session* s) {
             if (b > bp) k = 0;
             else if (b == bp) k = std::min(k, p - bb[bp] + 1);
           }
-          if (k > 0) segs.push_back({bra, kb[b], k, (long long)gp[bra].nprim * kmax[b]});
+          if (k > 0) segs.push_back({(int)coff[cb] + p, kb[b], k, (long long)gp[bra].nprim * kmax[b]});
         }
       }
       if (segs.empty()) continue;
