@@ -55,7 +55,8 @@ HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm) {
   h.offB = h.offA + sA;
   h.offT = h.offB + sB;
   h.offC = h.offT + nT;
-  h.offI = h.offC + nC;
+  h.offD = h.offC + nC;
+  h.offI = h.offD + (grad ? 0 : 6 * 100);  // J/K: D_ab, D_cd, D_ac, D_ad, D_bc, D_bd, up to 100 entries each
   h.lds = (int)sizeof(double) * (h.offI + (nI + 1) / 2);
   return h;
 }

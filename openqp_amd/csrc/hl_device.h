@@ -456,6 +456,19 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
     const double cj = 2.0 * deg * prm.sc, ck = -0.5 * deg * prm.se;
     const int oc = Q.ao_a, od = Q.ao_b;
     const int OA = h.oa, OB = h.ob, OC = h.oc, OD = h.od;
+    // the six density blocks of this quartet, staged in LDS (stride 100: up
+    // to 10 x 10 AO components for Cartesian f)
+    double* sD = lds + h.offD;
+    for (int t = lane; t < 6 * 100; t += HL_BLOCK) {
+      const int blk = t / 100, e = t - 100 * blk;
+      const int r0 = blk == 0 ? oa : blk == 1 ? oc : blk <= 3 ? oa : ob;
+      const int c0 = blk == 0 ? ob : blk == 1 ? od : (blk == 2 || blk == 4) ? oc : od;
+      const int nc = blk == 0 ? OB : blk == 1 ? OD : (blk == 2 || blk == 4) ? OC : OD;
+      const int nr = blk == 0 ? OA : blk == 1 ? OC : blk <= 3 ? OA : OB;
+      if (e < nr * nc) sD[t] = Dp[tri_idx(r0 + e / nc, c0 + e % nc)];
+    }
+    __syncthreads();
+    const double *Dab = sD, *Dcd = sD + 100, *Dac = sD + 200, *Dad = sD + 300, *Dbc = sD + 400, *Dbd = sD + 500;
     auto val = [&](int a, int b, int c, int d) {
       const double v = V[((a * OB + b) * OC + c) * OD + d];
       return fabs(v) < cut ? 0.0 : v;  // OpenQP element cutoff on the AO value
@@ -469,14 +482,14 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
       const int a = t / OB, b = t - a * OB;
       double s = 0.0;
       for (int c = 0; c < OC; ++c)
-        for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dp[tri_idx(oc + c, od + d)];
+        for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dcd[c * OD + d];
       put(oa + a, ob + b, cj * s);
     }
     for (int t = lane; t < OC * OD; t += HL_BLOCK) {
       const int c = t / OD, d = t - c * OD;
       double s = 0.0;
       for (int a = 0; a < OA; ++a)
-        for (int b = 0; b < OB; ++b) s += val(a, b, c, d) * Dp[tri_idx(oa + a, ob + b)];
+        for (int b = 0; b < OB; ++b) s += val(a, b, c, d) * Dab[a * OB + b];
       put(oc + c, od + d, cj * s);
     }
     // K_ac += V D_bd, K_ad += V D_bc, K_bc += V D_ad, K_bd += V D_ac
@@ -484,28 +497,28 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
       const int a = t / OC, c = t - a * OC;
       double s = 0.0;
       for (int b = 0; b < OB; ++b)
-        for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dp[tri_idx(ob + b, od + d)];
+        for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dbd[b * OD + d];
       put(oa + a, oc + c, ck * s);
     }
     for (int t = lane; t < OA * OD; t += HL_BLOCK) {
       const int a = t / OD, d = t - a * OD;
       double s = 0.0;
       for (int b = 0; b < OB; ++b)
-        for (int c = 0; c < OC; ++c) s += val(a, b, c, d) * Dp[tri_idx(ob + b, oc + c)];
+        for (int c = 0; c < OC; ++c) s += val(a, b, c, d) * Dbc[b * OC + c];
       put(oa + a, od + d, ck * s);
     }
     for (int t = lane; t < OB * OC; t += HL_BLOCK) {
       const int b = t / OC, c = t - b * OC;
       double s = 0.0;
       for (int a = 0; a < OA; ++a)
-        for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dp[tri_idx(oa + a, od + d)];
+        for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dad[a * OD + d];
       put(ob + b, oc + c, ck * s);
     }
     for (int t = lane; t < OB * OD; t += HL_BLOCK) {
       const int b = t / OD, d = t - b * OD;
       double s = 0.0;
       for (int a = 0; a < OA; ++a)
-        for (int c = 0; c < OC; ++c) s += val(a, b, c, d) * Dp[tri_idx(oa + a, oc + c)];
+        for (int c = 0; c < OC; ++c) s += val(a, b, c, d) * Dac[a * OC + c];
       put(ob + b, od + d, ck * s);
     }
   }
