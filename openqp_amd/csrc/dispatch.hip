@@ -353,9 +353,12 @@ void launch_cart_density(const double* D, int nsh, const int* am, const int* pur
 bool kernels_support(int lmax) { return lmax <= 3; }
 
 // classes with an f shell that have a class-templated kernel (the others run
-// on the run-time-class kernels k_jkh / k_gradh)
+// on the run-time-class kernels k_jkh / k_gradh); (dd|dd) also runs on
+// k_jkh: its streamed template needs 12-14 KB of spill scratch per lane
 bool jk_template(int la, int lb, int lc, int ld) {
   switch (la * 1000 + lb * 100 + lc * 10 + ld) {
+    case 2222:
+      return false;
     case 3000:
     case 3010:
     case 3011:
