@@ -119,3 +119,27 @@ def test_fshell_mixed_precision_energy(oracle):
     assert n32 > 0
     assert dE <= 1e-6
     fb.close()
+
+
+@pytest.mark.parametrize("harm", [None, 0])
+def test_fshell_runtime_class_kernels_only(oracle, monkeypatch, harm):
+    """OQPH_F_TEMPLATES=0 sends every f class to the run-time-class kernels
+    (k_jkh, k_gradh); J/K and gradient keep the same bars."""
+    from openqp_amd import lib as L
+    monkeypatch.setenv("OQPH_F_TEMPLATES", "0")
+    _say("f-shell run-time-class path", "pure" if harm is None else "cartesian")
+    bas = _basis(1, harm)
+    D, dp = density_for(oracle, bas, 5, seed=4)
+    Qo = oracle.schwarz(bas)
+    fo, st = oracle.fock_jk(bas, dp, Q=Qo)
+    go, _ = oracle.grad2(bas, dp, Q=Qo)
+    fb = L.FockBuilder(bas)
+    f = fb.fock_jk(dp)
+    err = np.abs(f - fo).max()
+    g = fb.grad2(dp)
+    gerr = np.abs(g - go).max()
+    _say("  max |f - f_oracle| =", err, "max |g - g_oracle| =", gerr)
+    assert err <= TOL
+    assert fb.last_stats.quartets_computed > 0
+    assert gerr <= 1e-8
+    fb.close()
