@@ -1,5 +1,5 @@
-// hl.hip -- launchers of the run-time-class kernels (classes with an f
-// shell) and the host-side sizing of their LDS layout.
+// hl.hip -- launchers of the run-time-class kernels (the classes without a
+// class template) and the host-side sizing of their LDS layout.
 #include <algorithm>
 
 #include "hl_device.h"

@@ -1,7 +1,9 @@
-// hl_device.h -- shell quartets of the classes with an f shell that have no
-// class-templated kernel (the larger ones, up to (ff|ff)), evaluated by one
-// wavefront per quartet with every intermediate in LDS, sized per class at
-// launch (HLDims).  Covers the Fock build, the Schwarz bounds and the 2e
+// hl_device.h -- shell quartets of the classes that have no class-templated
+// kernel: the larger classes with an f shell (up to (ff|ff)), and the d
+// classes whose templates would spill more than 8 KB per lane ((dd|dd) J/K,
+// (dd|dd) and (dd|dp) gradient).  One wavefront evaluates one quartet at a
+// time with every intermediate in LDS, sized per class at launch (HLDims).
+// Covers the Fock build, the Schwarz bounds of the f pair classes and the 2e
 // gradient for those classes; the algorithm is the one of the register
 // kernels (Rys quadrature, contracted (e0|f0), binomial HRR, AO transform,
 // OpenQP element cutoff and digestion), with loop bounds read at run time

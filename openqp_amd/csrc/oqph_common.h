@@ -51,7 +51,7 @@ struct WaveWork {
 };
 
 // One quartet class (la lb|lc ld), la >= lb, lc >= ld, for the kernels that
-// take the class at run time (classes with an f shell): extents of every
+// take the class at run time (classes without a template): extents of every
 // intermediate and the LDS layout (offsets in doubles), filled by hl_dims().
 struct HLDims {
   int la, lb, lc, ld, nr, pure;

@@ -156,7 +156,7 @@ struct oqph_session {
   struct GLaunch {
     int la, lb, lc, ld;
     WaveWork ww;
-    bool hl;     // run-time-class kernel (a class with an f shell), layout hd
+    bool hl;     // run-time-class kernel (a class without a template), layout hd
     HLDims hd;
   };
   struct WLaunch {
