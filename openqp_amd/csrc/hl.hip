@@ -56,7 +56,10 @@ HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm) {
   h.offT = h.offB + sB;
   h.offC = h.offT + nT;
   h.offD = h.offC + nC;
-  h.offI = h.offD + (grad ? 0 : 6 * 100);  // J/K: D_ab, D_cd, D_ac, D_ad, D_bc, D_bd, up to 100 entries each
+  // J/K: D_ab, D_cd, D_ac, D_ad, D_bc, D_bd, each at most (largest AO count)^2
+  const int omax = std::max(std::max(h.oa, h.ob), std::max(h.oc, h.od));
+  h.dstr = omax * omax;
+  h.offI = h.offD + (grad ? 0 : 6 * h.dstr);
   h.lds = (int)sizeof(double) * (h.offI + (nI + 1) / 2);
   return h;
 }

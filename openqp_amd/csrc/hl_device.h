@@ -458,11 +458,12 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
     const double cj = 2.0 * deg * prm.sc, ck = -0.5 * deg * prm.se;
     const int oc = Q.ao_a, od = Q.ao_b;
     const int OA = h.oa, OB = h.ob, OC = h.oc, OD = h.od;
-    // the six density blocks of this quartet, staged in LDS (stride 100: up
-    // to 10 x 10 AO components for Cartesian f)
+    // the six density blocks of this quartet, staged in LDS (block stride
+    // h.dstr = the square of the class's largest AO count)
     double* sD = lds + h.offD;
-    for (int t = lane; t < 6 * 100; t += HL_BLOCK) {
-      const int blk = t / 100, e = t - 100 * blk;
+    const int ds = h.dstr;
+    for (int t = lane; t < 6 * ds; t += HL_BLOCK) {
+      const int blk = t / ds, e = t - ds * blk;
       const int r0 = blk == 0 ? oa : blk == 1 ? oc : blk <= 3 ? oa : ob;
       const int c0 = blk == 0 ? ob : blk == 1 ? od : (blk == 2 || blk == 4) ? oc : od;
       const int nc = blk == 0 ? OB : blk == 1 ? OD : (blk == 2 || blk == 4) ? OC : OD;
@@ -470,7 +471,8 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
       if (e < nr * nc) sD[t] = Dp[tri_idx(r0 + e / nc, c0 + e % nc)];
     }
     __syncthreads();
-    const double *Dab = sD, *Dcd = sD + 100, *Dac = sD + 200, *Dad = sD + 300, *Dbc = sD + 400, *Dbd = sD + 500;
+    const double *Dab = sD, *Dcd = sD + ds, *Dac = sD + 2 * ds, *Dad = sD + 3 * ds, *Dbc = sD + 4 * ds,
+                 *Dbd = sD + 5 * ds;
     auto val = [&](int a, int b, int c, int d) {
       const double v = V[((a * OB + b) * OC + c) * OD + d];
       return fabs(v) < cut ? 0.0 : v;  // OpenQP element cutoff on the AO value

@@ -62,7 +62,7 @@ struct HLDims {
   int nt;                      // gradient: transferred 2D entries per axis, (la+2)(lb+2)(lc+2)(ld+1)
   int offG, offA, offB;        // LDS (doubles): 2D integrals, buffer A, buffer B
   int offT, offC, offI;        // LDS (doubles): transferred 2D table, HRR coefficients, int tables
-  int offD;                    // LDS (doubles): the six density blocks of one quartet (J/K)
+  int offD, dstr;              // LDS (doubles): the six density blocks of one quartet (J/K), block stride
   int lds;                     // bytes of dynamic LDS
 };
 
