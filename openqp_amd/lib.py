@@ -216,6 +216,22 @@ def grad1e(bas: Basis, D: np.ndarray, W: np.ndarray) -> np.ndarray:
     return g.reshape(bas.natm, 3)
 
 
+def routec_grad2(d_packed, xyz_bohr, nbf: int, hfscale: float = 1.0, coulscale: float = 1.0):
+    """Call the Route-C gradient symbol exactly as OpenQP's seam does (by reference,
+    ``xyz(3, natm)`` in bohr); returns (de (natm, 3), info)."""
+    L = load()
+    d = np.ascontiguousarray(d_packed, np.float64)
+    xyz = np.ascontiguousarray(np.asarray(xyz_bohr, np.float64).reshape(-1, 3))
+    natm = xyz.shape[0]
+    de = np.zeros(3 * natm)
+    nb, na, info = C.c_int(nbf), C.c_int(natm), C.c_int(-1)
+    hs, cs = C.c_double(hfscale), C.c_double(coulscale)
+    dp = C.POINTER(C.c_double)
+    L.routec_grad2(d.ctypes.data_as(dp), xyz.ctypes.data_as(dp), de.ctypes.data_as(dp), C.byref(nb), C.byref(na),
+                   C.byref(hs), C.byref(cs), C.byref(info))
+    return de.reshape(natm, 3), info.value
+
+
 def routec_fock_jk(d_packed, nbf: int, se: float = 1.0, sc: float = 1.0):
     """Call the Route-C symbol exactly as OpenQP's seam does (by reference); returns (f, info)."""
     L = load()

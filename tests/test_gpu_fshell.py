@@ -143,3 +143,40 @@ def test_fshell_runtime_class_kernels_only(oracle, monkeypatch, harm):
     assert fb.last_stats.quartets_computed > 0
     assert gerr <= 1e-8
     fb.close()
+
+
+@pytest.mark.parametrize("name,nw", [("cc-pvdz", 2), ("cc-pvtz", 1)])
+def test_routec_seam_symbols(oracle, name, nw):
+    """Both Route-C symbols called by reference as OpenQP's seam calls them
+    (routec_bridge.F90:33-61), on the default session: J/K against the oracle,
+    the 2e gradient at the session geometry and, through the xyz argument, at a
+    displaced geometry (the library refreshes its geometry)."""
+    from openqp_amd import lib as L
+    from openqp_amd import basis as B
+    if name == "cc-pvtz":
+        bas = _basis(nw)
+    else:
+        bas = small_system(name, nw=nw)
+    D, dp = density_for(oracle, bas, 5 * nw, seed=7)
+    fb = L.FockBuilder(bas)
+    fb.make_default()
+    f, info = L.routec_fock_jk(dp, bas.nbf)
+    assert info == 0
+    fo, _ = oracle.fock_jk(bas, dp, Q=oracle.schwarz(bas))
+    assert np.abs(f - fo).max() <= TOL
+    g, info = L.routec_grad2(dp, bas.xyz, bas.nbf)
+    assert info == 0
+    go, _ = oracle.grad2(bas, dp)
+    _say("  seam", name, "max |f - f_oracle|", np.abs(f - fo).max(), "max |g - g_oracle|", np.abs(g - go).max())
+    assert np.abs(g - go).max() <= 1e-8
+    xyz2 = bas.xyz.copy()
+    xyz2[0, 1] += 0.05
+    g2, info = L.routec_grad2(dp, xyz2, bas.nbf)
+    assert info == 0
+    go2, _ = oracle.grad2(bas.with_coords(xyz2), dp)
+    _say("  displaced geometry max |g - g_oracle|", np.abs(g2 - go2).max())
+    assert np.abs(g2 - go2).max() <= 1e-8
+    assert np.abs(g2 - g).max() > 1e-6   # the displacement is seen
+    _, info = L.routec_grad2(dp, xyz2, bas.nbf + 1)   # mismatched nbf declines
+    assert info != 0
+    fb.close()
