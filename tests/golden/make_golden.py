@@ -5,7 +5,8 @@ Fixtures: tests/golden/<name>.npz holding the packed density d, the packed
 5e-11) and the Schwarz table, for
   h2o8_631g      (H2O)8 / 6-31G, seeded orientations, 40 occupied (104 bf);
   h2o_631gs      H2O / 6-31G* Cartesian d at the OpenQP KAT geometry, 5 occupied;
-  h2o2_ccpvdz    (H2O)2 / cc-pVDZ pure d, 10 occupied.
+  h2o2_ccpvdz    (H2O)2 / cc-pVDZ pure d, 10 occupied;
+  h2o_ccpvtz     H2O / cc-pVTZ pure d and f at the OpenQP KAT geometry, 5 occupied.
 Run from the repo root: python tests/golden/make_golden.py
 """
 import os
@@ -34,3 +35,4 @@ if __name__ == "__main__":
     make("h2o8_631g", b, nocc)
     make("h2o_631gs", small_system("6-31g(d)"), 5)
     make("h2o2_ccpvdz", small_system("cc-pvdz", nw=2), 10)
+    make("h2o_ccpvtz", small_system("cc-pvtz"), 5)

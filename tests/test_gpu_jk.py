@@ -28,11 +28,12 @@ def _golden(name):
     return np.load(os.path.join(ROOT, "tests", "golden", name + ".npz"))
 
 
-@pytest.mark.parametrize("name", ["h2o_631gs", "h2o2_ccpvdz", "h2o8_631g"])
+@pytest.mark.parametrize("name", ["h2o_631gs", "h2o2_ccpvdz", "h2o8_631g", "h2o_ccpvtz"])
 def test_golden_parity(L, name):
     _say("golden parity", name)
     bas = {"h2o8_631g": lambda: workload("h2o8_631g")[0], "h2o_631gs": lambda: small_system("6-31g(d)"),
-           "h2o2_ccpvdz": lambda: small_system("cc-pvdz", nw=2)}[name]()
+           "h2o2_ccpvdz": lambda: small_system("cc-pvdz", nw=2),
+           "h2o_ccpvtz": lambda: small_system("cc-pvtz")}[name]()
     g = _golden(name)
     fb = L.FockBuilder(bas)
     Q = fb.schwarz()

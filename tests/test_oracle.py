@@ -248,7 +248,7 @@ def test_golden_fixtures_reproduce(oracle):
     import os
     from helpers import ROOT
     for name, bas in [("h2o8_631g", workload("h2o8_631g")[0]), ("h2o_631gs", small_system("6-31g(d)")),
-                      ("h2o2_ccpvdz", small_system("cc-pvdz", nw=2))]:
+                      ("h2o2_ccpvdz", small_system("cc-pvdz", nw=2)), ("h2o_ccpvtz", small_system("cc-pvtz"))]:
         g = np.load(os.path.join(ROOT, "tests", "golden", name + ".npz"))
         f, st = oracle.fock_jk(bas, g["d"], Q=g["Q"])
         assert np.abs(f - g["f"]).max() < 1e-12
