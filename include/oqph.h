@@ -67,9 +67,10 @@ typedef struct {
 int oqph_session_create(oqph_session **s, int device);
 int oqph_session_destroy(oqph_session *s);
 
-/* Basis: nsh shells; am, harmonic (1 = pure for l >= 2), atom (0-based),
- * nprim per shell; ex and cc concatenated per shell in shell order, cc
- * carrying OpenQP's primitive normalisation (basis_tools.F90:277-303). */
+/* Basis: nsh shells; am (0..3: s, p, d, f; larger declines with 1), harmonic
+ * (1 = pure for l >= 2), atom (0-based), nprim per shell; ex and cc
+ * concatenated per shell in shell order, cc carrying OpenQP's primitive
+ * normalisation (basis_tools.F90:277-303). */
 int oqph_session_set_basis(oqph_session *s, int nsh, const int *am, const int *harmonic, const int *atom,
                            const int *nprim, const double *ex, const double *cc);
 /* Geometry: natm atoms, xyz (3, natm) bohr, nuclear charges. */
