@@ -105,12 +105,12 @@ bool jkb_info_3120(bool, int*, size_t*, size_t);
 bool jkb_info_3200(bool, int*, size_t*, size_t);
 bool jkb_info_3210(bool, int*, size_t*, size_t);
 bool jkb_info_3300(bool, int*, size_t*, size_t);
-void launch_schwarz_00(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_10(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_11(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_20(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_21(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_22(bool, const ShellPair*, const PrimPair*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_00(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_10(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_11(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_20(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_21(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+void launch_schwarz_22(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
 
 __device__ __forceinline__ long long tri_idx_(int i, int j) {
   int a = i > j ? i : j, b = i > j ? j : i;
@@ -238,7 +238,7 @@ bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* stat
   }
 }
 
-bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                     const double* tab, const double* xf, double qcut, double* Q, hipStream_t s) {
   if (n == 0) return true;
   switch (la * 10 + lb) {

@@ -78,6 +78,17 @@ __device__ __forceinline__ double rsq64(double x) {
 
 // Load a primitive pair whose index is uniform across the wavefront through
 // the constant address space, so it is served by the scalar cache.
+__device__ __forceinline__ PrimJK load_uniform(const PrimJK* p, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(4))) double* q =
+      (const __attribute__((address_space(4))) double*)(p + i);
+  PrimJK r;
+  r.g = q[0]; r.Kg = q[1]; r.P[0] = q[2]; r.P[1] = q[3]; r.P[2] = q[4]; r.h = q[5];
+  return r;
+#else
+  return p[i];
+#endif
+}
 __device__ __forceinline__ PrimPair load_uniform(const PrimPair* p, int i) {
 #if defined(__HIP_DEVICE_COMPILE__)
   const __attribute__((address_space(4))) double* q =
@@ -253,8 +264,8 @@ template <typename R>
 struct PrimR {
   R g, Kg, P[3], h;
 };
-template <typename R>
-__device__ __forceinline__ PrimR<R> prim_r(const PrimPair& p) {
+template <typename R, typename PP>
+__device__ __forceinline__ PrimR<R> prim_r(const PP& p) {
   return {(R)p.g, (R)p.Kg, {(R)p.P[0], (R)p.P[1], (R)p.P[2]}, (R)p.h};
 }
 

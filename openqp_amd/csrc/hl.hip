@@ -75,7 +75,7 @@ void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStrea
   hipLaunchKernelGGL(k_jkh, dim3((unsigned)ww.nwaves), dim3(HL_BLOCK), h.lds, s, p, ww, h);
 }
 
-void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                      const double* rys, const double* xf, double qcut, double* Q, hipStream_t s) {
   if (n == 0) return;
   allow_lds(k_schwarzh, h.lds);

@@ -148,8 +148,8 @@ struct HLPrim {
   double PQx, PQy, PQz, PAx, PAy, PAz, QCx, QCy, QCz;
   double T, pref, rz, re, h2z, h2e, h2ze;
 };
-__device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q, const PrimPair& pp,
-                                          const PrimPair& qq) {
+template <typename PP>
+__device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q, const PP& pp, const PP& qq) {
   HLPrim s;
   const double zeta = pp.g, eta = qq.g, ze = zeta + eta;
   const double sq = rsq64(ze), ize = sq * sq, rho = zeta * eta * ize;
@@ -263,7 +263,7 @@ __device__ __forceinline__ void hl_xform_inplace(double* buf, int left, int n, i
 // block is then not formed).  Primitive screens as contract(): |K| < qcut
 // per pair and K_ab^2 K_cd^2 < qcut^2 (zeta + eta) per quartet.
 __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& P, const ShellPair& Q,
-                            const PrimPair* prims, const double* __restrict__ rys, const double* __restrict__ xf,
+                            const PrimJK* prims, const double* __restrict__ rys, const double* __restrict__ xf,
                             double qcut, double* lds, int& nprim) {
   const int lane = threadIdx.x;
   double* G = lds + h.offG;
@@ -277,10 +277,10 @@ __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& 
   nprim = 0;
   const double qcut2 = qcut * qcut;
   for (int iq = 0; iq < Q.nprim; ++iq) {
-    const PrimPair qq = load_uniform(prims, Q.prim0 + iq);
+    const PrimJK qq = load_uniform(prims, Q.prim0 + iq);
     if (fabs(qq.Kg) < qcut) continue;
     for (int ip = 0; ip < P.nprim; ++ip) {
-      const PrimPair pp = load_uniform(prims, P.prim0 + ip);
+      const PrimJK pp = load_uniform(prims, P.prim0 + ip);
       if (fabs(pp.Kg) < qcut) continue;
       const double fqz = pp.Kg * qq.Kg;
       if (fqz * fqz < qcut2 * (pp.g + qq.g)) continue;
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
 
 // --------------------------------------------------------- Schwarz bounds
 // One block per pair of the list: Q_ab = sqrt(max |(ab|ab)|) over the AO block.
-__global__ __launch_bounds__(HL_BLOCK) void k_schwarzh(const ShellPair* pairs, const PrimPair* prims, const int* list,
+__global__ __launch_bounds__(HL_BLOCK) void k_schwarzh(const ShellPair* pairs, const PrimJK* prims, const int* list,
                                                        int n, const double* rys, const double* xf, double qcut,
                                                        double* Qout, HLDims h) {
   extern __shared__ double lds[];

@@ -62,7 +62,7 @@ bool OQPH_JKB_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWo
 }
 
 #if XA == XC && XB == XD
-void OQPH_SCH_LAUNCHER(XA, XB)(bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+void OQPH_SCH_LAUNCHER(XA, XB)(bool pure, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                                const double* tab, const double* xf, double qcut, double* Q,
                                hipStream_t s) {
   const int bs = 64;

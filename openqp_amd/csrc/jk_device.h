@@ -79,7 +79,7 @@ __device__ __forceinline__ void xform_inplace(const double* __restrict__ xf, R* 
 // wavefront; its primitive pairs come from the LDS copy sbra when given, else
 // through the constant address space (scalar cache).
 template <int LA, int LB, int LC, int LD, typename R, bool UNIFORM_BRA = false>
-__device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
+__device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q, const PrimJK* __restrict__ prims,
                                          const double* __restrict__ tabN, const float* __restrict__ tab32N, R qcut,
                                          R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF],
                                          int& nprim_used, const PrimR<double>* sbra = nullptr) {
@@ -107,10 +107,10 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
   const int pp0 = UNIFORM_BRA ? __builtin_amdgcn_readfirstlane(P.prim0) : P.prim0;
   const int npp = UNIFORM_BRA ? __builtin_amdgcn_readfirstlane(P.nprim) : P.nprim;
   for (int iq = 0; iq < Q.nprim; ++iq) {
-    const PrimPair qq = prims[Q.prim0 + iq];
+    const PrimJK qq = prims[Q.prim0 + iq];
     if (fabs(qq.Kg) < (double)qcut) continue;
     for (int ip = 0; ip < npp; ++ip) {
-      const PrimPair pp = UNIFORM_BRA ? load_uniform(prims, pp0 + ip) : prims[pp0 + ip];
+      const PrimJK pp = UNIFORM_BRA ? load_uniform(prims, pp0 + ip) : prims[pp0 + ip];
       if (fabs(pp.Kg) < (double)qcut) continue;
       nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tabN, tab32N, acc) ? 1 : 0;
     }
@@ -213,7 +213,7 @@ struct Big {
 
 // Final-representation block of a quartet: V[a][b][c][d] in AO order.
 template <int LA, int LB, int LC, int LD, bool PURE, typename R, bool UNIFORM_BRA = false>
-__device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPair& Q, const PrimPair* __restrict__ prims,
+__device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPair& Q, const PrimJK* __restrict__ prims,
                                               const double* __restrict__ tabN, const float* __restrict__ tab32N,
                                               const double* __restrict__ xf, R qcut,
                                               R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used,
@@ -909,7 +909,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
 // (ints_exchange, source/integrals/int2.F90:1582-1737); the pair records and
 // primitive pairs passed here are built with the tight cutoffs of :1600-1604.
 template <int LA, int LB, bool PURE>
-__global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const PrimPair* prims, const int* list,
+__global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const PrimJK* prims, const int* list,
                                                  int n, const double* tab, const double* xf,
                                                  double qcut, double* Qout) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;

@@ -3,8 +3,9 @@
 // Device-resident layout (all in HBM, built once per basis+geometry):
 //   ShellPair  pairs[npair]      one record per significant shell pair, the
 //                                 higher-L shell first (swap recorded);
-//   PrimPair   prims[nprimpair]  primitive pairs of every shell pair, 64 B each,
-//                                 screened as OpenQP screens them
+//   PrimJK     prims[nprimpair]  primitive pairs of every shell pair, 48 B each
+//                                 (PrimPair, 64 B, for the gradient), screened
+//                                 as OpenQP screens them
 //                                 (source/integrals/int2_pairs.F90:179-266);
 //   double     rys[...]           Rys node/weight fit table (rys_table.inc).
 // Per build:
@@ -33,6 +34,16 @@ struct PrimPair {              // 64 bytes: one cache line per primitive pair
   double Kg;                   // K / zeta, K = sqrt(2) pi^(5/4) c_a c_b exp(-alpha beta/zeta AB^2)
   double P[3];                 // Gaussian product centre
   double a, b;                 // exponents of shell a and shell b
+  double h;                    // 1 / (2 zeta)
+};
+
+// The fields of a primitive pair that the J/K quadrature reads (48 B): the
+// J/K and Schwarz kernels stream these; the gradient keeps PrimPair, whose
+// exponents it needs for the derivatives.
+struct PrimJK {
+  double g;                    // zeta
+  double Kg;                   // K / zeta
+  double P[3];                 // Gaussian product centre
   double h;                    // 1 / (2 zeta)
 };
 
@@ -101,7 +112,7 @@ struct ColMap {
 // Arguments of one J/K build launch (shared by host and device code).
 struct JKParams {
   const ShellPair* pairs;
-  const PrimPair* prims;
+  const PrimJK* prims;
   const double* rys;
   const float* rys32;   // FP32 root fits (degree 6)
   const double* xform;  // [2][2][10][10] canonical Cartesian -> AO transforms (L = 2, 3)

@@ -31,7 +31,7 @@ bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, con
 bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
                 size_t lds, hipStream_t s);
 bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* static_lds, size_t lds_max);
-bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                     const double* tab, const double* xf, double qcut, double* Q, hipStream_t s);
 void launch_shell_density(const double* D, const double* D2, const int* ao_off, const int* nao, int nsh, double* dsh,
                           hipStream_t s);
@@ -42,7 +42,7 @@ bool jk_template(int la, int lb, int lc, int ld);
 bool grad_template(int la, int lb, int lc, int ld);
 HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm);
 void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStream_t s);
-void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimPair* prims, const int* list, int n,
+void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                      const double* rys, const double* xf, double qcut, double* Q, hipStream_t s);
 struct GradParams;
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s);
@@ -139,7 +139,7 @@ struct oqph_session {
   std::vector<std::vector<int>> cls;         // per class: pair ids sorted by Q desc
   std::vector<std::vector<double>> clsQ;     // matching Q values
   DBuf<ShellPair> d_pairs;
-  DBuf<PrimPair> d_prims;
+  DBuf<PrimJK> d_prims;
   DBuf<double> d_rys, d_xform;
   DBuf<float> d_rys32;
   DBuf<int> d_cls;                           // concatenated class lists
@@ -268,6 +268,13 @@ static int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
   return 0;
 }
 
+// the J/K fields of primitive pairs
+static std::vector<PrimJK> to_jk(const std::vector<PrimPair>& v) {
+  std::vector<PrimJK> r(v.size());
+  for (size_t k = 0; k < v.size(); ++k) r[k] = {v[k].g, v[k].Kg, {v[k].P[0], v[k].P[1], v[k].P[2]}, v[k].h};
+  return r;
+}
+
 static int prepare_geometry(oqph_session* s) {
   if (s->ready) return 0;
   if (s->nsh == 0 || s->natm == 0) {
@@ -300,13 +307,14 @@ static int prepare_geometry(oqph_session* s) {
   build_pairs(*s, Cut{1.0e-17, 50.0}, tp, tpr);
   const size_t np = tp.size();
   DBuf<ShellPair> dtp;
-  DBuf<PrimPair> dtpr;
+  DBuf<PrimJK> dtpr;
   DBuf<int> dlist;
   DBuf<double> dQ;
   if ((rc = dtp.ensure(np)) || (rc = dtpr.ensure(tpr.size())) || (rc = dlist.ensure(np)) || (rc = dQ.ensure(np)))
     return rc;
   if ((rc = upload(dtp.p, tp.data(), np * sizeof(ShellPair), s->stream))) return rc;
-  if (!tpr.empty() && (rc = upload(dtpr.p, tpr.data(), tpr.size() * sizeof(PrimPair), s->stream))) return rc;
+  const std::vector<PrimJK> tjk = to_jk(tpr);
+  if (!tjk.empty() && (rc = upload(dtpr.p, tjk.data(), tjk.size() * sizeof(PrimJK), s->stream))) return rc;
   std::vector<std::vector<int>> byc(10);
   for (size_t pid = 0; pid < np; ++pid) byc[class_of(tp[pid].la, tp[pid].lb)].push_back((int)pid);
   std::vector<int> flat;
@@ -381,13 +389,16 @@ static int prepare_geometry(oqph_session* s) {
   // are then the identity.
   {
     std::vector<ShellPair> dp(all.size());
-    std::vector<PrimPair> dpr;
+    std::vector<PrimJK> dpr;
     dpr.reserve(prims.size());
     for (size_t i = 0; i < all.size(); ++i) {
       dp[i] = s->pairs[all[i]];
       dp[i].prim0 = (int)dpr.size();
       const ShellPair& o = s->pairs[all[i]];
-      dpr.insert(dpr.end(), prims.begin() + o.prim0, prims.begin() + o.prim0 + o.nprim);
+      for (int k = o.prim0; k < o.prim0 + o.nprim; ++k) {
+        const PrimPair& q = prims[k];
+        dpr.push_back({q.g, q.Kg, {q.P[0], q.P[1], q.P[2]}, q.h});
+      }
     }
     std::vector<int> ident(all.size());
     for (size_t i = 0; i < all.size(); ++i) ident[i] = (int)i;
@@ -396,7 +407,7 @@ static int prepare_geometry(oqph_session* s) {
         (rc = s->d_cls.ensure(std::max<size_t>(all.size(), 1))))
       return rc;
     if (!dp.empty() && ((rc = upload(s->d_pairs.p, dp.data(), dp.size() * sizeof(ShellPair), s->stream)) ||
-                        (rc = upload(s->d_prims.p, dpr.data(), dpr.size() * sizeof(PrimPair), s->stream)) ||
+                        (rc = upload(s->d_prims.p, dpr.data(), dpr.size() * sizeof(PrimJK), s->stream)) ||
                         (rc = upload(s->d_cls.p, ident.data(), ident.size() * sizeof(int), s->stream))))
       return rc;
     HIPCHK(hipStreamSynchronize(s->stream));  // the host vectors end here
