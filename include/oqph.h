@@ -83,6 +83,16 @@ int oqph_session_set_thresholds(oqph_session *s, double int2e_cutoff, double gra
 int oqph_session_set_precision(oqph_session *s, int policy, double fp32_cutoff);
 /* Make s the session used by routec_fock_jk / routec_grad2. */
 int oqph_set_default_session(oqph_session *s);
+/* Build the geometry-dependent device state now (pair records, Schwarz
+ * bounds); otherwise the first build does it.  Counterpart of
+ * int2_compute_t%init + set_screening (source/integrals/int2.F90:245-289, 473-477). */
+int oqph_session_prepare(oqph_session *s);
+/* Restrict every following J/K build to the quartets OpenQP's MPI rank `rank`
+ * of `size` computes: those whose first pair in OpenQP's loop order has
+ * mod(ij_pair, size) == rank, ij_pair its 1-based position in the cost-sorted
+ * shell-pair map (source/integrals/int2.F90:759-761, 864-921).  Summing the
+ * results of all ranks gives the full build.  size = 1: everything. */
+int oqph_session_set_share(oqph_session *s, int rank, int size);
 
 /* Host-buffer builds (copy in, build, copy out). */
 int oqph_fock_jk(oqph_session *s, const double *d, double *f, int nfocks, double se, double sc,
@@ -93,6 +103,17 @@ int oqph_fock_jk_inc(oqph_session *s, const double *d, double *d_old, double *f,
                      double se, double sc, oqph_stats *stats);
 int oqph_grad2(oqph_session *s, const double *d, double *de, double hfscale, double coulscale,
                oqph_stats *stats);
+
+/* Range-separated (CAM) build, the two passes of int2_run_cam
+ * (source/integrals/int2.F90:538-584) as fock_jk runs them for a CAM
+ * functional (scf_addons.F90:1162-1166, scale_exch ignored there):
+ * f = G[1/r; coulomb alpha_coulomb, exchange alpha] +
+ *     G[erf(mu r)/r; coulomb beta_coulomb, exchange beta],
+ * each G with fock_jk's packed scaling; the attenuated pass screens with the
+ * attenuated Schwarz bounds (int2.F90:675-681).  fock_jk passes
+ * alpha_coulomb = 1, beta_coulomb = 0.  nfocks 1 or 2 as oqph_fock_jk. */
+int oqph_fock_jk_cam(oqph_session *s, const double *d, double *f, int nfocks, double alpha, double beta, double mu,
+                     double alpha_coulomb, double beta_coulomb, oqph_stats *stats);
 
 /* Device-resident builds: d_dev, f_dev are device pointers (ntri doubles) on
  * the session's device; stream 0 means the session stream.  f_dev is
@@ -135,6 +156,15 @@ int oqph_grad1e(int nsh, const int *am, const int *harmonic, const int *atom, co
 
 /* Basis dimensions of the session. */
 int oqph_session_nbf(oqph_session *s);
+
+/* Compiled resources of one class-templated kernel, read from the code
+ * object without launching it: kind 0 = J/K wavefront-task kernel, 1 = J/K
+ * bra-row kernel, 2 = 2e-gradient kernel, 3 = Schwarz kernel (class (la lb|la lb));
+ * out[4] = per-lane private segment bytes, static LDS bytes, registers, largest
+ * block.  Returns 1 when the class has no such kernel.  Classes whose kernel
+ * needs more than 8704 bytes of private segment per lane run on the
+ * run-time-class kernels instead.  No counterpart in the reference. */
+int oqph_kernel_resources(int kind, int la, int lb, int lc, int ld, int pure, int *out);
 
 const char *oqph_last_error(void);
 

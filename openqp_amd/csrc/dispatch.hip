@@ -6,111 +6,39 @@
 
 namespace oqph {
 
-void launch_jk_0000(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_1000(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_1010(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_1100(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_1110(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_1111(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2000(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2010(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2011(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2020(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2100(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2110(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2111(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2120(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2121(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2200(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2210(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2211(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2220(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2221(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_2222(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3000(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3010(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3011(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3020(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3021(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3030(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3100(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3110(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3120(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3200(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3210(bool, const JKParams&, const WaveWork&, hipStream_t);
-void launch_jk_3300(bool, const JKParams&, const WaveWork&, hipStream_t);
-bool launch_jkb_0000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_0000(bool, int*, size_t*, size_t);
-bool launch_jkb_1000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_1000(bool, int*, size_t*, size_t);
-bool launch_jkb_1010(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_1010(bool, int*, size_t*, size_t);
-bool launch_jkb_1100(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_1100(bool, int*, size_t*, size_t);
-bool launch_jkb_1110(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_1110(bool, int*, size_t*, size_t);
-bool launch_jkb_1111(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_1111(bool, int*, size_t*, size_t);
-bool launch_jkb_2000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2000(bool, int*, size_t*, size_t);
-bool launch_jkb_2010(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2010(bool, int*, size_t*, size_t);
-bool launch_jkb_2011(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2011(bool, int*, size_t*, size_t);
-bool launch_jkb_2020(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2020(bool, int*, size_t*, size_t);
-bool launch_jkb_2100(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2100(bool, int*, size_t*, size_t);
-bool launch_jkb_2110(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2110(bool, int*, size_t*, size_t);
-bool launch_jkb_2111(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2111(bool, int*, size_t*, size_t);
-bool launch_jkb_2120(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2120(bool, int*, size_t*, size_t);
-bool launch_jkb_2121(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2121(bool, int*, size_t*, size_t);
-bool launch_jkb_2200(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2200(bool, int*, size_t*, size_t);
-bool launch_jkb_2210(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2210(bool, int*, size_t*, size_t);
-bool launch_jkb_2211(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2211(bool, int*, size_t*, size_t);
-bool launch_jkb_2220(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2220(bool, int*, size_t*, size_t);
-bool launch_jkb_2221(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2221(bool, int*, size_t*, size_t);
-bool launch_jkb_2222(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3000(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3010(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3011(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3020(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3021(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3030(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3100(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3110(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3120(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3200(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3210(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool launch_jkb_3300(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t);
-bool jkb_info_2222(bool, int*, size_t*, size_t);
-bool jkb_info_3000(bool, int*, size_t*, size_t);
-bool jkb_info_3010(bool, int*, size_t*, size_t);
-bool jkb_info_3011(bool, int*, size_t*, size_t);
-bool jkb_info_3020(bool, int*, size_t*, size_t);
-bool jkb_info_3021(bool, int*, size_t*, size_t);
-bool jkb_info_3030(bool, int*, size_t*, size_t);
-bool jkb_info_3100(bool, int*, size_t*, size_t);
-bool jkb_info_3110(bool, int*, size_t*, size_t);
-bool jkb_info_3120(bool, int*, size_t*, size_t);
-bool jkb_info_3200(bool, int*, size_t*, size_t);
-bool jkb_info_3210(bool, int*, size_t*, size_t);
-bool jkb_info_3300(bool, int*, size_t*, size_t);
-void launch_schwarz_00(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_10(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_11(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_20(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_21(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
-void launch_schwarz_22(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, double, double*, hipStream_t);
+// Quartet classes with class-templated kernels.  OQPH_JK_CLASSES: J/K
+// (k_jk, k_jkb; jk_class.hip) -- the s/p/d classes and the small f classes;
+// OQPH_GRAD_CLASSES: 2e gradient (k_grad; grad_class.hip).  The lists match
+// CLASSES + F_JK and the gradient list of build.py (a CPU test checks it).
+#define OQPH_JK_CLASSES(X)                                                                                      \
+  X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(1, 1, 1, 0) X(1, 1, 1, 1) X(2, 0, 0, 0) X(2, 0, 1, 0) \
+  X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 0, 0) X(2, 1, 1, 0) X(2, 1, 1, 1) X(2, 1, 2, 0) X(2, 1, 2, 1) X(2, 2, 0, 0) \
+  X(2, 2, 1, 0) X(2, 2, 1, 1) X(2, 2, 2, 0) X(2, 2, 2, 1) X(2, 2, 2, 2) X(3, 0, 0, 0) X(3, 0, 1, 0) X(3, 0, 1, 1) \
+  X(3, 0, 2, 0) X(3, 0, 2, 1) X(3, 0, 3, 0) X(3, 1, 0, 0) X(3, 1, 1, 0) X(3, 1, 2, 0) X(3, 2, 0, 0) X(3, 2, 1, 0) \
+  X(3, 3, 0, 0)
+#define OQPH_GRAD_CLASSES(X)                                                                                    \
+  X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(1, 1, 1, 0) X(1, 1, 1, 1) X(2, 0, 0, 0) X(2, 0, 1, 0) \
+  X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 0, 0) X(2, 1, 1, 0) X(2, 1, 1, 1) X(2, 1, 2, 0) X(2, 1, 2, 1) X(2, 2, 0, 0) \
+  X(2, 2, 1, 0) X(2, 2, 1, 1) X(2, 2, 2, 0) X(3, 0, 0, 0) X(3, 0, 1, 0) X(3, 0, 1, 1) X(3, 0, 2, 0) X(3, 0, 3, 0) \
+  X(3, 1, 0, 0) X(3, 1, 1, 0) X(3, 2, 0, 0) X(3, 3, 0, 0)
+#define OQPH_SCHWARZ_CLASSES(X) X(0, 0) X(1, 0) X(1, 1) X(2, 0) X(2, 1) X(2, 2)
+
+#define OQPH_CODE(a, b, c, d) ((a) * 1000 + (b) * 100 + (c) * 10 + (d))
+
+#define OQPH_DECL_JK(a, b, c, d)                                                                            \
+  void launch_jk_##a##b##c##d(bool, const JKParams&, const WaveWork&, hipStream_t);                        \
+  bool launch_jkb_##a##b##c##d(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t); \
+  bool jkb_info_##a##b##c##d(bool, int*, size_t*, size_t);                                                 \
+  bool jk_res_##a##b##c##d(int, bool, KernelRes*);
+OQPH_JK_CLASSES(OQPH_DECL_JK)
+#define OQPH_DECL_GR(a, b, c, d)                                              \
+  void launch_grad_##a##b##c##d(const GradParams&, const WaveWork&, hipStream_t); \
+  bool grad_res_##a##b##c##d(KernelRes*);
+OQPH_GRAD_CLASSES(OQPH_DECL_GR)
+#define OQPH_DECL_SCH(a, b)                                                                                  \
+  void launch_schwarz_##a##b(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, \
+                             double, double, double*, hipStream_t);
+OQPH_SCHWARZ_CLASSES(OQPH_DECL_SCH)
 
 __device__ __forceinline__ long long tri_idx_(int i, int j) {
   int a = i > j ? i : j, b = i > j ? j : i;
@@ -119,40 +47,11 @@ __device__ __forceinline__ long long tri_idx_(int i, int j) {
 
 bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const WaveWork& cw, hipStream_t s) {
   if (cw.nwaves == 0) return true;
-  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
-    case 0: launch_jk_0000(pure, p, cw, s); return true;
-    case 1000: launch_jk_1000(pure, p, cw, s); return true;
-    case 1010: launch_jk_1010(pure, p, cw, s); return true;
-    case 1100: launch_jk_1100(pure, p, cw, s); return true;
-    case 1110: launch_jk_1110(pure, p, cw, s); return true;
-    case 1111: launch_jk_1111(pure, p, cw, s); return true;
-    case 2000: launch_jk_2000(pure, p, cw, s); return true;
-    case 2010: launch_jk_2010(pure, p, cw, s); return true;
-    case 2011: launch_jk_2011(pure, p, cw, s); return true;
-    case 2020: launch_jk_2020(pure, p, cw, s); return true;
-    case 2100: launch_jk_2100(pure, p, cw, s); return true;
-    case 2110: launch_jk_2110(pure, p, cw, s); return true;
-    case 2111: launch_jk_2111(pure, p, cw, s); return true;
-    case 2120: launch_jk_2120(pure, p, cw, s); return true;
-    case 2121: launch_jk_2121(pure, p, cw, s); return true;
-    case 2200: launch_jk_2200(pure, p, cw, s); return true;
-    case 2210: launch_jk_2210(pure, p, cw, s); return true;
-    case 2211: launch_jk_2211(pure, p, cw, s); return true;
-    case 2220: launch_jk_2220(pure, p, cw, s); return true;
-    case 2221: launch_jk_2221(pure, p, cw, s); return true;
-    case 2222: launch_jk_2222(pure, p, cw, s); return true;
-    case 3000: launch_jk_3000(pure, p, cw, s); return true;
-    case 3010: launch_jk_3010(pure, p, cw, s); return true;
-    case 3011: launch_jk_3011(pure, p, cw, s); return true;
-    case 3020: launch_jk_3020(pure, p, cw, s); return true;
-    case 3021: launch_jk_3021(pure, p, cw, s); return true;
-    case 3030: launch_jk_3030(pure, p, cw, s); return true;
-    case 3100: launch_jk_3100(pure, p, cw, s); return true;
-    case 3110: launch_jk_3110(pure, p, cw, s); return true;
-    case 3120: launch_jk_3120(pure, p, cw, s); return true;
-    case 3200: launch_jk_3200(pure, p, cw, s); return true;
-    case 3210: launch_jk_3210(pure, p, cw, s); return true;
-    case 3300: launch_jk_3300(pure, p, cw, s); return true;
+  switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): launch_jk_##a##b##c##d(pure, p, cw, s); return true;
+    OQPH_JK_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
     default: return false;
   }
 }
@@ -160,158 +59,71 @@ bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, con
 bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
                 size_t lds, hipStream_t s) {
   if (bw.nitem == 0) return true;
-  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
-    case 0: return launch_jkb_0000(pure, p, bw, cm, lds, s);
-    case 1000: return launch_jkb_1000(pure, p, bw, cm, lds, s);
-    case 1010: return launch_jkb_1010(pure, p, bw, cm, lds, s);
-    case 1100: return launch_jkb_1100(pure, p, bw, cm, lds, s);
-    case 1110: return launch_jkb_1110(pure, p, bw, cm, lds, s);
-    case 1111: return launch_jkb_1111(pure, p, bw, cm, lds, s);
-    case 2000: return launch_jkb_2000(pure, p, bw, cm, lds, s);
-    case 2010: return launch_jkb_2010(pure, p, bw, cm, lds, s);
-    case 2011: return launch_jkb_2011(pure, p, bw, cm, lds, s);
-    case 2020: return launch_jkb_2020(pure, p, bw, cm, lds, s);
-    case 2100: return launch_jkb_2100(pure, p, bw, cm, lds, s);
-    case 2110: return launch_jkb_2110(pure, p, bw, cm, lds, s);
-    case 2111: return launch_jkb_2111(pure, p, bw, cm, lds, s);
-    case 2120: return launch_jkb_2120(pure, p, bw, cm, lds, s);
-    case 2121: return launch_jkb_2121(pure, p, bw, cm, lds, s);
-    case 2200: return launch_jkb_2200(pure, p, bw, cm, lds, s);
-    case 2210: return launch_jkb_2210(pure, p, bw, cm, lds, s);
-    case 2211: return launch_jkb_2211(pure, p, bw, cm, lds, s);
-    case 2220: return launch_jkb_2220(pure, p, bw, cm, lds, s);
-    case 2221: return launch_jkb_2221(pure, p, bw, cm, lds, s);
-    case 2222: return launch_jkb_2222(pure, p, bw, cm, lds, s);
-    case 3000: return launch_jkb_3000(pure, p, bw, cm, lds, s);
-    case 3010: return launch_jkb_3010(pure, p, bw, cm, lds, s);
-    case 3011: return launch_jkb_3011(pure, p, bw, cm, lds, s);
-    case 3020: return launch_jkb_3020(pure, p, bw, cm, lds, s);
-    case 3021: return launch_jkb_3021(pure, p, bw, cm, lds, s);
-    case 3030: return launch_jkb_3030(pure, p, bw, cm, lds, s);
-    case 3100: return launch_jkb_3100(pure, p, bw, cm, lds, s);
-    case 3110: return launch_jkb_3110(pure, p, bw, cm, lds, s);
-    case 3120: return launch_jkb_3120(pure, p, bw, cm, lds, s);
-    case 3200: return launch_jkb_3200(pure, p, bw, cm, lds, s);
-    case 3210: return launch_jkb_3210(pure, p, bw, cm, lds, s);
-    case 3300: return launch_jkb_3300(pure, p, bw, cm, lds, s);
+  switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): return launch_jkb_##a##b##c##d(pure, p, bw, cm, lds, s);
+    OQPH_JK_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
     default: return false;
   }
 }
 
 // false when the class has no bra-row kernel
 bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* static_lds, size_t lds_max) {
-  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
-    case 0: return jkb_info_0000(pure, nrow, static_lds, lds_max);
-    case 1000: return jkb_info_1000(pure, nrow, static_lds, lds_max);
-    case 1010: return jkb_info_1010(pure, nrow, static_lds, lds_max);
-    case 1100: return jkb_info_1100(pure, nrow, static_lds, lds_max);
-    case 1110: return jkb_info_1110(pure, nrow, static_lds, lds_max);
-    case 1111: return jkb_info_1111(pure, nrow, static_lds, lds_max);
-    case 2000: return jkb_info_2000(pure, nrow, static_lds, lds_max);
-    case 2010: return jkb_info_2010(pure, nrow, static_lds, lds_max);
-    case 2011: return jkb_info_2011(pure, nrow, static_lds, lds_max);
-    case 2020: return jkb_info_2020(pure, nrow, static_lds, lds_max);
-    case 2100: return jkb_info_2100(pure, nrow, static_lds, lds_max);
-    case 2110: return jkb_info_2110(pure, nrow, static_lds, lds_max);
-    case 2111: return jkb_info_2111(pure, nrow, static_lds, lds_max);
-    case 2120: return jkb_info_2120(pure, nrow, static_lds, lds_max);
-    case 2121: return jkb_info_2121(pure, nrow, static_lds, lds_max);
-    case 2200: return jkb_info_2200(pure, nrow, static_lds, lds_max);
-    case 2210: return jkb_info_2210(pure, nrow, static_lds, lds_max);
-    case 2211: return jkb_info_2211(pure, nrow, static_lds, lds_max);
-    case 2220: return jkb_info_2220(pure, nrow, static_lds, lds_max);
-    case 2221: return jkb_info_2221(pure, nrow, static_lds, lds_max);
-    case 2222: return jkb_info_2222(pure, nrow, static_lds, lds_max);
-    case 3000: return jkb_info_3000(pure, nrow, static_lds, lds_max);
-    case 3010: return jkb_info_3010(pure, nrow, static_lds, lds_max);
-    case 3011: return jkb_info_3011(pure, nrow, static_lds, lds_max);
-    case 3020: return jkb_info_3020(pure, nrow, static_lds, lds_max);
-    case 3021: return jkb_info_3021(pure, nrow, static_lds, lds_max);
-    case 3030: return jkb_info_3030(pure, nrow, static_lds, lds_max);
-    case 3100: return jkb_info_3100(pure, nrow, static_lds, lds_max);
-    case 3110: return jkb_info_3110(pure, nrow, static_lds, lds_max);
-    case 3120: return jkb_info_3120(pure, nrow, static_lds, lds_max);
-    case 3200: return jkb_info_3200(pure, nrow, static_lds, lds_max);
-    case 3210: return jkb_info_3210(pure, nrow, static_lds, lds_max);
-    case 3300: return jkb_info_3300(pure, nrow, static_lds, lds_max);
+  switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): return jkb_info_##a##b##c##d(pure, nrow, static_lds, lds_max);
+    OQPH_JK_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
     default: return false;
   }
+}
+
+// Compiled resources of one class-templated kernel: kind 0 = k_jk, 1 = k_jkb,
+// 2 = k_grad, 3 = k_schwarz (lc, ld ignored); false when there is no such
+// kernel.  hipFuncGetAttributes reads the code object, nothing is launched.
+bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, KernelRes* r) {
+  if (kind == 0 || kind == 1 || kind == 3) {
+    if (kind == 3) { lc = la; ld = lb; }
+    switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): return jk_res_##a##b##c##d(kind, pure, r);
+      OQPH_JK_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
+      default: return false;
+    }
+  }
+  if (kind == 2) {
+    switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): return grad_res_##a##b##c##d(r);
+      OQPH_GRAD_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
+      default: return false;
+    }
+  }
+  return false;
 }
 
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
-                    const double* tab, const double* xf, double qcut, double* Q, hipStream_t s) {
+                    const double* tab, const double* xf, double qcut, double imu2, double* Q, hipStream_t s) {
   if (n == 0) return true;
   switch (la * 10 + lb) {
-    case 0: launch_schwarz_00(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
-    case 10: launch_schwarz_10(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
-    case 11: launch_schwarz_11(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
-    case 20: launch_schwarz_20(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
-    case 21: launch_schwarz_21(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
-    case 22: launch_schwarz_22(pure, pairs, prims, list, n, tab, xf, qcut, Q, s); return true;
+#define OQPH_CASE(a, b) \
+  case (a) * 10 + (b): launch_schwarz_##a##b(pure, pairs, prims, list, n, tab, xf, qcut, imu2, Q, s); return true;
+    OQPH_SCHWARZ_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
     default: return false;
   }
 }
 
-void launch_grad_0000(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_1000(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_1010(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_1100(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_1110(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_1111(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2000(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2010(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2011(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2020(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2100(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2110(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2111(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2120(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2121(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2200(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2210(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2211(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_2220(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3000(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3010(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3011(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3020(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3030(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3100(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3110(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3200(const GradParams&, const WaveWork&, hipStream_t);
-void launch_grad_3300(const GradParams&, const WaveWork&, hipStream_t);
-
 bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s) {
   if (ww.nwaves == 0) return true;
-  switch (la * 1000 + lb * 100 + lc * 10 + ld) {
-    case 0: launch_grad_0000(p, ww, s); return true;
-    case 1000: launch_grad_1000(p, ww, s); return true;
-    case 1010: launch_grad_1010(p, ww, s); return true;
-    case 1100: launch_grad_1100(p, ww, s); return true;
-    case 1110: launch_grad_1110(p, ww, s); return true;
-    case 1111: launch_grad_1111(p, ww, s); return true;
-    case 2000: launch_grad_2000(p, ww, s); return true;
-    case 2010: launch_grad_2010(p, ww, s); return true;
-    case 2011: launch_grad_2011(p, ww, s); return true;
-    case 2020: launch_grad_2020(p, ww, s); return true;
-    case 2100: launch_grad_2100(p, ww, s); return true;
-    case 2110: launch_grad_2110(p, ww, s); return true;
-    case 2111: launch_grad_2111(p, ww, s); return true;
-    case 2120: launch_grad_2120(p, ww, s); return true;
-    case 2121: launch_grad_2121(p, ww, s); return true;
-    case 2200: launch_grad_2200(p, ww, s); return true;
-    case 2210: launch_grad_2210(p, ww, s); return true;
-    case 2211: launch_grad_2211(p, ww, s); return true;
-    case 2220: launch_grad_2220(p, ww, s); return true;
-    case 3000: launch_grad_3000(p, ww, s); return true;
-    case 3010: launch_grad_3010(p, ww, s); return true;
-    case 3011: launch_grad_3011(p, ww, s); return true;
-    case 3020: launch_grad_3020(p, ww, s); return true;
-    case 3030: launch_grad_3030(p, ww, s); return true;
-    case 3100: launch_grad_3100(p, ww, s); return true;
-    case 3110: launch_grad_3110(p, ww, s); return true;
-    case 3200: launch_grad_3200(p, ww, s); return true;
-    case 3300: launch_grad_3300(p, ww, s); return true;
+  switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): launch_grad_##a##b##c##d(p, ww, s); return true;
+    OQPH_GRAD_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
     default: return false;
   }
 }

@@ -83,7 +83,18 @@ __device__ __forceinline__ PrimJK load_uniform(const PrimJK* p, int i) {
   const __attribute__((address_space(4))) double* q =
       (const __attribute__((address_space(4))) double*)(p + i);
   PrimJK r;
-  r.g = q[0]; r.Kg = q[1]; r.P[0] = q[2]; r.P[1] = q[3]; r.P[2] = q[4]; r.h = q[5];
+  r.g = q[0]; r.Kg = q[1]; r.X[0] = q[2]; r.X[1] = q[3]; r.X[2] = q[4]; r.h = q[5];
+  return r;
+#else
+  return p[i];
+#endif
+}
+__device__ __forceinline__ PrimF32 load_uniform(const PrimF32* p, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(4))) float* q =
+      (const __attribute__((address_space(4))) float*)(p + i);
+  PrimF32 r;
+  r.g = q[0]; r.Kg = q[1]; r.X[0] = q[2]; r.X[1] = q[3]; r.X[2] = q[4]; r.h = q[5];
   return r;
 #else
   return p[i];
@@ -259,31 +270,37 @@ __host__ __device__ constexpr long long flops_grad_prim_rt(int LA, int LB, int L
                           6 * (LA + 2) * (LB + 2) * (LC + 2) * (LD + 1) + 3 * 4 * nc);
 }
 
-// The fields of a primitive pair that the quadrature reads, in one precision.
+// The fields of a primitive pair that the quadrature reads, in one precision:
+// X is the product centre relative to the pair's first centre.
 template <typename R>
-struct PrimR {
-  R g, Kg, P[3], h;
+struct PrimV {
+  R g, Kg, X[3], h;
 };
 template <typename R, typename PP>
-__device__ __forceinline__ PrimR<R> prim_r(const PP& p) {
-  return {(R)p.g, (R)p.Kg, {(R)p.P[0], (R)p.P[1], (R)p.P[2]}, (R)p.h};
+__device__ __forceinline__ PrimV<R> prim_v(const PP& p) {
+  return {(R)p.g, (R)p.Kg, {(R)p.X[0], (R)p.X[1], (R)p.X[2]}, (R)p.h};
 }
 
 // Adds one primitive quartet's (e0|f0) to acc; returns false when the
 // primitive prefactor screen |K_ab K_cd| / sqrt(zeta+eta) < qcut drops it.
+// pp, qq: bra and ket primitive pairs (X = P - A and X = Q - C); AC = A - C,
+// formed in FP64 by the caller, so PA = X_p, QC = X_q and PQ = AC + X_p - X_q.
+// imu2 = 1/mu^2 selects the erf(mu r)/r operator: zeta + eta becomes
+// zeta + eta + zeta eta / mu^2 in every quantity of the quadrature
+// (OpenQP's attenuated Rys setup, source/integrals/int_rys.F90:180-182, 227);
+// imu2 = 0 is the plain Coulomb operator.
 // tabN / tab32N: the NROOT-root blocks of the FP64 / FP32 root tables.
-// PB, PK: PrimPair or PrimR (fields g, Kg, P, h).
-template <int LA, int LB, int LC, int LD, typename R, typename PB, typename PK>
-__device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, const double* __restrict__ Cc,
-                                               const PB& pp, const PK& qq, R qcut2,
-                                               const double* __restrict__ tabN, const float* __restrict__ tab32N,
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AC)[3], R qcut2,
+                                               R imu2, const double* __restrict__ tabN,
+                                               const float* __restrict__ tab32N,
                                                R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]) {
   using Dm = Dims<LA, LB, LC, LD>;
   constexpr int LAB = Dm::LAB, LCD = Dm::LCD, NR = Dm::NROOT;
   constexpr bool F32 = sizeof(R) == 4;
-  const R zeta = (R)pp.g, eta = (R)qq.g;
-  const R fqz = (R)pp.Kg * (R)qq.Kg;
-  const R ze = zeta + eta;
+  const R zeta = pp.g, eta = qq.g;
+  const R fqz = pp.Kg * qq.Kg;
+  const R ze = zeta + eta + zeta * eta * imu2;
   if (fqz * fqz < qcut2 * ze) return false;
   R ize, sq;
   if constexpr (F32) {
@@ -298,16 +315,16 @@ __device__ __forceinline__ bool rys_accumulate(const double* __restrict__ Pa, co
   R pq2 = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    PQ[d] = (R)pp.P[d] - (R)qq.P[d];
-    PA[d] = (R)pp.P[d] - (R)Pa[d];
-    QC[d] = (R)qq.P[d] - (R)Cc[d];
+    PA[d] = pp.X[d];
+    QC[d] = qq.X[d];
+    PQ[d] = AC[d] + pp.X[d] - qq.X[d];
     pq2 += PQ[d] * PQ[d];
   }
   const R T = rho * pq2;
   const R pref = fqz * sq;
   // rho/zeta = eta/(zeta+eta), rho/eta = zeta/(zeta+eta): no further divisions
   const R rz = eta * ize, re = zeta * ize;
-  const R h2z = (R)pp.h, h2e = (R)qq.h, h2ze = (R)0.5 * ize;
+  const R h2z = pp.h, h2e = qq.h, h2ze = (R)0.5 * ize;
   // one root: 2D integrals G_d(i, k) by the vertical recursion, then the
   // (e0|f0) products
   auto root = [&](const R t2, const R wr) {

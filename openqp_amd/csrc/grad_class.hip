@@ -13,4 +13,18 @@ void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const WaveWork& ww, h
   hipLaunchKernelGGL((k_grad<XA, XB, XC, XD>), dim3((unsigned)nb), dim3(bs), sizeof(double) * 3 * p.natm, s, p, ww);
 }
 
+#define OQPH_GR_RES_(a, b, c, d) grad_res_##a##b##c##d
+#define OQPH_GR_RES(a, b, c, d) OQPH_GR_RES_(a, b, c, d)
+
+// Compiled resources of this class's gradient kernel.
+bool OQPH_GR_RES(XA, XB, XC, XD)(KernelRes* r) {
+  hipFuncAttributes a;
+  if (hipFuncGetAttributes(&a, (const void*)k_grad<XA, XB, XC, XD>) != hipSuccess) return false;
+  r->private_bytes = (int)a.localSizeBytes;
+  r->lds_bytes = (int)a.sharedSizeBytes;
+  r->regs = a.numRegs;
+  r->max_threads = a.maxThreadsPerBlock;
+  return true;
+}
+
 }  // namespace oqph

@@ -76,11 +76,11 @@ void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStrea
 }
 
 void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
-                     const double* rys, const double* xf, double qcut, double* Q, hipStream_t s) {
+                     const double* rys, const double* xf, double qcut, double imu2, double* Q, hipStream_t s) {
   if (n == 0) return;
   allow_lds(k_schwarzh, h.lds);
   hipLaunchKernelGGL(k_schwarzh, dim3((unsigned)n), dim3(HL_BLOCK), h.lds, s, pairs, prims, list, n, rys, xf, qcut,
-                     Q, h);
+                     imu2, Q, h);
 }
 
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s) {

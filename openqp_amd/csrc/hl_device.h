@@ -148,8 +148,31 @@ struct HLPrim {
   double PQx, PQy, PQz, PAx, PAy, PAz, QCx, QCy, QCz;
   double T, pref, rz, re, h2z, h2e, h2ze;
 };
-template <typename PP>
-__device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q, const PP& pp, const PP& qq) {
+// J/K records (PrimJK): product centres relative to the pair's first centre,
+// PQ = (A - C) + X_p - X_q; imu2 = 1/mu^2 of the erf-attenuated operator
+// (zeta + eta -> zeta + eta + zeta eta / mu^2, as in rys_accumulate), 0 for 1/r.
+__device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q, const PrimJK& pp, const PrimJK& qq,
+                                          double imu2) {
+  HLPrim s;
+  const double zeta = pp.g, eta = qq.g, ze = zeta + eta + zeta * eta * imu2;
+  const double sq = rsq64(ze), ize = sq * sq, rho = zeta * eta * ize;
+  s.PQx = (P.A[0] - Q.A[0]) + pp.X[0] - qq.X[0];
+  s.PQy = (P.A[1] - Q.A[1]) + pp.X[1] - qq.X[1];
+  s.PQz = (P.A[2] - Q.A[2]) + pp.X[2] - qq.X[2];
+  s.PAx = pp.X[0]; s.PAy = pp.X[1]; s.PAz = pp.X[2];
+  s.QCx = qq.X[0]; s.QCy = qq.X[1]; s.QCz = qq.X[2];
+  s.T = rho * (s.PQx * s.PQx + s.PQy * s.PQy + s.PQz * s.PQz);
+  s.pref = pp.Kg * qq.Kg * sq;
+  s.rz = eta * ize;
+  s.re = zeta * ize;
+  s.h2z = pp.h;
+  s.h2e = qq.h;
+  s.h2ze = 0.5 * ize;
+  return s;
+}
+// gradient records (PrimPair): absolute product centres
+__device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q, const PrimPair& pp,
+                                          const PrimPair& qq) {
   HLPrim s;
   const double zeta = pp.g, eta = qq.g, ze = zeta + eta;
   const double sq = rsq64(ze), ize = sq * sq, rho = zeta * eta * ize;
@@ -264,7 +287,7 @@ __device__ __forceinline__ void hl_xform_inplace(double* buf, int left, int n, i
 // per pair and K_ab^2 K_cd^2 < qcut^2 (zeta + eta) per quartet.
 __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& P, const ShellPair& Q,
                             const PrimJK* prims, const double* __restrict__ rys, const double* __restrict__ xf,
-                            double qcut, double* lds, int& nprim) {
+                            double qcut, double imu2, double* lds, int& nprim) {
   const int lane = threadIdx.x;
   double* G = lds + h.offG;
   double* bufA = lds + h.offA;
@@ -283,9 +306,9 @@ __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& 
       const PrimJK pp = load_uniform(prims, P.prim0 + ip);
       if (fabs(pp.Kg) < qcut) continue;
       const double fqz = pp.Kg * qq.Kg;
-      if (fqz * fqz < qcut2 * (pp.g + qq.g)) continue;
+      if (fqz * fqz < qcut2 * (pp.g + qq.g + pp.g * qq.g * imu2)) continue;
       ++nprim;
-      const HLPrim s = hl_prim(P, Q, pp, qq);
+      const HLPrim s = hl_prim(P, Q, pp, qq, imu2);
       __syncthreads();
       hl_vrr(h, s, rys, G);
       __syncthreads();
@@ -417,21 +440,22 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
   const int ip = __builtin_amdgcn_readfirstlane(ww.seg_bra[seg]);
   const ShellPair P = prm.pairs[ip];
   int iq = 0;
-  bool skip = true;
+  bool skip = true, mine = true;
   if (active) {
     iq = ww.ket[ww.seg_k0[seg] + k];
     const ShellPair Q = prm.pairs[iq];
     const bool p_first = (P.si > Q.si) || (P.si == Q.si && P.sj >= Q.sj);
+    if (prm.own) mine = prm.own[p_first ? ip : iq] != 0;
     const double qbra = p_first ? P.Q : Q.Q;
     const int n = prm.nsh;
     const double* ds = prm.dsh;
     double dm = fmax(4.0 * ds[P.si * n + P.sj], 4.0 * ds[Q.si * n + Q.sj]);
     dm = fmax(dm, fmax(ds[P.sj * n + Q.sj], ds[P.sj * n + Q.si]));
     dm = fmax(dm, fmax(ds[P.si * n + Q.sj], ds[P.si * n + Q.si]));
-    skip = (qbra * prm.maxden < prm.cutoff) || (P.Q * Q.Q * dm < prm.cutoff);
+    skip = !mine || (qbra * prm.maxden < prm.cutoff) || (P.Q * Q.Q * dm < prm.cutoff);
   }
   unsigned long long* ctr = prm.counters + 8 * (int)(wave & (OQPH_CTR_SLOTS - 1));
-  count_wave(ctr + 1, active && skip);
+  count_wave(ctr + 1, active && mine && skip);
   count_wave(ctr + 0, !skip);
   unsigned long long todo = __ballot(!skip);
   const long long fprim = flops_prim_rt(h.la, h.lb, h.lc, h.ld);
@@ -447,7 +471,7 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
     const int jq = __builtin_amdgcn_readfirstlane(__shfl(iq, l));
     const ShellPair Q = prm.pairs[jq];
     int nprim = 0;
-    const double* V = hl_block(h, tb, P, Q, prm.prims, prm.rys, prm.xform, 1.0e-4 * cut, lds, nprim);
+    const double* V = hl_block(h, tb, P, Q, prm.prims, prm.rys, prm.xform, 1.0e-4 * cut, prm.inv_mu2, lds, nprim);
     prims_used += (unsigned long long)nprim;
     if (nprim == 0) continue;
     flops += (unsigned long long)(nprim * fprim + 12 * nout);
@@ -536,14 +560,14 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
 // One block per pair of the list: Q_ab = sqrt(max |(ab|ab)|) over the AO block.
 __global__ __launch_bounds__(HL_BLOCK) void k_schwarzh(const ShellPair* pairs, const PrimJK* prims, const int* list,
                                                        int n, const double* rys, const double* xf, double qcut,
-                                                       double* Qout, HLDims h) {
+                                                       double imu2, double* Qout, HLDims h) {
   extern __shared__ double lds[];
   const HLTabs tb = hl_tables(h, lds);
   if ((int)blockIdx.x >= n) return;
   const int ip = list[blockIdx.x];
   const ShellPair P = pairs[ip];
   int nprim = 0;
-  const double* V = hl_block(h, tb, P, P, prims, rys, xf, qcut, lds, nprim);
+  const double* V = hl_block(h, tb, P, P, prims, rys, xf, qcut, imu2, lds, nprim);
   double m = 0.0;
   if (nprim) {
     const int nv = h.oa * h.ob * h.oc * h.od;

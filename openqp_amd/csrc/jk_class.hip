@@ -61,17 +61,50 @@ bool OQPH_JKB_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWo
   }
 }
 
+#define OQPH_JK_RES_(a, b, c, d) jk_res_##a##b##c##d
+#define OQPH_JK_RES(a, b, c, d) OQPH_JK_RES_(a, b, c, d)
+
+static bool func_res(const void* f, KernelRes* r) {
+  hipFuncAttributes a;
+  if (hipFuncGetAttributes(&a, f) != hipSuccess) return false;
+  r->private_bytes = (int)a.localSizeBytes;
+  r->lds_bytes = (int)a.sharedSizeBytes;
+  r->regs = a.numRegs;
+  r->max_threads = a.maxThreadsPerBlock;
+  return true;
+}
+
+// Compiled resources of this class's kernels: kind 0 = k_jk, 1 = k_jkb,
+// 3 = k_schwarz (diagonal classes only).
+bool OQPH_JK_RES(XA, XB, XC, XD)(int kind, bool pure, KernelRes* r) {
+  if (kind == 0)
+    return func_res(pure ? (const void*)k_jk<XA, XB, XC, XD, true> : (const void*)k_jk<XA, XB, XC, XD, false>, r);
+  if (kind == 1) {
+    if constexpr (Big<XA, XB, XC, XD>::value) {
+      return false;
+    } else {
+      return func_res(pure ? (const void*)k_jkb<XA, XB, XC, XD, true> : (const void*)k_jkb<XA, XB, XC, XD, false>,
+                      r);
+    }
+  }
+#if XA == XC && XB == XD
+  if (kind == 3)
+    return func_res(pure ? (const void*)k_schwarz<XA, XB, true> : (const void*)k_schwarz<XA, XB, false>, r);
+#endif
+  return false;
+}
+
 #if XA == XC && XB == XD
 void OQPH_SCH_LAUNCHER(XA, XB)(bool pure, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
-                               const double* tab, const double* xf, double qcut, double* Q,
+                               const double* tab, const double* xf, double qcut, double imu2, double* Q,
                                hipStream_t s) {
   const int bs = 64;
   if (pure)
     hipLaunchKernelGGL((k_schwarz<XA, XB, true>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, pairs, prims, list, n, tab,
-                       xf, qcut, Q);
+                       xf, qcut, imu2, Q);
   else
     hipLaunchKernelGGL((k_schwarz<XA, XB, false>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, pairs, prims, list, n,
-                       tab, xf, qcut, Q);
+                       tab, xf, qcut, imu2, Q);
 }
 #endif
 

@@ -75,44 +75,58 @@ __device__ __forceinline__ void xform_inplace(const double* __restrict__ xf, R* 
 #define JKB_MAX_BRA_PRIMS 128
 
 // Contracted (e0|f0) of a quartet: Rys quadrature over every surviving
-// primitive quartet.  UNIFORM_BRA: the caller guarantees one bra per
+// primitive quartet.  The FP64 path reads the PrimJK records, the FP32 path
+// the PrimF32 copies (prims32).  UNIFORM_BRA: the caller guarantees one bra per
 // wavefront; its primitive pairs come from the LDS copy sbra when given, else
-// through the constant address space (scalar cache).
+// through the constant address space (scalar cache).  imu2: 1/mu^2 of the
+// erf-attenuated operator, 0 for 1/r.
 template <int LA, int LB, int LC, int LD, typename R, bool UNIFORM_BRA = false>
 __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q, const PrimJK* __restrict__ prims,
-                                         const double* __restrict__ tabN, const float* __restrict__ tab32N, R qcut,
+                                         const PrimF32* __restrict__ prims32, const double* __restrict__ tabN,
+                                         const float* __restrict__ tab32N, R qcut, R imu2,
                                          R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF],
-                                         int& nprim_used, const PrimR<double>* sbra = nullptr) {
+                                         int& nprim_used, const PrimV<double>* sbra = nullptr) {
   using Dm = Dims<LA, LB, LC, LD>;
+  constexpr bool F32 = sizeof(R) == 4;
 #pragma unroll
   for (int e = 0; e < Dm::NE; ++e)
 #pragma unroll
     for (int f = 0; f < Dm::NF; ++f) acc[e][f] = 0;
+  R AC[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) AC[d] = (R)(P.A[d] - Q.A[d]);  // formed in FP64
   const R qcut2 = qcut * qcut;
   nprim_used = 0;
+  auto ket = [&](int i) -> PrimV<R> {
+    if constexpr (F32) return prim_v<float>(prims32[i]);
+    else return prim_v<double>(prims[i]);
+  };
   if (UNIFORM_BRA && sbra) {
     const int npp = __builtin_amdgcn_readfirstlane(P.nprim);
     for (int iq = 0; iq < Q.nprim; ++iq) {
-      const PrimR<R> qq = prim_r<R>(prims[Q.prim0 + iq]);
+      const PrimV<R> qq = ket(Q.prim0 + iq);
       if (fabs(qq.Kg) < qcut) continue;
       for (int ip = 0; ip < npp; ++ip) {
-        const PrimR<double>& s = sbra[ip];
-        const PrimR<R> pp{(R)s.g, (R)s.Kg, {(R)s.P[0], (R)s.P[1], (R)s.P[2]}, (R)s.h};
+        const PrimV<R> pp = prim_v<R>(sbra[ip]);
         if (fabs(pp.Kg) < qcut) continue;
-        nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tabN, tab32N, acc) ? 1 : 0;
+        nprim_used += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AC, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
       }
     }
     return;
   }
   const int pp0 = UNIFORM_BRA ? __builtin_amdgcn_readfirstlane(P.prim0) : P.prim0;
   const int npp = UNIFORM_BRA ? __builtin_amdgcn_readfirstlane(P.nprim) : P.nprim;
+  auto bra = [&](int i) -> PrimV<R> {
+    if constexpr (F32) return prim_v<float>(UNIFORM_BRA ? load_uniform(prims32, i) : prims32[i]);
+    else return prim_v<double>(UNIFORM_BRA ? load_uniform(prims, i) : prims[i]);
+  };
   for (int iq = 0; iq < Q.nprim; ++iq) {
-    const PrimJK qq = prims[Q.prim0 + iq];
-    if (fabs(qq.Kg) < (double)qcut) continue;
+    const PrimV<R> qq = ket(Q.prim0 + iq);
+    if (fabs(qq.Kg) < qcut) continue;
     for (int ip = 0; ip < npp; ++ip) {
-      const PrimJK pp = UNIFORM_BRA ? load_uniform(prims, pp0 + ip) : prims[pp0 + ip];
-      if (fabs(pp.Kg) < (double)qcut) continue;
-      nprim_used += rys_accumulate<LA, LB, LC, LD, R>(P.A, Q.A, pp, qq, qcut2, tabN, tab32N, acc) ? 1 : 0;
+      const PrimV<R> pp = bra(pp0 + ip);
+      if (fabs(pp.Kg) < qcut) continue;
+      nprim_used += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AC, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
     }
   }
 }
@@ -214,13 +228,13 @@ struct Big {
 // Final-representation block of a quartet: V[a][b][c][d] in AO order.
 template <int LA, int LB, int LC, int LD, bool PURE, typename R, bool UNIFORM_BRA = false>
 __device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPair& Q, const PrimJK* __restrict__ prims,
-                                              const double* __restrict__ tabN, const float* __restrict__ tab32N,
-                                              const double* __restrict__ xf, R qcut,
-                                              R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used,
-                                              const PrimR<double>* sbra = nullptr) {
+                                              const PrimF32* __restrict__ prims32, const double* __restrict__ tabN,
+                                              const float* __restrict__ tab32N, const double* __restrict__ xf,
+                                              R qcut, R imu2, R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used,
+                                              const PrimV<double>* sbra = nullptr) {
   using Dm = Dims<LA, LB, LC, LD>;
   R acc[Dm::NE][Dm::NF];
-  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, tabN, tab32N, qcut, acc, nprim_used, sbra);
+  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, prims32, tabN, tab32N, qcut, imu2, acc, nprim_used, sbra);
   R AB[3], CD[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -308,11 +322,13 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
   ShellPair Q{};
   bool skip = true;
   double bound = 0.0;
+  bool mine = true;  // the quartet belongs to this share (OpenQP's MPI split, int2.F90:759-761)
   if (active) {
     iq = ww.ket[ww.seg_k0[seg] + k];
     Q = prm.pairs[iq];
     // ---- OpenQP screening (int2.F90:763-800)
     const bool p_first = (P.si > Q.si) || (P.si == Q.si && P.sj >= Q.sj);
+    if (prm.own) mine = prm.own[p_first ? ip : iq] != 0;
     const double qbra = p_first ? P.Q : Q.Q;
     const int n = prm.nsh;
     const double* ds = prm.dsh;
@@ -320,11 +336,11 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
     dm = fmax(dm, fmax(ds[P.sj * n + Q.sj], ds[P.sj * n + Q.si]));
     dm = fmax(dm, fmax(ds[P.si * n + Q.sj], ds[P.si * n + Q.si]));
     bound = P.Q * Q.Q * dm;
-    skip = (qbra * prm.maxden < prm.cutoff) || (bound < prm.cutoff);
+    skip = !mine || (qbra * prm.maxden < prm.cutoff) || (bound < prm.cutoff);
   }
   unsigned long long* ctr = prm.counters + 8 * (int)(wave & (OQPH_CTR_SLOTS - 1));
   // every lane of the wave reaches the counters (no early exit before them)
-  count_wave(ctr + 1, active && skip);
+  count_wave(ctr + 1, active && mine && skip);
   count_wave(ctr + 0, !skip);
   constexpr int OA = AO<LA, PURE>::N, OB = AO<LB, PURE>::N, OC = AO<LC, PURE>::N, OD = AO<LD, PURE>::N;
   if constexpr (Big<LA, LB, LC, LD>::value) {
@@ -333,8 +349,8 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
     double acc[Dm::NE][Dm::NF];
     int nprim = 0;
     if (!skip)
-      contract<LA, LB, LC, LD, double, true>(P, Q, prm.prims, tabN, tab32N, 1.0e-4 * prm.cutoff,
-                                             acc, nprim);
+      contract<LA, LB, LC, LD, double, true>(P, Q, prm.prims, prm.prims32, tabN, tab32N, 1.0e-4 * prm.cutoff,
+                                             prm.inv_mu2, acc, nprim);
     count_wave(ctr + 2, false);
     sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() +
                           (nprim ? 12ull * OA * OB * OC * OD : 0ull));
@@ -455,15 +471,16 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
     if constexpr (T32) {
       if (lowp) {
         float Vf[NBLK];
-        quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
-                                                         (float)(1.0e-4 * prm.cutoff), Vf, nprim);
+        quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, prm.prims32, tabN, tab32N, prm.xform,
+                                                         (float)(1.0e-4 * prm.cutoff), (float)prm.inv_mu2, Vf,
+                                                         nprim);
 #pragma unroll
         for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
       }
     }
     if (!lowp)
-      quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
-                                                        1.0e-4 * prm.cutoff, V, nprim);
+      quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, prm.prims32, tabN, tab32N, prm.xform,
+                                                        1.0e-4 * prm.cutoff, prm.inv_mu2, V, nprim);
   }
   count_wave(ctr + 2, lowp && !skip);
   sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() + (nprim ? 12ull * NOUT : 0ull));
@@ -636,7 +653,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   constexpr int OA = AO<LA, PURE>::N, OB = AO<LB, PURE>::N, OC = AO<LC, PURE>::N, OD = AO<LD, PURE>::N;
   constexpr int NROW = OA + OB, NOUT = OA * OB * OC * OD, NBLK = Dm::NBLK;
   __shared__ double s_t64[T64 ? rys_len<NR>() : 1];
-  __shared__ PrimR<double> s_bra[JKB_MAX_BRA_PRIMS];
+  __shared__ PrimV<double> s_bra[JKB_MAX_BRA_PRIMS];
   __shared__ float s_t32[T32 ? rys32_len<NR>() : 1];
   __shared__ double s_jab[OA * OB];
   // dynamic LDS: K rows [NROW][ncol] (accumulated); when cm.drows, also D rows
@@ -659,7 +676,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   const ShellPair P = prm.pairs[ip];
   const bool bra_lds = P.nprim <= JKB_MAX_BRA_PRIMS;  // block-uniform
   if (bra_lds)
-    for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra[i] = prim_r<double>(prm.prims[P.prim0 + i]);
+    for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra[i] = prim_v<double>(prm.prims[P.prim0 + i]);
   // the density entries the digestion reads in the bra rows a and b, gathered
   // once per block instead of once per quartet
   const bool drows = cm.drows != 0;  // launch-uniform
@@ -678,7 +695,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   __syncthreads();
   const double* tabN = T64 ? (const double*)s_t64 : prm.rys + RYS_OFF[NR];
   const float* tab32N = T32 ? (const float*)s_t32 : prm.rys32 + RYS32_OFF[NR];
-  const PrimR<double>* sbra = bra_lds ? (const PrimR<double>*)s_bra : nullptr;
+  const PrimV<double>* sbra = bra_lds ? (const PrimV<double>*)s_bra : nullptr;
   const int lane = (int)__lane_id(), wv = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
   const int oa = P.ao_a, ob = P.ao_b;
   const double* Dp = prm.D;
@@ -702,7 +719,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
       const bool active = k < n;
       int iq = 0;
       ShellPair Q{};
-      bool skip = true;
+      bool skip = true, mine = true;
       double bound = 0.0;
       if (active) {
         iq = bw.ket[kbase + k];
@@ -710,6 +727,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
         // ---- OpenQP screening (int2.F90:763-800); the bra shells' rows of
         // the shell-pair maxima come from LDS
         const bool p_first = (P.si > Q.si) || (P.si == Q.si && P.sj >= Q.sj);
+        if (prm.own) mine = prm.own[p_first ? ip : iq] != 0;
         const double qbra = p_first ? P.Q : Q.Q;
         double dm = fmax(4.0 * prm.dsh[P.si * nsh + P.sj], 4.0 * prm.dsh[Q.si * nsh + Q.sj]);
         if (drows) {
@@ -721,9 +739,9 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
           dm = fmax(dm, fmax(ds[P.si * nsh + Q.sj], ds[P.si * nsh + Q.si]));
         }
         bound = P.Q * Q.Q * dm;
-        skip = (qbra * prm.maxden < prm.cutoff) || (bound < prm.cutoff);
+        skip = !mine || (qbra * prm.maxden < prm.cutoff) || (bound < prm.cutoff);
       }
-      n_skip += (active && skip) ? 1 : 0;
+      n_skip += (active && mine && skip) ? 1 : 0;
       n_comp += skip ? 0 : 1;
       double V[NBLK];
 #pragma unroll
@@ -735,15 +753,17 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
         if constexpr (T32) {
           if (lowp) {
             float Vf[NBLK];
-            quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
-                                                             (float)(1.0e-4 * prm.cutoff), Vf, nprim, sbra);
+            quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, prm.prims32, tabN, tab32N,
+                                                             prm.xform, (float)(1.0e-4 * prm.cutoff),
+                                                             (float)prm.inv_mu2, Vf, nprim, sbra);
 #pragma unroll
             for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
           }
         }
         if (!lowp)
-          quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, tabN, tab32N, prm.xform,
-                                                            1.0e-4 * prm.cutoff, V, nprim, sbra);
+          quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, prm.prims32, tabN, tab32N,
+                                                            prm.xform, 1.0e-4 * prm.cutoff, prm.inv_mu2, V, nprim,
+                                                            sbra);
       }
       n_f32 += (lowp && !skip) ? 1 : 0;
       n_valid += nprim ? 1 : 0;
@@ -908,10 +928,12 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
 // Schwarz bounds Q_ab = sqrt(max |(ab|ab)|) for every pair of one class
 // (ints_exchange, source/integrals/int2.F90:1582-1737); the pair records and
 // primitive pairs passed here are built with the tight cutoffs of :1600-1604.
+// imu2 = 1/mu^2 gives the bounds of the erf-attenuated integrals (the mu2
+// argument of ints_exchange, :1597, :1625), 0 those of 1/r.
 template <int LA, int LB, bool PURE>
 __global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const PrimJK* prims, const int* list,
                                                  int n, const double* tab, const double* xf,
-                                                 double qcut, double* Qout) {
+                                                 double qcut, double imu2, double* Qout) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int ip = list[t];
@@ -921,8 +943,8 @@ __global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const P
   int nprim;
   if constexpr (Big<LA, LB, LA, LB>::value) {
     double acc[Dims<LA, LB, LA, LB>::NE][Dims<LA, LB, LA, LB>::NF];
-    contract<LA, LB, LA, LB, double>(P, P, prims, tab + RYS_OFF[Dims<LA, LB, LA, LB>::NROOT], nullptr, qcut, acc,
-                                     nprim);
+    contract<LA, LB, LA, LB, double>(P, P, prims, nullptr, tab + RYS_OFF[Dims<LA, LB, LA, LB>::NROOT], nullptr,
+                                     qcut, imu2, acc, nprim);
     if (nprim)
       stream_slices<LA, LB, LA, LB, PURE>(P, P, xf, acc, [&](int, int, double* S) {
 #pragma unroll
@@ -930,8 +952,8 @@ __global__ __launch_bounds__(256) void k_schwarz(const ShellPair* pairs, const P
       });
   } else {
     double V[Dims<LA, LB, LA, LB>::NBLK];
-    quartet_block<LA, LB, LA, LB, PURE, double>(P, P, prims, tab + RYS_OFF[Dims<LA, LB, LA, LB>::NROOT], nullptr,
-                                                xf, qcut, V, nprim);
+    quartet_block<LA, LB, LA, LB, PURE, double>(P, P, prims, nullptr, tab + RYS_OFF[Dims<LA, LB, LA, LB>::NROOT],
+                                                nullptr, xf, qcut, imu2, V, nprim);
     if (nprim) {
 #pragma unroll
       for (int i = 0; i < N; ++i) m = fmax(m, fabs(V[i]));

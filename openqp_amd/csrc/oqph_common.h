@@ -39,12 +39,22 @@ struct PrimPair {              // 64 bytes: one cache line per primitive pair
 
 // The fields of a primitive pair that the J/K quadrature reads (48 B): the
 // J/K and Schwarz kernels stream these; the gradient keeps PrimPair, whose
-// exponents it needs for the derivatives.
+// exponents it needs for the derivatives.  The product centre is stored
+// relative to the pair's first centre A (X = P - A), so the quadrature forms
+// PA = X, QC = X' and PQ = (A - C) + X - X' from small differences and never
+// from absolute coordinates: results do not depend on where the molecule sits.
 struct PrimJK {
   double g;                    // zeta
   double Kg;                   // K / zeta
-  double P[3];                 // Gaussian product centre
+  double X[3];                 // P - A
   double h;                    // 1 / (2 zeta)
+};
+
+// The same fields in FP32 (24 B), read by the FP32 quadrature of the mixed
+// mode: half the bytes of a PrimJK per primitive pair streamed.  The
+// differences X = P - A are formed in FP64 and then rounded.
+struct PrimF32 {
+  float g, Kg, X[3], h;
 };
 
 // Wavefront tasks of one class-pair launch (J/K build or gradient): segment
@@ -76,6 +86,19 @@ struct HLDims {
   int offD, dstr;              // LDS (doubles): the six density blocks of one quartet (J/K), block stride
   int lds;                     // bytes of dynamic LDS
 };
+
+// Compiled resources of one kernel (hipFuncGetAttributes).
+struct KernelRes {
+  int private_bytes;  // per-lane private segment (register spills)
+  int lds_bytes;      // static LDS per block
+  int regs;           // registers per lane reported by the runtime
+  int max_threads;    // largest block the kernel can be launched with
+};
+// Largest per-lane private segment a class-templated kernel may use before its
+// class is routed to the run-time-class kernels instead: large private
+// segments can fail to get scratch next to other kernels' reservations (a
+// 17 KB one aborted its queue with HSA_STATUS_ERROR_OUT_OF_RESOURCES).
+#define OQPH_PRIVATE_BUDGET 8704
 
 // Primitive-pair-count buckets of the class lists.
 #define OQPH_NBKT 6
@@ -113,6 +136,7 @@ struct ColMap {
 struct JKParams {
   const ShellPair* pairs;
   const PrimJK* prims;
+  const PrimF32* prims32;  // the same primitive pairs in FP32 (mixed mode)
   const double* rys;
   const float* rys32;   // FP32 root fits (degree 6)
   const double* xform;  // [2][2][10][10] canonical Cartesian -> AO transforms (L = 2, 3)
@@ -126,4 +150,7 @@ struct JKParams {
   double sc, se;
   int mixed;            // 1: quartets with bound below fp32_cut run in FP32
   double fp32_cut;
+  double inv_mu2;       // 1/mu^2 of erf(mu r)/r attenuated integrals; 0: plain 1/r
+  const unsigned char* own;  // per pair: 1 when this share owns quartets whose
+                             // reference-order first pair it is; null: all
 };

@@ -32,7 +32,8 @@ bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, co
                 size_t lds, hipStream_t s);
 bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* static_lds, size_t lds_max);
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
-                    const double* tab, const double* xf, double qcut, double* Q, hipStream_t s);
+                    const double* tab, const double* xf, double qcut, double imu2, double* Q, hipStream_t s);
+bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, KernelRes* r);
 void launch_shell_density(const double* D, const double* D2, const int* ao_off, const int* nao, int nsh, double* dsh,
                           hipStream_t s);
 void launch_add(const double* x, const double* y, double* z, long long n, hipStream_t s);
@@ -43,7 +44,7 @@ bool grad_template(int la, int lb, int lc, int ld);
 HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm);
 void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStream_t s);
 void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
-                     const double* rys, const double* xf, double qcut, double* Q, hipStream_t s);
+                     const double* rys, const double* xf, double qcut, double imu2, double* Q, hipStream_t s);
 struct GradParams;
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s);
 bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s);
@@ -89,6 +90,10 @@ template <typename T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
   int ensure(size_t m) {
     if (m <= n && p) return 0;
     if (p) hipFree(p);
@@ -140,6 +145,16 @@ struct oqph_session {
   std::vector<std::vector<double>> clsQ;     // matching Q values
   DBuf<ShellPair> d_pairs;
   DBuf<PrimJK> d_prims;
+  DBuf<PrimF32> d_prims32;                   // FP32 copies of d_prims (mixed mode)
+  std::vector<ShellPair> h_dpairs;           // host copy of d_pairs
+  std::vector<long long> dev_pid;            // pair id i(i+1)/2+j of each device pair
+  // erf-attenuated pass (CAM): pair records with the attenuated Schwarz bounds
+  DBuf<ShellPair> d_pairs_att;
+  double att_mu = -1.0;
+  // OpenQP's MPI share of the bra shell pairs (int2.F90:759-761): rank of size
+  int share_rank = 0, share_size = 1;
+  bool share_ready = false;
+  DBuf<unsigned char> d_own;                 // per device pair: 1 when the share owns it
   DBuf<double> d_rys, d_xform;
   DBuf<float> d_rys32;
   DBuf<int> d_cls;                           // concatenated class lists
@@ -267,12 +282,129 @@ static int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
   HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
   return 0;
 }
+// upload from a host buffer that may go out of scope right after: the copy
+// has completed when this returns (set-up paths only)
+static int upload_sync(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
 
-// the J/K fields of primitive pairs
-static std::vector<PrimJK> to_jk(const std::vector<PrimPair>& v) {
-  std::vector<PrimJK> r(v.size());
-  for (size_t k = 0; k < v.size(); ++k) r[k] = {v[k].g, v[k].Kg, {v[k].P[0], v[k].P[1], v[k].P[2]}, v[k].h};
-  return r;
+// J/K primitive-pair records of one shell pair: product centre relative to
+// the pair's first centre A, formed in FP64 (and the FP32 copy)
+static PrimJK jk_record(const PrimPair& q, const ShellPair& sp) {
+  return {q.g, q.Kg, {q.P[0] - sp.A[0], q.P[1] - sp.A[1], q.P[2] - sp.A[2]}, q.h};
+}
+static PrimF32 f32_record(const PrimJK& q) {
+  return {(float)q.g, (float)q.Kg, {(float)q.X[0], (float)q.X[1], (float)q.X[2]}, (float)q.h};
+}
+
+// Schwarz bounds Q = sqrt(max |(ab|ab)|) of every shell pair (pair id
+// i(i+1)/2 + j) with the tight cutoffs of ints_exchange (exponent 50,
+// quartet 1e-17; int2.F90:1600-1604); imu2 = 1/mu^2 gives the bounds of the
+// erf-attenuated integrals (0: 1/r).
+static int schwarz_bounds(oqph_session* s, double imu2, std::vector<double>& Qout) {
+  int rc;
+  std::vector<ShellPair> tp;
+  std::vector<PrimPair> tpr;
+  build_pairs(*s, Cut{1.0e-17, 50.0}, tp, tpr);
+  const size_t np = tp.size();
+  std::vector<PrimJK> tjk;
+  tjk.reserve(tpr.size());
+  for (size_t pid = 0; pid < np; ++pid)
+    for (int k = tp[pid].prim0; k < tp[pid].prim0 + tp[pid].nprim; ++k) tjk.push_back(jk_record(tpr[k], tp[pid]));
+  std::vector<std::vector<int>> byc(10);
+  for (size_t pid = 0; pid < np; ++pid) byc[class_of(tp[pid].la, tp[pid].lb)].push_back((int)pid);
+  std::vector<int> flat;
+  std::vector<size_t> foff(11, 0);
+  for (int c = 0; c < 10; ++c) {
+    foff[c] = flat.size();
+    flat.insert(flat.end(), byc[c].begin(), byc[c].end());
+  }
+  foff[10] = flat.size();
+  DBuf<ShellPair> dtp;
+  DBuf<PrimJK> dtpr;
+  DBuf<int> dlist;
+  DBuf<double> dQ;
+  if ((rc = dtp.ensure(np)) || (rc = dtpr.ensure(tjk.size())) || (rc = dlist.ensure(np)) || (rc = dQ.ensure(np)))
+    return rc;
+  if ((rc = upload_sync(dtp.p, tp.data(), np * sizeof(ShellPair), s->stream))) return rc;
+  if (!tjk.empty() && (rc = upload_sync(dtpr.p, tjk.data(), tjk.size() * sizeof(PrimJK), s->stream))) return rc;
+  if ((rc = upload_sync(dlist.p, flat.data(), flat.size() * sizeof(int), s->stream))) return rc;
+  HIPCHK(hipMemsetAsync(dQ.p, 0, np * sizeof(double), s->stream));
+  for (int la = 0; la <= s->lmax; ++la)
+    for (int lb = 0; lb <= la; ++lb) {
+      const int c = class_of(la, lb);
+      if (byc[c].empty()) continue;
+      KernelRes kr{};
+      const bool tmpl = la < 3 && kernel_resources(3, la, lb, la, lb, s->pure, &kr) &&
+                        kr.private_bytes <= OQPH_PRIVATE_BUDGET;
+      if (!tmpl) {
+        const HLDims h = hl_dims(la, lb, la, lb, s->pure, 0, 0);
+        if ((size_t)h.lds > s->lds_max) {
+          set_err("Schwarz class (%d %d): %d bytes of LDS exceed the device's %zu", la, lb, h.lds, s->lds_max);
+          HIPCHK(hipStreamSynchronize(s->stream));
+          return 1;
+        }
+        launch_schwarzh(h, dtp.p, dtpr.p, dlist.p + foff[c], (int)byc[c].size(), s->d_rys.p, s->d_xform.p, 1.0e-17,
+                        imu2, dQ.p, s->stream);
+        continue;
+      }
+      if (!launch_schwarz(la, lb, s->pure, dtp.p, dtpr.p, dlist.p + foff[c], (int)byc[c].size(), s->d_rys.p,
+                          s->d_xform.p, 1.0e-17, imu2, dQ.p, s->stream)) {
+        set_err("no Schwarz kernel for class (%d %d)", la, lb);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return 1;
+      }
+    }
+  Qout.assign(np, 0.0);
+  const hipError_t e1 = hipGetLastError();
+  const hipError_t e2 = hipMemcpyAsync(Qout.data(), dQ.p, np * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  const hipError_t e3 = hipStreamSynchronize(s->stream);  // before the temporaries are freed
+  HIPCHK(e1);
+  HIPCHK(e2);
+  HIPCHK(e3);
+  return 0;
+}
+
+// OpenQP's shell-pair work list order (int2_build_shell_pair_map,
+// source/integrals/int2.F90:864-921): pairs (i, j), i >= j (1-based), in
+// classes of log2 of the cost Q_ij nbf_i nbf_j i (Cartesian counts), most
+// expensive class first, i descending then j ascending inside a class.
+// Returns the 1-based position ij_pair of every pair id i(i+1)/2 + j (0-based
+// i, j); rank r of size n owns the pairs with mod(ij_pair, n) == r
+// (:759-761).
+static std::vector<long long> reference_pair_order(const oqph_session& s) {
+  const int nsh = s.nsh, NCLASS = 64, OFFS = 42;
+  auto cls = [&](int i, int j) {  // 0-based shells
+    const int nbi = ncart(s.am[i]), nbj = ncart(s.am[j]);
+    const double sw = std::max(s.Q[(size_t)i * (i + 1) / 2 + j], 1.0e-30);
+    const double cost = sw * (double)(nbi * nbj) * (double)(i + 1);
+    return NCLASS - std::max(0, std::min(NCLASS, (int)(std::log(cost) / std::log(2.0)) + OFFS));
+  };
+  std::vector<long long> off(NCLASS + 1, 0), cnt(NCLASS + 1, 0);
+  for (int i = nsh - 1; i >= 0; --i)
+    for (int j = 0; j <= i; ++j) ++cnt[cls(i, j)];
+  for (int c = 1; c <= NCLASS; ++c) off[c] = off[c - 1] + cnt[c - 1];
+  std::vector<long long> pos((size_t)nsh * (nsh + 1) / 2);
+  for (int i = nsh - 1; i >= 0; --i)
+    for (int j = 0; j <= i; ++j) pos[(size_t)i * (i + 1) / 2 + j] = ++off[cls(i, j)];
+  return pos;
+}
+
+// Per-device-pair ownership flags of the current (rank, size) share.
+static int prepare_share(oqph_session* s) {
+  if (s->share_size <= 1) return 0;
+  if (s->share_ready) return 0;
+  const std::vector<long long> pos = reference_pair_order(*s);
+  std::vector<unsigned char> own(s->dev_pid.size());
+  for (size_t k = 0; k < s->dev_pid.size(); ++k)
+    own[k] = (pos[s->dev_pid[k]] % s->share_size) == s->share_rank ? 1 : 0;
+  int rc;
+  if ((rc = s->d_own.ensure(std::max<size_t>(own.size(), 1)))) return rc;
+  if (!own.empty() && (rc = upload_sync(s->d_own.p, own.data(), own.size(), s->stream))) return rc;
+  s->share_ready = true;
+  return 0;
 }
 
 static int prepare_geometry(oqph_session* s) {
@@ -292,64 +424,23 @@ static int prepare_geometry(oqph_session* s) {
   size_t nt;
   const double* tab = rys_table_host(&nt);
   if ((rc = s->d_rys.ensure(nt))) return rc;
-  if ((rc = upload(s->d_rys.p, tab, nt * sizeof(double), s->stream))) return rc;
+  if ((rc = upload_sync(s->d_rys.p, tab, nt * sizeof(double), s->stream))) return rc;
   size_t n32;
   const float* tab32 = rys32_table_host(&n32);
   if ((rc = s->d_rys32.ensure(n32))) return rc;
-  if ((rc = upload(s->d_rys32.p, tab32, n32 * sizeof(float), s->stream))) return rc;
+  if ((rc = upload_sync(s->d_rys32.p, tab32, n32 * sizeof(float), s->stream))) return rc;
   double xf[2][2][10][10];
   build_xform(xf);
   if ((rc = s->d_xform.ensure(400))) return rc;
-  if ((rc = upload(s->d_xform.p, &xf[0][0][0][0], 400 * sizeof(double), s->stream))) return rc;
-  // Schwarz with the tight cutoffs of ints_exchange: exponent 50, quartet 1e-17
-  std::vector<ShellPair> tp;
-  std::vector<PrimPair> tpr;
-  build_pairs(*s, Cut{1.0e-17, 50.0}, tp, tpr);
-  const size_t np = tp.size();
-  DBuf<ShellPair> dtp;
-  DBuf<PrimJK> dtpr;
-  DBuf<int> dlist;
-  DBuf<double> dQ;
-  if ((rc = dtp.ensure(np)) || (rc = dtpr.ensure(tpr.size())) || (rc = dlist.ensure(np)) || (rc = dQ.ensure(np)))
-    return rc;
-  if ((rc = upload(dtp.p, tp.data(), np * sizeof(ShellPair), s->stream))) return rc;
-  const std::vector<PrimJK> tjk = to_jk(tpr);
-  if (!tjk.empty() && (rc = upload(dtpr.p, tjk.data(), tjk.size() * sizeof(PrimJK), s->stream))) return rc;
-  std::vector<std::vector<int>> byc(10);
-  for (size_t pid = 0; pid < np; ++pid) byc[class_of(tp[pid].la, tp[pid].lb)].push_back((int)pid);
-  std::vector<int> flat;
-  std::vector<size_t> foff(11, 0);
-  for (int c = 0; c < 10; ++c) {
-    foff[c] = flat.size();
-    flat.insert(flat.end(), byc[c].begin(), byc[c].end());
+  if ((rc = upload_sync(s->d_xform.p, &xf[0][0][0][0], 400 * sizeof(double), s->stream))) return rc;
+  {
+    const char* ft = getenv("OQPH_F_TEMPLATES");
+    s->f_templates = !(ft && ft[0] == '0');
+    const char* e = getenv("OQPH_JK_ROWS");
+    s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
   }
-  foff[10] = flat.size();
-  if ((rc = upload(dlist.p, flat.data(), flat.size() * sizeof(int), s->stream))) return rc;
-  HIPCHK(hipMemsetAsync(dQ.p, 0, np * sizeof(double), s->stream));
-  for (int la = 0; la <= s->lmax; ++la)
-    for (int lb = 0; lb <= la; ++lb) {
-      const int c = class_of(la, lb);
-      if (la == 3) {
-        const HLDims h = hl_dims(la, lb, la, lb, s->pure, 0, 0);
-        if ((size_t)h.lds > s->lds_max) {
-          set_err("Schwarz class (%d %d): %d bytes of LDS exceed the device's %zu", la, lb, h.lds, s->lds_max);
-          return 1;
-        }
-        launch_schwarzh(h, dtp.p, dtpr.p, dlist.p + foff[c], (int)byc[c].size(), s->d_rys.p, s->d_xform.p, 1.0e-17,
-                        dQ.p, s->stream);
-        continue;
-      }
-      if (!launch_schwarz(la, lb, s->pure, dtp.p, dtpr.p, dlist.p + foff[c], (int)byc[c].size(), s->d_rys.p,
-                          s->d_xform.p, 1.0e-17, dQ.p, s->stream)) {
-        set_err("no Schwarz kernel for class (%d %d)", la, lb);
-        return 1;
-      }
-    }
-  HIPCHK(hipGetLastError());
-  s->Q.resize(np);
-  HIPCHK(hipMemcpyAsync(s->Q.data(), dQ.p, np * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  dtp.release(); dtpr.release(); dlist.release(); dQ.release();
+  if ((rc = schwarz_bounds(s, 0.0, s->Q))) return rc;
+  const size_t np = s->Q.size();
   // regular pair data (int2.F90:266-272): quartet 1e-4 c, exponent 25 ln 10
   std::vector<PrimPair> prims;
   build_pairs(*s, Cut{1.0e-4 * s->cutoff, 25.0 * std::log(10.0)}, s->pairs, prims);
@@ -383,6 +474,7 @@ static int prepare_geometry(oqph_session* s) {
     all.insert(all.end(), v.begin(), v.end());
   }
   s->cls_off[10] = all.size();
+  s->dev_pid.assign(all.begin(), all.end());
   // Device pair and primitive-pair records in class-list order: device pair
   // id i is s->pairs[all[i]], so the 64 kets of a wavefront (consecutive
   // list entries) read consecutive records; the class lists on the device
@@ -390,33 +482,37 @@ static int prepare_geometry(oqph_session* s) {
   {
     std::vector<ShellPair> dp(all.size());
     std::vector<PrimJK> dpr;
+    std::vector<PrimF32> dpr32;
     dpr.reserve(prims.size());
+    dpr32.reserve(prims.size());
     for (size_t i = 0; i < all.size(); ++i) {
       dp[i] = s->pairs[all[i]];
       dp[i].prim0 = (int)dpr.size();
       const ShellPair& o = s->pairs[all[i]];
       for (int k = o.prim0; k < o.prim0 + o.nprim; ++k) {
-        const PrimPair& q = prims[k];
-        dpr.push_back({q.g, q.Kg, {q.P[0], q.P[1], q.P[2]}, q.h});
+        dpr.push_back(jk_record(prims[k], o));
+        dpr32.push_back(f32_record(dpr.back()));
       }
     }
+    s->h_dpairs = dp;
     std::vector<int> ident(all.size());
     for (size_t i = 0; i < all.size(); ++i) ident[i] = (int)i;
     if ((rc = s->d_pairs.ensure(std::max<size_t>(dp.size(), 1))) ||
         (rc = s->d_prims.ensure(std::max<size_t>(dpr.size(), 1))) ||
+        (rc = s->d_prims32.ensure(std::max<size_t>(dpr.size(), 1))) ||
         (rc = s->d_cls.ensure(std::max<size_t>(all.size(), 1))))
       return rc;
-    if (!dp.empty() && ((rc = upload(s->d_pairs.p, dp.data(), dp.size() * sizeof(ShellPair), s->stream)) ||
-                        (rc = upload(s->d_prims.p, dpr.data(), dpr.size() * sizeof(PrimJK), s->stream)) ||
-                        (rc = upload(s->d_cls.p, ident.data(), ident.size() * sizeof(int), s->stream))))
+    if (!dp.empty() && ((rc = upload_sync(s->d_pairs.p, dp.data(), dp.size() * sizeof(ShellPair), s->stream)) ||
+                        (rc = upload_sync(s->d_prims.p, dpr.data(), dpr.size() * sizeof(PrimJK), s->stream)) ||
+                        (rc = upload_sync(s->d_prims32.p, dpr32.data(), dpr32.size() * sizeof(PrimF32), s->stream)) ||
+                        (rc = upload_sync(s->d_cls.p, ident.data(), ident.size() * sizeof(int), s->stream))))
       return rc;
-    HIPCHK(hipStreamSynchronize(s->stream));  // the host vectors end here
   }
   if ((rc = s->d_kbs.ensure(kbs_all.size() + 2))) return rc;
-  if ((rc = upload(s->d_kbs.p, kbs_all.data(), kbs_all.size() * sizeof(int), s->stream))) return rc;
+  if ((rc = upload_sync(s->d_kbs.p, kbs_all.data(), kbs_all.size() * sizeof(int), s->stream))) return rc;
   if ((rc = s->d_aooff.ensure(s->nsh)) || (rc = s->d_nao.ensure(s->nsh))) return rc;
-  if ((rc = upload(s->d_aooff.p, s->aooff.data(), s->nsh * sizeof(int), s->stream))) return rc;
-  if ((rc = upload(s->d_nao.p, s->nao.data(), s->nsh * sizeof(int), s->stream))) return rc;
+  if ((rc = upload_sync(s->d_aooff.p, s->aooff.data(), s->nsh * sizeof(int), s->stream))) return rc;
+  if ((rc = upload_sync(s->d_nao.p, s->nao.data(), s->nsh * sizeof(int), s->stream))) return rc;
   // LDS column spaces of the bra-row kernel: the AOs of the shells of each
   // angular momentum, in shell order
   s->h_colidx.assign(s->nsh, 0);
@@ -435,15 +531,9 @@ static int prepare_geometry(oqph_session* s) {
     s->ncolL[L] = off;
   }
   if ((rc = s->d_colidx.ensure(s->nsh)) || (rc = s->d_colao.ensure(s->h_colao.size()))) return rc;
-  if ((rc = upload(s->d_colidx.p, s->h_colidx.data(), s->nsh * sizeof(int), s->stream)) ||
-      (rc = upload(s->d_colao.p, s->h_colao.data(), s->h_colao.size() * sizeof(int), s->stream)))
+  if ((rc = upload_sync(s->d_colidx.p, s->h_colidx.data(), s->nsh * sizeof(int), s->stream)) ||
+      (rc = upload_sync(s->d_colao.p, s->h_colao.data(), s->h_colao.size() * sizeof(int), s->stream)))
     return rc;
-  {
-    const char* ft = getenv("OQPH_F_TEMPLATES");
-    s->f_templates = !(ft && ft[0] == '0');
-    const char* e = getenv("OQPH_JK_ROWS");
-    s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
-  }
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
   if ((rc = s->d_D.ensure(ntri)) || (rc = s->d_F.ensure(ntri)) || (rc = s->d_dsh.ensure((size_t)s->nsh * s->nsh)) ||
       (rc = s->d_ctr.ensure(8)))
@@ -451,7 +541,28 @@ static int prepare_geometry(oqph_session* s) {
   HIPCHK(hipStreamSynchronize(s->stream));
   s->cached_maxden = -1.0;
   s->gcut_ready = -1.0;
+  s->att_mu = -1.0;
+  s->share_ready = false;
   s->ready = true;
+  return 0;
+}
+
+// Device pair records of the erf-attenuated pass (mu): the regular records
+// with Q replaced by the attenuated Schwarz bound, which the in-kernel screens
+// read (int2.F90:675-681).  The attenuated bound never exceeds the regular
+// one (erfc(mu r)/r is a positive kernel too), so the regular coarse lists
+// stay a superset and the kernels' exact screens decide.
+static int prepare_attenuated(oqph_session* s, double mu) {
+  if (s->att_mu == mu) return 0;
+  std::vector<double> Qa;
+  int rc = schwarz_bounds(s, 1.0 / (mu * mu), Qa);
+  if (rc) return rc;
+  std::vector<ShellPair> dp = s->h_dpairs;
+  for (size_t k = 0; k < dp.size(); ++k) dp[k].Q = Qa[s->dev_pid[k]];
+  if ((rc = s->d_pairs_att.ensure(std::max<size_t>(dp.size(), 1)))) return rc;
+  if (!dp.empty() && (rc = upload_sync(s->d_pairs_att.p, dp.data(), dp.size() * sizeof(ShellPair), s->stream)))
+    return rc;
+  s->att_mu = mu;
   return 0;
 }
 
@@ -462,6 +573,12 @@ static int prepare_grad(oqph_session* s) {
   int rc = prepare_geometry(s);
   if (rc) return rc;
   if (s->gcut_ready == s->grad_cutoff) return 0;
+  // the per-atom gradient sums are staged in LDS (3 natm doubles per block)
+  if (24 * (size_t)s->natm + 16384 > s->lds_max) {
+    set_err("gradient: %d atoms need %zu bytes of LDS per block for the per-atom sums (device: %zu)", s->natm,
+            24 * (size_t)s->natm, s->lds_max);
+    return 1;
+  }
   std::vector<ShellPair> gp;
   std::vector<PrimPair> gpr;
   build_pairs(*s, Cut{1.0e-20, 20.0 * std::log(10.0)}, gp, gpr);
@@ -507,9 +624,9 @@ static int prepare_grad(oqph_session* s) {
       (rc = s->d_gprims.ensure(std::max<size_t>(dgpr.size(), 1))) ||
       (rc = s->d_gcls.ensure(std::max<size_t>(all.size(), 1))))
     return rc;
-  if (!all.empty() && ((rc = upload(s->d_gpairs.p, dgp.data(), dgp.size() * sizeof(ShellPair), s->stream)) ||
-                       (rc = upload(s->d_gprims.p, dgpr.data(), dgpr.size() * sizeof(PrimPair), s->stream)) ||
-                       (rc = upload(s->d_gcls.p, ident.data(), ident.size() * sizeof(int), s->stream))))
+  if (!all.empty() && ((rc = upload_sync(s->d_gpairs.p, dgp.data(), dgp.size() * sizeof(ShellPair), s->stream)) ||
+                       (rc = upload_sync(s->d_gprims.p, dgpr.data(), dgpr.size() * sizeof(PrimPair), s->stream)) ||
+                       (rc = upload_sync(s->d_gcls.p, ident.data(), ident.size() * sizeof(int), s->stream))))
     return rc;
   // segments (bra p, ket bucket b): the kets with Q_p Q_k >= grad_cutoff form
   // a prefix of the bucket; within one class each unordered pair is taken
@@ -571,10 +688,10 @@ static int prepare_grad(oqph_session* s) {
   const size_t nseg = sbra.size();
   if ((rc = s->d_gseg.ensure(3 * nseg + 1)) || (rc = s->d_gchoff.ensure(choff.size() + 1))) return rc;
   if (nseg) {
-    if ((rc = upload(s->d_gseg.p, sbra.data(), nseg * sizeof(int), s->stream)) ||
-        (rc = upload(s->d_gseg.p + nseg, sk0.data(), nseg * sizeof(int), s->stream)) ||
-        (rc = upload(s->d_gseg.p + 2 * nseg, sn.data(), nseg * sizeof(int), s->stream)) ||
-        (rc = upload(s->d_gchoff.p, choff.data(), choff.size() * sizeof(long long), s->stream)))
+    if ((rc = upload_sync(s->d_gseg.p, sbra.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload_sync(s->d_gseg.p + nseg, sk0.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload_sync(s->d_gseg.p + 2 * nseg, sn.data(), nseg * sizeof(int), s->stream)) ||
+        (rc = upload_sync(s->d_gchoff.p, choff.data(), choff.size() * sizeof(long long), s->stream)))
       return rc;
   }
   s->glaunches.clear();
@@ -589,6 +706,11 @@ static int prepare_grad(oqph_session* s) {
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
     L.hl = !grad_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3));
+    if (!L.hl) {  // templates whose private segment exceeds the budget run on k_gradh
+      KernelRes kr{};
+      L.hl = !kernel_resources(2, p.la, p.lb, p.lc, p.ld, s->pure, &kr) || kr.private_bytes > OQPH_PRIVATE_BUDGET ||
+             (size_t)kr.lds_bytes + 24 * (size_t)s->natm > s->lds_max;
+    }
     if (L.hl) {
       L.hd = hl_dims(p.la, p.lb, p.lc, p.ld, s->pure, 1, s->natm);
       if ((size_t)L.hd.lds > s->lds_max) {
@@ -612,10 +734,10 @@ static int prepare_grad(oqph_session* s) {
       (rc = s->d_pure.ensure(s->nsh)) || (rc = s->d_Dc.ensure((size_t)ncf * ncf)) ||
       (rc = s->d_de.ensure(3 * (size_t)s->natm * OQPH_DE_SLOTS)))
     return rc;
-  if ((rc = upload(s->d_coff.p, cof.data(), s->nsh * sizeof(int), s->stream)) ||
-      (rc = upload(s->d_satom.p, s->atom.data(), s->nsh * sizeof(int), s->stream)) ||
-      (rc = upload(s->d_am.p, s->am.data(), s->nsh * sizeof(int), s->stream)) ||
-      (rc = upload(s->d_pure.p, pu.data(), s->nsh * sizeof(int), s->stream)))
+  if ((rc = upload_sync(s->d_coff.p, cof.data(), s->nsh * sizeof(int), s->stream)) ||
+      (rc = upload_sync(s->d_satom.p, s->atom.data(), s->nsh * sizeof(int), s->stream)) ||
+      (rc = upload_sync(s->d_am.p, s->am.data(), s->nsh * sizeof(int), s->stream)) ||
+      (rc = upload_sync(s->d_pure.p, pu.data(), s->nsh * sizeof(int), s->stream)))
     return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   s->gcut_ready = s->grad_cutoff;
@@ -644,6 +766,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
     int nitem = 0, ncolC = 0, ncol = 0;
     size_t lds = 0;
     bool drows = false;
+    bool hl = false;  // run-time-class kernel (k_jkh)
   };
   s->h_items.clear();
   std::vector<Pend> pend;
@@ -699,8 +822,15 @@ static int prepare_lists(oqph_session* s, double maxden) {
       int nrow = 0;
       size_t slds = 0;
       // run-time-class kernel for the f classes without a class template
-      const bool hl = !jk_template(la, lb, lc, ld) || (!s->f_templates && (la == 3 || lc == 3));
-      bool rows = !hl && s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max);
+      // templates whose private segment exceeds the budget run on k_jkh instead
+      KernelRes kr0{}, kr1{};
+      const bool tmpl = jk_template(la, lb, lc, ld) && !(!s->f_templates && (la == 3 || lc == 3));
+      const bool fit0 = tmpl && kernel_resources(0, la, lb, lc, ld, s->pure, &kr0) &&
+                        kr0.private_bytes <= OQPH_PRIVATE_BUDGET;
+      const bool fit1 = tmpl && kernel_resources(1, la, lb, lc, ld, s->pure, &kr1) &&
+                        kr1.private_bytes <= OQPH_PRIVATE_BUDGET;
+      const bool hl = !fit0 && !fit1;
+      bool rows = fit1 && s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max);
       // dynamic LDS: K rows (nrow x ncol), plus D rows (nrow x ncol) and the
       // bra rows of dsh (2 x nsh) when those fit as well
       const size_t dynk = (size_t)nrow * ncol * sizeof(double);
@@ -708,7 +838,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
       const bool drows = rows && s->rows_mode != 3 && slds + dynd <= s->lds_max;
       const size_t dynb = drows ? dynd : dynk;
       rows = rows && slds + dynb <= s->lds_max;
-      if (rows && s->rows_mode == 1) {
+      if (rows && s->rows_mode == 1 && fit0) {
         const double old_at = (double)nket * (oc * od + (oa + obn) * (oc + od));
         const double new_at = (double)nket * oc * od + (double)nbra * nrow * ncol;
         rows = new_at < 0.5 * old_at;
@@ -742,6 +872,11 @@ static int prepare_lists(oqph_session* s, double maxden) {
         pend.push_back(q);
         continue;
       }
+      if (!fit0 && !hl) {
+        set_err("class (%d %d|%d %d): its bra-row kernel does not fit in LDS and its wavefront kernel exceeds the "
+                "private-segment budget", la, lb, lc, ld);
+        return 1;
+      }
       const size_t seg0 = s->h_seg_bra.size(), ch0 = s->h_choff.size();
       struct Seg {
         int bra, k0, n;
@@ -770,6 +905,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
       const int nseg = (int)(s->h_seg_bra.size() - seg0);
       s->h_choff.push_back(waves);  // terminator of this launch's prefix
       pend.push_back({la, lb, lc, ld, ck, seg0, ch0, nseg, waves});
+      pend.back().hl = hl;
     }
   int rc;
   const size_t nseg = s->h_seg_bra.size();
@@ -812,7 +948,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
     L.ww.ket = s->d_cls.p + s->cls_off[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
-    L.hl = !jk_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3));
+    L.hl = p.hl;
     if (L.hl) {
       L.hd = hl_dims(p.la, p.lb, p.lc, p.ld, s->pure, 0, 0);
       if ((size_t)L.hd.lds > s->lds_max) {
@@ -829,11 +965,14 @@ static int prepare_lists(oqph_session* s, double maxden) {
 }
 
 // G[D] on device buffers; D already on the device.  The screening densities
-// scr_a (and scr_b) default to D; zero_f = false accumulates into dF.
+// scr_a (and scr_b) default to D; zero_f = false accumulates into dF.  mu > 0:
+// the erf(mu r)/r attenuated pass (pair records with the attenuated bounds).
 static int build_device(oqph_session* s, const double* dD, double* dF, double se, double sc, oqph_stats* st,
                         hipStream_t stream, const double* scr_a = nullptr, const double* scr_b = nullptr,
-                        bool zero_f = true) {
+                        bool zero_f = true, double mu = 0.0) {
   int rc;
+  if (mu > 0.0 && (rc = prepare_attenuated(s, mu))) return rc;
+  if ((rc = prepare_share(s))) return rc;
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
   launch_shell_density(scr_a ? scr_a : dD, scr_b, s->d_aooff.p, s->d_nao.p, s->nsh, s->d_dsh.p, stream);
   HIPCHK(hipMemsetAsync(s->d_ctr.p, 0, 8 * sizeof(unsigned long long), stream));
@@ -861,8 +1000,11 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     }
   }
   JKParams p{};
-  p.pairs = s->d_pairs.p;
+  p.pairs = mu > 0.0 ? s->d_pairs_att.p : s->d_pairs.p;
   p.prims = s->d_prims.p;
+  p.prims32 = s->d_prims32.p;
+  p.inv_mu2 = mu > 0.0 ? 1.0 / (mu * mu) : 0.0;
+  p.own = s->share_size > 1 ? s->d_own.p : nullptr;
   p.rys = s->d_rys.p;
   p.rys32 = s->d_rys32.p;
   p.xform = s->d_xform.p;
@@ -1109,7 +1251,7 @@ int oqph_schwarz(oqph_session* s, double* q) {
 // shlden does for nfocks = 2 (int2.F90:999-1047), so they keep the same
 // quartets as the reference's single pass.  d and f hold alpha then beta.
 static int build_uhf(oqph_session* s, const double* d_dev, double* f_dev, double se, double sc, oqph_stats* st,
-                     hipStream_t stream) {
+                     hipStream_t stream, bool zero_f = true, double mu = 0.0) {
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
   int rc;
   if ((rc = s->d_Dt.ensure(ntri))) return rc;
@@ -1117,16 +1259,30 @@ static int build_uhf(oqph_session* s, const double* d_dev, double* f_dev, double
   double *fa = f_dev, *fb = f_dev + ntri;
   launch_add(da, db, s->d_Dt.p, (long long)ntri, stream);
   oqph_stats s1{}, s2{}, s3{};
-  if ((rc = build_device(s, s->d_Dt.p, fa, 0.0, sc, &s1, stream, da, db, true))) return rc;
-  HIPCHK(hipMemcpyAsync(fb, fa, ntri * sizeof(double), hipMemcpyDeviceToDevice, stream));
-  if ((rc = build_device(s, da, fa, 2.0 * se, 0.0, &s2, stream, da, db, false))) return rc;
-  if ((rc = build_device(s, db, fb, 2.0 * se, 0.0, &s3, stream, da, db, false))) return rc;
+  if (zero_f) {
+    if ((rc = build_device(s, s->d_Dt.p, fa, 0.0, sc, &s1, stream, da, db, true, mu))) return rc;
+    HIPCHK(hipMemcpyAsync(fb, fa, ntri * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  } else {
+    if ((rc = build_device(s, s->d_Dt.p, fa, 0.0, sc, &s1, stream, da, db, false, mu))) return rc;
+    if ((rc = build_device(s, s->d_Dt.p, fb, 0.0, sc, nullptr, stream, da, db, false, mu))) return rc;
+  }
+  if ((rc = build_device(s, da, fa, 2.0 * se, 0.0, &s2, stream, da, db, false, mu))) return rc;
+  if ((rc = build_device(s, db, fb, 2.0 * se, 0.0, &s3, stream, da, db, false, mu))) return rc;
   if (st) {
     *st = s1;
     st->kernel_ms = s1.kernel_ms + s2.kernel_ms + s3.kernel_ms;
     st->model_flops = s1.model_flops + s2.model_flops + s3.model_flops;
   }
   return 0;
+}
+
+static void add_stats(oqph_stats* a, const oqph_stats& b) {
+  a->quartets_computed += b.quartets_computed;
+  a->quartets_screened += b.quartets_screened;
+  a->quartets_fp32 += b.quartets_fp32;
+  a->kernel_ms += b.kernel_ms;
+  a->model_flops += b.model_flops;
+  a->prim_quartets += b.prim_quartets;
 }
 
 int oqph_fock_jk_device(oqph_session* s, const double* d_dev, double* f_dev, int nfocks, double se, double sc,
@@ -1267,6 +1423,82 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
     HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     st->kernel_ms = ms;
   }
+  return 0;
+}
+
+// CAM build (int2_run_cam, source/integrals/int2.F90:538-584): pass 1 with
+// 1/r, Coulomb scale alpha_coulomb and exchange scale alpha; pass 2 with
+// erf(mu r)/r, Coulomb scale beta_coulomb and exchange scale beta; both
+// accumulate into f, which then carries fock_jk's scaling.
+int oqph_fock_jk_cam(oqph_session* s, const double* d, double* f, int nfocks, double alpha, double beta, double mu,
+                     double alpha_coulomb, double beta_coulomb, oqph_stats* st) {
+  if (!s || !d || !f) return 4;
+  if (int fc = fault_code()) {
+    set_err("OQPH_FAULT injected decline");
+    return fc;
+  }
+  if (nfocks != 1 && nfocks != 2) {
+    set_err("nfocks=%d: only nfocks = 1 (RHF) and 2 (UHF/ROHF) are supported", nfocks);
+    return 1;
+  }
+  if (!(mu > 0.0)) {
+    set_err("fock_jk_cam: mu must be positive");
+    return 4;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  int rc = prepare_geometry(s);
+  if (rc) return rc;
+  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+  const size_t nt = ntri * (size_t)nfocks;
+  if ((rc = s->d_D.ensure(nt)) || (rc = s->d_F.ensure(nt))) return rc;
+  if ((rc = upload(s->d_D.p, d, nt * sizeof(double), s->stream))) return rc;
+  oqph_stats s1{}, s2{};
+  if (nfocks == 2) {
+    if ((rc = build_uhf(s, s->d_D.p, s->d_F.p, alpha, alpha_coulomb, &s1, s->stream, true, 0.0))) return rc;
+    if ((rc = build_uhf(s, s->d_D.p, s->d_F.p, beta, beta_coulomb, &s2, s->stream, false, mu))) return rc;
+  } else {
+    if ((rc = build_device(s, s->d_D.p, s->d_F.p, alpha, alpha_coulomb, &s1, s->stream, nullptr, nullptr, true, 0.0)))
+      return rc;
+    if ((rc = build_device(s, s->d_D.p, s->d_F.p, beta, beta_coulomb, &s2, s->stream, nullptr, nullptr, false, mu)))
+      return rc;
+  }
+  HIPCHK(hipMemcpyAsync(f, s->d_F.p, nt * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (st) {
+    *st = s1;
+    add_stats(st, s2);
+  }
+  return 0;
+}
+
+int oqph_session_set_share(oqph_session* s, int rank, int size) {
+  if (!s || size < 1 || rank < 0 || rank >= size) {
+    set_err("set_share: need 0 <= rank < size");
+    return 4;
+  }
+  s->share_rank = rank;
+  s->share_size = size;
+  s->share_ready = false;
+  return 0;
+}
+
+int oqph_session_prepare(oqph_session* s) {
+  if (!s) return 4;
+  HIPCHK(hipSetDevice(s->device));
+  return prepare_geometry(s);
+}
+
+int oqph_kernel_resources(int kind, int la, int lb, int lc, int ld, int pure, int* out) {
+  if (!out) return 4;
+  KernelRes r{};
+  if (!kernel_resources(kind, la, lb, lc, ld, pure != 0, &r)) {
+    set_err("no class-templated kernel of kind %d for (%d %d|%d %d)", kind, la, lb, lc, ld);
+    return 1;
+  }
+  out[0] = r.private_bytes;
+  out[1] = r.lds_bytes;
+  out[2] = r.regs;
+  out[3] = r.max_threads;
   return 0;
 }
 

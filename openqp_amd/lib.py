@@ -28,7 +28,8 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_session_set_basis", "oqph_session_set_geometry", "oqph_session_set_thresholds",
            "oqph_session_set_precision", "oqph_set_default_session", "oqph_fock_jk", "oqph_fock_jk_inc",
            "oqph_grad2", "oqph_fock_jk_device", "oqph_schwarz", "oqph_session_nbf", "oqph_last_error",
-           "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e"]
+           "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e",
+           "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources"]
 
 
 class OqphError(RuntimeError):
@@ -64,6 +65,11 @@ def load(path: str | None = None):
     L.oqph_fock_jk_device.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                                       C.POINTER(Stats), C.c_void_p]
     L.oqph_schwarz.argtypes = [vp, _pd]
+    L.oqph_session_prepare.argtypes = [vp]
+    L.oqph_session_set_share.argtypes = [vp, C.c_int, C.c_int]
+    L.oqph_fock_jk_cam.argtypes = [vp, _pd, _pd, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.POINTER(Stats)]
+    L.oqph_kernel_resources.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _pi]
     L.oqph_session_nbf.argtypes = [vp]
     L.oqph_last_error.restype = C.c_char_p
     L.oqph_session_set_profiling.argtypes = [vp, C.c_int]
@@ -113,6 +119,22 @@ class FockBuilder:
 
     def set_precision(self, policy: int, fp32_cutoff: float = 0.0):
         _check(load().oqph_session_set_precision(self._s, policy, fp32_cutoff), "set_precision")
+
+    def set_share(self, rank: int, size: int):
+        """Restrict builds to OpenQP MPI rank `rank` of `size` (int2.F90:759-761)."""
+        _check(load().oqph_session_set_share(self._s, rank, size), "set_share")
+
+    def prepare(self):
+        _check(load().oqph_session_prepare(self._s), "prepare")
+
+    def fock_jk_cam(self, d_packed, alpha: float, beta: float, mu: float, alpha_coulomb: float = 1.0,
+                    beta_coulomb: float = 0.0, nfocks: int = 1) -> np.ndarray:
+        """Packed CAM G = G[1/r; alpha_coulomb, alpha] + G[erf(mu r)/r; beta_coulomb, beta] (int2_run_cam)."""
+        d = np.ascontiguousarray(d_packed, np.float64)
+        f = np.zeros_like(d)
+        _check(load().oqph_fock_jk_cam(self._s, d, f, nfocks, alpha, beta, mu, alpha_coulomb, beta_coulomb,
+                                       C.byref(self.last_stats)), "fock_jk_cam")
+        return f
 
     def make_default(self):
         _check(load().oqph_set_default_session(self._s), "set_default_session")
@@ -185,6 +207,13 @@ class FockBuilder:
             self.close()
         except Exception:
             pass
+
+
+def kernel_resources(kind: int, la: int, lb: int, lc: int, ld: int, pure: bool):
+    """(private bytes per lane, static LDS, registers, largest block) of a class-templated kernel, or None."""
+    out = np.zeros(4, np.int32)
+    rc = load().oqph_kernel_resources(kind, la, lb, lc, ld, 1 if pure else 0, out)
+    return None if rc != 0 else tuple(int(x) for x in out)
 
 
 def int1e(bas: Basis):

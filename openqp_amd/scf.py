@@ -10,10 +10,14 @@ Restates the semantics of OpenQP's scf_driver for closed shells
   (scf.F90:768-773);
 * convergence: DIIS error max|X^T (F D S - S D F) X| below ``conv`` and the
   energy change below ``econv``;
-* Pulay DIIS on the orthonormal-basis commutator.
+* Pulay DIIS on the orthonormal-basis commutator; the B matrix is scaled by
+  its largest diagonal entry and the oldest vectors are dropped while it is
+  ill-conditioned (reciprocal condition number below 1e-12), so the
+  extrapolation coefficients stay meaningful near convergence.
 
-The initial guess is the core Hamiltonian (converged energies do not depend
-on it for the closed-shell systems used here).
+Initial guess: OpenQP's extended-Hueckel guess (``guess="huckel"``,
+source/modules/guess_huckel.F90:33; openqp_amd/guess.py) or the core
+Hamiltonian (``guess="core"``).
 """
 from __future__ import annotations
 
@@ -21,7 +25,6 @@ import time
 from dataclasses import dataclass, field
 
 import numpy as np
-import scipy.linalg as sl
 
 from . import basis as B
 from .lib import FockBuilder, int1e
@@ -43,13 +46,16 @@ class SCFResult:
 class RHF:
     def __init__(self, bas: B.Basis, builder: FockBuilder | None = None, nocc: int | None = None,
                  conv: float = 1e-6, econv: float = 1e-8, max_iter: int = 100, incremental: bool = True,
-                 diis_size: int = 8):
+                 diis_size: int = 8, guess: str = "huckel"):
         self.bas = bas
         self.builder = builder or FockBuilder(bas)
         self.nocc = nocc if nocc is not None else int(round(bas.zn.sum())) // 2
         self.conv, self.econv, self.max_iter = conv, econv, max_iter
         self.incremental = incremental
         self.diis_size = diis_size
+        if guess not in ("huckel", "core"):
+            raise ValueError(f"unknown guess {guess!r} (huckel, core)")
+        self.guess = guess
         S, T, V = int1e(bas)
         self.S, self.H = S, T + V
         self.enuc = B.nuclear_repulsion(bas.zn, bas.xyz)
@@ -64,9 +70,43 @@ class RHF:
         Cocc = C[:, :self.nocc]
         return 2.0 * Cocc @ Cocc.T, C, e
 
+    def initial_density(self) -> np.ndarray:
+        if self.guess == "huckel":
+            from .guess import huckel_guess
+            C, _ = huckel_guess(self.bas, self.nocc)
+            Cocc = C[:, :self.nocc]
+            return 2.0 * Cocc @ Cocc.T
+        return self._density(self.H)[0]
+
+    @staticmethod
+    def diis_coefficients(errs) -> np.ndarray:
+        """Pulay coefficients for the stored error vectors: B scaled by its
+        largest diagonal entry; the oldest vectors are dropped while B is
+        ill-conditioned (rcond < 1e-12).  Returns one coefficient per stored
+        vector (zeros for the dropped ones)."""
+        n = len(errs)
+        G = np.array([[float(np.sum(errs[i] * errs[j])) for j in range(n)] for i in range(n)])
+        for k in range(n - 1):
+            m = n - k
+            Gk = G[k:, k:]
+            scale = float(np.max(np.diag(Gk)))
+            Bm = -np.ones((m + 1, m + 1))
+            Bm[m, m] = 0.0
+            Bm[:m, :m] = Gk / scale
+            if 1.0 / np.linalg.cond(Bm) < 1.0e-12:
+                continue
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            c = np.zeros(n)
+            c[k:] = np.linalg.solve(Bm, rhs)[:m]
+            return c
+        c = np.zeros(n)
+        c[-1] = 1.0
+        return c
+
     def run(self, D0: np.ndarray | None = None) -> SCFResult:
         nbf = self.bas.nbf
-        D = self._density(self.H)[0] if D0 is None else D0
+        D = self.initial_density() if D0 is None else D0
         dp = B.pack_tri(D)
         d_old = np.zeros_like(dp)
         f_old = np.zeros_like(dp)
@@ -99,20 +139,9 @@ class RHF:
             if len(focks) > self.diis_size:
                 focks.pop(0)
                 errs.pop(0)
-            n = len(focks)
-            if n > 1:
-                Bm = -np.ones((n + 1, n + 1))
-                Bm[n, n] = 0.0
-                for i in range(n):
-                    for j in range(n):
-                        Bm[i, j] = np.sum(errs[i] * errs[j])
-                rhs = np.zeros(n + 1)
-                rhs[n] = -1.0
-                try:
-                    c = sl.solve(Bm, rhs)[:n]
-                    Fx = sum(ci * Fi for ci, Fi in zip(c, focks))
-                except sl.LinAlgError:
-                    Fx = F
+            if len(focks) > 1:
+                c = self.diis_coefficients(errs)
+                Fx = sum(ci * Fi for ci, Fi in zip(c, focks) if ci != 0.0)
             else:
                 Fx = F
             D, C, eps = self._density(Fx)

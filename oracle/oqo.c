@@ -237,6 +237,7 @@ static int nao_of(const oqo_basis *b, int s) {
 typedef struct {
   double integral, pair, quartet, expo;
   double dab2, dabcut;   /* gradient: skip primitive quartets with fqz^2 dabmax^2 < dabcut ze */
+  double imu2;           /* 1/mu^2 of the erf(mu r)/r operator; 0: 1/r */
 } oqo_cut;
 
 static oqo_cut make_cut(double c) { /* int2.F90:266-272 */
@@ -346,7 +347,8 @@ static int raw_block(int La, int Lb, int Lc, int Ld, const double *A, const doub
       double sp = p->K * p->ginv;
       if (fabs(sp) < cut->quartet) continue;             /* int_rotaxis.F90:295 */
       double fqz = sp * sq, zeta = p->g, eta = q->g, ze = zeta + eta;
-      if (fqz * fqz < qc2 * ze) continue;                /* int_rotaxis.F90:629 */
+      /* int_rotaxis.F90:629; attenuated: zeta + eta + zeta eta / mu^2 (int_rys.F90:227-232) */
+      if (fqz * fqz < qc2 * (ze + zeta * eta * cut->imu2)) continue;
       if (cut->dabcut > 0.0 && fqz * fqz * cut->dab2 < cut->dabcut * ze) continue; /* grd2_rys.F90:376 */
       any = 1;
       double rho = zeta * eta / ze;
@@ -361,8 +363,19 @@ static int raw_block(int La, int Lb, int Lc, int Ld, const double *A, const doub
       }
       /* K includes sqrt(2) pi^(5/4) c c exp(); product/(zeta eta sqrt(ze)) = 2 pi^(5/2)... */
       double pref = fqz / sqrt(ze) * wfac(mode, p, q);
-      oqo_boys(Mt, rho * pq2, F);
-      for (int m = 0; m <= Mt; ++m) WI(0, 0, m) = pref * F[m];
+      if (cut->imu2 > 0.0) {
+        /* erf(mu r)/r: the auxiliary integrals become w^(m+1/2) F_m(w T) with
+         * w = mu^2 / (mu^2 + rho), which obey the same Obara-Saika recursion
+         * since w^(m+1/2) F_m(w T) = (-d/dT)^m [w^(1/2) F_0(w T)] (Ahlrichs,
+         * Phys. Chem. Chem. Phys. 8, 3072 (2006)); 1/mu^2 -> 0 gives 1/r back. */
+        double om = 1.0 / (1.0 + rho * cut->imu2);
+        oqo_boys(Mt, om * rho * pq2, F);
+        double f = sqrt(om);
+        for (int m = 0; m <= Mt; ++m) { WI(0, 0, m) = pref * f * F[m]; f *= om; }
+      } else {
+        oqo_boys(Mt, rho * pq2, F);
+        for (int m = 0; m <= Mt; ++m) WI(0, 0, m) = pref * F[m];
+      }
       double i2z = 0.5 / zeta, i2e = 0.5 / eta, i2ze = 0.5 / ze, rz = rho / zeta, re = rho / eta;
       /* bra VRR: [e0|00]^(m) */
       for (int e = 1; e < ne; ++e) {
@@ -543,12 +556,23 @@ static int max_prim(const oqo_basis *bs) {
 
 /* Schwarz table: Q_ab = sqrt(max |(ab|ab)|) with the tight cutoffs of
  * int2.F90:1600-1604 (ic 1e-15, pair/quartet 1e-17, exponent 50). */
+void oqo_schwarz_mu(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                    const int *nprim, const int *poff, const int *aooff, const double *ex,
+                    const double *cc, const double *xyz, double imu2, double *Q);
 void oqo_schwarz(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                  const int *nprim, const int *poff, const int *aooff, const double *ex,
                  const double *cc, const double *xyz, double *Q) {
+  oqo_schwarz_mu(nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, 0.0, Q);
+}
+
+/* The same for the erf(mu r)/r operator, imu2 = 1/mu^2 (ints_exchange with
+ * mu2, int2.F90:1597, 1625). */
+void oqo_schwarz_mu(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                    const int *nprim, const int *poff, const int *aooff, const double *ex,
+                    const double *cc, const double *xyz, double imu2, double *Q) {
   build_tables();
   oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
-  oqo_cut cut = {1.0e-15, 1.0e-17, 1.0e-17, 50.0, 0.0, 0.0};
+  oqo_cut cut = {1.0e-15, 1.0e-17, 1.0e-17, 50.0, 0.0, 0.0, imu2};
   int mp = max_prim(&bs);
 #pragma omp parallel
   {
@@ -574,25 +598,41 @@ void oqo_schwarz(int nsh, int nbf, int natm, const int *am, const int *harm, con
 }
 
 /* Full final-representation ERI block for one shell quartet (tests) */
+int oqo_eri_quartet_mu(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                       const int *nprim, const int *poff, const int *aooff, const double *ex,
+                       const double *cc, const double *xyz, int i, int j, int k, int l, double cutoff,
+                       double imu2, double *out);
 int oqo_eri_quartet(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                     const int *nprim, const int *poff, const int *aooff, const double *ex,
                     const double *cc, const double *xyz, int i, int j, int k, int l, double cutoff,
                     double *out) {
+  return oqo_eri_quartet_mu(nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, i, j, k, l, cutoff,
+                            0.0, out);
+}
+
+/* The same block for the erf(mu r)/r operator, imu2 = 1/mu^2. */
+int oqo_eri_quartet_mu(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                       const int *nprim, const int *poff, const int *aooff, const double *ex,
+                       const double *cc, const double *xyz, int i, int j, int k, int l, double cutoff,
+                       double imu2, double *out) {
   build_tables();
   oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
   oqo_cut cut = make_cut(cutoff);
+  cut.imu2 = imu2;
   if (cutoff <= 0.0) { cut.pair = cut.quartet = 0.0; cut.expo = 1e300; }
   int mp = max_prim(&bs);
   work_t w;
   work_init(&w);
   ppair *bp = (ppair *)malloc(sizeof(ppair) * mp * mp), *bq = (ppair *)malloc(sizeof(ppair) * mp * mp);
   double *raw = (double *)malloc(sizeof(double) * BLK), *tmp = (double *)malloc(sizeof(double) * BLK);
-  int r = eri_shell(&bs, i, j, k, l, &cut, &w, bp, bq, out, raw, tmp);
-  if (!r) {
-    int n = nao_of(&bs, i) * nao_of(&bs, j) * nao_of(&bs, k) * nao_of(&bs, l);
-    memset(out, 0, sizeof(double) * n);
-  }
-  free(bp); free(bq); free(raw); free(tmp);
+  /* the projections ping-pong between two full-size buffers: the caller's
+   * out holds only the final block */
+  double *blk = (double *)malloc(sizeof(double) * BLK);
+  int r = eri_shell(&bs, i, j, k, l, &cut, &w, bp, bq, blk, raw, tmp);
+  int n = nao_of(&bs, i) * nao_of(&bs, j) * nao_of(&bs, k) * nao_of(&bs, l);
+  if (r) memcpy(out, blk, sizeof(double) * n);
+  else memset(out, 0, sizeof(double) * n);
+  free(bp); free(bq); free(raw); free(tmp); free(blk);
   work_free(&w);
   return r;
 }
@@ -745,34 +785,42 @@ static double max6(double a, double b, double c, double d, double e, double f) {
  * stats[0] = nschwz (skipped quartets), stats[1] = computed quartets,
  * stats[2] = stored integrals.
  */
-int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
-                     const int *nprim, const int *poff, const int *aooff, const double *ex,
-                     const double *cc, const double *xyz, const double *Q, const double *d, double *f,
-                     double se, double sc, double cutoff, long *stats, int rank, int size);
+static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                      const int *nprim, const int *poff, const int *aooff, const double *ex,
+                      const double *cc, const double *xyz, const double *Q, const double *Qmap, const double *d,
+                      double *f, double se, double sc, double cutoff, double imu2, long *stats, int rank, int size);
 
 int oqo_fock_jk(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                 const int *nprim, const int *poff, const int *aooff, const double *ex,
                 const double *cc, const double *xyz, const double *Q, const double *d, double *f,
                 double se, double sc, double cutoff, long *stats) {
-  return oqo_fock_jk_part(nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, d, f, se, sc,
-                          cutoff, stats, 0, 1);
+  return fock_jk_nf(1, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, NULL, d, f, se, sc,
+                    cutoff, 0.0, stats, 0, 1);
 }
 
-/* Same build restricted to the shell pairs (i, j) that rank `rank` of `size`
- * owns under OpenQP's MPI decomposition mod(ij_pair, size) == rank
- * (int2.F90:759-761), ij_pair counting (i, j) pairs in loop order.  With
- * size > 1 this is a representative 1/size sample of the work. */
-static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
-                      const int *nprim, const int *poff, const int *aooff, const double *ex,
-                      const double *cc, const double *xyz, const double *Q, const double *d, double *f,
-                      double se, double sc, double cutoff, long *stats, int rank, int size);
-
+/* Same build restricted to the quartets rank `rank` of `size` computes under
+ * OpenQP's MPI decomposition mod(ij_pair, size) == rank (int2.F90:759-761),
+ * ij_pair the 1-based position of the outer pair (i, j) in the cost-sorted
+ * shell-pair map (int2_build_shell_pair_map, :864-921). */
 int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                      const int *nprim, const int *poff, const int *aooff, const double *ex,
                      const double *cc, const double *xyz, const double *Q, const double *d, double *f,
                      double se, double sc, double cutoff, long *stats, int rank, int size) {
-  return fock_jk_nf(1, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, d, f, se, sc, cutoff,
-                    stats, rank, size);
+  return fock_jk_nf(1, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, NULL, d, f, se, sc,
+                    cutoff, 0.0, stats, rank, size);
+}
+
+/* General form: nfocks 1 (RHF) or 2 (UHF/ROHF); Q screens the quartets, Qmap
+ * (NULL: Q) orders the shell-pair map -- OpenQP orders it by the regular
+ * bounds also in the attenuated pass (int2.F90:883-892); imu2 = 1/mu^2 of the
+ * erf(mu r)/r operator (0: 1/r). */
+int oqo_fock_jk_ex(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                   const int *nprim, const int *poff, const int *aooff, const double *ex,
+                   const double *cc, const double *xyz, const double *Q, const double *Qmap, const double *d,
+                   double *f, int nfocks, double se, double sc, double cutoff, double imu2, long *stats, int rank,
+                   int size) {
+  return fock_jk_nf(nfocks, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, Qmap, d, f, se, sc,
+                    cutoff, imu2, stats, rank, size);
 }
 
 /* UHF/ROHF build (nfocks = 2): d and f hold the alpha then the beta packed
@@ -783,23 +831,57 @@ int oqo_fock_jk_uhf(int nsh, int nbf, int natm, const int *am, const int *harm, 
                     const int *nprim, const int *poff, const int *aooff, const double *ex,
                     const double *cc, const double *xyz, const double *Q, const double *d, double *f,
                     double se, double sc, double cutoff, long *stats) {
-  return fock_jk_nf(2, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, d, f, se, sc, cutoff,
-                    stats, 0, 1);
+  return fock_jk_nf(2, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, NULL, d, f, se, sc,
+                    cutoff, 0.0, stats, 0, 1);
+}
+
+/* int2_build_shell_pair_map (int2.F90:864-921): 1-based position of pair
+ * (i, j), i >= j (0-based here), in the work list sorted by 64 classes of
+ * log2(Q_ij nbf_i nbf_j i), nbf the Cartesian counts, most expensive class
+ * first, i descending then j ascending inside a class. */
+static long *pair_map_positions(int nsh, const int *am, const double *Q) {
+  const int NCLASS = 64, OFFS = 42;
+  long cnt[65] = {0}, off[65] = {0};
+  long *pos = (long *)malloc(sizeof(long) * (size_t)nsh * (nsh + 1) / 2);
+#define OQO_CLS(i, j)                                                                       \
+  ({                                                                                        \
+    double sw_ = Q[(i) * nsh + (j)] > 1.0e-30 ? Q[(i) * nsh + (j)] : 1.0e-30;               \
+    double cost_ = sw_ * (double)(NCART(am[i]) * NCART(am[j])) * (double)((i) + 1);         \
+    int t_ = (int)(log(cost_) / log(2.0)) + OFFS;                                           \
+    t_ = t_ < 0 ? 0 : (t_ > NCLASS ? NCLASS : t_);                                          \
+    NCLASS - t_;                                                                            \
+  })
+  for (int i = nsh - 1; i >= 0; --i)
+    for (int j = 0; j <= i; ++j) ++cnt[OQO_CLS(i, j)];
+  for (int c = 1; c <= NCLASS; ++c) off[c] = off[c - 1] + cnt[c - 1];
+  for (int i = nsh - 1; i >= 0; --i)
+    for (int j = 0; j <= i; ++j) pos[(long)i * (i + 1) / 2 + j] = ++off[OQO_CLS(i, j)];
+#undef OQO_CLS
+  return pos;
+}
+
+/* the pair-map position of every pair, for tests */
+void oqo_pair_map(int nsh, const int *am, const double *Q, long *out) {
+  long *pos = pair_map_positions(nsh, am, Q);
+  memcpy(out, pos, sizeof(long) * (size_t)nsh * (nsh + 1) / 2);
+  free(pos);
 }
 
 static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                       const int *nprim, const int *poff, const int *aooff, const double *ex,
-                      const double *cc, const double *xyz, const double *Q, const double *d, double *f,
-                      double se, double sc, double cutoff, long *stats, int rank, int size) {
+                      const double *cc, const double *xyz, const double *Q, const double *Qmap, const double *d,
+                      double *f, double se, double sc, double cutoff, double imu2, long *stats, int rank, int size) {
   build_tables();
   oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
   oqo_cut cut = make_cut(cutoff);
+  cut.imu2 = imu2;
   long ntri = (long)nbf * (nbf + 1) / 2;
   double *dsh = (double *)malloc(sizeof(double) * nsh * nsh);
   shell_density(&bs, d, nfocks, ntri, dsh);
   double maxden = 0.0;
   for (int x = 0; x < nsh * nsh; ++x)
     if (fabs(dsh[x]) > maxden) maxden = fabs(dsh[x]);
+  long *pos = size > 1 ? pair_map_positions(nsh, am, Qmap ? Qmap : Q) : NULL;
   int mp = max_prim(&bs);
   long nschwz = 0, nquart = 0, nints = 0;
   int nthr = 1;
@@ -824,7 +906,7 @@ static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, con
 #pragma omp for schedule(dynamic, 1)
     for (int i = 0; i < nsh; ++i)
       for (int j = 0; j <= i; ++j) {
-        if (size > 1 && (((long)i * (i + 1) / 2 + j + 1) % size) != rank) continue;
+        if (pos && (pos[(long)i * (i + 1) / 2 + j] % size) != rank) continue;
         /* screen_ij (int2.F90:963-971) */
         if (Q[i * nsh + j] * maxden < cutoff) {
           nschwz += (long)(i + 1) * i / 2 + (j + 1);
@@ -861,6 +943,7 @@ static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, con
     for (long i = 1; i <= nbf; ++i) f[fk * ntri + tri(i, i) - 1] *= 2.0;
   free(fthr);
   free(dsh);
+  free(pos);
   if (stats) { stats[0] = nschwz; stats[1] = nquart; stats[2] = nints; }
   return 0;
 }

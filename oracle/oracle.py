@@ -35,8 +35,15 @@ def lib():
         L.oqo_c2s.argtypes = [C.c_int, _pd]
         L.oqo_cart_order.argtypes = [C.c_int, _pi]
         L.oqo_schwarz.argtypes = _BAS + [_pd]
+        L.oqo_schwarz_mu.argtypes = _BAS + [C.c_double, _pd]
         L.oqo_eri_quartet.argtypes = _BAS + [C.c_int] * 4 + [C.c_double, _pd]
         L.oqo_eri_quartet.restype = C.c_int
+        L.oqo_eri_quartet_mu.argtypes = _BAS + [C.c_int] * 4 + [C.c_double, C.c_double, _pd]
+        L.oqo_eri_quartet_mu.restype = C.c_int
+        _pl = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.oqo_pair_map.argtypes = [C.c_int, _pi, _pd, _pl]
+        L.oqo_fock_jk_ex.argtypes = _BAS + [_pd, C.c_void_p, _pd, _pd, C.c_int, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, _pl, C.c_int, C.c_int]
         L.oqo_fock_jk.argtypes = _BAS + [_pd, _pd, _pd, C.c_double, C.c_double, C.c_double,
                                          np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.oqo_fock_jk_part.argtypes = L.oqo_fock_jk.argtypes + [C.c_int, C.c_int]
@@ -70,17 +77,56 @@ def cart_order(l: int) -> np.ndarray:
     return out.reshape(nc, 3)
 
 
-def schwarz(bas) -> np.ndarray:
+def schwarz(bas, mu=None) -> np.ndarray:
+    """Schwarz table sqrt(max|(ab|ab)|); mu: of the erf(mu r)/r integrals."""
     Q = np.zeros(bas.nsh * bas.nsh)
-    lib().oqo_schwarz(*bas.carray(), Q)
+    if mu is None:
+        lib().oqo_schwarz(*bas.carray(), Q)
+    else:
+        lib().oqo_schwarz_mu(*bas.carray(), 1.0 / (mu * mu), Q)
     return Q.reshape(bas.nsh, bas.nsh)
 
 
-def eri_quartet(bas, i, j, k, l, cutoff=0.0) -> np.ndarray:
+def eri_quartet(bas, i, j, k, l, cutoff=0.0, mu=None) -> np.ndarray:
+    """Final-representation ERI block; mu: erf(mu r)/r operator instead of 1/r."""
     n = bas.naos()
     out = np.zeros(int(n[i] * n[j] * n[k] * n[l]))
-    lib().oqo_eri_quartet(*bas.carray(), i, j, k, l, cutoff, out)
+    if mu is None:
+        lib().oqo_eri_quartet(*bas.carray(), i, j, k, l, cutoff, out)
+    else:
+        lib().oqo_eri_quartet_mu(*bas.carray(), i, j, k, l, cutoff, 1.0 / (mu * mu), out)
     return out.reshape(n[i], n[j], n[k], n[l])
+
+
+def pair_map(bas, Q) -> np.ndarray:
+    """1-based position ij_pair of every shell pair i >= j (index i(i+1)/2 + j) in
+    OpenQP's cost-sorted shell-pair map (int2_build_shell_pair_map)."""
+    out = np.zeros(bas.nsh * (bas.nsh + 1) // 2, dtype=np.int64)
+    lib().oqo_pair_map(bas.nsh, np.ascontiguousarray(bas.am, np.int32), np.ascontiguousarray(Q.reshape(-1)), out)
+    return out
+
+
+def fock_jk_general(bas, d, nfocks=1, se=1.0, sc=1.0, cutoff=5e-11, Q=None, Qmap=None, mu=None, rank=0, size=1):
+    """Packed f (nfocks * ntri) and stats for 1/r (mu None) or erf(mu r)/r, on
+    the share (rank, size); Q screens, Qmap orders the shell-pair map."""
+    if Q is None:
+        Q = schwarz(bas, mu)
+    f = np.zeros(nfocks * bas.ntri)
+    stats = np.zeros(3, dtype=np.int64)
+    qm = None if Qmap is None else np.ascontiguousarray(Qmap.reshape(-1), np.float64)
+    lib().oqo_fock_jk_ex(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)),
+                         None if qm is None else qm.ctypes.data, np.ascontiguousarray(d, np.float64), f, nfocks,
+                         se, sc, cutoff, 0.0 if mu is None else 1.0 / (mu * mu), stats, rank, size)
+    return f, stats
+
+
+def fock_jk_cam(bas, d, alpha, beta, mu, alpha_coulomb=1.0, beta_coulomb=0.0, nfocks=1, cutoff=5e-11):
+    """CAM build of int2_run_cam (int2.F90:538-584): 1/r pass (Coulomb
+    alpha_coulomb, exchange alpha) plus erf(mu r)/r pass (Coulomb beta_coulomb,
+    exchange beta) screened with the attenuated bounds."""
+    f1, s1 = fock_jk_general(bas, d, nfocks, alpha, alpha_coulomb, cutoff)
+    f2, s2 = fock_jk_general(bas, d, nfocks, beta, beta_coulomb, cutoff, mu=mu)
+    return f1 + f2, s1 + s2
 
 
 def eri_tensor(bas) -> np.ndarray:

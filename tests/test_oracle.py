@@ -314,3 +314,130 @@ def test_uhf_build_matches_brute_force(oracle, name):
     for Ds, fs in ((Da, fa), (Db, fb)):
         K = np.einsum("ikjl,kl->ij", eri, Ds)
         assert np.abs(B.unpack_tri(fs, n) - (sc * J - se * K)).max() < 1e-12
+
+
+# ------------------------------------------------------------------ erf-attenuated (CAM) integrals
+def _ssss_erf(e, R, mu):
+    """(ss|ss) of normalised s primitives with erf(mu r12)/r12, from
+    erf(mu r)/r = (2/sqrt(pi)) int_0^mu exp(-u^2 r^2) du and the Gaussian-operator
+    integral int int exp(-p r1P^2 - q r2Q^2 - u^2 r12^2) =
+    (pi^2 / (p q))^(3/2) (rho / (rho + u^2))^(3/2) exp(-rho u^2 / (rho + u^2) PQ^2),
+    integrated numerically over u (independent of the Boys-function route)."""
+    from scipy.integrate import quad
+    a, b, c, d = e
+    A, Bc, C, D = [np.array(x, float) for x in R]
+    p, q = a + b, c + d
+    P, Q = (a * A + b * Bc) / p, (c * C + d * D) / q
+    K = math.exp(-a * b / p * np.sum((A - Bc) ** 2) - c * d / q * np.sum((C - D) ** 2))
+    rho, pq2 = p * q / (p + q), float(np.sum((P - Q) ** 2))
+    f = lambda u: (rho / (rho + u * u)) ** 1.5 * math.exp(-rho * u * u / (rho + u * u) * pq2)
+    val, _ = quad(f, 0.0, mu, epsabs=0.0, epsrel=2e-14, limit=200)
+    norm = np.prod([(2 * x / math.pi) ** 0.75 for x in e])
+    return 2.0 / math.sqrt(math.pi) * (math.pi ** 2 / (p * q)) ** 1.5 * K * val * norm
+
+
+@pytest.mark.parametrize("mu", [0.05, 0.33, 1.0, 4.0])
+def test_eri_erf_ssss_by_quadrature(oracle, mu):
+    R = [[0, 0, 0], [0.3, -0.7, 1.1], [1.5, 0.2, -0.4], [-0.9, 1.3, 0.6]]
+    e = [1.3, 0.4, 2.2, 0.75]
+    bas = _prim_basis([0, 0, 0, 0], e, R)
+    v = oracle.eri_quartet(bas, 0, 1, 2, 3, mu=mu)[0, 0, 0, 0]
+    assert v == pytest.approx(_ssss_erf(e, R, mu), rel=1e-12)
+
+
+def test_eri_erf_p_by_centre_derivative(oracle):
+    """<x_A s| = (1/2a) d/dA_x <s|: the attenuated p block against FD of the
+    quadrature closed form (pins the recursion at finite mu, not just the limits)."""
+    R = [[0.1, -0.2, 0.3], [0.3, -0.7, 1.1], [1.5, 0.2, -0.4], [-0.9, 1.3, 0.6]]
+    e = [0.9, 0.4, 1.2, 0.75]
+    mu, h, a = 0.6, 1e-4, e[0]
+    s_un = lambda RA: _ssss_erf(e, [RA] + R[1:], mu) / (2 * a / math.pi) ** 0.75
+    bas = _prim_basis([1, 0, 0, 0], e, R)
+    vp = oracle.eri_quartet(bas, 0, 1, 2, 3, mu=mu)[:, 0, 0, 0]
+    Np = (2 * a / math.pi) ** 0.75 * math.sqrt(4 * a)
+    for d in range(3):
+        Rp, Rm = [list(R[0]) for _ in range(2)]
+        Rp[d] += h
+        Rm[d] -= h
+        fd = (s_un(Rp) - s_un(Rm)) / (2 * h) / (2 * a)
+        assert vp[d] == pytest.approx(fd * Np, rel=1e-7, abs=1e-12)
+
+
+def test_eri_erf_limits_all_shells(oracle):
+    """mu -> infinity gives 1/r; mu -> 0 gives (2 mu / sqrt(pi)) S_ab S_cd, since
+    erf(mu r)/r = 2 mu / sqrt(pi) (1 - mu^2 r^2 / 3 + ...): pins every angular
+    momentum (s..f, pure) and component order of the attenuated path."""
+    bas = small_system("cc-pvtz")
+    S, _, _ = oracle.int1e(bas)
+    naos, off = bas.naos(), bas.aooff
+    rng = np.random.default_rng(1)
+    quartets = [(i, j, k, l) for i, j, k, l in rng.integers(0, bas.nsh, size=(40, 4))]
+    quartets += [(int(np.argmax(bas.am)),) * 4]
+    for (i, j, k, l) in quartets:
+        reg = oracle.eri_quartet(bas, i, j, k, l)
+        big = oracle.eri_quartet(bas, i, j, k, l, mu=1.0e7)
+        assert np.abs(big - reg).max() <= 1e-9 * max(1.0, np.abs(reg).max())
+        mu = 1.0e-4
+        small = oracle.eri_quartet(bas, i, j, k, l, mu=mu) / mu
+        sab = S[off[i]:off[i] + naos[i], off[j]:off[j] + naos[j]]
+        scd = S[off[k]:off[k] + naos[k], off[l]:off[l] + naos[l]]
+        lim = 2.0 / math.sqrt(math.pi) * np.einsum("ab,cd->abcd", sab, scd)
+        assert np.abs(small - lim).max() <= 1e-6 * max(1.0, np.abs(lim).max())
+
+
+def test_attenuated_schwarz_never_exceeds_regular(oracle):
+    """erfc(mu r)/r is a positive kernel, so (ab|ab)_erf <= (ab|ab)_1/r
+    elementwise on the diagonal: the attenuated bounds never exceed the regular
+    ones (the regular coarse lists are then a superset for the attenuated pass)."""
+    bas = small_system("cc-pvdz", nw=2)
+    Q = oracle.schwarz(bas)
+    for mu in (0.2, 0.33, 1.5):
+        Qa = oracle.schwarz(bas, mu=mu)
+        assert (Qa <= Q * (1 + 1e-13)).all()
+        assert Qa.max() > 0.05 * Q.max()
+
+
+def test_cam_build_limits(oracle):
+    """CAM build = G[1/r; a_c, alpha] + G[erf; b_c, beta]: beta = 0, b_c = 0 gives
+    the plain build with exchange scale alpha; a huge mu makes the two passes
+    the same operator (sum of the scales)."""
+    bas = small_system("cc-pvdz", nw=2)
+    D, dp = density_for(oracle, bas, 10)
+    plain, _ = oracle.fock_jk(bas, dp, se=0.19, sc=1.0)
+    cam, _ = oracle.fock_jk_cam(bas, dp, alpha=0.19, beta=0.0, mu=0.33, beta_coulomb=0.0)
+    assert np.abs(cam - plain).max() < 1e-13
+    full, _ = oracle.fock_jk(bas, dp, se=0.65, sc=1.0)
+    cam2, _ = oracle.fock_jk_cam(bas, dp, alpha=0.19, beta=0.46, mu=1.0e8, beta_coulomb=0.0)
+    assert np.abs(cam2 - full).max() < 1e-9
+
+
+def test_pair_map_is_a_permutation_sorted_by_cost(oracle):
+    """int2_build_shell_pair_map: ij_pair is a permutation of 1..npair, and the
+    cost class log2(Q_ij nbf_i nbf_j i) never increases along it."""
+    bas = small_system("cc-pvdz", nw=2)
+    Q = oracle.schwarz(bas)
+    pos = oracle.pair_map(bas, Q)
+    n = bas.nsh * (bas.nsh + 1) // 2
+    assert sorted(pos.tolist()) == list(range(1, n + 1))
+    nc = [(l + 1) * (l + 2) // 2 for l in bas.am]
+    cls = np.zeros(n)
+    for i in range(bas.nsh):
+        for j in range(i + 1):
+            cost = max(Q[i, j], 1e-30) * nc[i] * nc[j] * (i + 1)
+            cls[pos[i * (i + 1) // 2 + j] - 1] = 64 - max(0, min(64, int(math.log(cost) / math.log(2.0)) + 42))
+    assert np.all(np.diff(cls) >= 0)
+
+
+def test_rank_shares_sum_to_full_build(oracle):
+    bas = small_system("cc-pvdz", nw=2)
+    D, dp = density_for(oracle, bas, 10)
+    Q = oracle.schwarz(bas)
+    full, st = oracle.fock_jk(bas, dp, Q=Q)
+    tot = np.zeros_like(full)
+    nq = 0
+    for r in range(3):
+        f, s = oracle.fock_jk(bas, dp, Q=Q, rank=r, size=3)
+        tot += f
+        nq += s[1]
+    assert np.abs(tot - full).max() < 1e-12
+    assert nq == st[1]
