@@ -10,7 +10,17 @@ module oqph_bridge
   private
   public :: routec_fock_jk, routec_grad2
   public :: oqph_session_create, oqph_session_destroy, oqph_session_set_basis, &
-            oqph_session_set_geometry, oqph_session_set_thresholds, oqph_set_default_session
+            oqph_session_set_geometry, oqph_session_set_thresholds, oqph_set_default_session, &
+            oqph_session_set_precision, oqph_session_set_share, oqph_session_prepare, &
+            oqph_fock_jk, oqph_fock_jk_inc, oqph_fock_jk_cam, oqph_grad2
+  public :: oqph_stats
+
+  !> Build statistics (oqph_stats of include/oqph.h).
+  type, bind(C) :: oqph_stats
+    integer(c_long_long) :: quartets_computed, quartets_screened, quartets_fp32
+    real(c_double) :: kernel_ms, model_flops
+    integer(c_long_long) :: prim_quartets
+  end type
 
   interface
     subroutine routec_fock_jk(d, f, nbf, nfocks, se, sc, info) bind(C, name="routec_fock_jk")
@@ -61,6 +71,64 @@ module oqph_bridge
     integer(c_int) function oqph_set_default_session(s) bind(C, name="oqph_set_default_session")
       import :: c_ptr, c_int
       type(c_ptr), value :: s
+    end function
+    !> policy 0 = FP64, 1 = mixed (FP32 below fp32_cutoff)
+    integer(c_int) function oqph_session_set_precision(s, policy, fp32_cutoff) &
+        bind(C, name="oqph_session_set_precision")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: s
+      integer(c_int), value :: policy
+      real(c_double), value :: fp32_cutoff
+    end function
+    !> OpenQP MPI share: rank of size (int2.F90:759-761)
+    integer(c_int) function oqph_session_set_share(s, rank, size) bind(C, name="oqph_session_set_share")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: s
+      integer(c_int), value :: rank, size
+    end function
+    integer(c_int) function oqph_session_prepare(s) bind(C, name="oqph_session_prepare")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: s
+    end function
+    integer(c_int) function oqph_fock_jk(s, d, f, nfocks, se, sc, stats) bind(C, name="oqph_fock_jk")
+      import :: c_ptr, c_int, c_double, oqph_stats
+      type(c_ptr), value :: s
+      real(c_double), intent(in) :: d(*)
+      real(c_double), intent(inout) :: f(*)
+      integer(c_int), value :: nfocks
+      real(c_double), value :: se, sc
+      type(oqph_stats), intent(out) :: stats
+    end function
+    !> incremental build (scf_addons.F90:1207-1210): f = G[d - d_old] + f_old, then
+    !> f_old <- f and d_old <- d
+    integer(c_int) function oqph_fock_jk_inc(s, d, d_old, f, f_old, nfocks, se, sc, stats) &
+        bind(C, name="oqph_fock_jk_inc")
+      import :: c_ptr, c_int, c_double, oqph_stats
+      type(c_ptr), value :: s
+      real(c_double), intent(in) :: d(*)
+      real(c_double), intent(inout) :: d_old(*), f(*), f_old(*)
+      integer(c_int), value :: nfocks
+      real(c_double), value :: se, sc
+      type(oqph_stats), intent(out) :: stats
+    end function
+    !> range-separated build (int2_run_cam, int2.F90:538-584)
+    integer(c_int) function oqph_fock_jk_cam(s, d, f, nfocks, alpha, beta, mu, alpha_coulomb, beta_coulomb, stats) &
+        bind(C, name="oqph_fock_jk_cam")
+      import :: c_ptr, c_int, c_double, oqph_stats
+      type(c_ptr), value :: s
+      real(c_double), intent(in) :: d(*)
+      real(c_double), intent(inout) :: f(*)
+      integer(c_int), value :: nfocks
+      real(c_double), value :: alpha, beta, mu, alpha_coulomb, beta_coulomb
+      type(oqph_stats), intent(out) :: stats
+    end function
+    integer(c_int) function oqph_grad2(s, d, de, hfscale, coulscale, stats) bind(C, name="oqph_grad2")
+      import :: c_ptr, c_int, c_double, oqph_stats
+      type(c_ptr), value :: s
+      real(c_double), intent(in) :: d(*)
+      real(c_double), intent(inout) :: de(*)
+      real(c_double), value :: hfscale, coulscale
+      type(oqph_stats), intent(out) :: stats
     end function
   end interface
 end module oqph_bridge

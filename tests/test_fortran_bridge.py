@@ -37,8 +37,17 @@ def test_fortran_driver_matches_oracle(tmp_path, oracle):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120).stdout
     print(out, flush=True)
     info = int(out.split("info=")[1].split()[0])
-    sumf = float(out.split("sumf=")[1].split()[0])
     assert info == 0
+    vals = {}
+    for ln in out.splitlines():
+        if ")=" in ln:
+            k, v = ln.split("=")
+            vals[k.strip()] = float(v)
+    f_drv = np.array([vals[f"f({i})"] for i in range(1, 22)])
+    finc = np.array([vals[f"finc({i})"] for i in range(1, 22)])
+    fmx = np.array([vals[f"fmx({i})"] for i in range(1, 22)])
+    assert int(out.split("inc_rc=")[1].split()[0]) == 0
+    assert int(out.split("mixed_rc=")[1].split()[0]) == 0
     ex = np.array([10.0, 0.8, 0.9, 0.5])
     am = np.array([0, 0, 1, 0], np.int32)
     cc = np.array([1.0 / np.sqrt(B.gauss_norm(2 * e, l)) for e, l in zip(ex, am)])
@@ -47,4 +56,7 @@ def test_fortran_driver_matches_oracle(tmp_path, oracle):
                   np.array([[0, 0, 0], [0, 0, 1.4]], np.float64), np.array([8.0, 1.0]))
     d = 0.01 * np.arange(1, 22)
     f, _ = oracle.fock_jk(bas, d)
-    assert abs(sumf - f.sum()) < 1e-10
+    # every element of the seam build, the incremental build and the mixed build
+    assert np.abs(f_drv - f).max() < 1e-10
+    assert np.abs(finc - f).max() < 1e-10
+    assert np.abs(fmx - f).max() < 1e-6
