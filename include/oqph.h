@@ -84,7 +84,8 @@ int oqph_session_set_precision(oqph_session *s, int policy, double fp32_cutoff);
 /* Make s the session used by routec_fock_jk / routec_grad2. */
 int oqph_set_default_session(oqph_session *s);
 /* Build the geometry-dependent device state now (pair records, Schwarz
- * bounds); otherwise the first build does it.  Counterpart of
+ * bounds, the rank share's ownership flags); otherwise the first build does
+ * it.  Counterpart of
  * int2_compute_t%init + set_screening (source/integrals/int2.F90:245-289, 473-477). */
 int oqph_session_prepare(oqph_session *s);
 /* Restrict every following J/K build to the quartets OpenQP's MPI rank `rank`
@@ -120,6 +121,24 @@ int oqph_fock_jk_cam(oqph_session *s, const double *d, double *f, int nfocks, do
  * overwritten.  Inputs already in HBM: no host transfer in the call. */
 int oqph_fock_jk_device(oqph_session *s, const double *d_dev, double *f_dev, int nfocks, double se, double sc,
                         oqph_stats *stats, void *stream);
+
+/* No-allocation RHF build.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * oqph_fock_jk_device with nfocks = 1; d_dev, f_dev are device pointers
+ * supplied by the caller.  The build's own device memory -- shell-pair
+ * density maxima, statistics counters, work lists -- lives in a static array
+ * of OQPH_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds
+ * until the process ends; a basis/density that needs more declines with
+ * code 1 and a message giving the size.  The call performs no hipMalloc,
+ * hipFree or other device allocation: the session must have been prepared
+ * first (oqph_session_prepare, which does allocate; also after a change of
+ * basis, geometry, thresholds or share).  All calls on one GPU share its
+ * array: calls that overlap on one GPU (other streams, other host threads'
+ * kernels still running) race; the library serialises its own host side. */
+#define OQPH_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int oqph_fock_jk_noalloc(oqph_session *s, const double *d_dev, double *f_dev, double se, double sc,
+                         oqph_stats *stats, void *stream);
 
 /* Per-launch profile of the next builds (HIP events around each class
  * launch); oqph_last_profile copies up to maxn entries of the last build:

@@ -179,10 +179,11 @@ struct oqph_session {
     bool rows;   // bra-row kernel (bw, cm, lds) instead of wavefront tasks (ww)
     bool hl;     // run-time-class kernel on the wavefront tasks (f shells), layout hd
     HLDims hd;
-    WaveWork ww;
+    WaveWork ww;   // the list pointers are set per build (bind_lists)
     BraWork bw;
     ColMap cm;
     size_t lds;
+    size_t seg0, ch0, item0;  // offsets of this launch in the work lists
   };
   // bra-row kernel: LDS column spaces per angular momentum and its work items
   std::vector<int> h_colidx, h_colao, colbase, ncolL;
@@ -197,6 +198,9 @@ struct oqph_session {
   // kernels (k_jkh, k_gradh) instead of the class templates where they exist
   bool f_templates = true;
   std::vector<WLaunch> launches;
+  unsigned long long list_gen = 0;       // bumped whenever the host work lists change
+  unsigned long long list_uploaded = 0;  // list_gen of the copy in d_seg / d_choff / d_items
+  unsigned long long serial = 0;         // process-unique session number
   DBuf<int> d_seg;                 // seg_bra | seg_k0 | seg_n of all launches
   DBuf<long long> d_choff;
   std::vector<int> h_seg_bra, h_seg_k0, h_seg_n;
@@ -221,6 +225,34 @@ struct oqph_session {
 
 static oqph_session* g_default = nullptr;
 static std::mutex g_mtx;
+static unsigned long long g_serial = 0;  // last session serial handed out (under g_mtx)
+
+// ------------------------------------------------ no-allocation scratch
+// Device memory of oqph_fock_jk_noalloc: the shell-pair density maxima, the
+// statistics counters and the work lists of one build, in a static array
+// that every GPU the library runs on holds until the process ends.  Host code
+// never uses the array's name as an address: each device's copy is looked up
+// once with hipGetSymbolAddress (oqph_session_create, device current) and
+// that pointer is what kernels, memsets and copies receive.
+static __device__ __align__(256) unsigned char scratch[OQPH_NOALLOC_SCRATCH_BYTES];
+#define OQPH_MAX_DEVICES 64
+struct ScratchSlot {
+  unsigned char* ptr = nullptr;     // this device's copy of scratch
+  int looked_up = 0;                // 1 after hipGetSymbolAddress ran (ptr null if it failed)
+  unsigned long long serial = 0;    // session and list generation whose work
+  unsigned long long gen = 0;       // lists the copy holds
+};
+static ScratchSlot g_scratch[OQPH_MAX_DEVICES];
+
+// once per device, with that device current (called under g_mtx)
+static void scratch_lookup(int device) {
+  if (device < 0 || device >= OQPH_MAX_DEVICES || g_scratch[device].looked_up) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch[device].looked_up = 1;
+  g_scratch[device].ptr = (e == hipSuccess) ? (unsigned char*)ptr : nullptr;
+  if (e != hipSuccess) (void)hipGetLastError();  // leave no sticky error for the next call
+}
 
 #include "ao_xform.h"
 using oqph_host::build_xform;
@@ -907,26 +939,14 @@ static int prepare_lists(oqph_session* s, double maxden) {
       pend.push_back({la, lb, lc, ld, ck, seg0, ch0, nseg, waves});
       pend.back().hl = hl;
     }
-  int rc;
-  const size_t nseg = s->h_seg_bra.size();
-  if ((rc = s->d_seg.ensure(3 * nseg + 1)) || (rc = s->d_choff.ensure(s->h_choff.size() + 1))) return rc;
-  if (nseg) {
-    if ((rc = upload(s->d_seg.p, s->h_seg_bra.data(), nseg * sizeof(int), s->stream)) ||
-        (rc = upload(s->d_seg.p + nseg, s->h_seg_k0.data(), nseg * sizeof(int), s->stream)) ||
-        (rc = upload(s->d_seg.p + 2 * nseg, s->h_seg_n.data(), nseg * sizeof(int), s->stream)) ||
-        (rc = upload(s->d_choff.p, s->h_choff.data(), s->h_choff.size() * sizeof(long long), s->stream)))
-      return rc;
-  }
-  if (!s->h_items.empty()) {
-    if ((rc = s->d_items.ensure(s->h_items.size()))) return rc;
-    if ((rc = upload(s->d_items.p, s->h_items.data(), s->h_items.size() * sizeof(int), s->stream))) return rc;
-  }
   for (auto& p : pend) {
     oqph_session::WLaunch L{};
     L.la = p.la; L.lb = p.lb; L.lc = p.lc; L.ld = p.ld;
     L.rows = p.rows;
+    L.seg0 = p.seg0;
+    L.ch0 = p.ch0;
+    L.item0 = p.item0;
     if (p.rows) {
-      L.bw.item = s->d_items.p + p.item0;
       L.bw.ket = s->d_cls.p + s->cls_off[p.ck];
       for (int b = 0; b <= NBKT; ++b) L.bw.kb[b] = s->kbs[p.ck][b];
       L.bw.nitem = p.nitem;
@@ -941,10 +961,6 @@ static int prepare_lists(oqph_session* s, double maxden) {
       s->launches.push_back(L);
       continue;
     }
-    L.ww.seg_bra = s->d_seg.p + p.seg0;
-    L.ww.seg_k0 = s->d_seg.p + nseg + p.seg0;
-    L.ww.seg_n = s->d_seg.p + 2 * nseg + p.seg0;
-    L.ww.choff = s->d_choff.p + p.ch0;
     L.ww.ket = s->d_cls.p + s->cls_off[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
@@ -959,26 +975,112 @@ static int prepare_lists(oqph_session* s, double maxden) {
     }
     s->launches.push_back(L);
   }
-  HIPCHK(hipStreamSynchronize(s->stream));  // host vectors stay alive; keep uploads ordered
+  ++s->list_gen;
   s->cached_maxden = maxden;
   return 0;
 }
 
+// Device work lists of the current launches: element counts of seg (three
+// int arrays of nseg), choff (long long) and items (int).
+struct ListSizes {
+  size_t seg, choff, items;
+};
+static ListSizes list_sizes(const oqph_session* s) {
+  return {3 * s->h_seg_bra.size() + 1, s->h_choff.size() + 1, std::max<size_t>(s->h_items.size(), 1)};
+}
+
+// Copy the host work lists into device buffers of at least list_sizes();
+// synchronises the stream, so the host vectors may change afterwards.
+static int upload_lists(oqph_session* s, int* seg, long long* choff, int* items, hipStream_t stream) {
+  const size_t nseg = s->h_seg_bra.size();
+  int rc;
+  if (nseg) {
+    if ((rc = upload(seg, s->h_seg_bra.data(), nseg * sizeof(int), stream)) ||
+        (rc = upload(seg + nseg, s->h_seg_k0.data(), nseg * sizeof(int), stream)) ||
+        (rc = upload(seg + 2 * nseg, s->h_seg_n.data(), nseg * sizeof(int), stream)) ||
+        (rc = upload(choff, s->h_choff.data(), s->h_choff.size() * sizeof(long long), stream)))
+      return rc;
+  }
+  if (!s->h_items.empty() &&
+      (rc = upload(items, s->h_items.data(), s->h_items.size() * sizeof(int), stream)))
+    return rc;
+  HIPCHK(hipStreamSynchronize(stream));
+  return 0;
+}
+
+// The session's own list buffers (grown on demand), refreshed when the host
+// lists changed since the last upload.
+static int session_lists(oqph_session* s, hipStream_t stream) {
+  if (s->list_uploaded == s->list_gen) return 0;
+  const ListSizes n = list_sizes(s);
+  int rc;
+  if ((rc = s->d_seg.ensure(n.seg)) || (rc = s->d_choff.ensure(n.choff)) || (rc = s->d_items.ensure(n.items)))
+    return rc;
+  if ((rc = upload_lists(s, s->d_seg.p, s->d_choff.p, s->d_items.p, stream))) return rc;
+  s->list_uploaded = s->list_gen;
+  return 0;
+}
+
+// Point one launch's work descriptors at a device copy of the lists.
+static void bind_lists(const oqph_session* s, oqph_session::WLaunch& L, const int* seg, const long long* choff,
+                       const int* items) {
+  const size_t nseg = s->h_seg_bra.size();
+  if (L.rows) {
+    L.bw.item = items + L.item0;
+    return;
+  }
+  L.ww.seg_bra = seg + L.seg0;
+  L.ww.seg_k0 = seg + nseg + L.seg0;
+  L.ww.seg_n = seg + 2 * nseg + L.seg0;
+  L.ww.choff = choff + L.ch0;
+}
+
+// The no-allocation entry point's scratch: the calling device's copy of the
+// static array, and a host record of which session's work lists it holds.
+struct StaticScratch {
+  unsigned char* base;        // device address from hipGetSymbolAddress
+  size_t bytes;
+  unsigned long long* serial;  // session serial and list generation of the
+  unsigned long long* gen;     // lists last uploaded into it
+};
+
 // G[D] on device buffers; D already on the device.  The screening densities
 // scr_a (and scr_b) default to D; zero_f = false accumulates into dF.  mu > 0:
 // the erf(mu r)/r attenuated pass (pair records with the attenuated bounds).
+// ss: run in the static scratch of the no-allocation entry point instead of
+// the session's buffers (no device allocation; profiling is off there).
 static int build_device(oqph_session* s, const double* dD, double* dF, double se, double sc, oqph_stats* st,
                         hipStream_t stream, const double* scr_a = nullptr, const double* scr_b = nullptr,
-                        bool zero_f = true, double mu = 0.0) {
+                        bool zero_f = true, double mu = 0.0, const StaticScratch* ss = nullptr) {
   int rc;
   if (mu > 0.0 && (rc = prepare_attenuated(s, mu))) return rc;
   if ((rc = prepare_share(s))) return rc;
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
-  launch_shell_density(scr_a ? scr_a : dD, scr_b, s->d_aooff.p, s->d_nao.p, s->nsh, s->d_dsh.p, stream);
-  HIPCHK(hipMemsetAsync(s->d_ctr.p, 0, 8 * sizeof(unsigned long long), stream));
-  launch_max_reduce(s->d_dsh.p, (long long)s->nsh * s->nsh, s->d_ctr.p + 4, stream);
+  const size_t nsh2 = (size_t)s->nsh * s->nsh;
+  // scratch layout: counters 0..7 (density maximum), dsh, per-launch
+  // counters, work lists; 256-byte aligned slices
+  size_t off = 0;
+  auto carve = [&](size_t bytes) -> unsigned char* {
+    unsigned char* q = ss->base + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  unsigned long long* ctr0 = s->d_ctr.p;
+  double* dsh = s->d_dsh.p;
+  if (ss) {
+    ctr0 = (unsigned long long*)carve(8 * sizeof(unsigned long long));
+    dsh = (double*)carve(nsh2 * sizeof(double));
+    if (off > ss->bytes) {
+      set_err("no-allocation build: %zu shells need %zu bytes of scratch for the shell-pair maxima alone; the "
+              "static scratch holds %zu", (size_t)s->nsh, off, ss->bytes);
+      return 1;
+    }
+  }
+  launch_shell_density(scr_a ? scr_a : dD, scr_b, s->d_aooff.p, s->d_nao.p, s->nsh, dsh, stream);
+  HIPCHK(hipMemsetAsync(ctr0, 0, 8 * sizeof(unsigned long long), stream));
+  launch_max_reduce(dsh, (long long)nsh2, ctr0 + 4, stream);
   unsigned long long mbits = 0;
-  HIPCHK(hipMemcpyAsync(&mbits, s->d_ctr.p + 4, sizeof(mbits), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(&mbits, ctr0 + 4, sizeof(mbits), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
   double maxden;
   memcpy(&maxden, &mbits, sizeof(double));
@@ -990,9 +1092,38 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   if ((rc = prepare_lists(s, maxden))) return rc;
   const size_t nl = s->launches.size();
   const size_t nc = 8 * (size_t)OQPH_CTR_SLOTS;  // counters per launch
-  if ((rc = s->d_ctr.ensure(8 + nc * nl))) return rc;
-  HIPCHK(hipMemsetAsync(s->d_ctr.p + 8, 0, nc * nl * sizeof(unsigned long long), stream));
-  if (s->profile && s->pev.size() < 2 * nl) {
+  unsigned long long* ctr = nullptr;  // per-launch counters
+  int* lseg;
+  long long* lchoff;
+  int* litems;
+  if (ss) {
+    const ListSizes n = list_sizes(s);
+    ctr = (unsigned long long*)carve(nc * std::max<size_t>(nl, 1) * sizeof(unsigned long long));
+    lseg = (int*)carve(n.seg * sizeof(int));
+    lchoff = (long long*)carve(n.choff * sizeof(long long));
+    litems = (int*)carve(n.items * sizeof(int));
+    if (off > ss->bytes) {
+      set_err("no-allocation build: this basis and density need %zu bytes of scratch (shell-pair maxima, "
+              "counters and work lists); the static scratch holds %zu", off, ss->bytes);
+      return 1;
+    }
+    if (*ss->serial != s->serial || *ss->gen != s->list_gen) {
+      if ((rc = upload_lists(s, lseg, lchoff, litems, stream))) return rc;
+      *ss->serial = s->serial;
+      *ss->gen = s->list_gen;
+    }
+  } else {
+    if ((rc = session_lists(s, stream))) return rc;
+    if ((rc = s->d_ctr.ensure(8 + nc * nl))) return rc;
+    ctr = s->d_ctr.p + 8;
+    lseg = s->d_seg.p;
+    lchoff = s->d_choff.p;
+    litems = s->d_items.p;
+  }
+  for (auto& L : s->launches) bind_lists(s, L, lseg, lchoff, litems);
+  HIPCHK(hipMemsetAsync(ctr, 0, nc * nl * sizeof(unsigned long long), stream));
+  const bool profile = s->profile && !ss;
+  if (profile && s->pev.size() < 2 * nl) {
     for (size_t i = s->pev.size(); i < 2 * nl; ++i) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
@@ -1010,8 +1141,8 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   p.xform = s->d_xform.p;
   p.D = dD;
   p.F = dF;
-  p.dsh = s->d_dsh.p;
-  p.counters = s->d_ctr.p;
+  p.dsh = dsh;
+  p.counters = ctr;
   p.nsh = s->nsh;
   p.cutoff = s->cutoff;
   p.maxden = maxden;
@@ -1022,8 +1153,8 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   HIPCHK(hipEventRecord(s->ev0, stream));
   for (size_t i = 0; i < nl; ++i) {
     auto& L = s->launches[i];
-    p.counters = s->d_ctr.p + 8 + nc * i;
-    if (s->profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
+    p.counters = ctr + nc * i;
+    if (profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
     bool ok = true;
     if (L.hl) launch_jkh(L.hd, p, L.ww, stream);
     else if (L.rows) ok = launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, stream);
@@ -1032,20 +1163,20 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
       set_err("no J/K kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }
-    if (s->profile) HIPCHK(hipEventRecord(s->pev[2 * i + 1], stream));
+    if (profile) HIPCHK(hipEventRecord(s->pev[2 * i + 1], stream));
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, stream));
-  if (st || s->profile) {
+  if (st || profile) {
     std::vector<unsigned long long> raw(nc * nl);
-    if (nl) HIPCHK(hipMemcpyAsync(raw.data(), s->d_ctr.p + 8, nc * nl * sizeof(unsigned long long),
+    if (nl) HIPCHK(hipMemcpyAsync(raw.data(), ctr, nc * nl * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     s->prof_ctr.assign(8 * nl, 0ull);
     for (size_t i = 0; i < nl; ++i)
       for (size_t k = 0; k < nc; ++k) s->prof_ctr[8 * i + k % 8] += raw[nc * i + k];
     s->prof_ms.assign(nl, 0.f);
-    if (s->profile)
+    if (profile)
       for (size_t i = 0; i < nl; ++i) HIPCHK(hipEventElapsedTime(&s->prof_ms[i], s->pev[2 * i], s->pev[2 * i + 1]));
   }
   if (st) {
@@ -1093,6 +1224,8 @@ int oqph_session_create(oqph_session** out, int device) {
   }
   *out = s;
   std::lock_guard<std::mutex> lk(g_mtx);
+  s->serial = ++g_serial;
+  scratch_lookup(device);
   if (!g_default) g_default = s;
   return 0;
 }
@@ -1304,6 +1437,35 @@ int oqph_fock_jk_device(oqph_session* s, const double* d_dev, double* f_dev, int
   return build_device(s, d_dev, f_dev, se, sc, st, st_);
 }
 
+int oqph_fock_jk_noalloc(oqph_session* s, const double* d_dev, double* f_dev, double se, double sc, oqph_stats* st,
+                         void* stream) {
+  if (!s || !d_dev || !f_dev) return 4;
+  if (int fc = fault_code()) {
+    set_err("OQPH_FAULT injected decline");
+    return fc;
+  }
+  if (!s->ready) {
+    set_err("no-allocation build: the session is not prepared (call oqph_session_prepare after setting the basis, "
+            "geometry and thresholds; preparing allocates)");
+    return 4;
+  }
+  if (s->share_size > 1 && !s->share_ready) {
+    set_err("no-allocation build: the rank share is not prepared (call oqph_session_prepare after "
+            "oqph_session_set_share)");
+    return 4;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  std::lock_guard<std::mutex> lk(g_mtx);  // the scratch tag of the device
+  ScratchSlot* slot = (s->device < OQPH_MAX_DEVICES) ? &g_scratch[s->device] : nullptr;
+  if (!slot || !slot->ptr) {
+    set_err("no-allocation build: the static scratch of device %d is unavailable", s->device);
+    return 2;
+  }
+  const StaticScratch ss{slot->ptr, (size_t)OQPH_NOALLOC_SCRATCH_BYTES, &slot->serial, &slot->gen};
+  hipStream_t st_ = stream ? (hipStream_t)stream : s->stream;
+  return build_device(s, d_dev, f_dev, se, sc, st, st_, nullptr, nullptr, true, 0.0, &ss);
+}
+
 int oqph_fock_jk(oqph_session* s, const double* d, double* f, int nfocks, double se, double sc, oqph_stats* st) {
   if (!s || !d || !f) return 4;
   if (int fc = fault_code()) {
@@ -1485,7 +1647,9 @@ int oqph_session_set_share(oqph_session* s, int rank, int size) {
 int oqph_session_prepare(oqph_session* s) {
   if (!s) return 4;
   HIPCHK(hipSetDevice(s->device));
-  return prepare_geometry(s);
+  int rc = prepare_geometry(s);
+  if (rc) return rc;
+  return prepare_share(s);
 }
 
 int oqph_kernel_resources(int kind, int la, int lb, int lc, int ld, int pure, int* out) {

@@ -29,7 +29,8 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_session_set_precision", "oqph_set_default_session", "oqph_fock_jk", "oqph_fock_jk_inc",
            "oqph_grad2", "oqph_fock_jk_device", "oqph_schwarz", "oqph_session_nbf", "oqph_last_error",
            "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e",
-           "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources"]
+           "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources",
+           "oqph_fock_jk_noalloc"]
 
 
 class OqphError(RuntimeError):
@@ -64,6 +65,8 @@ def load(path: str | None = None):
     L.oqph_grad2.argtypes = [vp, _pd, _pd, C.c_double, C.c_double, C.POINTER(Stats)]
     L.oqph_fock_jk_device.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                                       C.POINTER(Stats), C.c_void_p]
+    L.oqph_fock_jk_noalloc.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.POINTER(Stats),
+                                       C.c_void_p]
     L.oqph_schwarz.argtypes = [vp, _pd]
     L.oqph_session_prepare.argtypes = [vp]
     L.oqph_session_set_share.argtypes = [vp, C.c_int, C.c_int]
@@ -125,6 +128,7 @@ class FockBuilder:
         _check(load().oqph_session_set_share(self._s, rank, size), "set_share")
 
     def prepare(self):
+        """Build the geometry-dependent device state (and the rank share) now."""
         _check(load().oqph_session_prepare(self._s), "prepare")
 
     def fock_jk_cam(self, d_packed, alpha: float, beta: float, mu: float, alpha_coulomb: float = 1.0,
@@ -189,6 +193,15 @@ class FockBuilder:
         st = Stats()
         _check(load().oqph_fock_jk_device(self._s, C.c_void_p(d_dev_ptr), C.c_void_p(f_dev_ptr), 1, se, sc,
                                           C.byref(st), C.c_void_p(stream)), "fock_jk_device")
+        return st
+
+    def fock_jk_noalloc(self, d_dev_ptr: int, f_dev_ptr: int, se: float = 1.0, sc: float = 1.0,
+                        stream: int = 0) -> Stats:
+        """RHF build through the no-allocation entry point (the session must be
+        prepared: ``prepare()``)."""
+        st = Stats()
+        _check(load().oqph_fock_jk_noalloc(self._s, C.c_void_p(d_dev_ptr), C.c_void_p(f_dev_ptr), se, sc,
+                                           C.byref(st), C.c_void_p(stream)), "fock_jk_noalloc")
         return st
 
     def grad2(self, d_packed, hfscale: float = 1.0, coulscale: float = 1.0) -> np.ndarray:
