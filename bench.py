@@ -258,11 +258,20 @@ def main():
         t = time.perf_counter()
         for _ in range(steps):
             st_last = fb.fock_jk_device(d_dev.data_ptr(), fout.data_ptr(), stream=stream)
+            # a class may run as two launches (grouped and segmented bra-row
+            # kernels): per-class figures are per build
+            seen = set()
             for p in fb.last_profile():
-                q = prof.setdefault(p["cls"], {"ms": 0.0, "flops": 0.0, "quartets": 0, "prims": 0, "n": 0,
-                                               "kernel": p["kernel"]})
+                q = prof.setdefault(p["cls"], {"ms": 0.0, "flops": 0.0, "quartets": 0, "prims": 0, "prims_eval": 0,
+                                               "screened": 0, "slots": 0, "n": 0, "kernel": p["kernel"]})
                 q["ms"] += p["ms"]; q["flops"] += p["flops"]; q["quartets"] += p["quartets"]
-                q["prims"] += p["prims"]; q["n"] += 1
+                q["prims"] += p["prims"]; q["prims_eval"] += p["prims_eval"]
+                q["screened"] += p["screened"]; q["slots"] += p["slots"]
+                if p["cls"] not in seen:
+                    q["n"] += 1
+                    seen.add(p["cls"])
+                if p["kernel"] not in q["kernel"].split("+"):
+                    q["kernel"] += "+" + p["kernel"]
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / steps
         fb.set_profiling(False)
@@ -359,7 +368,10 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
         "per_class": {str(k): {"kernel": v["kernel"], "ms_fp64": round(v["ms"] / v["n"], 3),
                                "ms_mixed": round(profmx[k]["ms"] / profmx[k]["n"], 3) if k in profmx else None,
-                               "quartets": v["quartets"] // v["n"], "prim_quartets": v["prims"] // v["n"]}
+                               "quartets": v["quartets"] // v["n"], "prim_quartets": v["prims"] // v["n"],
+                               "prim_quartets_evaluated": (profmx.get(k, v)["prims_eval"] // profmx.get(k, v)["n"])
+                               or None,
+                               "screened": v["screened"] // v["n"], "ket_lanes": v["slots"] // v["n"]}
                       for k, v in sorted(prof64.items())},
     }
     print(json.dumps(out), flush=True)

@@ -144,12 +144,22 @@ int oqph_fock_jk_noalloc(oqph_session *s, const double *d_dev, double *f_dev, do
  * launch); oqph_last_profile copies up to maxn entries of the last build:
  * class code la*1000+lb*100+lc*10+ld (plus 10000 when the launch used the
  * bra-row kernel k_jkb rather than k_jk, 20000 for the run-time-class kernel
- * k_jkh of the classes with an f shell), device ms, model FLOPs, computed
+ * k_jkh of the classes with an f shell, 30000 for the grouped bra-row kernel
+ * k_jkg), device ms, model FLOPs, computed
  * quartets, primitive quartets.  Returns the number of launches.
  * No counterpart in the seam: measurement hooks for benchmarks. */
 int oqph_session_set_profiling(oqph_session *s, int on);
 int oqph_last_profile(oqph_session *s, int maxn, int *cls, double *ms, double *flops, long long *quartets,
                       long long *prims);
+/* The raw statistics counters of the last build, 8 per launch in launch
+ * order: computed quartets, quartets skipped by the density screen, FP32
+ * quartets, model FLOPs, primitive quartets (segmented count), primitive
+ * quartets evaluated (the grouped kernel evaluates a chunk's shared primitive
+ * quartet once for all members; 0 for the other kernels), ket lanes visited
+ * (64 per ket iteration of a wavefront: computed + screened quartets over it
+ * is the lane occupancy of the screening loop), 0.  Returns the number of
+ * launches copied. */
+int oqph_last_counters(oqph_session *s, int maxn, unsigned long long *ctr);
 
 /* Schwarz table (nsh*nsh, symmetric) of the current geometry. */
 int oqph_schwarz(oqph_session *s, double *q);

@@ -49,7 +49,7 @@ def _cost(c):
 
 # headers each translation unit depends on (cache key = unit + these + flags)
 DEPS = {
-    "jk_class.hip": ["jk_device.h", "eri_device.h", "oqph_common.h", "rys_table.inc", "rys_table_f32.inc"],
+    "jk_class.hip": ["jk_device.h", "jkg_device.h", "eri_device.h", "oqph_common.h", "rys_table.inc", "rys_table_f32.inc"],
     "dispatch.hip": ["eri_device.h", "grad_device.h", "grad_params.h", "oqph_common.h", "rys_table.inc", "rys_table_f32.inc"],
     "session.hip": ["oqph_common.h", "grad_params.h", "ao_xform.h", "../../include/oqph.h"],
     "grad_class.hip": ["grad_device.h", "grad_params.h", "eri_device.h", "oqph_common.h", "rys_table.inc", "rys_table_f32.inc"],

@@ -28,6 +28,9 @@ namespace oqph {
 #define OQPH_DECL_JK(a, b, c, d)                                                                            \
   void launch_jk_##a##b##c##d(bool, const JKParams&, const WaveWork&, hipStream_t);                        \
   bool launch_jkb_##a##b##c##d(bool, const JKParams&, const BraWork&, const ColMap&, size_t, hipStream_t); \
+  bool launch_jkg_##a##b##c##d(bool, const JKParams&, const BraWork&, const ColMap&, const ChunkView&, size_t,  \
+                               hipStream_t);                                                                \
+  int jkg_info_##a##b##c##d(bool, size_t*, size_t);                                                         \
   bool jkb_info_##a##b##c##d(bool, int*, size_t*, size_t);                                                 \
   bool jk_res_##a##b##c##d(int, bool, KernelRes*);
 OQPH_JK_CLASSES(OQPH_DECL_JK)
@@ -68,6 +71,29 @@ bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, co
   }
 }
 
+bool launch_jkg(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
+                const ChunkView& cv, size_t lds, hipStream_t s) {
+  if (bw.nitem == 0) return true;
+  switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): return launch_jkg_##a##b##c##d(pure, p, bw, cm, cv, lds, s);
+    OQPH_JK_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
+    default: return false;
+  }
+}
+
+// members per chunk of the class's grouped bra-row kernel; 0: none
+int jkg_info(int la, int lb, int lc, int ld, bool pure, size_t* static_lds, size_t lds_max) {
+  switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): return jkg_info_##a##b##c##d(pure, static_lds, lds_max);
+    OQPH_JK_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
+    default: return 0;
+  }
+}
+
 // false when the class has no bra-row kernel
 bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* static_lds, size_t lds_max) {
   switch (OQPH_CODE(la, lb, lc, ld)) {
@@ -80,10 +106,10 @@ bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* stat
 }
 
 // Compiled resources of one class-templated kernel: kind 0 = k_jk, 1 = k_jkb,
-// 2 = k_grad, 3 = k_schwarz (lc, ld ignored); false when there is no such
+// 2 = k_grad, 3 = k_schwarz (lc, ld ignored), 4 = k_jkg; false when there is no such
 // kernel.  hipFuncGetAttributes reads the code object, nothing is launched.
 bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, KernelRes* r) {
-  if (kind == 0 || kind == 1 || kind == 3) {
+  if (kind == 0 || kind == 1 || kind == 3 || kind == 4) {
     if (kind == 3) { lc = la; ld = lb; }
     switch (OQPH_CODE(la, lb, lc, ld)) {
 #define OQPH_CASE(a, b, c, d) \

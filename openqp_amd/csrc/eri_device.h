@@ -281,6 +281,11 @@ __device__ __forceinline__ PrimV<R> prim_v(const PP& p) {
   return {(R)p.g, (R)p.Kg, {(R)p.X[0], (R)p.X[1], (R)p.X[2]}, (R)p.h};
 }
 
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void rys_add(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AC)[3], R ze, R sq, R ize,
+                                        R pref, const double* __restrict__ tabN, const float* __restrict__ tab32N,
+                                        R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]);
+
 // Adds one primitive quartet's (e0|f0) to acc; returns false when the
 // primitive prefactor screen |K_ab K_cd| / sqrt(zeta+eta) < qcut drops it.
 // pp, qq: bra and ket primitive pairs (X = P - A and X = Q - C); AC = A - C,
@@ -295,8 +300,6 @@ __device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R
                                                R imu2, const double* __restrict__ tabN,
                                                const float* __restrict__ tab32N,
                                                R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]) {
-  using Dm = Dims<LA, LB, LC, LD>;
-  constexpr int LAB = Dm::LAB, LCD = Dm::LCD, NR = Dm::NROOT;
   constexpr bool F32 = sizeof(R) == 4;
   const R zeta = pp.g, eta = qq.g;
   const R fqz = pp.Kg * qq.Kg;
@@ -310,6 +313,21 @@ __device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R
     sq = rsq64(ze);
     ize = sq * sq;
   }
+  rys_add<LA, LB, LC, LD, R>(pp, qq, AC, ze, sq, ize, fqz * sq, tabN, tab32N, acc);
+  return true;
+}
+
+// The quadrature of one primitive quartet without the screen: adds the
+// (e0|f0) products, scaled by pref, to acc.  ze = zeta + eta (+ zeta eta /
+// mu^2 for the attenuated operator), sq = 1/sqrt(ze), ize = 1/ze.
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void rys_add(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AC)[3], R ze, R sq, R ize,
+                                        R pref, const double* __restrict__ tabN, const float* __restrict__ tab32N,
+                                        R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]) {
+  using Dm = Dims<LA, LB, LC, LD>;
+  constexpr int LAB = Dm::LAB, LCD = Dm::LCD, NR = Dm::NROOT;
+  constexpr bool F32 = sizeof(R) == 4;
+  const R zeta = pp.g, eta = qq.g;
   const R rho = zeta * eta * ize;
   R PQ[3], PA[3], QC[3];
   R pq2 = 0;
@@ -321,7 +339,6 @@ __device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R
     pq2 += PQ[d] * PQ[d];
   }
   const R T = rho * pq2;
-  const R pref = fqz * sq;
   // rho/zeta = eta/(zeta+eta), rho/eta = zeta/(zeta+eta): no further divisions
   const R rz = eta * ize, re = zeta * ize;
   const R h2z = pp.h, h2e = qq.h, h2ze = (R)0.5 * ize;
@@ -383,7 +400,6 @@ __device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R
       root(t2, wr);
     }
   }
-  return true;
 }
 
 // ---------------------------------------------------------------- HRR

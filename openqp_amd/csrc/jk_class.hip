@@ -3,6 +3,7 @@
 // classes (XA XB | XA XB) the Schwarz kernel.  Built once per class with
 // -DXA=.. -DXB=.. -DXC=.. -DXD=.. so the classes compile in parallel.
 #include "jk_device.h"
+#include "jkg_device.h"
 
 #define OQPH_CAT2(a, b, c, d) a##b##c##d
 #define OQPH_NAME(p, a, b, c, d) OQPH_CAT2(p, a, b, c) ## d
@@ -61,6 +62,45 @@ bool OQPH_JKB_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWo
   }
 }
 
+#define OQPH_JKG_LAUNCHER_(a, b, c, d) launch_jkg_##a##b##c##d
+#define OQPH_JKG_LAUNCHER(a, b, c, d) OQPH_JKG_LAUNCHER_(a, b, c, d)
+#define OQPH_JKG_INFO_(a, b, c, d) jkg_info_##a##b##c##d
+#define OQPH_JKG_INFO(a, b, c, d) OQPH_JKG_INFO_(a, b, c, d)
+
+// Grouped bra-row kernel of this class: members per chunk (0 when the class
+// has none, i.e. one member: k_jkb serves it) and static LDS bytes; raises
+// the kernel's dynamic-LDS limit to lds_max.
+int OQPH_JKG_INFO(XA, XB, XC, XD)(bool pure, size_t* static_lds, size_t lds_max) {
+  if constexpr (Big<XA, XB, XC, XD>::value || JKG<XA, XB, XC, XD>::nm < 2) {
+    return 0;
+  } else {
+    const void* f = pure ? (const void*)k_jkg<XA, XB, XC, XD, true> : (const void*)k_jkg<XA, XB, XC, XD, false>;
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, f) != hipSuccess) return 0;
+    if (a.localSizeBytes > OQPH_PRIVATE_BUDGET) return 0;
+    if (lds_max > a.sharedSizeBytes &&
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max - a.sharedSizeBytes)) !=
+            hipSuccess)
+      return 0;
+    *static_lds = a.sharedSizeBytes;
+    return JKG<XA, XB, XC, XD>::nm;
+  }
+}
+
+bool OQPH_JKG_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
+                                       const ChunkView& cv, size_t lds, hipStream_t s) {
+  if constexpr (Big<XA, XB, XC, XD>::value || JKG<XA, XB, XC, XD>::nm < 2) {
+    return false;
+  } else {
+    const int bs = JKG<XA, XB, XC, XD>::threads;
+    if (pure)
+      hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm, cv);
+    else
+      hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm, cv);
+    return true;
+  }
+}
+
 #define OQPH_JK_RES_(a, b, c, d) jk_res_##a##b##c##d
 #define OQPH_JK_RES(a, b, c, d) OQPH_JK_RES_(a, b, c, d)
 
@@ -75,7 +115,7 @@ static bool func_res(const void* f, KernelRes* r) {
 }
 
 // Compiled resources of this class's kernels: kind 0 = k_jk, 1 = k_jkb,
-// 3 = k_schwarz (diagonal classes only).
+// 3 = k_schwarz (diagonal classes only), 4 = k_jkg.
 bool OQPH_JK_RES(XA, XB, XC, XD)(int kind, bool pure, KernelRes* r) {
   if (kind == 0)
     return func_res(pure ? (const void*)k_jk<XA, XB, XC, XD, true> : (const void*)k_jk<XA, XB, XC, XD, false>, r);
@@ -84,6 +124,14 @@ bool OQPH_JK_RES(XA, XB, XC, XD)(int kind, bool pure, KernelRes* r) {
       return false;
     } else {
       return func_res(pure ? (const void*)k_jkb<XA, XB, XC, XD, true> : (const void*)k_jkb<XA, XB, XC, XD, false>,
+                      r);
+    }
+  }
+  if (kind == 4) {
+    if constexpr (Big<XA, XB, XC, XD>::value || JKG<XA, XB, XC, XD>::nm < 2) {
+      return false;
+    } else {
+      return func_res(pure ? (const void*)k_jkg<XA, XB, XC, XD, true> : (const void*)k_jkg<XA, XB, XC, XD, false>,
                       r);
     }
   }

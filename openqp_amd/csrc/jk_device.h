@@ -85,7 +85,8 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
                                          const PrimF32* __restrict__ prims32, const double* __restrict__ tabN,
                                          const float* __restrict__ tab32N, R qcut, R imu2,
                                          R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF],
-                                         int& nprim_used, const PrimV<double>* sbra = nullptr) {
+                                         int& nprim_used, const PrimV<double>* sbra = nullptr,
+                                         const PrimV<float>* sbra32 = nullptr) {
   using Dm = Dims<LA, LB, LC, LD>;
   constexpr bool F32 = sizeof(R) == 4;
 #pragma unroll
@@ -107,7 +108,9 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
       const PrimV<R> qq = ket(Q.prim0 + iq);
       if (fabs(qq.Kg) < qcut) continue;
       for (int ip = 0; ip < npp; ++ip) {
-        const PrimV<R> pp = prim_v<R>(sbra[ip]);
+        PrimV<R> pp;
+        if constexpr (F32) pp = sbra32 ? sbra32[ip] : prim_v<float>(sbra[ip]);
+        else pp = sbra[ip];
         if (fabs(pp.Kg) < qcut) continue;
         nprim_used += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AC, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
       }
@@ -231,10 +234,12 @@ __device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPai
                                               const PrimF32* __restrict__ prims32, const double* __restrict__ tabN,
                                               const float* __restrict__ tab32N, const double* __restrict__ xf,
                                               R qcut, R imu2, R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used,
-                                              const PrimV<double>* sbra = nullptr) {
+                                              const PrimV<double>* sbra = nullptr,
+                                              const PrimV<float>* sbra32 = nullptr) {
   using Dm = Dims<LA, LB, LC, LD>;
   R acc[Dm::NE][Dm::NF];
-  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, prims32, tabN, tab32N, qcut, imu2, acc, nprim_used, sbra);
+  contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, prims32, tabN, tab32N, qcut, imu2, acc, nprim_used, sbra,
+                                           sbra32);
   R AB[3], CD[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -654,6 +659,11 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   constexpr int NROW = OA + OB, NOUT = OA * OB * OC * OD, NBLK = Dm::NBLK;
   __shared__ double s_t64[T64 ? rys_len<NR>() : 1];
   __shared__ PrimV<double> s_bra[JKB_MAX_BRA_PRIMS];
+  // FP32 copy of the bra primitives for the smallest classes, whose FP32
+  // quadrature is short enough that converting the FP64 copy per primitive
+  // quartet shows (5 % of (ps|ss)); larger classes keep the LDS for blocks
+  constexpr bool B32 = T32 && Dm::NE * Dm::NF <= 4;
+  __shared__ PrimV<float> s_bra32[B32 ? JKB_MAX_BRA_PRIMS : 1];
   __shared__ float s_t32[T32 ? rys32_len<NR>() : 1];
   __shared__ double s_jab[OA * OB];
   // dynamic LDS: K rows [NROW][ncol] (accumulated); when cm.drows, also D rows
@@ -677,6 +687,9 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   const bool bra_lds = P.nprim <= JKB_MAX_BRA_PRIMS;  // block-uniform
   if (bra_lds)
     for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra[i] = prim_v<double>(prm.prims[P.prim0 + i]);
+  if constexpr (B32)
+    if (bra_lds && prm.mixed)
+      for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra32[i] = prim_v<float>(prm.prims32[P.prim0 + i]);
   // the density entries the digestion reads in the bra rows a and b, gathered
   // once per block instead of once per quartet
   const bool drows = cm.drows != 0;  // launch-uniform
@@ -696,6 +709,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   const double* tabN = T64 ? (const double*)s_t64 : prm.rys + RYS_OFF[NR];
   const float* tab32N = T32 ? (const float*)s_t32 : prm.rys32 + RYS32_OFF[NR];
   const PrimV<double>* sbra = bra_lds ? (const PrimV<double>*)s_bra : nullptr;
+  const PrimV<float>* sbra32 = (B32 && bra_lds && prm.mixed) ? (const PrimV<float>*)s_bra32 : nullptr;
   const int lane = (int)__lane_id(), wv = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
   const int oa = P.ao_a, ob = P.ao_b;
   const double* Dp = prm.D;
@@ -755,7 +769,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
             float Vf[NBLK];
             quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, prm.prims32, tabN, tab32N,
                                                              prm.xform, (float)(1.0e-4 * prm.cutoff),
-                                                             (float)prm.inv_mu2, Vf, nprim, sbra);
+                                                             (float)prm.inv_mu2, Vf, nprim, sbra, sbra32);
 #pragma unroll
             for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
           }

@@ -103,8 +103,9 @@ struct KernelRes {
 // Primitive-pair-count buckets of the class lists.
 #define OQPH_NBKT 6
 
-// Bra-row work of one J/K class-pair launch: block i takes bra pair item[i*7]
-// and, for each ket bucket b, the first item[i*7 + 1 + b] kets of that bucket
+// Bra-row work of one J/K class-pair launch: block i takes bra pair
+// item[i*(1+OQPH_NBKT)] and, for each ket bucket b, the first
+// item[i*(1+OQPH_NBKT) + 1 + b] kets of that bucket
 // (ket[kb[b] ...]); blocks are ordered by decreasing cost.
 struct BraWork {
   const int* item;
@@ -112,6 +113,33 @@ struct BraWork {
   int kb[OQPH_NBKT + 1];
   int nitem;
 };
+
+// Grouped bra-row work (k_jkg).  Segmented shells contracted from one set of
+// primitive exponents on one atom (cc-pVDZ oxygen: 1s and 2s over nine s
+// exponents) form a family; the shell pairs of one class whose two shells
+// come from the same two families share every primitive pair's exponents and
+// centres and differ only in the contraction coefficients.  A chunk is up to
+// OQPH_JKG_MAXMEM such pairs: its primitive records hold the common fields
+// once and one K/zeta per member, so a block evaluates the quadrature of a
+// primitive quartet once for all members.
+#define OQPH_JKG_MAXMEM 4
+// Chunk c starts at pool[hdr[c]]:
+//   [0] nmem  [1] first primitive record (offset in doubles)  [2] nprim
+//   [3] nrow (K/D rows: the AOs of the members' distinct shells)  [4] nshl
+//   (distinct shells)  [5] stride (doubles per primitive record, 6 + nmem)
+//   [6] offset of the row AO list  [7] offset of the shell list (both from
+//   the chunk start)  [8 + 5m ...] member m: device pair id, first K row of
+//   its shell a, first K row of its shell b, shell slot of si, of sj.
+// Primitive record: zeta, X = P - A (3), 1/(2 zeta), max_m |K_m/zeta|,
+// K_m/zeta per member (0 where the member's own pair screens dropped it).
+struct ChunkView {
+  const int* hdr;
+  const int* pool;
+  const double* cp;
+  const float* cp32;  // the same records in FP32 (mixed mode)
+};
+#define OQPH_JKG_HDR 8
+#define OQPH_JKG_MEM 5
 
 // LDS column space of the K rows of one launch: the AOs of the shells of
 // angular momentum lc (C space) and, when ld != lc, of ld (D space, after C).

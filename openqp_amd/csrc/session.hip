@@ -18,6 +18,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -31,6 +33,9 @@ bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, con
 bool launch_jkb(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
                 size_t lds, hipStream_t s);
 bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* static_lds, size_t lds_max);
+bool launch_jkg(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
+                const ChunkView& cv, size_t lds, hipStream_t s);
+int jkg_info(int la, int lb, int lc, int ld, bool pure, size_t* static_lds, size_t lds_max);
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                     const double* tab, const double* xf, double qcut, double imu2, double* Q, hipStream_t s);
 bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, KernelRes* r);
@@ -121,8 +126,15 @@ static inline int class_of(int la, int lb) { return la * (la + 1) / 2 + lb; }  /
 static inline int nao_of(int l, bool pure) { return (pure && l >= 2) ? 2 * l + 1 : (l + 1) * (l + 2) / 2; }
 // primitive-pair-count buckets of the class lists
 static constexpr int NBKT = OQPH_NBKT;
+// Primitive-pair-count buckets: the 64 kets of a wavefront come from one
+// bucket, and its lanes wait for the most contracted of them (14 buckets
+// growing by 1.3x measured no faster overall on (H2O)64/cc-pVDZ: the s/p
+// classes gained 5-8 %, the d classes lost as much to partial wavefronts).
 static inline int bucket_of(int nprim) {
-  return nprim <= 1 ? 0 : nprim <= 4 ? 1 : nprim <= 9 ? 2 : nprim <= 20 ? 3 : nprim <= 40 ? 4 : 5;
+  static const int hi[OQPH_NBKT - 1] = {1, 4, 9, 20, 40};
+  int b = 0;
+  while (b < OQPH_NBKT - 1 && nprim > hi[b]) ++b;
+  return b;
 }
 
 struct oqph_session {
@@ -174,8 +186,26 @@ struct oqph_session {
     bool hl;     // run-time-class kernel (a class without a template), layout hd
     HLDims hd;
   };
+  // Grouped bra-row kernel (k_jkg): per shell, the shell whose primitive set
+  // it is contracted from (fam[i] = i for a family's master); per bra class
+  // and member cap, the chunks of pairs that share primitive pairs
+  std::vector<int> fam;
+  struct ChunkSet {
+    std::vector<int> hdr, pool;          // host copies of the device tables
+    std::vector<std::vector<int>> pos;   // per chunk: members' class-list positions
+    std::vector<int> ncp, nrow, nshl, stride;
+    DBuf<int> d_hdr, d_pool;
+    DBuf<double> d_cp;
+    DBuf<float> d_cp32;
+    ChunkView view() const { return {d_hdr.p, d_pool.p, d_cp.p, d_cp32.p}; }
+  };
+  std::map<int, std::unique_ptr<ChunkSet>> chunksets;  // key: bra class * 16 + cap
+  int jkg_mode = 1;  // OQPH_JKG: 0 = segmented bra-row kernel only, 1 = default class pairs, 2 = all
+  int jkg_cap = 0;   // OQPH_JKG_CAP: at most this many members per chunk (0: the kernel's)
   struct WLaunch {
     int la, lb, lc, ld;
+    bool grouped = false;  // grouped bra-row kernel (k_jkg; bw, cm, cv, lds)
+    ChunkView cv{};
     bool rows;   // bra-row kernel (bw, cm, lds) instead of wavefront tasks (ww)
     bool hl;     // run-time-class kernel on the wavefront tasks (f shells), layout hd
     HLDims hd;
@@ -184,6 +214,7 @@ struct oqph_session {
     ColMap cm;
     size_t lds;
     size_t seg0, ch0, item0;  // offsets of this launch in the work lists
+    long long slots = 0;      // ket lanes the launch's wavefronts visit (64 per ket iteration)
   };
   // bra-row kernel: LDS column spaces per angular momentum and its work items
   std::vector<int> h_colidx, h_colao, colbase, ncolL;
@@ -439,6 +470,202 @@ static int prepare_share(oqph_session* s) {
   return 0;
 }
 
+// Families of the grouped bra-row kernel: shell i is evaluated with the
+// primitive set of fam[i], the first shell on the same atom with the same
+// angular momentum and the same exponents (cc-pVDZ oxygen: 1s and 2s over one
+// set of nine s exponents).  Shells whose exponents are a proper subset (the
+// single-primitive 3s) stay apart: with them a chunk would evaluate its whole
+// primitive grid for kets that only the small member keeps.
+static void build_families(oqph_session* s) {
+  const int n = s->nsh;
+  s->fam.assign(n, 0);
+  auto subset = [&](int i, int j) {  // exponents of i all among those of j
+    for (int p = 0; p < s->nprim[i]; ++p) {
+      const double e = s->ex[s->poff[i] + p];
+      bool found = false;
+      for (int q = 0; q < s->nprim[j] && !found; ++q) found = s->ex[s->poff[j] + q] == e;
+      if (!found) return false;
+    }
+    return true;
+  };
+  for (int i = 0; i < n; ++i) {
+    int m = i;
+    for (int j = 0; j < n; ++j)
+      if (j < m && s->atom[j] == s->atom[i] && s->am[j] == s->am[i] && s->nprim[j] == s->nprim[i] && subset(i, j))
+        m = j;
+    s->fam[i] = m;
+  }
+}
+
+// Chunks of bra class cb with at most cap members: the class's pairs grouped
+// by the families of their two shells (same two atoms, same primitive
+// exponents), members in decreasing primitive-pair count, cut into chunks of
+// cap.  Each chunk's primitive records cover the union of its members'
+// primitive pairs (each member keeps its own K/zeta, 0 where its pair
+// screens dropped the pair); its K rows are the AOs of the members' distinct
+// shells.
+static int build_chunkset(oqph_session* s, int cb, int cap, const std::vector<PrimPair>& prims,
+                          oqph_session::ChunkSet& cs) {
+  const std::vector<int>& list = s->cls[cb];
+  std::map<std::pair<int, int>, std::vector<int>> groups;  // (fam a, fam b) -> positions
+  std::vector<std::pair<int, int>> order;
+  for (int p = 0; p < (int)list.size(); ++p) {
+    const ShellPair& sp = s->pairs[list[p]];
+    const std::pair<int, int> key{s->fam[sp.sa], s->fam[sp.sb]};
+    auto it = groups.find(key);
+    if (it == groups.end()) {
+      order.push_back(key);
+      groups[key] = {p};
+    } else {
+      it->second.push_back(p);
+    }
+  }
+  std::vector<double> cp;
+  std::vector<float> cp32;
+  for (const auto& key : order) {
+    std::vector<int> mem = groups[key];
+    std::stable_sort(mem.begin(), mem.end(),
+                     [&](int x, int y) { return s->pairs[list[x]].nprim > s->pairs[list[y]].nprim; });
+    for (size_t c0 = 0; c0 < mem.size(); c0 += cap) {
+      const int nm = (int)std::min<size_t>(cap, mem.size() - c0);
+      const ShellPair& p0 = s->pairs[list[mem[c0]]];
+      // union of the members' primitive pairs, keyed by the two exponents
+      std::map<std::pair<double, double>, int> gi;
+      std::vector<const PrimPair*> base;
+      std::vector<std::vector<double>> kg;
+      for (int m = 0; m < nm; ++m) {
+        const ShellPair& sp = s->pairs[list[mem[c0 + m]]];
+        for (int k = sp.prim0; k < sp.prim0 + sp.nprim; ++k) {
+          const PrimPair& q = prims[k];
+          auto r = gi.emplace(std::make_pair(q.a, q.b), (int)base.size());
+          if (r.second) {
+            base.push_back(&q);
+            kg.emplace_back(nm, 0.0);
+          }
+          kg[r.first->second][m] = q.Kg;
+        }
+      }
+      // distinct shells of the members, their K rows and dsh slots
+      std::vector<int> shl, row0;
+      int nrow = 0;
+      auto slot = [&](int sh) {
+        for (size_t t = 0; t < shl.size(); ++t)
+          if (shl[t] == sh) return (int)t;
+        shl.push_back(sh);
+        row0.push_back(nrow);
+        nrow += s->nao[sh];
+        return (int)shl.size() - 1;
+      };
+      std::vector<int> ent;
+      for (int m = 0; m < nm; ++m) {
+        const ShellPair& sp = s->pairs[list[mem[c0 + m]]];
+        const int ta = slot(sp.sa), tb = slot(sp.sb);
+        const int ti = slot(sp.si), tj = slot(sp.sj);
+        ent.insert(ent.end(), {(int)s->cls_off[cb] + mem[c0 + m], row0[ta], row0[tb], ti, tj});
+      }
+      const int stride = 6 + nm;
+      const int c = (int)cs.hdr.size();
+      cs.hdr.push_back((int)cs.pool.size());
+      const int o = (int)cs.pool.size();
+      const int hdr_len = OQPH_JKG_HDR + OQPH_JKG_MEM * nm;
+      cs.pool.insert(cs.pool.end(), {nm, (int)cp.size(), (int)base.size(), nrow, (int)shl.size(), stride,
+                                     hdr_len, hdr_len + nrow});
+      cs.pool.insert(cs.pool.end(), ent.begin(), ent.end());
+      for (size_t t = 0; t < shl.size(); ++t)
+        for (int k = 0; k < s->nao[shl[t]]; ++k) cs.pool.push_back(s->aooff[shl[t]] + k);
+      cs.pool.insert(cs.pool.end(), shl.begin(), shl.end());
+      (void)o;
+      for (size_t g = 0; g < base.size(); ++g) {
+        const PrimPair& q = *base[g];
+        double km = 0.0;
+        for (int m = 0; m < nm; ++m) km = std::max(km, std::fabs(kg[g][m]));
+        const double rec[6] = {q.g, q.P[0] - p0.A[0], q.P[1] - p0.A[1], q.P[2] - p0.A[2], q.h, km};
+        for (double v : rec) {
+          cp.push_back(v);
+          cp32.push_back((float)v);
+        }
+        for (int m = 0; m < nm; ++m) {
+          cp.push_back(kg[g][m]);
+          cp32.push_back((float)kg[g][m]);
+        }
+      }
+      std::vector<int> pos(mem.begin() + c0, mem.begin() + c0 + nm);
+      cs.pos.push_back(pos);
+      cs.ncp.push_back((int)base.size());
+      cs.nrow.push_back(nrow);
+      cs.nshl.push_back((int)shl.size());
+      cs.stride.push_back(stride);
+      (void)c;
+    }
+  }
+  int rc;
+  if ((rc = cs.d_hdr.ensure(std::max<size_t>(cs.hdr.size(), 1))) ||
+      (rc = cs.d_pool.ensure(std::max<size_t>(cs.pool.size(), 1))) ||
+      (rc = cs.d_cp.ensure(std::max<size_t>(cp.size(), 1))) || (rc = cs.d_cp32.ensure(std::max<size_t>(cp32.size(), 1))))
+    return rc;
+  if (!cs.hdr.empty() &&
+      ((rc = upload_sync(cs.d_hdr.p, cs.hdr.data(), cs.hdr.size() * sizeof(int), s->stream)) ||
+       (rc = upload_sync(cs.d_pool.p, cs.pool.data(), cs.pool.size() * sizeof(int), s->stream)) ||
+       (rc = upload_sync(cs.d_cp.p, cp.data(), cp.size() * sizeof(double), s->stream)) ||
+       (rc = upload_sync(cs.d_cp32.p, cp32.data(), cp32.size() * sizeof(float), s->stream))))
+    return rc;
+  return 0;
+}
+
+// The member cap of a (bra class, ket class) launch of the grouped kernel;
+// 0 when the launch keeps the segmented kernels.  By default the grouped
+// kernel runs (ss|ss) and (ps|ps), the class pairs where it measured faster
+// than the segmented bra-row kernel on (H2O)64/cc-pVDZ ((ss|ss) FP64 72 -> 61
+// ms, (ps|ps) mixed 78 -> 74 ms); for (ps|ss), (ds|ss) and (ds|ps) its
+// larger register file (three waves per SIMD instead of four) cost more than
+// the shared quadrature saved.  OQPH_JKG=2 sends every class pair that has
+// a grouped kernel to it.
+static int jkg_cap_for(const oqph_session* s, int la, int lb, int lc, int ld) {
+  if (!s->jkg_mode) return 0;
+  if (s->jkg_mode == 1 && !((la == 0 && lb == 0 && lc == 0 && ld == 0) || (la == 1 && lb == 0 && lc == 1 && ld == 0)))
+    return 0;
+  size_t st = 0;
+  int nm = jkg_info(la, lb, lc, ld, s->pure, &st, s->lds_max);
+  if (s->jkg_cap > 0) nm = std::min(nm, s->jkg_cap);
+  return nm >= 2 ? nm : 0;
+}
+
+// Chunk sets of every (bra class, cap) a launch can use, built with the
+// geometry so that builds never allocate them.
+static int prepare_chunks(oqph_session* s, const std::vector<PrimPair>& prims) {
+  s->chunksets.clear();
+  if (!s->jkg_mode) return 0;
+  const int ncls = class_of(s->lmax, s->lmax) + 1;
+  for (int cb = 0; cb < ncls; ++cb) {
+    if (s->cls[cb].empty()) continue;
+    // does the class have any pair that shares its primitives with another?
+    std::map<std::pair<int, int>, int> cnt;
+    bool shared = false;
+    for (int pid : s->cls[cb]) {
+      const ShellPair& sp = s->pairs[pid];
+      shared |= ++cnt[{s->fam[sp.sa], s->fam[sp.sb]}] > 1;
+    }
+    if (!shared) continue;
+    int la = 0, lb = 0;
+    for (int x = 0; x <= 3; ++x)
+      for (int y = 0; y <= x; ++y)
+        if (class_of(x, y) == cb) { la = x; lb = y; }
+    for (int ck = 0; ck <= cb; ++ck) {
+      int lc = 0, ld = 0;
+      for (int x = 0; x <= 3; ++x)
+        for (int y = 0; y <= x; ++y)
+          if (class_of(x, y) == ck) { lc = x; ld = y; }
+      const int cap = jkg_cap_for(s, la, lb, lc, ld);
+      if (cap < 2 || s->chunksets.count(cb * 16 + cap)) continue;
+      auto cs = std::make_unique<oqph_session::ChunkSet>();
+      int rc = build_chunkset(s, cb, cap, prims, *cs);
+      if (rc) return rc;
+      s->chunksets[cb * 16 + cap] = std::move(cs);
+    }
+  }
+  return 0;
+}
+
 static int prepare_geometry(oqph_session* s) {
   if (s->ready) return 0;
   if (s->nsh == 0 || s->natm == 0) {
@@ -470,6 +697,10 @@ static int prepare_geometry(oqph_session* s) {
     s->f_templates = !(ft && ft[0] == '0');
     const char* e = getenv("OQPH_JK_ROWS");
     s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
+    const char* g = getenv("OQPH_JKG");
+    s->jkg_mode = (g && (g[0] == '0' || g[0] == '2')) ? g[0] - '0' : 1;
+    const char* gc = getenv("OQPH_JKG_CAP");
+    s->jkg_cap = gc ? atoi(gc) : 0;
   }
   if ((rc = schwarz_bounds(s, 0.0, s->Q))) return rc;
   const size_t np = s->Q.size();
@@ -540,6 +771,8 @@ static int prepare_geometry(oqph_session* s) {
                         (rc = upload_sync(s->d_cls.p, ident.data(), ident.size() * sizeof(int), s->stream))))
       return rc;
   }
+  build_families(s);
+  if ((rc = prepare_chunks(s, prims))) return rc;
   if ((rc = s->d_kbs.ensure(kbs_all.size() + 2))) return rc;
   if ((rc = upload_sync(s->d_kbs.p, kbs_all.data(), kbs_all.size() * sizeof(int), s->stream))) return rc;
   if ((rc = s->d_aooff.ensure(s->nsh)) || (rc = s->d_nao.ensure(s->nsh))) return rc;
@@ -799,6 +1032,9 @@ static int prepare_lists(oqph_session* s, double maxden) {
     size_t lds = 0;
     bool drows = false;
     bool hl = false;  // run-time-class kernel (k_jkh)
+    bool grouped = false;  // grouped bra-row kernel (k_jkg) on chunk set cv
+    ChunkView cv{};
+    long long slots = 0;
   };
   s->h_items.clear();
   std::vector<Pend> pend;
@@ -876,12 +1112,78 @@ static int prepare_lists(oqph_session* s, double maxden) {
         rows = new_at < 0.5 * old_at;
       }
       if (rows) {
+        // grouped bra-row kernel: one block per chunk of pairs sharing primitives
+        const int cap = jkg_cap_for(s, la, lb, lc, ld);
+        const auto csit = cap >= 2 ? s->chunksets.find(cb * 16 + cap) : s->chunksets.end();
+        // bras the grouped launch takes (members of chunks of two or more);
+        // the single pairs stay on the segmented bra-row kernel below
+        std::vector<char> grouped_bra;
+        if (csit != s->chunksets.end()) {
+          const oqph_session::ChunkSet& cs = *csit->second;
+          size_t st = 0;
+          jkg_info(la, lb, lc, ld, s->pure, &st, s->lds_max);
+          struct Itg {
+            int c;
+            long long cost;
+            std::array<int, NBKT> n;
+          };
+          std::vector<Itg> its;
+          size_t need_d = 0, need_n = 0;
+          for (int ci = 0; ci < (int)cs.pos.size(); ++ci) {
+            if (cs.pos[ci].size() < 2) continue;
+            Itg t{ci, 0, {}};
+            long long any = 0;
+            for (int b = 0; b < NBKT; ++b) {
+              int m = 0;
+              for (int p : cs.pos[ci]) m = std::max(m, cnt[(size_t)p * NBKT + b]);
+              t.n[b] = m;
+              any += m;
+              t.cost += (long long)m * kmax[b];
+            }
+            if (any == 0) continue;
+            t.cost *= cs.ncp[ci];
+            its.push_back(t);
+            const size_t rows_b = (size_t)cs.nrow[ci] * ncol * sizeof(double);
+            const size_t prims_b = (size_t)cs.ncp[ci] * cs.stride[ci] * (sizeof(double) + sizeof(float));
+            need_n = std::max(need_n, rows_b + prims_b);
+            need_d = std::max(need_d, 2 * rows_b + (size_t)cs.nshl[ci] * s->nsh * sizeof(double) + prims_b);
+          }
+          const bool gd = s->rows_mode != 3 && st + need_d <= s->lds_max;
+          if (!its.empty() && (gd || st + need_n <= s->lds_max)) {
+            grouped_bra.assign(nb, 0);
+            for (const Itg& t : its)
+              for (int p : cs.pos[t.c]) grouped_bra[p] = 1;
+            std::stable_sort(its.begin(), its.end(), [](const Itg& x, const Itg& y) { return x.cost > y.cost; });
+            const size_t i0 = s->h_items.size();
+            long long slots = 0;
+            for (const Itg& t : its) {
+              s->h_items.push_back(t.c);
+              for (int b = 0; b < NBKT; ++b) {
+                s->h_items.push_back(t.n[b]);
+                slots += 64ll * ((t.n[b] + 63) / 64);
+              }
+            }
+            Pend q{la, lb, lc, ld, ck, 0, 0, 0, 0};
+            q.rows = true;
+            q.grouped = true;
+            q.cv = cs.view();
+            q.item0 = i0;
+            q.nitem = (int)its.size();
+            q.ncolC = ncC;
+            q.ncol = ncol;
+            q.lds = gd ? need_d : need_n;
+            q.drows = gd;
+            q.slots = slots;
+            pend.push_back(q);
+          }
+        }
         struct It {
           int p;
           long long cost;
         };
         std::vector<It> its;
         for (int p = 0; p < nb; ++p) {
+          if (!grouped_bra.empty() && grouped_bra[p]) continue;
           long long cost = 0;
           for (int b = 0; b < NBKT; ++b) cost += (long long)cnt[(size_t)p * NBKT + b] * kmax[b];
           if (cost > 0) its.push_back({p, cost * s->pairs[s->cls[cb][p]].nprim});
@@ -889,9 +1191,13 @@ static int prepare_lists(oqph_session* s, double maxden) {
         // longest blocks first
         std::stable_sort(its.begin(), its.end(), [](const It& x, const It& y) { return x.cost > y.cost; });
         const size_t i0 = s->h_items.size();
+        long long slots = 0;
         for (const It& t : its) {
           s->h_items.push_back((int)s->cls_off[cb] + t.p);  // device pair id
-          for (int b = 0; b < NBKT; ++b) s->h_items.push_back(cnt[(size_t)t.p * NBKT + b]);
+          for (int b = 0; b < NBKT; ++b) {
+            s->h_items.push_back(cnt[(size_t)t.p * NBKT + b]);
+            slots += 64ll * ((cnt[(size_t)t.p * NBKT + b] + 63) / 64);
+          }
         }
         Pend q{la, lb, lc, ld, ck, 0, 0, 0, 0};
         q.rows = true;
@@ -901,6 +1207,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
         q.ncol = ncol;
         q.lds = dynb;
         q.drows = drows;
+        q.slots = slots;
         pend.push_back(q);
         continue;
       }
@@ -938,11 +1245,15 @@ static int prepare_lists(oqph_session* s, double maxden) {
       s->h_choff.push_back(waves);  // terminator of this launch's prefix
       pend.push_back({la, lb, lc, ld, ck, seg0, ch0, nseg, waves});
       pend.back().hl = hl;
+      pend.back().slots = 64ll * waves;
     }
   for (auto& p : pend) {
     oqph_session::WLaunch L{};
     L.la = p.la; L.lb = p.lb; L.lc = p.lc; L.ld = p.ld;
     L.rows = p.rows;
+    L.grouped = p.grouped;
+    L.cv = p.cv;
+    L.slots = p.slots;
     L.seg0 = p.seg0;
     L.ch0 = p.ch0;
     L.item0 = p.item0;
@@ -1157,6 +1468,7 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     if (profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
     bool ok = true;
     if (L.hl) launch_jkh(L.hd, p, L.ww, stream);
+    else if (L.grouped) ok = launch_jkg(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.cv, L.lds, stream);
     else if (L.rows) ok = launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, stream);
     else ok = launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream);
     if (!ok) {
@@ -1173,8 +1485,10 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
                                   hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     s->prof_ctr.assign(8 * nl, 0ull);
-    for (size_t i = 0; i < nl; ++i)
+    for (size_t i = 0; i < nl; ++i) {
       for (size_t k = 0; k < nc; ++k) s->prof_ctr[8 * i + k % 8] += raw[nc * i + k];
+      s->prof_ctr[8 * i + 6] = (unsigned long long)s->launches[i].slots;
+    }
     s->prof_ms.assign(nl, 0.f);
     if (profile)
       for (size_t i = 0; i < nl; ++i) HIPCHK(hipEventElapsedTime(&s->prof_ms[i], s->pev[2 * i], s->pev[2 * i + 1]));
@@ -1360,12 +1674,19 @@ int oqph_last_profile(oqph_session* s, int maxn, int* cls, double* ms, double* f
   for (int i = 0; i < n; ++i) {
     const auto& L = s->launches[i];
     const unsigned long long* c = i * 8 + 8 <= (int)s->prof_ctr.size() ? &s->prof_ctr[8 * i] : nullptr;
-    if (cls) cls[i] = (L.hl ? 20000 : L.rows ? 10000 : 0) + L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
+    if (cls) cls[i] = (L.hl ? 20000 : L.grouped ? 30000 : L.rows ? 10000 : 0) + L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
     if (ms) ms[i] = i < (int)s->prof_ms.size() ? s->prof_ms[i] : 0.0;
     if (flops) flops[i] = c ? (double)c[3] : 0.0;
     if (quartets) quartets[i] = c ? (long long)c[0] : 0;
     if (prims) prims[i] = c ? (long long)c[4] : 0;
   }
+  return n;
+}
+
+int oqph_last_counters(oqph_session* s, int maxn, unsigned long long* ctr) {
+  if (!s || !ctr) return -4;
+  const int n = (int)std::min<size_t>((size_t)maxn, s->prof_ctr.size() / 8);
+  for (int i = 0; i < 8 * n; ++i) ctr[i] = s->prof_ctr[i];
   return n;
 }
 

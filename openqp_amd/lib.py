@@ -30,7 +30,7 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_grad2", "oqph_fock_jk_device", "oqph_schwarz", "oqph_session_nbf", "oqph_last_error",
            "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e",
            "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources",
-           "oqph_fock_jk_noalloc"]
+           "oqph_fock_jk_noalloc", "oqph_last_counters"]
 
 
 class OqphError(RuntimeError):
@@ -78,6 +78,7 @@ def load(path: str | None = None):
     L.oqph_session_set_profiling.argtypes = [vp, C.c_int]
     _pl = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
     L.oqph_last_profile.argtypes = [vp, C.c_int, _pi, _pd, _pd, _pl, _pl]
+    L.oqph_last_counters.argtypes = [vp, C.c_int, np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")]
     L.oqph_int1e.argtypes = [C.c_int, _pi, _pi, _pi, _pi, _pd, _pd, C.c_int, _pd, _pd, C.c_void_p, C.c_void_p,
                              C.c_void_p]
     L.oqph_grad1e.argtypes = [C.c_int, _pi, _pi, _pi, _pi, _pd, _pd, C.c_int, _pd, _pd, C.c_void_p, C.c_void_p,
@@ -149,15 +150,19 @@ class FockBuilder:
     def last_profile(self):
         """Per-launch (class code, ms, model FLOPs, quartets, primitive quartets) of the last build.
 
-        cls = la*1000 + lb*100 + lc*10 + ld; kernel is "k_jkb" (bra-row kernel), "k_jk"
-        (wavefront tasks) or "k_jkh" (run-time class, the classes with an f shell)."""
+        cls = la*1000 + lb*100 + lc*10 + ld; kernel is "k_jkb" (bra-row kernel), "k_jkg"
+        (grouped bra-row kernel), "k_jk" (wavefront tasks) or "k_jkh" (run-time class,
+        the classes with an f shell)."""
         n = 64
         cls = np.zeros(n, np.int32); ms = np.zeros(n); fl = np.zeros(n)
         q = np.zeros(n, np.int64); pq = np.zeros(n, np.int64)
         m = load().oqph_last_profile(self._s, n, cls, ms, fl, q, pq)
-        kern = lambda c: "k_jkh" if c >= 20000 else "k_jkb" if c >= 10000 else "k_jk"
+        kern = lambda c: "k_jkg" if c >= 30000 else "k_jkh" if c >= 20000 else "k_jkb" if c >= 10000 else "k_jk"
+        ctr = np.zeros(8 * n, np.uint64)
+        load().oqph_last_counters(self._s, n, ctr)
         return [dict(cls=int(cls[i]) % 10000, kernel=kern(int(cls[i])), ms=float(ms[i]),
-                     flops=float(fl[i]), quartets=int(q[i]), prims=int(pq[i]))
+                     flops=float(fl[i]), quartets=int(q[i]), prims=int(pq[i]), prims_eval=int(ctr[8 * i + 5]),
+                     screened=int(ctr[8 * i + 1]), slots=int(ctr[8 * i + 6]))
                 for i in range(max(m, 0))]
 
     def schwarz(self) -> np.ndarray:
