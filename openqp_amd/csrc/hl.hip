@@ -59,7 +59,9 @@ HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm) {
   // J/K: D_ab, D_cd, D_ac, D_ad, D_bc, D_bd, each at most (largest AO count)^2
   const int omax = std::max(std::max(h.oa, h.ob), std::max(h.oc, h.od));
   h.dstr = omax * omax;
-  h.offI = h.offD + (grad ? 0 : 6 * h.dstr);
+  h.n4 = grad ? (la + 1) * (lb + 1) * (lc + 1) * (ld + 1) : 0;
+  h.offR = (h.offD + (grad ? 0 : 6 * h.dstr) + 3) & ~3;  // 32-byte aligned records
+  h.offI = h.offR + 12 * h.n4;
   h.lds = (int)sizeof(double) * (h.offI + (nI + 1) / 2);
   return h;
 }

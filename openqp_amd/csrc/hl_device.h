@@ -618,7 +618,8 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
   const int oa = prm.coff[P.sa], ob = prm.coff[P.sb];
   const long long fprim = flops_grad_prim_rt(h.la, h.lb, h.lc, h.ld);
   unsigned long long ncomp = 0ull, nskip = 0ull, flops = 0ull, prims_used = 0ull;
-  const int nrk = h.ni * h.nk, nk = h.nk, nt = h.nt;
+  const int nrk = h.ni * h.nk, nk = h.nk, nt = h.nt, n4 = h.n4;
+  double* R = lds + h.offR;
   const int NTb = h.lb + 2, NTc = h.lc + 2, NTd = h.ld + 1;
   const int sa = NTb * NTc * NTd, sb = NTc * NTd, sc = NTd;  // T strides of ia, ib, ic (id: 1)
   const int lab1 = h.la + h.lb + 1, lcd1 = h.lc + h.ld + 1;
@@ -689,6 +690,24 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
             }
           }
           __syncthreads();
+          // per axis and exponent tuple (ia, ib, ic, id) of the components:
+          // the transferred 2D integral and its A, B and C derivatives
+          //   d/dA = 2 alpha (ia+1) - ia (ia-1)  (likewise B, C), stored as
+          // four consecutive doubles, so a component reads two b128 words per
+          // axis instead of gathering seven table entries
+          for (int idx = lane; idx < 3 * n4; idx += HL_BLOCK) {
+            const int x = idx / n4, t = idx - x * n4;
+            const int id = t % (h.ld + 1), t1 = t / (h.ld + 1);
+            const int ic = t1 % (h.lc + 1), t2 = t1 / (h.lc + 1);
+            const int ib = t2 % (h.lb + 1), ia = t2 / (h.lb + 1);
+            const double* tx = T + x * nt + ia * sa + ib * sb + ic * sc + id;
+            double* rr = R + 4 * idx;
+            rr[0] = tx[0];
+            rr[1] = a2 * tx[sa] - (ia > 0 ? (double)ia * tx[-sa] : 0.0);
+            rr[2] = b2 * tx[sb] - (ib > 0 ? (double)ib * tx[-sb] : 0.0);
+            rr[3] = c2 * tx[sc] - (ic > 0 ? (double)ic * tx[-sc] : 0.0);
+          }
+          __syncthreads();
           HLStep st(lane, NCD);
           for (int t = lane; t < NG; t += HL_BLOCK, st.next()) {
             const double g = Gam[t];
@@ -699,11 +718,13 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
             for (int x = 0; x < 3; ++x) {
               const int ia = (pab >> (8 + 4 * x)) & 15, ib = (pab >> (20 + 4 * x)) & 15;
               const int ic = (pcd >> (8 + 4 * x)) & 15, id = (pcd >> (20 + 4 * x)) & 15;
-              const double* tx = T + x * nt + ia * sa + ib * sb + ic * sc + id;
-              I0[x] = tx[0];
-              dA[x] = a2 * tx[sa] - (ia > 0 ? (double)ia * tx[-sa] : 0.0);
-              dB[x] = b2 * tx[sb] - (ib > 0 ? (double)ib * tx[-sb] : 0.0);
-              dC[x] = c2 * tx[sc] - (ic > 0 ? (double)ic * tx[-sc] : 0.0);
+              const int tt = ((ia * (h.lb + 1) + ib) * (h.lc + 1) + ic) * (h.ld + 1) + id;
+              const double2 v0 = *(const double2*)(R + 4 * (x * n4 + tt));
+              const double2 v1 = *(const double2*)(R + 4 * (x * n4 + tt) + 2);
+              I0[x] = v0.x;
+              dA[x] = v0.y;
+              dB[x] = v1.x;
+              dC[x] = v1.y;
             }
             const double gyz = g * I0[1] * I0[2], gxz = g * I0[0] * I0[2], gxy = g * I0[0] * I0[1];
             f0x += dA[0] * gyz; f0y += dA[1] * gxz; f0z += dA[2] * gxy;

@@ -644,8 +644,14 @@ struct JKB {
 #endif
   static constexpr int threads =
       (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF <= 4) ? OQPH_JKB_SMALL_THREADS : 512;
-  // (ss|ss) fits in 64 VGPRs without spilling: eight waves per SIMD
-  static constexpr int min_waves = (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF == 1) ? 8 : 1;
+  // (ss|ss) fits in 64 VGPRs without spilling: eight waves per SIMD;
+  // OQPH_JKB_WAVES3 sets the target for (e0|f0) sets of 2..4 ((ps|ss))
+#ifndef OQPH_JKB_WAVES3
+#define OQPH_JKB_WAVES3 1
+#endif
+  static constexpr int min_waves = (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF == 1)  ? 8
+                                   : (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF <= 4) ? OQPH_JKB_WAVES3
+                                                                                                 : 1;
 };
 
 template <int LA, int LB, int LC, int LD, bool PURE>

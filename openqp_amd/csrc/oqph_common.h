@@ -84,6 +84,9 @@ struct HLDims {
   int offG, offA, offB;        // LDS (doubles): 2D integrals, buffer A, buffer B
   int offT, offC, offI;        // LDS (doubles): transferred 2D table, HRR coefficients, int tables
   int offD, dstr;              // LDS (doubles): the six density blocks of one quartet (J/K), block stride
+  int offR, n4;                // gradient, LDS (doubles): per axis and component exponent tuple
+                               // (a <= la, b <= lb, c <= lc, d <= ld; n4 of them) the value and the
+                               // A, B, C derivatives of the transferred 2D integral, [3][n4][4]
   int lds;                     // bytes of dynamic LDS
 };
 

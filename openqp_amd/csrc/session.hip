@@ -228,6 +228,10 @@ struct oqph_session {
   // OQPH_F_TEMPLATES=0: every class with an f shell on the run-time-class
   // kernels (k_jkh, k_gradh) instead of the class templates where they exist
   bool f_templates = true;
+  // OQPH_GRAD_HL_NG=n: gradient classes with more than n Cartesian components
+  // per quartet run on the run-time-class kernel k_gradh (0: templates wherever
+  // they exist and fit)
+  int grad_hl_ng = 0;
   std::vector<WLaunch> launches;
   unsigned long long list_gen = 0;       // bumped whenever the host work lists change
   unsigned long long list_uploaded = 0;  // list_gen of the copy in d_seg / d_choff / d_items
@@ -695,6 +699,8 @@ static int prepare_geometry(oqph_session* s) {
   {
     const char* ft = getenv("OQPH_F_TEMPLATES");
     s->f_templates = !(ft && ft[0] == '0');
+    const char* gh = getenv("OQPH_GRAD_HL_NG");
+    s->grad_hl_ng = gh ? atoi(gh) : 0;
     const char* e = getenv("OQPH_JK_ROWS");
     s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
     const char* g = getenv("OQPH_JKG");
@@ -970,7 +976,8 @@ static int prepare_grad(oqph_session* s) {
     L.ww.ket = s->d_gcls.p + coff[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
-    L.hl = !grad_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3));
+    L.hl = !grad_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3)) ||
+           (s->grad_hl_ng > 0 && ncart(p.la) * ncart(p.lb) * ncart(p.lc) * ncart(p.ld) > s->grad_hl_ng);
     if (!L.hl) {  // templates whose private segment exceeds the budget run on k_gradh
       KernelRes kr{};
       L.hl = !kernel_resources(2, p.la, p.lb, p.lc, p.ld, s->pure, &kr) || kr.private_bytes > OQPH_PRIVATE_BUDGET ||

@@ -44,11 +44,12 @@ class Stats(C.Structure):
 
 
 def load(path: str | None = None):
-    """Load liboqph.so; raises if it has not been built."""
+    """Load liboqph.so (the in-tree build, or the file $OQPH_LIB names); raises
+    if it has not been built."""
     global _lib
     if _lib is not None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("OQPH_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise OqphError(f"liboqph not built: {p} missing (run openqp_amd/build.py)")
     L = C.CDLL(p)

@@ -90,6 +90,43 @@ def test_bra_row_kernel_matches_wave_kernel(L, name):
         assert out["0"][1:] == out[mode][1:]
 
 
+@pytest.mark.parametrize("basis,nw,nocc", [("cc-pvdz", 3, 15), ("cc-pvtz", 1, 5)])
+def test_grouped_kernel_modes(L, oracle, basis, nw, nocc):
+    """The grouped bra-row kernel (pairs whose shells share a primitive set:
+    cc-pVXZ 1s/2s) against the oracle and against the segmented kernels:
+    OQPH_JKG=0 (off), 1 (default class pairs), 2 (every class pair with a
+    grouped kernel).  Same G to 1e-12, same computed and screened counts."""
+    _say("grouped kernel modes", basis, nw)
+    bas = small_system(basis, nw=nw, seed=9)
+    D, dp = density_for(oracle, bas, nocc, seed=19)
+    fo, so = oracle.fock_jk(bas, dp)
+    out, kinds = {}, {}
+    old = os.environ.get("OQPH_JKG")
+    try:
+        for mode in ("0", "1", "2"):
+            os.environ["OQPH_JKG"] = mode
+            fb = L.FockBuilder(bas)
+            f = fb.fock_jk(dp)
+            out[mode] = (f, fb.last_stats.quartets_computed, fb.last_stats.quartets_screened)
+            kinds[mode] = {p["kernel"] for p in fb.last_profile()}
+            fb.close()
+    finally:
+        if old is None:
+            os.environ.pop("OQPH_JKG", None)
+        else:
+            os.environ["OQPH_JKG"] = old
+    for mode in ("0", "1", "2"):
+        err = np.abs(out[mode][0] - fo).max()
+        _say("  OQPH_JKG", mode, "max |f - f_oracle| =", err, "quartets", out[mode][1], int(so[1]))
+        assert err <= TOL
+        assert out[mode][1] == int(so[1])
+    for mode in ("1", "2"):
+        assert np.abs(out["0"][0] - out[mode][0]).max() <= 1e-12
+        assert out["0"][1:] == out[mode][1:]
+    assert "k_jkg" not in kinds["0"]
+    assert "k_jkg" in kinds["2"]
+
+
 @pytest.mark.parametrize("name,nw,nocc", [("cc-pvdz", 3, 15), ("6-31g(d)", 4, 20)])
 def test_parity_fresh_density(L, oracle, name, nw, nocc):
     _say("fresh parity", name, nw)
