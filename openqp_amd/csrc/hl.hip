@@ -77,6 +77,12 @@ void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStrea
   hipLaunchKernelGGL(k_jkh, dim3((unsigned)ww.nwaves), dim3(HL_BLOCK), h.lds, s, p, ww, h);
 }
 
+void launch_jkh_multi(const JKParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s) {
+  if (nblocks == 0) return;
+  allow_lds(k_jkh_multi, lds);
+  hipLaunchKernelGGL(k_jkh_multi, dim3((unsigned)nblocks), dim3(HL_BLOCK), lds, s, p, tl);
+}
+
 void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                      const double* rys, const double* xf, double qcut, double imu2, double* Q, hipStream_t s) {
   if (n == 0) return;

@@ -423,10 +423,10 @@ __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& 
 // Wavefront tasks as k_jk: a block takes one 64-ket chunk of one segment,
 // screens its kets lane-parallel with OpenQP's tests, then evaluates and
 // digests the surviving quartets one after another, all 64 lanes on each.
-__global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_jkh(JKParams prm, WaveWork ww, HLDims h) {
-  extern __shared__ double lds[];
+// One 64-ket chunk (wave) of one class's wavefront tasks, by one block.
+__device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& ww, const HLDims& h, long long wave,
+                                          double* lds) {
   const HLTabs tb = hl_tables(h, lds);
-  const long long wave = blockIdx.x;
   if (wave >= ww.nwaves) return;  // block-uniform
   int lo = 0, hi = ww.nseg;
   while (hi - lo > 1) {
@@ -554,6 +554,39 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
     if (flops) atomicAdd(ctr + 3, flops);
     if (prims_used) atomicAdd(ctr + 4, prims_used);
   }
+}
+
+__global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_jkh(JKParams prm, WaveWork ww, HLDims h) {
+  extern __shared__ double lds[];
+  jkh_chunk(prm, ww, h, (long long)blockIdx.x, lds);
+}
+
+// Several run-time classes in one launch: block b takes wave b - wave0 of the
+// task (class) whose range holds it; the tasks come sorted by decreasing
+// cost per wave, so the most expensive chunks start first and the launch ends
+// on the cheapest.  The task tables hold offsets; the list bases are launch
+// arguments (the lists may live in different buffers from build to build).
+__global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_jkh_multi(
+    JKParams prm, HLTaskList tl) {
+  extern __shared__ double lds[];
+  const long long b = blockIdx.x;
+  int lo = 0, hi = tl.ntask;
+  while (hi - lo > 1) {  // scalar binary search over the task prefix
+    const int mid = (lo + hi) >> 1;
+    if (tl.task[mid].wave0 <= b) lo = mid; else hi = mid;
+  }
+  const int t = __builtin_amdgcn_readfirstlane(lo);
+  const HLTask& tk = tl.task[t];
+  WaveWork ww;
+  ww.seg_bra = tl.seg + tk.seg0;
+  ww.seg_k0 = tl.seg + tl.nseg + tk.seg0;
+  ww.seg_n = tl.seg + 2 * tl.nseg + tk.seg0;
+  ww.choff = tl.choff + tk.ch0;
+  ww.ket = tl.ket + tk.ket0;
+  ww.nseg = tk.nseg;
+  ww.nwaves = tk.nwaves;
+  const HLDims h = tk.h;
+  jkh_chunk(prm, ww, h, b - tk.wave0, lds);
 }
 
 // --------------------------------------------------------- Schwarz bounds

@@ -90,6 +90,23 @@ struct HLDims {
   int lds;                     // bytes of dynamic LDS
 };
 
+// One class of a multi-class launch of the run-time-class kernels: its
+// layout, the offsets of its wavefront tasks in the launch's lists, and its
+// range of blocks [wave0, wave0 + nwaves).
+struct HLTask {
+  HLDims h;
+  long long wave0, nwaves;
+  int seg0, ch0, nseg, ket0;
+};
+struct HLTaskList {
+  const HLTask* task;
+  int ntask;
+  const int* seg;          // seg_bra | seg_k0 | seg_n of all launches (nseg each)
+  int nseg;
+  const long long* choff;
+  const int* ket;          // class lists
+};
+
 // Compiled resources of one kernel (hipFuncGetAttributes).
 struct KernelRes {
   int private_bytes;  // per-lane private segment (register spills)

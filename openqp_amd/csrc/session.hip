@@ -48,6 +48,7 @@ bool jk_template(int la, int lb, int lc, int ld);
 bool grad_template(int la, int lb, int lc, int ld);
 HLDims hl_dims(int la, int lb, int lc, int ld, int pure, int grad, int natm);
 void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStream_t s);
+void launch_jkh_multi(const JKParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s);
 void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                      const double* rys, const double* xf, double qcut, double imu2, double* Q, hipStream_t s);
 struct GradParams;
@@ -123,6 +124,14 @@ struct DBuf {
 
 static inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 static inline int class_of(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+// Rys model FLOPs of one primitive quartet of a class (eri_device.h's
+// flops_prim_rt), for ordering work by cost on the host
+static inline long long flops_prim_host(int la, int lb, int lc, int ld) {
+  const int lab = la + lb, lcd = lc + ld, nr = (lab + lcd) / 2 + 1;
+  const int nc = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+  return (long long)nr * (56 + 12 * (lab + 1) * (lcd + 1) + 6 * (la + 1) * (lb + 1) * (lcd + 1) +
+                          6 * (la + 1) * (lb + 1) * (lc + 1) * (ld + 1) + 3 * nc);
+}
 static inline int nao_of(int l, bool pure) { return (pure && l >= 2) ? 2 * l + 1 : (l + 1) * (l + 2) / 2; }
 // primitive-pair-count buckets of the class lists
 static constexpr int NBKT = OQPH_NBKT;
@@ -215,6 +224,12 @@ struct oqph_session {
     size_t lds;
     size_t seg0, ch0, item0;  // offsets of this launch in the work lists
     long long slots = 0;      // ket lanes the launch's wavefronts visit (64 per ket iteration)
+    // multi-class launch of the run-time-class kernel: tasks [task0, task0 +
+    // ntask) of the task list, nblocks blocks, lds bytes (the largest class)
+    bool multi = false;
+    int task0 = 0, ntask = 0;
+    long long nblocks = 0;
+    HLTaskList tl{};
   };
   // bra-row kernel: LDS column spaces per angular momentum and its work items
   std::vector<int> h_colidx, h_colao, colbase, ncolL;
@@ -238,8 +253,11 @@ struct oqph_session {
   unsigned long long serial = 0;         // process-unique session number
   DBuf<int> d_seg;                 // seg_bra | seg_k0 | seg_n of all launches
   DBuf<long long> d_choff;
+  DBuf<HLTask> d_tasks;            // classes of the multi-class run-time-class launches
   std::vector<int> h_seg_bra, h_seg_k0, h_seg_n;
   std::vector<long long> h_choff;
+  std::vector<HLTask> h_tasks;
+  int hl_multi = 1;                // OQPH_HL_MULTI=0: one launch per run-time class
   // gradient state (pair data with the grd2 cutoffs, coarse-screened lists)
   DBuf<ShellPair> d_gpairs;
   DBuf<PrimPair> d_gprims;
@@ -703,6 +721,8 @@ static int prepare_geometry(oqph_session* s) {
     s->grad_hl_ng = gh ? atoi(gh) : 0;
     const char* e = getenv("OQPH_JK_ROWS");
     s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
+    const char* hm = getenv("OQPH_HL_MULTI");
+    s->hl_multi = (hm && hm[0] == '0') ? 0 : 1;
     const char* g = getenv("OQPH_JKG");
     s->jkg_mode = (g && (g[0] == '0' || g[0] == '2')) ? g[0] - '0' : 1;
     const char* gc = getenv("OQPH_JKG_CAP");
@@ -1293,6 +1313,68 @@ static int prepare_lists(oqph_session* s, double maxden) {
     }
     s->launches.push_back(L);
   }
+  // the run-time-class launches (classes with an f shell, (dd|dd)) merged
+  // into one launch per occupancy tier: a launch is sized for the LDS of its
+  // largest class, so only classes that fit the same number of blocks per CU
+  // (LDS per CU over the class's LDS, at most 4 waves per SIMD of one-wave
+  // blocks) share a launch, and no class loses occupancy by the merge
+  s->h_tasks.clear();
+  if (s->hl_multi) {
+    std::vector<oqph_session::WLaunch> merged;
+    std::map<int, std::vector<size_t>> tiers;
+    for (size_t i = 0; i < s->launches.size(); ++i) {
+      const auto& L = s->launches[i];
+      if (!L.hl || L.ww.nwaves == 0) continue;
+      const int per_cu = (int)std::min<size_t>(16, s->lds_max / std::max(L.hd.lds, 1));
+      tiers[per_cu].push_back(i);
+    }
+    std::vector<std::vector<size_t>> tier;
+    for (auto& kv : tiers) tier.push_back(kv.second);
+    std::vector<char> drop(s->launches.size(), 0);
+    for (auto& t : tier) {
+      if (t.size() < 2) continue;
+      // most expensive chunks first: model FLOPs of one primitive quartet
+      std::stable_sort(t.begin(), t.end(), [&](size_t x, size_t y) {
+        const auto &a = s->launches[x], &b = s->launches[y];
+        return flops_prim_host(a.la, a.lb, a.lc, a.ld) > flops_prim_host(b.la, b.lb, b.lc, b.ld);
+      });
+      oqph_session::WLaunch M{};
+      const auto& L0 = s->launches[t[0]];
+      M.la = L0.la; M.lb = L0.lb; M.lc = L0.lc; M.ld = L0.ld;
+      M.hl = true;
+      M.multi = true;
+      M.task0 = (int)s->h_tasks.size();
+      M.ntask = (int)t.size();
+      size_t lds = 0;
+      long long w0 = 0;
+      for (size_t i : t) {
+        const auto& L = s->launches[i];
+        HLTask k{};
+        k.h = L.hd;
+        k.wave0 = w0;
+        k.nwaves = L.ww.nwaves;
+        k.seg0 = (int)L.seg0;
+        k.ch0 = (int)L.ch0;
+        k.nseg = L.ww.nseg;
+        k.ket0 = (int)s->cls_off[class_of(L.lc, L.ld)];
+        s->h_tasks.push_back(k);
+        w0 += L.ww.nwaves;
+        lds = std::max(lds, (size_t)L.hd.lds);
+        M.slots += L.slots;
+        drop[i] = 1;
+      }
+      M.nblocks = w0;
+      M.lds = lds;
+      merged.push_back(M);
+    }
+    if (!merged.empty()) {
+      std::vector<oqph_session::WLaunch> keep;
+      for (size_t i = 0; i < s->launches.size(); ++i)
+        if (!drop[i]) keep.push_back(s->launches[i]);
+      keep.insert(keep.end(), merged.begin(), merged.end());
+      s->launches.swap(keep);
+    }
+  }
   ++s->list_gen;
   s->cached_maxden = maxden;
   return 0;
@@ -1301,15 +1383,16 @@ static int prepare_lists(oqph_session* s, double maxden) {
 // Device work lists of the current launches: element counts of seg (three
 // int arrays of nseg), choff (long long) and items (int).
 struct ListSizes {
-  size_t seg, choff, items;
+  size_t seg, choff, items, tasks;
 };
 static ListSizes list_sizes(const oqph_session* s) {
-  return {3 * s->h_seg_bra.size() + 1, s->h_choff.size() + 1, std::max<size_t>(s->h_items.size(), 1)};
+  return {3 * s->h_seg_bra.size() + 1, s->h_choff.size() + 1, std::max<size_t>(s->h_items.size(), 1),
+          std::max<size_t>(s->h_tasks.size(), 1)};
 }
 
 // Copy the host work lists into device buffers of at least list_sizes();
 // synchronises the stream, so the host vectors may change afterwards.
-static int upload_lists(oqph_session* s, int* seg, long long* choff, int* items, hipStream_t stream) {
+static int upload_lists(oqph_session* s, int* seg, long long* choff, int* items, HLTask* tasks, hipStream_t stream) {
   const size_t nseg = s->h_seg_bra.size();
   int rc;
   if (nseg) {
@@ -1322,6 +1405,9 @@ static int upload_lists(oqph_session* s, int* seg, long long* choff, int* items,
   if (!s->h_items.empty() &&
       (rc = upload(items, s->h_items.data(), s->h_items.size() * sizeof(int), stream)))
     return rc;
+  if (!s->h_tasks.empty() &&
+      (rc = upload(tasks, s->h_tasks.data(), s->h_tasks.size() * sizeof(HLTask), stream)))
+    return rc;
   HIPCHK(hipStreamSynchronize(stream));
   return 0;
 }
@@ -1332,17 +1418,27 @@ static int session_lists(oqph_session* s, hipStream_t stream) {
   if (s->list_uploaded == s->list_gen) return 0;
   const ListSizes n = list_sizes(s);
   int rc;
-  if ((rc = s->d_seg.ensure(n.seg)) || (rc = s->d_choff.ensure(n.choff)) || (rc = s->d_items.ensure(n.items)))
+  if ((rc = s->d_seg.ensure(n.seg)) || (rc = s->d_choff.ensure(n.choff)) || (rc = s->d_items.ensure(n.items)) ||
+      (rc = s->d_tasks.ensure(n.tasks)))
     return rc;
-  if ((rc = upload_lists(s, s->d_seg.p, s->d_choff.p, s->d_items.p, stream))) return rc;
+  if ((rc = upload_lists(s, s->d_seg.p, s->d_choff.p, s->d_items.p, s->d_tasks.p, stream))) return rc;
   s->list_uploaded = s->list_gen;
   return 0;
 }
 
 // Point one launch's work descriptors at a device copy of the lists.
 static void bind_lists(const oqph_session* s, oqph_session::WLaunch& L, const int* seg, const long long* choff,
-                       const int* items) {
+                       const int* items, const HLTask* tasks) {
   const size_t nseg = s->h_seg_bra.size();
+  if (L.multi) {
+    L.tl.task = tasks + L.task0;
+    L.tl.ntask = L.ntask;
+    L.tl.seg = seg;
+    L.tl.nseg = (int)nseg;
+    L.tl.choff = choff;
+    L.tl.ket = s->d_cls.p;
+    return;
+  }
   if (L.rows) {
     L.bw.item = items + L.item0;
     return;
@@ -1414,19 +1510,21 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   int* lseg;
   long long* lchoff;
   int* litems;
+  HLTask* ltasks;
   if (ss) {
     const ListSizes n = list_sizes(s);
     ctr = (unsigned long long*)carve(nc * std::max<size_t>(nl, 1) * sizeof(unsigned long long));
     lseg = (int*)carve(n.seg * sizeof(int));
     lchoff = (long long*)carve(n.choff * sizeof(long long));
     litems = (int*)carve(n.items * sizeof(int));
+    ltasks = (HLTask*)carve(n.tasks * sizeof(HLTask));
     if (off > ss->bytes) {
       set_err("no-allocation build: this basis and density need %zu bytes of scratch (shell-pair maxima, "
               "counters and work lists); the static scratch holds %zu", off, ss->bytes);
       return 1;
     }
     if (*ss->serial != s->serial || *ss->gen != s->list_gen) {
-      if ((rc = upload_lists(s, lseg, lchoff, litems, stream))) return rc;
+      if ((rc = upload_lists(s, lseg, lchoff, litems, ltasks, stream))) return rc;
       *ss->serial = s->serial;
       *ss->gen = s->list_gen;
     }
@@ -1437,8 +1535,9 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     lseg = s->d_seg.p;
     lchoff = s->d_choff.p;
     litems = s->d_items.p;
+    ltasks = s->d_tasks.p;
   }
-  for (auto& L : s->launches) bind_lists(s, L, lseg, lchoff, litems);
+  for (auto& L : s->launches) bind_lists(s, L, lseg, lchoff, litems, ltasks);
   HIPCHK(hipMemsetAsync(ctr, 0, nc * nl * sizeof(unsigned long long), stream));
   const bool profile = s->profile && !ss;
   if (profile && s->pev.size() < 2 * nl) {
@@ -1474,7 +1573,8 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     p.counters = ctr + nc * i;
     if (profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
     bool ok = true;
-    if (L.hl) launch_jkh(L.hd, p, L.ww, stream);
+    if (L.multi) launch_jkh_multi(p, L.tl, L.nblocks, (int)L.lds, stream);
+    else if (L.hl) launch_jkh(L.hd, p, L.ww, stream);
     else if (L.grouped) ok = launch_jkg(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.cv, L.lds, stream);
     else if (L.rows) ok = launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, stream);
     else ok = launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream);
@@ -1681,7 +1781,10 @@ int oqph_last_profile(oqph_session* s, int maxn, int* cls, double* ms, double* f
   for (int i = 0; i < n; ++i) {
     const auto& L = s->launches[i];
     const unsigned long long* c = i * 8 + 8 <= (int)s->prof_ctr.size() ? &s->prof_ctr[8 * i] : nullptr;
-    if (cls) cls[i] = (L.hl ? 20000 : L.grouped ? 30000 : L.rows ? 10000 : 0) + L.la * 1000 + L.lb * 100 + L.lc * 10 + L.ld;
+    // a multi-class launch reports 40000 + its number of classes
+    if (cls) cls[i] = L.multi ? 40000 + L.ntask
+                              : (L.hl ? 20000 : L.grouped ? 30000 : L.rows ? 10000 : 0) + L.la * 1000 + L.lb * 100 +
+                                    L.lc * 10 + L.ld;
     if (ms) ms[i] = i < (int)s->prof_ms.size() ? s->prof_ms[i] : 0.0;
     if (flops) flops[i] = c ? (double)c[3] : 0.0;
     if (quartets) quartets[i] = c ? (long long)c[0] : 0;

@@ -145,6 +145,37 @@ def test_fshell_runtime_class_kernels_only(oracle, monkeypatch, harm):
     fb.close()
 
 
+def test_fshell_multi_class_launch(oracle, monkeypatch):
+    """The run-time classes of a build merged into one persistent launch per
+    LDS tier (k_jkh_multi, the default) against one launch per class
+    (OQPH_HL_MULTI=0) and the oracle: same G to 1e-12, same quartet counts,
+    and with every f class on the run-time-class kernel the merged launch
+    covers several classes."""
+    from openqp_amd import lib as L
+    monkeypatch.setenv("OQPH_F_TEMPLATES", "0")
+    bas = _basis(2, None)
+    D, dp = density_for(oracle, bas, 10, seed=14)
+    fo, st = oracle.fock_jk(bas, dp)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("OQPH_HL_MULTI", mode)
+        fb = L.FockBuilder(bas)
+        f = fb.fock_jk(dp)
+        prof = fb.last_profile()
+        out[mode] = (f, fb.last_stats.quartets_computed, fb.last_stats.quartets_screened,
+                     [p["cls"] for p in prof if p["kernel"] == "k_jkh_multi"])
+        fb.close()
+    for mode in ("0", "1"):
+        err = np.abs(out[mode][0] - fo).max()
+        _say("  OQPH_HL_MULTI", mode, "max |f - f_oracle| =", err, "multi launches", out[mode][3])
+        assert err <= TOL
+        assert out[mode][1] == int(st[1])
+    assert np.abs(out["0"][0] - out["1"][0]).max() <= 1e-12
+    assert out["0"][1:3] == out["1"][1:3]
+    assert out["0"][3] == []
+    assert out["1"][3] and max(c - 40000 for c in out["1"][3]) >= 2
+
+
 @pytest.mark.parametrize("name,nw", [("cc-pvdz", 2), ("cc-pvtz", 1)])
 def test_routec_seam_symbols(oracle, name, nw):
     """Both Route-C symbols called by reference as OpenQP's seam calls them
