@@ -91,6 +91,12 @@ void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prim
                      imu2, Q, h);
 }
 
+void launch_gradh_multi(const GradParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s) {
+  if (nblocks == 0) return;
+  allow_lds(k_gradh_multi, lds);
+  hipLaunchKernelGGL(k_gradh_multi, dim3((unsigned)nblocks), dim3(HL_BLOCK), lds, s, p, tl);
+}
+
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s) {
   if (ww.nwaves == 0) return;
   allow_lds(k_gradh, h.lds);

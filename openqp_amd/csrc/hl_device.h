@@ -617,13 +617,13 @@ __global__ __launch_bounds__(HL_BLOCK) void k_schwarzh(const ShellPair* pairs, c
 // transfer the 2D integrals to every (a, b, c, d) exponent the derivatives
 // need (T, LDS) and contract them with their Gamma entries; per-atom sums
 // go through LDS.
-__global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_gradh(GradParams prm, WaveWork ww, HLDims h) {
-  extern __shared__ double lds[];
+// One chunk (wave) of one gradient class's wavefront tasks, by one block.
+__device__ __forceinline__ void gradh_chunk(const GradParams& prm, const WaveWork& ww, const HLDims& h, long long wave,
+                                            double* lds) {
   const HLTabs tb = hl_tables(h, lds);
   double* s_de = lds + h.offB;  // 3 natm
   for (int i = threadIdx.x; i < 3 * prm.natm; i += HL_BLOCK) s_de[i] = 0.0;
   __syncthreads();
-  const long long wave = blockIdx.x;
   if (wave >= ww.nwaves) return;  // block-uniform
   int lo = 0, hi = ww.nseg;
   while (hi - lo > 1) {
@@ -797,6 +797,35 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
   __syncthreads();
   for (int i = lane; i < 3 * prm.natm; i += HL_BLOCK)
     if (s_de[i] != 0.0) unsafeAtomicAdd(prm.de + (size_t)(blockIdx.x & (OQPH_DE_SLOTS - 1)) * 3 * prm.natm + i, s_de[i]);
+}
+
+__global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_gradh(GradParams prm, WaveWork ww, HLDims h) {
+  extern __shared__ double lds[];
+  gradh_chunk(prm, ww, h, (long long)blockIdx.x, lds);
+}
+
+// Several gradient run-time classes in one launch (as k_jkh_multi).
+__global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_gradh_multi(
+    GradParams prm, HLTaskList tl) {
+  extern __shared__ double lds[];
+  const long long b = blockIdx.x;
+  int lo = 0, hi = tl.ntask;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tl.task[mid].wave0 <= b) lo = mid; else hi = mid;
+  }
+  const int t = __builtin_amdgcn_readfirstlane(lo);
+  const HLTask& tk = tl.task[t];
+  WaveWork ww;
+  ww.seg_bra = tl.seg + tk.seg0;
+  ww.seg_k0 = tl.seg + tl.nseg + tk.seg0;
+  ww.seg_n = tl.seg + 2 * tl.nseg + tk.seg0;
+  ww.choff = tl.choff + tk.ch0;
+  ww.ket = tl.ket + tk.ket0;
+  ww.nseg = tk.nseg;
+  ww.nwaves = tk.nwaves;
+  const HLDims h = tk.h;
+  gradh_chunk(prm, ww, h, b - tk.wave0, lds);
 }
 
 }  // namespace oqph

@@ -53,6 +53,7 @@ void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prim
                      const double* rys, const double* xf, double qcut, double imu2, double* Q, hipStream_t s);
 struct GradParams;
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s);
+void launch_gradh_multi(const GradParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s);
 bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s);
 void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
                          const double* xf, int ncf, double* Dc, hipStream_t s);
@@ -194,6 +195,10 @@ struct oqph_session {
     WaveWork ww;
     bool hl;     // run-time-class kernel (a class without a template), layout hd
     HLDims hd;
+    bool multi = false;  // several run-time classes in one launch (tl, nblocks, lds)
+    HLTaskList tl{};
+    long long nblocks = 0;
+    int lds = 0;
   };
   // Grouped bra-row kernel (k_jkg): per shell, the shell whose primitive set
   // it is contracted from (fam[i] = i for a family's master); per bra class
@@ -254,6 +259,7 @@ struct oqph_session {
   DBuf<int> d_seg;                 // seg_bra | seg_k0 | seg_n of all launches
   DBuf<long long> d_choff;
   DBuf<HLTask> d_tasks;            // classes of the multi-class run-time-class launches
+  DBuf<HLTask> d_gtasks;           // the same for the gradient
   std::vector<int> h_seg_bra, h_seg_k0, h_seg_n;
   std::vector<long long> h_choff;
   std::vector<HLTask> h_tasks;
@@ -1012,6 +1018,69 @@ static int prepare_grad(oqph_session* s) {
       }
     }
     s->glaunches.push_back(L);
+  }
+  // run-time classes merged into one launch per occupancy tier (as the J/K
+  // build's k_jkh_multi)
+  if (s->hl_multi) {
+    std::map<int, std::vector<size_t>> tiers;
+    for (size_t i = 0; i < s->glaunches.size(); ++i) {
+      const auto& L = s->glaunches[i];
+      if (!L.hl || L.ww.nwaves == 0) continue;
+      tiers[(int)std::min<size_t>(16, s->lds_max / std::max(L.hd.lds, 1))].push_back(i);
+    }
+    std::vector<HLTask> tasks;
+    std::vector<oqph_session::GLaunch> merged;
+    std::vector<char> drop(s->glaunches.size(), 0);
+    for (auto& kv : tiers) {
+      auto& t = kv.second;
+      if (t.size() < 2) continue;
+      std::stable_sort(t.begin(), t.end(), [&](size_t x, size_t y) {
+        const auto &a = s->glaunches[x], &b = s->glaunches[y];
+        return flops_prim_host(a.la, a.lb, a.lc, a.ld) > flops_prim_host(b.la, b.lb, b.lc, b.ld);
+      });
+      oqph_session::GLaunch M{};
+      const auto& L0 = s->glaunches[t[0]];
+      M.la = L0.la; M.lb = L0.lb; M.lc = L0.lc; M.ld = L0.ld;
+      M.hl = true;
+      M.multi = true;
+      M.tl.ntask = (int)t.size();
+      M.tl.task = (const HLTask*)(uintptr_t)tasks.size();  // offset until the table is on the device
+      M.tl.seg = s->d_gseg.p;
+      M.tl.nseg = (int)nseg;
+      M.tl.choff = s->d_gchoff.p;
+      M.tl.ket = s->d_gcls.p;
+      long long w0 = 0;
+      for (size_t i : t) {
+        const auto& L = s->glaunches[i];
+        HLTask k{};
+        k.h = L.hd;
+        k.wave0 = w0;
+        k.nwaves = L.ww.nwaves;
+        k.seg0 = (int)(L.ww.seg_bra - s->d_gseg.p);
+        k.ch0 = (int)(L.ww.choff - s->d_gchoff.p);
+        k.nseg = L.ww.nseg;
+        k.ket0 = (int)(L.ww.ket - s->d_gcls.p);
+        tasks.push_back(k);
+        w0 += L.ww.nwaves;
+        M.lds = std::max(M.lds, L.hd.lds);
+        drop[i] = 1;
+      }
+      M.nblocks = w0;
+      merged.push_back(M);
+    }
+    if (!merged.empty()) {
+      if ((rc = s->d_gtasks.ensure(tasks.size())) ||
+          (rc = upload_sync(s->d_gtasks.p, tasks.data(), tasks.size() * sizeof(HLTask), s->stream)))
+        return rc;
+      std::vector<oqph_session::GLaunch> keep;
+      for (size_t i = 0; i < s->glaunches.size(); ++i)
+        if (!drop[i]) keep.push_back(s->glaunches[i]);
+      for (auto& M : merged) {
+        M.tl.task = s->d_gtasks.p + (uintptr_t)M.tl.task;
+        keep.push_back(M);
+      }
+      s->glaunches.swap(keep);
+    }
   }
   // Cartesian offsets, shell atoms and flags
   std::vector<int> cof(s->nsh), pu(s->nsh);
@@ -1981,7 +2050,9 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   for (size_t i = 0; i < nl; ++i) {
     auto& L = s->glaunches[i];
     p.counters = s->d_ctr.p + 8 + nc * i;
-    if (L.hl) {
+    if (L.multi) {
+      launch_gradh_multi(p, L.tl, L.nblocks, L.lds, stream);
+    } else if (L.hl) {
       launch_gradh(L.hd, p, L.ww, stream);
     } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, stream)) {
       set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
