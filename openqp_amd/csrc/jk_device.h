@@ -642,8 +642,14 @@ struct JKB {
 // faster per build than 1024 and 6 % faster than 256 on (H2O)64/cc-pVDZ
 #define OQPH_JKB_SMALL_THREADS 512
 #endif
-  static constexpr int threads =
-      (Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF <= 4) ? OQPH_JKB_SMALL_THREADS : 512;
+  // (e0|f0) sets of 5..9 ((ps|ps), (pp|ss), (ds|ss)): their blocks' K and
+  // density rows span every p (or d) AO of the molecule, so one block fills
+  // a CU's LDS; 1024 threads give that block 16 waves (the 128-VGPR cap of
+  // that size costs these classes little: (pp|ss) 26.3 -> 21.2 ms, (ds|ss)
+  // 22.3 -> 18.1 ms mixed on (H2O)64/cc-pVDZ, while the larger classes
+  // spill heavily under it)
+  static constexpr int NEF = Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF;
+  static constexpr int threads = NEF <= 4 ? OQPH_JKB_SMALL_THREADS : (NEF <= 9 ? 1024 : 512);
   // (ss|ss) fits in 64 VGPRs without spilling: eight waves per SIMD;
   // OQPH_JKB_WAVES3 sets the target for (e0|f0) sets of 2..4 ((ps|ss))
 #ifndef OQPH_JKB_WAVES3

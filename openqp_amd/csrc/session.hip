@@ -252,6 +252,8 @@ struct oqph_session {
   // per quartet run on the run-time-class kernel k_gradh (0: templates wherever
   // they exist and fit)
   int grad_hl_ng = 0;
+  // OQPH_JK_HL_NG=n: the same for the J/K build (k_jkh)
+  int jk_hl_ng = 0;
   std::vector<WLaunch> launches;
   unsigned long long list_gen = 0;       // bumped whenever the host work lists change
   unsigned long long list_uploaded = 0;  // list_gen of the copy in d_seg / d_choff / d_items
@@ -725,6 +727,8 @@ static int prepare_geometry(oqph_session* s) {
     s->f_templates = !(ft && ft[0] == '0');
     const char* gh = getenv("OQPH_GRAD_HL_NG");
     s->grad_hl_ng = gh ? atoi(gh) : 0;
+    const char* jh = getenv("OQPH_JK_HL_NG");
+    s->jk_hl_ng = jh ? atoi(jh) : 0;
     const char* e = getenv("OQPH_JK_ROWS");
     s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
     const char* hm = getenv("OQPH_HL_MULTI");
@@ -1188,7 +1192,8 @@ static int prepare_lists(oqph_session* s, double maxden) {
       // run-time-class kernel for the f classes without a class template
       // templates whose private segment exceeds the budget run on k_jkh instead
       KernelRes kr0{}, kr1{};
-      const bool tmpl = jk_template(la, lb, lc, ld) && !(!s->f_templates && (la == 3 || lc == 3));
+      const bool tmpl = jk_template(la, lb, lc, ld) && !(!s->f_templates && (la == 3 || lc == 3)) &&
+                        !(s->jk_hl_ng > 0 && ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > s->jk_hl_ng);
       const bool fit0 = tmpl && kernel_resources(0, la, lb, lc, ld, s->pure, &kr0) &&
                         kr0.private_bytes <= OQPH_PRIVATE_BUDGET;
       const bool fit1 = tmpl && kernel_resources(1, la, lb, lc, ld, s->pure, &kr1) &&
