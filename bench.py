@@ -5,7 +5,10 @@ Workload (default): 64 H2O on a 4x4x4 grid (3.0 A spacing, seeded random
 orientations), cc-pVDZ (spherical d), 1536 basis functions, a seeded
 S-orthonormal density of 320 occupied orbitals (D = 2 C C^T), int2e_cutoff
 5e-11.  One step = one J+K build (G = J - K/2 for that density) with the
-density and result resident in HBM.  The plain FP64 build and the mixed
+density and result resident in HBM; the class launches of a build are spread
+over the session's four streams.  Per-class times and the roofline of the
+dominant kernel come from further builds whose launches are serialised and
+timed one by one with HIP events.  The plain FP64 build and the mixed
 build (FP32 for quartets whose Schwarz x density bound is below the FP32
 cutoff, FP64 accumulation) are both timed; ``value`` is the mixed rate when
 its energy error |dE2| = |0.5 Tr D (G_mixed - G_fp64)| is within 1 uEh,
@@ -214,6 +217,9 @@ def main():
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--no-fshell", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-builds", type=int, default=3,
+                    help="builds after the timed ones with per-launch HIP events (launches serialised): the "
+                         "per-class times and the roofline of the dominant kernel")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: measures one GPU only (--gpus {a.gpus} requested); across GPUs the bra "
@@ -246,19 +252,25 @@ def main():
     fb.schwarz()
     log(f"[bench] geometry setup (pairs + Schwarz) {time.perf_counter() - t0:.2f} s")
 
-    def run(fout, steps, warm, precision, cut=0.0):
+    def run(fout, steps, warm, precision, cut=0.0, profile_builds=1):
+        """Time `steps` builds (the class launches spread over the session's
+        streams), then run `profile_builds` more with per-launch HIP events,
+        which serialise the launches so that each is timed alone."""
         fb.set_precision(precision, cut)
         for _ in range(warm):
             fb.fock_jk_device(d_dev.data_ptr(), fout.data_ptr(), stream=stream)
         torch.cuda.synchronize()
-        fb.set_profiling(True)
-        prof = {}
-        launches = {}
         st_last = None
-        torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
             st_last = fb.fock_jk_device(d_dev.data_ptr(), fout.data_ptr(), stream=stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        prof = {}
+        launches = {}
+        fb.set_profiling(True)
+        for _ in range(profile_builds):
+            fb.fock_jk_device(d_dev.data_ptr(), fout.data_ptr(), stream=stream)
             # a class may run as two launches (grouped and segmented bra-row
             # kernels): per-class figures are per build
             seen = set()
@@ -276,13 +288,12 @@ def main():
                 if p["kernel"] not in q["kernel"].split("+"):
                     q["kernel"] += "+" + p["kernel"]
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t) / steps
         fb.set_profiling(False)
         prof["_launches"] = launches
         return dt, st_last, prof
 
     log("[bench] FP64 builds")
-    dt64, st64, prof64 = run(f64, a.steps, a.warmup, 0)
+    dt64, st64, prof64 = run(f64, a.steps, a.warmup, 0, profile_builds=a.profile_builds)
     log(f"[bench] FP64 {dt64 * 1e3:.1f} ms/build, quartets {st64.quartets_computed}, "
         f"model {st64.model_flops / 1e12:.3f} TFLOP")
     e64_ref = trace_energy(dp, f64.cpu().numpy(), bas.nbf)
@@ -290,7 +301,7 @@ def main():
     sweep = []
     cuts = [a.fp32_cutoff] if a.fp32_cutoff is not None else [1e-3, 1e-4, 1e-5, 1e-6]
     for cut in cuts:
-        dts, sts, _ = run(fmx, 1, 1, 1, cut)
+        dts, sts, _ = run(fmx, 1, 1, 1, cut, profile_builds=0)
         de = abs(trace_energy(dp, fmx.cpu().numpy(), bas.nbf) - e64_ref)
         sweep.append({"fp32_cutoff": cut, "ms": round(dts * 1e3, 3), "dE2_hartree": de,
                       "fp32_quartets": sts.quartets_fp32})
@@ -299,7 +310,7 @@ def main():
     best = min(ok, key=lambda x: x["ms"]) if ok else min(sweep, key=lambda x: x["dE2_hartree"])
     a.fp32_cutoff = best["fp32_cutoff"]
     log(f"[bench] mixed builds (fp32 cutoff {a.fp32_cutoff:g})")
-    dtmx, stmx, profmx = run(fmx, a.steps, a.warmup, 1, a.fp32_cutoff)
+    dtmx, stmx, profmx = run(fmx, a.steps, a.warmup, 1, a.fp32_cutoff, profile_builds=a.profile_builds)
     log(f"[bench] mixed {dtmx * 1e3:.1f} ms/build, fp32 quartets {stmx.quartets_fp32}")
 
     g64 = f64.cpu().numpy()

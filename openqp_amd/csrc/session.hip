@@ -147,10 +147,19 @@ static inline int bucket_of(int nprim) {
   return b;
 }
 
+// Extra streams of a session: the class launches of one build (or gradient)
+// are independent (each adds into F or de with atomics), so they are spread
+// over the session stream and these, which lets the small launches of a
+// small molecule fill CUs that one launch alone leaves idle.
+#define OQPH_SIDE_STREAMS 3
+
 struct oqph_session {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int nstreams = 1 + OQPH_SIDE_STREAMS;  // OQPH_STREAMS (1..4): streams a build's launches use
+  hipStream_t side[OQPH_SIDE_STREAMS] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[OQPH_SIDE_STREAMS] = {};
   // basis
   int nsh = 0, nbf = 0, natm = 0, lmax = 0;
   bool pure = false;
@@ -1086,6 +1095,16 @@ static int prepare_grad(oqph_session* s) {
       s->glaunches.swap(keep);
     }
   }
+  // longest launches first, so that the launches spread over the streams
+  // start with the expensive ones (estimated by chunks x model FLOPs)
+  std::stable_sort(s->glaunches.begin(), s->glaunches.end(),
+                   [](const oqph_session::GLaunch& x, const oqph_session::GLaunch& y) {
+                     auto cost = [](const oqph_session::GLaunch& L) {
+                       return (double)(L.multi ? L.nblocks : L.ww.nwaves) *
+                              (double)flops_prim_host(L.la, L.lb, L.lc, L.ld) * (L.hl ? 4.0 : 1.0);
+                     };
+                     return cost(x) > cost(y);
+                   });
   // Cartesian offsets, shell atoms and flags
   std::vector<int> cof(s->nsh), pu(s->nsh);
   int ncf = 0;
@@ -1532,6 +1551,29 @@ struct StaticScratch {
   unsigned long long* gen;     // lists last uploaded into it
 };
 
+// Fork the side streams off `main` (they wait for everything enqueued on it
+// so far), pick the stream of launch i, and join them back into `main`.
+// n: the streams used (1: everything on main; per-launch profiles use 1,
+// so each launch's events time it alone on the device).
+static int streams_fork(oqph_session* s, hipStream_t main, int n) {
+  if (n < 2) return 0;
+  HIPCHK(hipEventRecord(s->ev_fork, main));
+  for (int j = 0; j + 1 < n; ++j) HIPCHK(hipStreamWaitEvent(s->side[j], s->ev_fork, 0));
+  return 0;
+}
+static hipStream_t stream_of(const oqph_session* s, hipStream_t main, size_t i, int n) {
+  const int k = (int)(i % (size_t)n);
+  return k == 0 ? main : s->side[k - 1];
+}
+static int streams_join(oqph_session* s, hipStream_t main, int n) {
+  if (n < 2) return 0;
+  for (int j = 0; j + 1 < n; ++j) {
+    HIPCHK(hipEventRecord(s->ev_join[j], s->side[j]));
+    HIPCHK(hipStreamWaitEvent(main, s->ev_join[j], 0));
+  }
+  return 0;
+}
+
 // G[D] on device buffers; D already on the device.  The screening densities
 // scr_a (and scr_b) default to D; zero_f = false accumulates into dF.  mu > 0:
 // the erf(mu r)/r attenuated pass (pair records with the attenuated bounds).
@@ -1642,23 +1684,27 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   p.mixed = s->policy;
   p.fp32_cut = s->fp32_cut;
   HIPCHK(hipEventRecord(s->ev0, stream));
+  const int nst = profile ? 1 : s->nstreams;
+  if ((rc = streams_fork(s, stream, nst))) return rc;
   for (size_t i = 0; i < nl; ++i) {
     auto& L = s->launches[i];
+    const hipStream_t ls = stream_of(s, stream, i, nst);
     p.counters = ctr + nc * i;
-    if (profile) HIPCHK(hipEventRecord(s->pev[2 * i], stream));
+    if (profile) HIPCHK(hipEventRecord(s->pev[2 * i], ls));
     bool ok = true;
-    if (L.multi) launch_jkh_multi(p, L.tl, L.nblocks, (int)L.lds, stream);
-    else if (L.hl) launch_jkh(L.hd, p, L.ww, stream);
-    else if (L.grouped) ok = launch_jkg(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.cv, L.lds, stream);
-    else if (L.rows) ok = launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, stream);
-    else ok = launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, stream);
+    if (L.multi) launch_jkh_multi(p, L.tl, L.nblocks, (int)L.lds, ls);
+    else if (L.hl) launch_jkh(L.hd, p, L.ww, ls);
+    else if (L.grouped) ok = launch_jkg(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.cv, L.lds, ls);
+    else if (L.rows) ok = launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, ls);
+    else ok = launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, ls);
     if (!ok) {
       set_err("no J/K kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }
-    if (profile) HIPCHK(hipEventRecord(s->pev[2 * i + 1], stream));
+    if (profile) HIPCHK(hipEventRecord(s->pev[2 * i + 1], ls));
   }
   HIPCHK(hipGetLastError());
+  if ((rc = streams_join(s, stream, nst))) return rc;
   HIPCHK(hipEventRecord(s->ev1, stream));
   if (st || profile) {
     std::vector<unsigned long long> raw(nc * nl);
@@ -1717,6 +1763,21 @@ int oqph_session_create(oqph_session** out, int device) {
     delete s;
     return 2;
   }
+  {
+    const char* ns = getenv("OQPH_STREAMS");
+    if (ns) s->nstreams = std::max(1, std::min(1 + OQPH_SIDE_STREAMS, atoi(ns)));
+    // side streams at the default priority, like the session stream
+    for (int j = 0; j + 1 < s->nstreams; ++j)
+      if (hipStreamCreateWithFlags(&s->side[j], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&s->ev_join[j], hipEventDisableTiming) != hipSuccess) {
+        set_err("stream/event creation failed");
+        return 2;
+      }
+    if (s->nstreams > 1 && hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess) {
+      set_err("stream/event creation failed");
+      return 2;
+    }
+  }
   *out = s;
   std::lock_guard<std::mutex> lk(g_mtx);
   s->serial = ++g_serial;
@@ -1741,6 +1802,11 @@ int oqph_session_destroy(oqph_session* s) {
   for (auto e : s->pev) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
+  if (s->ev_fork) hipEventDestroy(s->ev_fork);
+  for (int j = 0; j < OQPH_SIDE_STREAMS; ++j) {
+    if (s->ev_join[j]) hipEventDestroy(s->ev_join[j]);
+    if (s->side[j]) hipStreamDestroy(s->side[j]);
+  }
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
   return 0;
@@ -2052,19 +2118,23 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   p.cs = coulscale;
   p.hs = hfscale;
   HIPCHK(hipEventRecord(s->ev0, stream));
+  const int nst = s->profile ? 1 : s->nstreams;
+  if ((rc = streams_fork(s, stream, nst))) return rc;
   for (size_t i = 0; i < nl; ++i) {
     auto& L = s->glaunches[i];
+    const hipStream_t ls = stream_of(s, stream, i, nst);
     p.counters = s->d_ctr.p + 8 + nc * i;
     if (L.multi) {
-      launch_gradh_multi(p, L.tl, L.nblocks, L.lds, stream);
+      launch_gradh_multi(p, L.tl, L.nblocks, L.lds, ls);
     } else if (L.hl) {
-      launch_gradh(L.hd, p, L.ww, stream);
-    } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, stream)) {
+      launch_gradh(L.hd, p, L.ww, ls);
+    } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, ls)) {
       set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }
   }
   HIPCHK(hipGetLastError());
+  if ((rc = streams_join(s, stream, nst))) return rc;
   HIPCHK(hipEventRecord(s->ev1, stream));
   std::vector<double> deraw(nde * OQPH_DE_SLOTS);
   HIPCHK(hipMemcpyAsync(deraw.data(), s->d_de.p, deraw.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
