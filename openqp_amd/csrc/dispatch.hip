@@ -108,9 +108,11 @@ bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* stat
 // Compiled resources of one class-templated kernel: kind 0 = k_jk, 1 = k_jkb,
 // 2 = k_grad, 3 = k_schwarz (lc, ld ignored), 4 = k_jkg; false when there is no such
 // kernel.  hipFuncGetAttributes reads the code object, nothing is launched.
+// kind | 16 (k_jk, k_jkb, k_jkg): the open-shell variant
 bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, KernelRes* r) {
-  if (kind == 0 || kind == 1 || kind == 3 || kind == 4) {
-    if (kind == 3) { lc = la; ld = lb; }
+  const int base = kind & 15;
+  if (base == 0 || base == 1 || base == 3 || base == 4) {
+    if (base == 3) { lc = la; ld = lb; }
     switch (OQPH_CODE(la, lb, lc, ld)) {
 #define OQPH_CASE(a, b, c, d) \
   case OQPH_CODE(a, b, c, d): return jk_res_##a##b##c##d(kind, pure, r);
@@ -278,14 +280,18 @@ void launch_shell_density(const double* D, const double* D2, const int* ao_off, 
                      dsh);
 }
 
-__global__ void k_add(const double* x, const double* y, double* z, long long n) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    z[i] = x[i] + y[i];
+// p = c (a + b), m = c (a - b), elementwise over n doubles; p == a and m == b
+// (in place) are allowed: each element is read before it is written
+__global__ void k_sumdiff(const double* a, const double* b, double* p, double* m, double c, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double x = a[i], y = b[i];
+    p[i] = c * (x + y);
+    m[i] = c * (x - y);
+  }
 }
 
-// z = x + y (n doubles)
-void launch_add(const double* x, const double* y, double* z, long long n, hipStream_t s) {
-  hipLaunchKernelGGL(k_add, dim3(1024), dim3(256), 0, s, x, y, z, n);
+void launch_sumdiff(const double* a, const double* b, double* p, double* m, double c, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_sumdiff, dim3(1024), dim3(256), 0, s, a, b, p, m, c, n);
 }
 
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s) {

@@ -474,7 +474,7 @@ __device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& w
     const double* V = hl_block(h, tb, P, Q, prm.prims, prm.rys, prm.xform, 1.0e-4 * cut, prm.inv_mu2, lds, nprim);
     prims_used += (unsigned long long)nprim;
     if (nprim == 0) continue;
-    flops += (unsigned long long)(nprim * fprim + 12 * nout);
+    flops += (unsigned long long)(nprim * fprim + (prm.F2 ? 24 : 12) * nout);
     double deg = 1.0;
     if (P.sa == P.sb) deg *= 0.5;
     if (Q.sa == Q.sb) deg *= 0.5;
@@ -501,9 +501,9 @@ __device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& w
       const double v = V[((a * OB + b) * OC + c) * OD + d];
       return fabs(v) < cut ? 0.0 : v;  // OpenQP element cutoff on the AO value
     };
-    auto put = [&](int x, int y, double v) {
+    auto put = [&](double* Ft, int x, int y, double v) {
       v *= (x == y ? 2.0 : 1.0);
-      if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+      if (v != 0.0) fatomic(Ft + tri_idx(x, y), v);
     };
     // J_ab and J_cd
     for (int t = lane; t < OA * OB; t += HL_BLOCK) {
@@ -511,43 +511,59 @@ __device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& w
       double s = 0.0;
       for (int c = 0; c < OC; ++c)
         for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dcd[c * OD + d];
-      put(oa + a, ob + b, cj * s);
+      put(F, oa + a, ob + b, cj * s);
     }
     for (int t = lane; t < OC * OD; t += HL_BLOCK) {
       const int c = t / OD, d = t - c * OD;
       double s = 0.0;
       for (int a = 0; a < OA; ++a)
         for (int b = 0; b < OB; ++b) s += val(a, b, c, d) * Dab[a * OB + b];
-      put(oc + c, od + d, cj * s);
+      put(F, oc + c, od + d, cj * s);
     }
     // K_ac += V D_bd, K_ad += V D_bc, K_bc += V D_ad, K_bd += V D_ac
-    for (int t = lane; t < OA * OC; t += HL_BLOCK) {
-      const int a = t / OC, c = t - a * OC;
-      double s = 0.0;
-      for (int b = 0; b < OB; ++b)
-        for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dbd[b * OD + d];
-      put(oa + a, oc + c, ck * s);
-    }
-    for (int t = lane; t < OA * OD; t += HL_BLOCK) {
-      const int a = t / OD, d = t - a * OD;
-      double s = 0.0;
-      for (int b = 0; b < OB; ++b)
-        for (int c = 0; c < OC; ++c) s += val(a, b, c, d) * Dbc[b * OC + c];
-      put(oa + a, od + d, ck * s);
-    }
-    for (int t = lane; t < OB * OC; t += HL_BLOCK) {
-      const int b = t / OC, c = t - b * OC;
-      double s = 0.0;
-      for (int a = 0; a < OA; ++a)
-        for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dad[a * OD + d];
-      put(ob + b, oc + c, ck * s);
-    }
-    for (int t = lane; t < OB * OD; t += HL_BLOCK) {
-      const int b = t / OD, d = t - b * OD;
-      double s = 0.0;
-      for (int a = 0; a < OA; ++a)
-        for (int c = 0; c < OC; ++c) s += val(a, b, c, d) * Dac[a * OC + c];
-      put(ob + b, od + d, ck * s);
+    auto kblocks = [&](double* Ft) {
+      for (int t = lane; t < OA * OC; t += HL_BLOCK) {
+        const int a = t / OC, c = t - a * OC;
+        double s = 0.0;
+        for (int b = 0; b < OB; ++b)
+          for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dbd[b * OD + d];
+        put(Ft, oa + a, oc + c, ck * s);
+      }
+      for (int t = lane; t < OA * OD; t += HL_BLOCK) {
+        const int a = t / OD, d = t - a * OD;
+        double s = 0.0;
+        for (int b = 0; b < OB; ++b)
+          for (int c = 0; c < OC; ++c) s += val(a, b, c, d) * Dbc[b * OC + c];
+        put(Ft, oa + a, od + d, ck * s);
+      }
+      for (int t = lane; t < OB * OC; t += HL_BLOCK) {
+        const int b = t / OC, c = t - b * OC;
+        double s = 0.0;
+        for (int a = 0; a < OA; ++a)
+          for (int d = 0; d < OD; ++d) s += val(a, b, c, d) * Dad[a * OD + d];
+        put(Ft, ob + b, oc + c, ck * s);
+      }
+      for (int t = lane; t < OB * OD; t += HL_BLOCK) {
+        const int b = t / OD, d = t - b * OD;
+        double s = 0.0;
+        for (int a = 0; a < OA; ++a)
+          for (int c = 0; c < OC; ++c) s += val(a, b, c, d) * Dac[a * OC + c];
+        put(Ft, ob + b, od + d, ck * s);
+      }
+    };
+    kblocks(F);
+    if (prm.F2) {  // launch-uniform: the open-shell pass, D2's exchange blocks into F2
+      __syncthreads();
+      for (int t = lane + 2 * ds; t < 6 * ds; t += HL_BLOCK) {
+        const int blk = t / ds, e = t - ds * blk;
+        const int r0 = blk <= 3 ? oa : ob;
+        const int c0 = (blk == 2 || blk == 4) ? oc : od;
+        const int nc = (blk == 2 || blk == 4) ? OC : OD;
+        const int nr = blk <= 3 ? OA : OB;
+        if (e < nr * nc) sD[t] = prm.D2[tri_idx(r0 + e / nc, c0 + e % nc)];
+      }
+      __syncthreads();
+      kblocks(prm.F2);
     }
   }
   if (lane == 0) {

@@ -2,6 +2,8 @@
 // (XA XB | XC XD), both Cartesian and pure AO variants, and for the diagonal
 // classes (XA XB | XA XB) the Schwarz kernel.  Built once per class with
 // -DXA=.. -DXB=.. -DXC=.. -DXD=.. so the classes compile in parallel.
+#include <algorithm>
+
 #include "jk_device.h"
 #include "jkg_device.h"
 
@@ -15,13 +17,63 @@ namespace oqph {
 #define OQPH_SCH_LAUNCHER_(a, b) launch_schwarz_##a##b
 #define OQPH_SCH_LAUNCHER(a, b) OQPH_SCH_LAUNCHER_(a, b)
 
-void OQPH_JK_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const WaveWork& ww, hipStream_t s) {
+// Each kernel comes in four variants: Cartesian or pure AOs, and closed-shell
+// or open-shell pass (p.F2 set: the exchange of a second density as well).
+template <bool TWO>
+static void jk_launch(bool pure, const JKParams& p, const WaveWork& ww, hipStream_t s) {
   const int bs = JK_BLOCK;
   const long long nb = (ww.nwaves * 64 + bs - 1) / bs;
   if (pure)
-    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, true>), dim3((unsigned)nb), dim3(bs), 0, s, p, ww);
+    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, true, TWO>), dim3((unsigned)nb), dim3(bs), 0, s, p, ww);
   else
-    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, false>), dim3((unsigned)nb), dim3(bs), 0, s, p, ww);
+    hipLaunchKernelGGL((k_jk<XA, XB, XC, XD, false, TWO>), dim3((unsigned)nb), dim3(bs), 0, s, p, ww);
+}
+
+void OQPH_JK_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const WaveWork& ww, hipStream_t s) {
+  if (p.F2) jk_launch<true>(pure, p, ww, s);
+  else jk_launch<false>(pure, p, ww, s);
+}
+
+static const void* jk_fn(int kind, bool pure, bool two) {
+  if (kind == 0) {
+    if (two) return pure ? (const void*)k_jk<XA, XB, XC, XD, true, true> : (const void*)k_jk<XA, XB, XC, XD, false, true>;
+    return pure ? (const void*)k_jk<XA, XB, XC, XD, true, false> : (const void*)k_jk<XA, XB, XC, XD, false, false>;
+  }
+  if (kind == 1) {
+    if constexpr (Big<XA, XB, XC, XD>::value) {
+      return nullptr;
+    } else {
+      if (two)
+        return pure ? (const void*)k_jkb<XA, XB, XC, XD, true, true> : (const void*)k_jkb<XA, XB, XC, XD, false, true>;
+      return pure ? (const void*)k_jkb<XA, XB, XC, XD, true, false> : (const void*)k_jkb<XA, XB, XC, XD, false, false>;
+    }
+  }
+  if (kind == 4) {
+    if constexpr (Big<XA, XB, XC, XD>::value || JKG<XA, XB, XC, XD>::nm < 2) {
+      return nullptr;
+    } else {
+      if (two)
+        return pure ? (const void*)k_jkg<XA, XB, XC, XD, true, true> : (const void*)k_jkg<XA, XB, XC, XD, false, true>;
+      return pure ? (const void*)k_jkg<XA, XB, XC, XD, true, false> : (const void*)k_jkg<XA, XB, XC, XD, false, false>;
+    }
+  }
+  return nullptr;
+}
+
+// raises the dynamic-LDS limit of both variants of kernel `kind` to what
+// lds_max leaves next to its static LDS; returns the static LDS bytes
+static bool allow_dyn_lds(int kind, bool pure, size_t lds_max, size_t* static_lds) {
+  for (int two = 0; two < 2; ++two) {
+    const void* f = jk_fn(kind, pure, two != 0);
+    hipFuncAttributes a;
+    if (!f || hipFuncGetAttributes(&a, f) != hipSuccess) return false;
+    if (lds_max > a.sharedSizeBytes &&
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max - a.sharedSizeBytes)) !=
+            hipSuccess)
+      return false;
+    *static_lds = std::max(*static_lds, (size_t)a.sharedSizeBytes);
+  }
+  return true;
 }
 
 #define OQPH_JKB_LAUNCHER_(a, b, c, d) launch_jkb_##a##b##c##d
@@ -35,15 +87,10 @@ bool OQPH_JKB_INFO(XA, XB, XC, XD)(bool pure, int* nrow, size_t* static_lds, siz
   if constexpr (Big<XA, XB, XC, XD>::value) {
     return false;
   } else {
-    const void* f = pure ? (const void*)k_jkb<XA, XB, XC, XD, true> : (const void*)k_jkb<XA, XB, XC, XD, false>;
-    hipFuncAttributes a;
-    if (hipFuncGetAttributes(&a, f) != hipSuccess) return false;
-    if (lds_max > a.sharedSizeBytes &&
-        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max - a.sharedSizeBytes)) !=
-            hipSuccess)
-      return false;
+    size_t st = 0;
+    if (!allow_dyn_lds(1, pure, lds_max, &st)) return false;
     *nrow = pure ? AO<XA, true>::N + AO<XB, true>::N : AO<XA, false>::N + AO<XB, false>::N;
-    *static_lds = a.sharedSizeBytes;
+    *static_lds = st;
     return true;
   }
 }
@@ -54,10 +101,18 @@ bool OQPH_JKB_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWo
     return false;
   } else {
     const int bs = JKB<XA, XB, XC, XD>::threads;
-    if (pure)
-      hipLaunchKernelGGL((k_jkb<XA, XB, XC, XD, true>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm);
-    else
-      hipLaunchKernelGGL((k_jkb<XA, XB, XC, XD, false>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm);
+    if (p.F2) {
+      if (pure)
+        hipLaunchKernelGGL((k_jkb<XA, XB, XC, XD, true, true>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm);
+      else
+        hipLaunchKernelGGL((k_jkb<XA, XB, XC, XD, false, true>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm);
+    } else {
+      if (pure)
+        hipLaunchKernelGGL((k_jkb<XA, XB, XC, XD, true, false>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm);
+      else
+        hipLaunchKernelGGL((k_jkb<XA, XB, XC, XD, false, false>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw,
+                           cm);
+    }
     return true;
   }
 }
@@ -74,15 +129,14 @@ int OQPH_JKG_INFO(XA, XB, XC, XD)(bool pure, size_t* static_lds, size_t lds_max)
   if constexpr (Big<XA, XB, XC, XD>::value || JKG<XA, XB, XC, XD>::nm < 2) {
     return 0;
   } else {
-    const void* f = pure ? (const void*)k_jkg<XA, XB, XC, XD, true> : (const void*)k_jkg<XA, XB, XC, XD, false>;
-    hipFuncAttributes a;
-    if (hipFuncGetAttributes(&a, f) != hipSuccess) return 0;
-    if (a.localSizeBytes > OQPH_PRIVATE_BUDGET) return 0;
-    if (lds_max > a.sharedSizeBytes &&
-        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max - a.sharedSizeBytes)) !=
-            hipSuccess)
-      return 0;
-    *static_lds = a.sharedSizeBytes;
+    for (int two = 0; two < 2; ++two) {
+      hipFuncAttributes a;
+      if (hipFuncGetAttributes(&a, jk_fn(4, pure, two != 0)) != hipSuccess) return 0;
+      if (a.localSizeBytes > OQPH_PRIVATE_BUDGET) return 0;
+    }
+    size_t st = 0;
+    if (!allow_dyn_lds(4, pure, lds_max, &st)) return 0;
+    *static_lds = st;
     return JKG<XA, XB, XC, XD>::nm;
   }
 }
@@ -93,10 +147,14 @@ bool OQPH_JKG_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWo
     return false;
   } else {
     const int bs = JKG<XA, XB, XC, XD>::threads;
-    if (pure)
-      hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm, cv);
-    else
-      hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false>), dim3((unsigned)bw.nitem), dim3(bs), lds, s, p, bw, cm, cv);
+    const dim3 g((unsigned)bw.nitem), b(bs);
+    if (p.F2) {
+      if (pure) hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true, true>), g, b, lds, s, p, bw, cm, cv);
+      else hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false, true>), g, b, lds, s, p, bw, cm, cv);
+    } else {
+      if (pure) hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true, false>), g, b, lds, s, p, bw, cm, cv);
+      else hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false, false>), g, b, lds, s, p, bw, cm, cv);
+    }
     return true;
   }
 }
@@ -115,25 +173,14 @@ static bool func_res(const void* f, KernelRes* r) {
 }
 
 // Compiled resources of this class's kernels: kind 0 = k_jk, 1 = k_jkb,
-// 3 = k_schwarz (diagonal classes only), 4 = k_jkg.
+// 3 = k_schwarz (diagonal classes only), 4 = k_jkg; kind | 16: the
+// open-shell variant.
 bool OQPH_JK_RES(XA, XB, XC, XD)(int kind, bool pure, KernelRes* r) {
-  if (kind == 0)
-    return func_res(pure ? (const void*)k_jk<XA, XB, XC, XD, true> : (const void*)k_jk<XA, XB, XC, XD, false>, r);
-  if (kind == 1) {
-    if constexpr (Big<XA, XB, XC, XD>::value) {
-      return false;
-    } else {
-      return func_res(pure ? (const void*)k_jkb<XA, XB, XC, XD, true> : (const void*)k_jkb<XA, XB, XC, XD, false>,
-                      r);
-    }
-  }
-  if (kind == 4) {
-    if constexpr (Big<XA, XB, XC, XD>::value || JKG<XA, XB, XC, XD>::nm < 2) {
-      return false;
-    } else {
-      return func_res(pure ? (const void*)k_jkg<XA, XB, XC, XD, true> : (const void*)k_jkg<XA, XB, XC, XD, false>,
-                      r);
-    }
+  const bool two = (kind & 16) != 0;
+  kind &= 15;
+  if (kind == 0 || kind == 1 || kind == 4) {
+    const void* f = jk_fn(kind, pure, two);
+    return f && func_res(f, r);
   }
 #if XA == XC && XB == XD
   if (kind == 3)

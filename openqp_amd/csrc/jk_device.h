@@ -292,7 +292,9 @@ struct RysLds {
   static constexpr bool f64 = rys_len<NR>() * 8 <= 36 * 1024;
 };
 
-template <int LA, int LB, int LC, int LD, bool PURE>
+// TWO: the open-shell pass (prm.D2 / prm.F2 set), compiled separately so the
+// closed-shell kernels keep their register allocation.
+template <int LA, int LB, int LC, int LD, bool PURE, bool TWO>
 __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
   using Dm = Dims<LA, LB, LC, LD>;
   constexpr int NR = Dm::NROOT;
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
                                              prm.inv_mu2, acc, nprim);
     count_wave(ctr + 2, false);
     sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() +
-                          (nprim ? 12ull * OA * OB * OC * OD : 0ull));
+                          (nprim ? (TWO ? 24ull : 12ull) * OA * OB * OC * OD : 0ull));
     sum_wave(ctr + 4, (unsigned long long)nprim);
     if (skip || nprim == 0) return;
     double deg = 1.0;
@@ -459,6 +461,76 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
         if (v != 0.0) fatomic(F + tri_idx(x, y), v);
       }
     }
+    if constexpr (!TWO) return;
+    // open-shell pass: the exchange part for D2 into F2, a second sweep over
+    // the AO slices of the same contracted integrals
+    const double* D2 = prm.D2;
+    double* F2 = prm.F2;
+#pragma unroll
+    for (int a = 0; a < OA; ++a) {
+#pragma unroll
+      for (int c = 0; c < OC; ++c) Dac[a * OC + c] = D2[tri_idx(oa + a, oc + c)];
+#pragma unroll
+      for (int d = 0; d < OD; ++d) Dad[a * OD + d] = D2[tri_idx(oa + a, od + d)];
+    }
+#pragma unroll
+    for (int b = 0; b < OB; ++b) {
+#pragma unroll
+      for (int c = 0; c < OC; ++c) Dbc[b * OC + c] = D2[tri_idx(ob + b, oc + c)];
+#pragma unroll
+      for (int d = 0; d < OD; ++d) Dbd[b * OD + d] = D2[tri_idx(ob + b, od + d)];
+    }
+#pragma unroll
+    for (int i = 0; i < OA * OC; ++i) Kac[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < OA * OD; ++i) Kad[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < OB * OC; ++i) Kbc[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < OB * OD; ++i) Kbd[i] = 0.0;
+    stream_slices<LA, LB, LC, LD, PURE>(P, Q, prm.xform, acc, [&](int a, int b, double* S) {
+#pragma unroll
+      for (int c = 0; c < OC; ++c)
+#pragma unroll
+        for (int d = 0; d < OD; ++d) {
+          double v = S[c * OD + d];
+          if (fabs(v) < cut) v = 0.0;
+          Kac[a * OC + c] += v * Dbd[b * OD + d];
+          Kad[a * OD + d] += v * Dbc[b * OC + c];
+          Kbc[b * OC + c] += v * Dad[a * OD + d];
+          Kbd[b * OD + d] += v * Dac[a * OC + c];
+        }
+    });
+#pragma unroll
+    for (int a = 0; a < OA; ++a) {
+#pragma unroll
+      for (int c = 0; c < OC; ++c) {
+        const int x = oa + a, y = oc + c;
+        const double v = ck * Kac[a * OC + c] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F2 + tri_idx(x, y), v);
+      }
+#pragma unroll
+      for (int d = 0; d < OD; ++d) {
+        const int x = oa + a, y = od + d;
+        const double v = ck * Kad[a * OD + d] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F2 + tri_idx(x, y), v);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < OB; ++b) {
+#pragma unroll
+      for (int c = 0; c < OC; ++c) {
+        const int x = ob + b, y = oc + c;
+        const double v = ck * Kbc[b * OC + c] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F2 + tri_idx(x, y), v);
+      }
+#pragma unroll
+      for (int d = 0; d < OD; ++d) {
+        const int x = ob + b, y = od + d;
+        const double v = ck * Kbd[b * OD + d] * (x == y ? 2.0 : 1.0);
+        if (v != 0.0) fatomic(F2 + tri_idx(x, y), v);
+      }
+    }
     return;
   } else {
   constexpr int NOUT = OA * OB * OC * OD;
@@ -488,7 +560,8 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
                                                         1.0e-4 * prm.cutoff, prm.inv_mu2, V, nprim);
   }
   count_wave(ctr + 2, lowp && !skip);
-  sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() + (nprim ? 12ull * NOUT : 0ull));
+  sum_wave(ctr + 3, (unsigned long long)nprim * flops_prim<LA, LB, LC, LD>() +
+                        (nprim ? (TWO ? 24ull : 12ull) * NOUT : 0ull));
   sum_wave(ctr + 4, (unsigned long long)nprim);
   const bool valid = !skip && nprim > 0;
   // all lanes stay converged through the digestion; invalid lanes carry V = 0
@@ -549,24 +622,25 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
         if (v != 0.0) fatomic(F + tri_idx(x, y), v);
       }
   }
-  // K blocks: K_ac += V D_bd, K_ad += V D_bc, K_bc += V D_ad, K_bd += V D_ac
-  {
+  // K blocks: K_ac += V D_bd, K_ad += V D_bc, K_bc += V D_ad, K_bd += V D_ac,
+  // for D into F and, in the open-shell pass, for D2 into F2
+#define VV(a, b, cc, d) V[(((a) * OB + (b)) * OC + (cc)) * OD + (d)]
+  auto kblock = [&](const double* __restrict__ Dk, double* Fk) {
     double Dbd[OB * OD], Dbc[OB * OC], Dad[OA * OD], Dac[OA * OC];
 #pragma unroll
     for (int b = 0; b < OB; ++b) {
 #pragma unroll
-      for (int d = 0; d < OD; ++d) Dbd[b * OD + d] = Dp[tri_idx(ob + b, od + d)];
+      for (int d = 0; d < OD; ++d) Dbd[b * OD + d] = Dk[tri_idx(ob + b, od + d)];
 #pragma unroll
-      for (int cc = 0; cc < OC; ++cc) Dbc[b * OC + cc] = Dp[tri_idx(ob + b, oc + cc)];
+      for (int cc = 0; cc < OC; ++cc) Dbc[b * OC + cc] = Dk[tri_idx(ob + b, oc + cc)];
     }
 #pragma unroll
     for (int a = 0; a < OA; ++a) {
 #pragma unroll
-      for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dp[tri_idx(oa + a, od + d)];
+      for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dk[tri_idx(oa + a, od + d)];
 #pragma unroll
-      for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dp[tri_idx(oa + a, oc + cc)];
+      for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dk[tri_idx(oa + a, oc + cc)];
     }
-#define VV(a, b, cc, d) V[(((a) * OB + (b)) * OC + (cc)) * OD + (d)]
 #pragma unroll
     for (int a = 0; a < OA; ++a)
 #pragma unroll
@@ -578,7 +652,7 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
           for (int d = 0; d < OD; ++d) s += VV(a, b, cc, d) * Dbd[b * OD + d];
         const int x = oa + a, y = oc + cc;
         const double v = ck * s * (x == y ? 2.0 : 1.0);
-        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+        if (v != 0.0) fatomic(Fk + tri_idx(x, y), v);
       }
 #pragma unroll
     for (int a = 0; a < OA; ++a)
@@ -591,7 +665,7 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
           for (int cc = 0; cc < OC; ++cc) s += VV(a, b, cc, d) * Dbc[b * OC + cc];
         const int x = oa + a, y = od + d;
         const double v = ck * s * (x == y ? 2.0 : 1.0);
-        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+        if (v != 0.0) fatomic(Fk + tri_idx(x, y), v);
       }
 #pragma unroll
     for (int b = 0; b < OB; ++b)
@@ -604,7 +678,7 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
           for (int d = 0; d < OD; ++d) s += VV(a, b, cc, d) * Dad[a * OD + d];
         const int x = ob + b, y = oc + cc;
         const double v = ck * s * (x == y ? 2.0 : 1.0);
-        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+        if (v != 0.0) fatomic(Fk + tri_idx(x, y), v);
       }
 #pragma unroll
     for (int b = 0; b < OB; ++b)
@@ -617,10 +691,12 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
           for (int cc = 0; cc < OC; ++cc) s += VV(a, b, cc, d) * Dac[a * OC + cc];
         const int x = ob + b, y = od + d;
         const double v = ck * s * (x == y ? 2.0 : 1.0);
-        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+        if (v != 0.0) fatomic(Fk + tri_idx(x, y), v);
       }
+  };
 #undef VV
-  }
+  kblock(Dp, F);
+  if constexpr (TWO) kblock(prm.D2, prm.F2);
   }
 }
 
@@ -660,7 +736,7 @@ struct JKB {
                                                                                                  : 1;
 };
 
-template <int LA, int LB, int LC, int LD, bool PURE>
+template <int LA, int LB, int LC, int LD, bool PURE, bool TWO>
 __global__ __launch_bounds__((JKB<LA, LB, LC, LD>::threads))
 __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(JKParams prm, BraWork bw, ColMap cm) {
   using Dm = Dims<LA, LB, LC, LD>;
@@ -680,13 +756,20 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   __shared__ double s_jab[OA * OB];
   // dynamic LDS: K rows [NROW][ncol] (accumulated); when cm.drows, also D rows
   // [NROW][ncol] (the density at the same positions) and the bra shells' rows
-  // of the shell-pair density maxima [2][nsh]
+  // of the shell-pair density maxima [2][nsh]; in the open-shell pass (F2
+  // set) the K rows of D2 and, with cm.drows, the D2 rows follow
   extern __shared__ double s_dyn[];
   const int ncol = cm.ncol, nk = NROW * ncol, nsh = prm.nsh;
+  const bool drows = cm.drows != 0;  // launch-uniform
+  constexpr bool two = TWO;
   double* s_k = s_dyn;
   double* s_d = s_dyn + nk;
   double* s_ds = s_dyn + 2 * nk;
+  double* s_k2 = drows ? s_ds + 2 * nsh : s_dyn + nk;
+  double* s_d2 = s_k2 + nk;
   for (int i = threadIdx.x; i < nk; i += blockDim.x) s_k[i] = 0.0;
+  if (two)
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) s_k2[i] = 0.0;
   if (threadIdx.x < OA * OB) s_jab[threadIdx.x] = 0.0;
   if constexpr (T64)
     for (int i = threadIdx.x; i < rys_len<NR>(); i += blockDim.x) s_t64[i] = prm.rys[RYS_OFF[NR] + i];
@@ -704,13 +787,13 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
       for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra32[i] = prim_v<float>(prm.prims32[P.prim0 + i]);
   // the density entries the digestion reads in the bra rows a and b, gathered
   // once per block instead of once per quartet
-  const bool drows = cm.drows != 0;  // launch-uniform
   if (drows) {
     for (int i = threadIdx.x; i < nk; i += blockDim.x) {
       const int r = i / ncol, col = i - r * ncol;
       const int x = r < OA ? P.ao_a + r : P.ao_b + (r - OA);
       const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
       s_d[i] = prm.D[tri_idx(x, y)];
+      if (two) s_d2[i] = prm.D2[tri_idx(x, y)];
     }
     for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
       s_ds[i] = prm.dsh[P.si * nsh + i];
@@ -835,38 +918,38 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
       }
       // K rows a (0..OA-1) and b (OA..OA+OB-1) in LDS; the density in those
       // rows comes from the LDS copy at the same positions
-      if (valid) {
+      auto krows = [&](const double* sd, const double* __restrict__ Dg, double* sk) {
         const int colC = cm.colidx[Q.sa], colD = cm.dofs + cm.colidx[Q.sb];
         double Dbd[OB * OD], Dbc[OB * OC], Dad[OA * OD], Dac[OA * OC];
         if (drows) {
 #pragma unroll
           for (int bb = 0; bb < OB; ++bb) {
 #pragma unroll
-            for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = s_d[(OA + bb) * ncol + colD + d];
+            for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = sd[(OA + bb) * ncol + colD + d];
 #pragma unroll
-            for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = s_d[(OA + bb) * ncol + colC + cc];
+            for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = sd[(OA + bb) * ncol + colC + cc];
           }
 #pragma unroll
           for (int a = 0; a < OA; ++a) {
 #pragma unroll
-            for (int d = 0; d < OD; ++d) Dad[a * OD + d] = s_d[a * ncol + colD + d];
+            for (int d = 0; d < OD; ++d) Dad[a * OD + d] = sd[a * ncol + colD + d];
 #pragma unroll
-            for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = s_d[a * ncol + colC + cc];
+            for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = sd[a * ncol + colC + cc];
           }
         } else {
 #pragma unroll
           for (int bb = 0; bb < OB; ++bb) {
 #pragma unroll
-            for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = Dp[tri_idx(ob + bb, od + d)];
+            for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = Dg[tri_idx(ob + bb, od + d)];
 #pragma unroll
-            for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = Dp[tri_idx(ob + bb, oc + cc)];
+            for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = Dg[tri_idx(ob + bb, oc + cc)];
           }
 #pragma unroll
           for (int a = 0; a < OA; ++a) {
 #pragma unroll
-            for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dp[tri_idx(oa + a, od + d)];
+            for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dg[tri_idx(oa + a, od + d)];
 #pragma unroll
-            for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dp[tri_idx(oa + a, oc + cc)];
+            for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dg[tri_idx(oa + a, oc + cc)];
           }
         }
 #pragma unroll
@@ -878,7 +961,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
             for (int bb = 0; bb < OB; ++bb)
 #pragma unroll
               for (int d = 0; d < OD; ++d) t += VV(a, bb, cc, d) * Dbd[bb * OD + d];
-            if (t != 0.0) atomicAdd(&s_k[a * ncol + colC + cc], ck * t);
+            if (t != 0.0) atomicAdd(&sk[a * ncol + colC + cc], ck * t);
           }
 #pragma unroll
           for (int d = 0; d < OD; ++d) {
@@ -887,7 +970,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
             for (int bb = 0; bb < OB; ++bb)
 #pragma unroll
               for (int cc = 0; cc < OC; ++cc) t += VV(a, bb, cc, d) * Dbc[bb * OC + cc];
-            if (t != 0.0) atomicAdd(&s_k[a * ncol + colD + d], ck * t);
+            if (t != 0.0) atomicAdd(&sk[a * ncol + colD + d], ck * t);
           }
         }
 #pragma unroll
@@ -899,7 +982,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
             for (int a = 0; a < OA; ++a)
 #pragma unroll
               for (int d = 0; d < OD; ++d) t += VV(a, bb, cc, d) * Dad[a * OD + d];
-            if (t != 0.0) atomicAdd(&s_k[(OA + bb) * ncol + colC + cc], ck * t);
+            if (t != 0.0) atomicAdd(&sk[(OA + bb) * ncol + colC + cc], ck * t);
           }
 #pragma unroll
           for (int d = 0; d < OD; ++d) {
@@ -908,9 +991,13 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
             for (int a = 0; a < OA; ++a)
 #pragma unroll
               for (int cc = 0; cc < OC; ++cc) t += VV(a, bb, cc, d) * Dac[a * OC + cc];
-            if (t != 0.0) atomicAdd(&s_k[(OA + bb) * ncol + colD + d], ck * t);
+            if (t != 0.0) atomicAdd(&sk[(OA + bb) * ncol + colD + d], ck * t);
           }
         }
+      };
+      if (valid) {
+        krows(s_d, Dp, s_k);
+        if constexpr (TWO) krows(s_d2, prm.D2, s_k2);
       }
 #undef VV
     }
@@ -925,7 +1012,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
     unsigned long long* ctr = prm.counters + 8 * (int)((blockIdx.x * nw + wv) & (OQPH_CTR_SLOTS - 1));
     unsigned long long v[5] = {n_comp, n_skip, n_f32,
                                (unsigned long long)n_prim * flops_prim<LA, LB, LC, LD>() +
-                                   (unsigned long long)n_valid * 12ull * NOUT,
+                                   (unsigned long long)n_valid * (two ? 24ull : 12ull) * NOUT,
                                n_prim};
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -944,6 +1031,15 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
     const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
     fatomic(F + tri_idx(x, y), v * (x == y ? 2.0 : 1.0));
   }
+  if (two)
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+      const double v = s_k2[i];
+      if (v == 0.0) continue;
+      const int r = i / ncol, col = i - r * ncol;
+      const int x = r < OA ? oa + r : ob + (r - OA);
+      const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
+      fatomic(prm.F2 + tri_idx(x, y), v * (x == y ? 2.0 : 1.0));
+    }
   if (threadIdx.x < OA * OB) {
     const double v = s_jab[threadIdx.x];
     const int x = oa + (int)threadIdx.x / OB, y = ob + (int)threadIdx.x % OB;

@@ -133,7 +133,7 @@ __device__ __forceinline__ int contract_group(const ShellPair& Q, const double (
   return ng;
 }
 
-template <int LA, int LB, int LC, int LD, bool PURE>
+template <int LA, int LB, int LC, int LD, bool PURE, bool TWO>
 __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm, ChunkView cv) {
   using Dm = Dims<LA, LB, LC, LD>;
   constexpr int NM = JKG<LA, LB, LC, LD>::nm;
@@ -164,14 +164,21 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
   const int* shl = ch + ch[7];
   const int ncol = cm.ncol, nk = nrow * ncol, nsh = prm.nsh;
   const bool drows = cm.drows != 0;  // launch-uniform
+  constexpr bool two = TWO;  // the open-shell pass
+  // dynamic LDS: K rows, D rows and dsh rows (drows), the K rows and D rows
+  // of D2 (open-shell pass), the chunk's primitive records
   double* s_k = s_dyn;
   double* s_d = s_k + nk;
   double* s_ds = s_d + (drows ? nk : 0);
-  double* s_cp = s_ds + (drows ? (size_t)nshl * nsh : 0);
+  double* s_k2 = s_ds + (drows ? (size_t)nshl * nsh : 0);
+  double* s_d2 = s_k2 + (two ? nk : 0);
+  double* s_cp = s_d2 + (two && drows ? nk : 0);
   float* s_cp32 = (float*)(s_cp + (size_t)ncp * stride);
   const int P0 = __builtin_amdgcn_readfirstlane(ch[OQPH_JKG_HDR]);
   const ShellPair PA = prm.pairs[P0];  // the members' common centres A, B
   for (int i = threadIdx.x; i < nk; i += blockDim.x) s_k[i] = 0.0;
+  if (two)
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) s_k2[i] = 0.0;
   for (int i = threadIdx.x; i < NM * OA * OB; i += blockDim.x) s_jab[i] = 0.0;
   if constexpr (T64)
     for (int i = threadIdx.x; i < rys_len<NR>(); i += blockDim.x) s_t64[i] = prm.rys[RYS_OFF[NR] + i];
@@ -219,6 +226,7 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
       const int r = i / ncol, col = i - r * ncol;
       const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
       s_d[i] = prm.D[tri_idx(rowao[r], y)];
+      if (two) s_d2[i] = prm.D2[tri_idx(rowao[r], y)];
     }
     for (int i = threadIdx.x; i < nshl * nsh; i += blockDim.x) {
       const int sl = i / nsh, c = i - sl * nsh;
@@ -393,8 +401,8 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
           for (int ab = 0; ab < OA * OB; ++ab) t += V[ab * OC * OD + cd] * dab[ab];
           tcd[cd] += cj * t;
         }
-        if (valid) {
-          // K rows of the member's shells a and b (LDS rows ra.., rb..)
+        // K rows of the member's shells a and b (LDS rows ra.., rb..)
+        auto krows = [&](const double* sd, const double* __restrict__ Dg, double* sk) {
           const int ra = mi[m][1], rb = mi[m][2];
           const int oa = mi[m][9], ob = mi[m][10];
           double Dbd[OB * OD], Dbc[OB * OC], Dad[OA * OD], Dac[OA * OC];
@@ -402,31 +410,31 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
 #pragma unroll
             for (int bb = 0; bb < OB; ++bb) {
 #pragma unroll
-              for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = s_d[(rb + bb) * ncol + colD + d];
+              for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = sd[(rb + bb) * ncol + colD + d];
 #pragma unroll
-              for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = s_d[(rb + bb) * ncol + colC + cc];
+              for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = sd[(rb + bb) * ncol + colC + cc];
             }
 #pragma unroll
             for (int a = 0; a < OA; ++a) {
 #pragma unroll
-              for (int d = 0; d < OD; ++d) Dad[a * OD + d] = s_d[(ra + a) * ncol + colD + d];
+              for (int d = 0; d < OD; ++d) Dad[a * OD + d] = sd[(ra + a) * ncol + colD + d];
 #pragma unroll
-              for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = s_d[(ra + a) * ncol + colC + cc];
+              for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = sd[(ra + a) * ncol + colC + cc];
             }
           } else {
 #pragma unroll
             for (int bb = 0; bb < OB; ++bb) {
 #pragma unroll
-              for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = Dp[tri_idx(ob + bb, od + d)];
+              for (int d = 0; d < OD; ++d) Dbd[bb * OD + d] = Dg[tri_idx(ob + bb, od + d)];
 #pragma unroll
-              for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = Dp[tri_idx(ob + bb, oc + cc)];
+              for (int cc = 0; cc < OC; ++cc) Dbc[bb * OC + cc] = Dg[tri_idx(ob + bb, oc + cc)];
             }
 #pragma unroll
             for (int a = 0; a < OA; ++a) {
 #pragma unroll
-              for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dp[tri_idx(oa + a, od + d)];
+              for (int d = 0; d < OD; ++d) Dad[a * OD + d] = Dg[tri_idx(oa + a, od + d)];
 #pragma unroll
-              for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dp[tri_idx(oa + a, oc + cc)];
+              for (int cc = 0; cc < OC; ++cc) Dac[a * OC + cc] = Dg[tri_idx(oa + a, oc + cc)];
             }
           }
 #pragma unroll
@@ -438,7 +446,7 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
               for (int bb = 0; bb < OB; ++bb)
 #pragma unroll
                 for (int d = 0; d < OD; ++d) t += VV(a, bb, cc, d) * Dbd[bb * OD + d];
-              if (t != 0.0) atomicAdd(&s_k[(ra + a) * ncol + colC + cc], ck * t);
+              if (t != 0.0) atomicAdd(&sk[(ra + a) * ncol + colC + cc], ck * t);
             }
 #pragma unroll
             for (int d = 0; d < OD; ++d) {
@@ -447,7 +455,7 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
               for (int bb = 0; bb < OB; ++bb)
 #pragma unroll
                 for (int cc = 0; cc < OC; ++cc) t += VV(a, bb, cc, d) * Dbc[bb * OC + cc];
-              if (t != 0.0) atomicAdd(&s_k[(ra + a) * ncol + colD + d], ck * t);
+              if (t != 0.0) atomicAdd(&sk[(ra + a) * ncol + colD + d], ck * t);
             }
           }
 #pragma unroll
@@ -459,7 +467,7 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
               for (int a = 0; a < OA; ++a)
 #pragma unroll
                 for (int d = 0; d < OD; ++d) t += VV(a, bb, cc, d) * Dad[a * OD + d];
-              if (t != 0.0) atomicAdd(&s_k[(rb + bb) * ncol + colC + cc], ck * t);
+              if (t != 0.0) atomicAdd(&sk[(rb + bb) * ncol + colC + cc], ck * t);
             }
 #pragma unroll
             for (int d = 0; d < OD; ++d) {
@@ -468,9 +476,13 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
               for (int a = 0; a < OA; ++a)
 #pragma unroll
                 for (int cc = 0; cc < OC; ++cc) t += VV(a, bb, cc, d) * Dac[a * OC + cc];
-              if (t != 0.0) atomicAdd(&s_k[(rb + bb) * ncol + colD + d], ck * t);
+              if (t != 0.0) atomicAdd(&sk[(rb + bb) * ncol + colD + d], ck * t);
             }
           }
+        };
+        if (valid) {
+          krows(s_d, Dp, s_k);
+          if constexpr (TWO) krows(s_d2, prm.D2, s_k2);
         }
 #undef VV
       }
@@ -490,7 +502,7 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
     unsigned long long* ctr = prm.counters + 8 * (int)((blockIdx.x * nw + wv) & (OQPH_CTR_SLOTS - 1));
     unsigned long long v[6] = {n_comp, n_skip, n_f32,
                                (unsigned long long)n_prim * flops_prim<LA, LB, LC, LD>() +
-                                   (unsigned long long)n_valid * 12ull * NOUT,
+                                   (unsigned long long)n_valid * (two ? 24ull : 12ull) * NOUT,
                                n_prim, n_grp};
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -509,6 +521,15 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
     const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
     fatomic(F + tri_idx(x, y), v * (x == y ? 2.0 : 1.0));
   }
+  if (two)
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+      const double v = s_k2[i];
+      if (v == 0.0) continue;
+      const int r = i / ncol, col = i - r * ncol;
+      const int x = rowao[r];
+      const int y = col < cm.ncolC ? cm.aoC[col] : cm.aoD[col - cm.ncolC];
+      fatomic(prm.F2 + tri_idx(x, y), v * (x == y ? 2.0 : 1.0));
+    }
   for (int i = threadIdx.x; i < nmem * OA * OB; i += blockDim.x) {
     const double v = s_jab[i];
     if (v == 0.0) continue;

@@ -201,4 +201,9 @@ struct JKParams {
   double inv_mu2;       // 1/mu^2 of erf(mu r)/r attenuated integrals; 0: plain 1/r
   const unsigned char* own;  // per pair: 1 when this share owns quartets whose
                              // reference-order first pair it is; null: all
+  // Second exchange density of the single-pass open-shell build: when F2 is
+  // set, every quartet also adds its exchange part (scale se, no Coulomb)
+  // for the packed density D2 into F2, from the same integrals.
+  const double* D2;
+  double* F2;
 };

@@ -41,7 +41,7 @@ bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const Pri
 bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, KernelRes* r);
 void launch_shell_density(const double* D, const double* D2, const int* ao_off, const int* nao, int nsh, double* dsh,
                           hipStream_t s);
-void launch_add(const double* x, const double* y, double* z, long long n, hipStream_t s);
+void launch_sumdiff(const double* a, const double* b, double* p, double* m, double c, long long n, hipStream_t s);
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
 bool kernels_support(int lmax);
 bool jk_template(int la, int lb, int lc, int ld);
@@ -198,6 +198,7 @@ struct oqph_session {
   DBuf<long long> d_off;
   DBuf<unsigned long long> d_ctr;
   double cached_maxden = -1.0;
+  bool cached_two = false;  // lists prepared for the open-shell pass (K rows of two densities in LDS)
   std::vector<long long> h_off;
   struct GLaunch {
     int la, lb, lc, ld;
@@ -1133,8 +1134,8 @@ static int prepare_grad(oqph_session* s) {
 // class with itself each unordered pair is taken once (the bra is the later
 // entry of the (bucket, Q) order).  Non-empty (p, b) segments are stored with
 // a prefix over their 64-ket chunks.
-static int prepare_lists(oqph_session* s, double maxden) {
-  if (maxden == s->cached_maxden) return 0;
+static int prepare_lists(oqph_session* s, double maxden, bool two) {
+  if (maxden == s->cached_maxden && two == s->cached_two) return 0;
   s->launches.clear();
   s->h_seg_bra.clear();
   s->h_seg_k0.clear();
@@ -1213,16 +1214,19 @@ static int prepare_lists(oqph_session* s, double maxden) {
       KernelRes kr0{}, kr1{};
       const bool tmpl = jk_template(la, lb, lc, ld) && !(!s->f_templates && (la == 3 || lc == 3)) &&
                         !(s->jk_hl_ng > 0 && ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > s->jk_hl_ng);
-      const bool fit0 = tmpl && kernel_resources(0, la, lb, lc, ld, s->pure, &kr0) &&
+      const int variant = two ? 16 : 0;  // the open-shell kernels when two
+      const bool fit0 = tmpl && kernel_resources(0 | variant, la, lb, lc, ld, s->pure, &kr0) &&
                         kr0.private_bytes <= OQPH_PRIVATE_BUDGET;
-      const bool fit1 = tmpl && kernel_resources(1, la, lb, lc, ld, s->pure, &kr1) &&
+      const bool fit1 = tmpl && kernel_resources(1 | variant, la, lb, lc, ld, s->pure, &kr1) &&
                         kr1.private_bytes <= OQPH_PRIVATE_BUDGET;
       const bool hl = !fit0 && !fit1;
       bool rows = fit1 && s->rows_mode > 0 && jkb_info(la, lb, lc, ld, s->pure, &nrow, &slds, s->lds_max);
       // dynamic LDS: K rows (nrow x ncol), plus D rows (nrow x ncol) and the
-      // bra rows of dsh (2 x nsh) when those fit as well
-      const size_t dynk = (size_t)nrow * ncol * sizeof(double);
-      const size_t dynd = (2 * (size_t)nrow * ncol + 2 * (size_t)s->nsh) * sizeof(double);
+      // bra rows of dsh (2 x nsh) when those fit as well; the open-shell
+      // pass holds the K rows (and D rows) of both densities
+      const size_t nd = two ? 2 : 1;
+      const size_t dynk = nd * nrow * ncol * sizeof(double);
+      const size_t dynd = (2 * nd * nrow * ncol + 2 * (size_t)s->nsh) * sizeof(double);
       const bool drows = rows && s->rows_mode != 3 && slds + dynd <= s->lds_max;
       const size_t dynb = drows ? dynd : dynk;
       rows = rows && slds + dynb <= s->lds_max;
@@ -1265,8 +1269,8 @@ static int prepare_lists(oqph_session* s, double maxden) {
             its.push_back(t);
             const size_t rows_b = (size_t)cs.nrow[ci] * ncol * sizeof(double);
             const size_t prims_b = (size_t)cs.ncp[ci] * cs.stride[ci] * (sizeof(double) + sizeof(float));
-            need_n = std::max(need_n, rows_b + prims_b);
-            need_d = std::max(need_d, 2 * rows_b + (size_t)cs.nshl[ci] * s->nsh * sizeof(double) + prims_b);
+            need_n = std::max(need_n, nd * rows_b + prims_b);
+            need_d = std::max(need_d, 2 * nd * rows_b + (size_t)cs.nshl[ci] * s->nsh * sizeof(double) + prims_b);
           }
           const bool gd = s->rows_mode != 3 && st + need_d <= s->lds_max;
           if (!its.empty() && (gd || st + need_n <= s->lds_max)) {
@@ -1470,6 +1474,7 @@ static int prepare_lists(oqph_session* s, double maxden) {
   }
   ++s->list_gen;
   s->cached_maxden = maxden;
+  s->cached_two = two;
   return 0;
 }
 
@@ -1581,7 +1586,8 @@ static int streams_join(oqph_session* s, hipStream_t main, int n) {
 // the session's buffers (no device allocation; profiling is off there).
 static int build_device(oqph_session* s, const double* dD, double* dF, double se, double sc, oqph_stats* st,
                         hipStream_t stream, const double* scr_a = nullptr, const double* scr_b = nullptr,
-                        bool zero_f = true, double mu = 0.0, const StaticScratch* ss = nullptr) {
+                        bool zero_f = true, double mu = 0.0, const StaticScratch* ss = nullptr,
+                        const double* dD2 = nullptr, double* dF2 = nullptr) {
   int rc;
   if (mu > 0.0 && (rc = prepare_attenuated(s, mu))) return rc;
   if ((rc = prepare_share(s))) return rc;
@@ -1615,11 +1621,12 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   double maxden;
   memcpy(&maxden, &mbits, sizeof(double));
   if (zero_f) HIPCHK(hipMemsetAsync(dF, 0, ntri * sizeof(double), stream));
+  if (zero_f && dF2) HIPCHK(hipMemsetAsync(dF2, 0, ntri * sizeof(double), stream));
   if (maxden <= 0.0) {
     if (st) memset(st, 0, sizeof(*st));
     return 0;
   }
-  if ((rc = prepare_lists(s, maxden))) return rc;
+  if ((rc = prepare_lists(s, maxden, dF2 != nullptr))) return rc;
   const size_t nl = s->launches.size();
   const size_t nc = 8 * (size_t)OQPH_CTR_SLOTS;  // counters per launch
   unsigned long long* ctr = nullptr;  // per-launch counters
@@ -1674,6 +1681,8 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   p.xform = s->d_xform.p;
   p.D = dD;
   p.F = dF;
+  p.D2 = dF2 ? dD2 : nullptr;
+  p.F2 = dF2;
   p.dsh = dsh;
   p.counters = ctr;
   p.nsh = s->nsh;
@@ -1949,34 +1958,30 @@ int oqph_schwarz(oqph_session* s, double* q) {
   return 0;
 }
 
-// UHF/ROHF (nfocks = 2, int2_urohf_data_t): with G(X; se, sc) = sc J[X] -
-// se K[X] / 2 (the RHF build), F_s = G(Da + Db; 0, sc) + G(Ds; 2 se, 0).  All
-// three builds screen with the shell maxima over both spin densities, as
-// shlden does for nfocks = 2 (int2.F90:999-1047), so they keep the same
-// quartets as the reference's single pass.  d and f hold alpha then beta.
+// UHF/ROHF (nfocks = 2, int2_urohf_data_t, int2.F90:1488-1578): F_s =
+// sc J[Da + Db] - se K[Ds] (packed, halved and diagonal-doubled as for RHF),
+// in one pass over the integrals.  With D+ = Da + Db and D- = Da - Db,
+// K[Da] = (K[D+] + K[D-]) / 2 and K[Db] = (K[D+] - K[D-]) / 2, so
+//   F_a = G(D+; se, sc) + X,  F_b = G(D+; se, sc) - X,  X = -se K[D-] / 2:
+// the build digests every quartet for D+ (Coulomb and exchange, into fa) and
+// for D- (exchange only, into fb), then forms fa + fb and fa - fb.  The
+// quartet screens use the shell maxima over both spin densities, as shlden
+// does for nfocks = 2 (int2.F90:999-1047).  d and f hold alpha then beta;
+// zero_f = false adds to what f holds (its alpha/beta pair is first taken to
+// the same half-sum / half-difference form).
 static int build_uhf(oqph_session* s, const double* d_dev, double* f_dev, double se, double sc, oqph_stats* st,
                      hipStream_t stream, bool zero_f = true, double mu = 0.0) {
   const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
   int rc;
-  if ((rc = s->d_Dt.ensure(ntri))) return rc;
+  if ((rc = s->d_Dt.ensure(2 * ntri))) return rc;
   const double *da = d_dev, *db = d_dev + ntri;
   double *fa = f_dev, *fb = f_dev + ntri;
-  launch_add(da, db, s->d_Dt.p, (long long)ntri, stream);
-  oqph_stats s1{}, s2{}, s3{};
-  if (zero_f) {
-    if ((rc = build_device(s, s->d_Dt.p, fa, 0.0, sc, &s1, stream, da, db, true, mu))) return rc;
-    HIPCHK(hipMemcpyAsync(fb, fa, ntri * sizeof(double), hipMemcpyDeviceToDevice, stream));
-  } else {
-    if ((rc = build_device(s, s->d_Dt.p, fa, 0.0, sc, &s1, stream, da, db, false, mu))) return rc;
-    if ((rc = build_device(s, s->d_Dt.p, fb, 0.0, sc, nullptr, stream, da, db, false, mu))) return rc;
-  }
-  if ((rc = build_device(s, da, fa, 2.0 * se, 0.0, &s2, stream, da, db, false, mu))) return rc;
-  if ((rc = build_device(s, db, fb, 2.0 * se, 0.0, &s3, stream, da, db, false, mu))) return rc;
-  if (st) {
-    *st = s1;
-    st->kernel_ms = s1.kernel_ms + s2.kernel_ms + s3.kernel_ms;
-    st->model_flops = s1.model_flops + s2.model_flops + s3.model_flops;
-  }
+  double *dp = s->d_Dt.p, *dm = s->d_Dt.p + ntri;
+  launch_sumdiff(da, db, dp, dm, 1.0, (long long)ntri, stream);
+  if (!zero_f) launch_sumdiff(fa, fb, fa, fb, 0.5, (long long)ntri, stream);
+  if ((rc = build_device(s, dp, fa, se, sc, st, stream, da, db, zero_f, mu, nullptr, dm, fb))) return rc;
+  launch_sumdiff(fa, fb, fa, fb, 1.0, (long long)ntri, stream);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

@@ -226,3 +226,68 @@ def test_uhf_parity(L, oracle, name, nw):
     assert err <= TOL
     assert fb.last_stats.quartets_computed == int(st[1])
     fb.close()
+
+
+def _spin_densities(oracle, bas, na, nb, seeds=(11, 12)):
+    from openqp_amd import basis as B
+    S = oracle.int1e(bas)[0]
+    ds = []
+    for k, sd in ((na, seeds[0]), (nb, seeds[1])):
+        rng = np.random.default_rng(sd)
+        C = rng.standard_normal((bas.nbf, k))
+        Lc = np.linalg.cholesky(C.T @ S @ C)
+        C = C @ np.linalg.inv(Lc).T
+        ds.append(B.pack_tri(C @ C.T))
+    return ds
+
+
+@pytest.mark.parametrize("basis,nw", [("cc-pvdz", 3), ("cc-pvtz", 1)])
+def test_uhf_single_pass_kernel_modes(L, oracle, basis, nw):
+    """The single-pass open-shell build (one digestion of D+ = Da + Db with
+    Coulomb and exchange, one of D- = Da - Db with exchange only, from the
+    same integrals) on every kernel family: wavefront-task kernel
+    (OQPH_JK_ROWS=0), bra-row kernel with and without the LDS density rows
+    (2, 3), grouped kernel on every class pair (OQPH_JKG=2).  Each against
+    the oracle's int2_urohf_data_t restatement (1e-10) with the oracle's
+    computed-quartet count."""
+    _say("UHF single pass, kernel modes", basis, nw)
+    bas = small_system(basis, nw=nw, seed=4)
+    da, db = _spin_densities(oracle, bas, 5 * nw, 5 * nw - 1)
+    Q = oracle.schwarz(bas)
+    ra, rb, st = oracle.fock_jk_uhf(bas, da, db, se=0.8, sc=1.0, cutoff=5e-11, Q=Q)
+    saved = {k: os.environ.get(k) for k in ("OQPH_JK_ROWS", "OQPH_JKG")}
+    try:
+        for rows, jkg in (("0", "0"), ("2", "1"), ("3", "1"), ("1", "2")):
+            os.environ["OQPH_JK_ROWS"] = rows
+            os.environ["OQPH_JKG"] = jkg
+            fb = L.FockBuilder(bas)
+            fa, fbb = fb.fock_jk_uhf(da, db, se=0.8)
+            kinds = sorted({p["kernel"] for p in fb.last_profile()})
+            err = max(np.abs(fa - ra).max(), np.abs(fbb - rb).max())
+            _say("  rows", rows, "jkg", jkg, "max |f - f_oracle| =", err, "quartets",
+                 fb.last_stats.quartets_computed, int(st[1]), "kernels", len(kinds))
+            assert err <= TOL
+            assert fb.last_stats.quartets_computed == int(st[1])
+            fb.close()
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def test_uhf_equal_spins_is_rhf(L, oracle):
+    """Da = Db = D/2 makes D- = 0: both spin Fock matrices equal the RHF
+    build of D (sc J[D] - se K[D]/2), elementwise to rounding.  (No
+    screening: the open-shell screen uses max(|Da|, |Db|), half the RHF
+    density bound here.)"""
+    bas = small_system("cc-pvdz", nw=3, seed=6)
+    D, dp = density_for(oracle, bas, 15, seed=8)
+    fb = L.FockBuilder(bas, cutoff=1e-20)
+    g = fb.fock_jk(dp, se=0.6, sc=0.9)
+    fa, fbb = fb.fock_jk_uhf(0.5 * dp, 0.5 * dp, se=0.6, sc=0.9)
+    _say("  equal spins: max |fa - g|", np.abs(fa - g).max(), "max |fb - g|", np.abs(fbb - g).max())
+    assert np.abs(fa - g).max() <= 1e-12
+    assert np.abs(fbb - g).max() <= 1e-12
+    fb.close()
