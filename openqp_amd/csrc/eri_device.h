@@ -154,7 +154,7 @@ __device__ __forceinline__ void rys_roots(double T, const double* __restrict__ t
 template <int N>
 __device__ __forceinline__ void rys_roots32(float T, const float* __restrict__ tabN, float* x, float* w) {
   if (T >= (float)RYS_TASYM[N]) {
-    const float it = __builtin_amdgcn_rcpf(T), rs = __builtin_amdgcn_rsqf(T);
+    const float rs = __builtin_amdgcn_rsqf(T), it = rs * rs;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       x[i] = (float)RYS_HERM[RYS_HOFF[N] + i] * it;
@@ -181,8 +181,9 @@ __device__ __forceinline__ void rys_root_one(R T, const double* __restrict__ tab
                                              int r, R& x, R& w) {
   if constexpr (sizeof(R) == 4) {
     if (T >= (float)RYS_TASYM[N]) {
-      x = (float)RYS_HERM[RYS_HOFF[N] + r] * __builtin_amdgcn_rcpf(T);
-      w = (float)RYS_HERM[RYS_HOFF[N] + N + r] * __builtin_amdgcn_rsqf(T);
+      const float rs = __builtin_amdgcn_rsqf(T);
+      x = (float)RYS_HERM[RYS_HOFF[N] + r] * (rs * rs);
+      w = (float)RYS_HERM[RYS_HOFF[N] + N + r] * rs;
       return;
     }
     const int k = (int)T;
@@ -282,21 +283,38 @@ __device__ __forceinline__ PrimV<R> prim_v(const PP& p) {
 }
 
 template <int LA, int LB, int LC, int LD, typename R>
-__device__ __forceinline__ void rys_add(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AC)[3], R ze, R sq, R ize,
+__device__ __forceinline__ void rys_add(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AQ)[3], R ze, R sq, R ize,
                                         R pref, const double* __restrict__ tabN, const float* __restrict__ tab32N,
                                         R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]);
 
+// The FP32 quadrature of the small classes folds the ket's product centre
+// into the centre difference once per ket primitive pair: AQ = (A - C) - X_q,
+// and PQ = AQ + X_p in the loop over bra primitives ((ps|ss) mixed 92 -> 89
+// ms on (H2O)64/cc-pVDZ, with the ket prefetch).  Elsewhere AQ = A - C and
+// PQ = AQ + X_p - X_q per quartet: FP64 needs all of the ket's 48-byte record
+// before its bra loop then ((ps|ss) FP64 122 -> 139 ms), and the larger
+// classes have no registers to spare.
+template <int LA, int LB, int LC, int LD, typename R>
+constexpr bool hoist_ket() {
+  return sizeof(R) == 4 && Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF <= 9;
+}
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void ket_offset(const R (&AC)[3], const PrimV<R>& qq, R (&AQ)[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) AQ[d] = hoist_ket<LA, LB, LC, LD, R>() ? AC[d] - qq.X[d] : AC[d];
+}
+
 // Adds one primitive quartet's (e0|f0) to acc; returns false when the
 // primitive prefactor screen |K_ab K_cd| / sqrt(zeta+eta) < qcut drops it.
-// pp, qq: bra and ket primitive pairs (X = P - A and X = Q - C); AC = A - C,
-// formed in FP64 by the caller, so PA = X_p, QC = X_q and PQ = AC + X_p - X_q.
+// pp, qq: bra and ket primitive pairs (X = P - A and X = Q - C); AQ from
+// ket_offset, so PA = X_p, QC = X_q and PQ = AQ + X_p (- X_q).
 // imu2 = 1/mu^2 selects the erf(mu r)/r operator: zeta + eta becomes
 // zeta + eta + zeta eta / mu^2 in every quantity of the quadrature
 // (OpenQP's attenuated Rys setup, source/integrals/int_rys.F90:180-182, 227);
 // imu2 = 0 is the plain Coulomb operator.
 // tabN / tab32N: the NROOT-root blocks of the FP64 / FP32 root tables.
 template <int LA, int LB, int LC, int LD, typename R>
-__device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AC)[3], R qcut2,
+__device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AQ)[3], R qcut2,
                                                R imu2, const double* __restrict__ tabN,
                                                const float* __restrict__ tab32N,
                                                R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]) {
@@ -305,15 +323,10 @@ __device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R
   const R fqz = pp.Kg * qq.Kg;
   const R ze = zeta + eta + zeta * eta * imu2;
   if (fqz * fqz < qcut2 * ze) return false;
-  R ize, sq;
-  if constexpr (F32) {
-    sq = __builtin_amdgcn_rsqf(ze);
-    ize = __builtin_amdgcn_rcpf(ze);
-  } else {
-    sq = rsq64(ze);
-    ize = sq * sq;
-  }
-  rys_add<LA, LB, LC, LD, R>(pp, qq, AC, ze, sq, ize, fqz * sq, tabN, tab32N, acc);
+  // 1/sqrt(ze) and its square: one transcendental (FP32: hardware rsq)
+  const R sq = F32 ? (R)__builtin_amdgcn_rsqf((float)ze) : (R)rsq64((double)ze);
+  const R ize = sq * sq;
+  rys_add<LA, LB, LC, LD, R>(pp, qq, AQ, ze, sq, ize, fqz * sq, tabN, tab32N, acc);
   return true;
 }
 
@@ -321,7 +334,7 @@ __device__ __forceinline__ bool rys_accumulate(const PrimV<R>& pp, const PrimV<R
 // (e0|f0) products, scaled by pref, to acc.  ze = zeta + eta (+ zeta eta /
 // mu^2 for the attenuated operator), sq = 1/sqrt(ze), ize = 1/ze.
 template <int LA, int LB, int LC, int LD, typename R>
-__device__ __forceinline__ void rys_add(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AC)[3], R ze, R sq, R ize,
+__device__ __forceinline__ void rys_add(const PrimV<R>& pp, const PrimV<R>& qq, const R (&AQ)[3], R ze, R sq, R ize,
                                         R pref, const double* __restrict__ tabN, const float* __restrict__ tab32N,
                                         R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF]) {
   using Dm = Dims<LA, LB, LC, LD>;
@@ -335,7 +348,7 @@ __device__ __forceinline__ void rys_add(const PrimV<R>& pp, const PrimV<R>& qq, 
   for (int d = 0; d < 3; ++d) {
     PA[d] = pp.X[d];
     QC[d] = qq.X[d];
-    PQ[d] = AC[d] + pp.X[d] - qq.X[d];
+    PQ[d] = hoist_ket<LA, LB, LC, LD, R>() ? AQ[d] + pp.X[d] : AQ[d] + pp.X[d] - qq.X[d];
     pq2 += PQ[d] * PQ[d];
   }
   const R T = rho * pq2;

@@ -104,15 +104,30 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
   };
   if (UNIFORM_BRA && sbra) {
     const int npp = __builtin_amdgcn_readfirstlane(P.nprim);
-    for (int iq = 0; iq < Q.nprim; ++iq) {
-      const PrimV<R> qq = ket(Q.prim0 + iq);
+    // small classes, FP32: the next ket primitive pair is loaded while the
+    // current one runs against the bra's primitives (where registers are
+    // short, i.e. the larger classes and FP64, it costs more than it hides)
+    constexpr bool PREF = F32 && Dm::NE * Dm::NF <= 9;
+    const int nq = Q.nprim;
+    PrimV<R> qn{};
+    if (PREF && nq > 0) qn = ket(Q.prim0);
+    for (int iq = 0; iq < nq; ++iq) {
+      PrimV<R> qq;
+      if constexpr (PREF) {
+        qq = qn;
+        if (iq + 1 < nq) qn = ket(Q.prim0 + iq + 1);
+      } else {
+        qq = ket(Q.prim0 + iq);
+      }
       if (fabs(qq.Kg) < qcut) continue;
+      R AQ[3];
+      ket_offset<LA, LB, LC, LD, R>(AC, qq, AQ);
       for (int ip = 0; ip < npp; ++ip) {
         PrimV<R> pp;
         if constexpr (F32) pp = sbra32 ? sbra32[ip] : prim_v<float>(sbra[ip]);
         else pp = sbra[ip];
         if (fabs(pp.Kg) < qcut) continue;
-        nprim_used += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AC, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
+        nprim_used += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AQ, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
       }
     }
     return;
@@ -126,10 +141,12 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
   for (int iq = 0; iq < Q.nprim; ++iq) {
     const PrimV<R> qq = ket(Q.prim0 + iq);
     if (fabs(qq.Kg) < qcut) continue;
+    R AQ[3];
+    ket_offset<LA, LB, LC, LD, R>(AC, qq, AQ);
     for (int ip = 0; ip < npp; ++ip) {
       const PrimV<R> pp = bra(pp0 + ip);
       if (fabs(pp.Kg) < qcut) continue;
-      nprim_used += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AC, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
+      nprim_used += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AQ, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
     }
   }
 }

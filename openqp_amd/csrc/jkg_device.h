@@ -79,12 +79,27 @@ __device__ __forceinline__ int contract_group(const ShellPair& Q, const double (
   for (int d = 0; d < 3; ++d) AC[d] = (R)(Ac[d] - Q.A[d]);  // formed in FP64
   const R qcut2 = qcut * qcut;
   int ng = 0;
-  for (int iq = 0; iq < Q.nprim; ++iq) {
+  auto ket = [&](int i) -> PrimV<R> {
+    if constexpr (F32) return prim_v<float>(prims32[i]);
+    else return prim_v<double>(prims[i]);
+  };
+  // small classes, FP32: the next ket primitive pair is loaded while the current one runs
+  constexpr bool PREF = F32 && NE * NF <= 9;
+  const int nq = Q.nprim;
+  PrimV<R> qn{};
+  if (PREF && nq > 0) qn = ket(Q.prim0);
+  for (int iq = 0; iq < nq; ++iq) {
     PrimV<R> qq;
-    if constexpr (F32) qq = prim_v<float>(prims32[Q.prim0 + iq]);
-    else qq = prim_v<double>(prims[Q.prim0 + iq]);
+    if constexpr (PREF) {
+      qq = qn;
+      if (iq + 1 < nq) qn = ket(Q.prim0 + iq + 1);
+    } else {
+      qq = ket(Q.prim0 + iq);
+    }
     const R kq = qq.Kg;
     if (fabs(kq) < qcut) continue;
+    R AQ[3];
+    ket_offset<LA, LB, LC, LD, R>(AC, qq, AQ);
     for (int ip = 0; ip < ncp; ++ip) {
       const R* r = rec + ip * stride;
       PrimV<R> pp;
@@ -98,20 +113,14 @@ __device__ __forceinline__ int contract_group(const ShellPair& Q, const double (
       const R ze = zeta + eta + zeta * eta * imu2;
       const R fq = pp.Kg * kq;  // the largest member prefactor: no member passes below it
       if (fq * fq < qcut2 * ze) continue;
-      R ize, sq;
-      if constexpr (F32) {
-        sq = __builtin_amdgcn_rsqf(ze);
-        ize = __builtin_amdgcn_rcpf(ze);
-      } else {
-        sq = rsq64(ze);
-        ize = sq * sq;
-      }
+      const R sq = F32 ? (R)__builtin_amdgcn_rsqf((float)ze) : (R)rsq64((double)ze);
+      const R ize = sq * sq;
       R val[NE][NF];
 #pragma unroll
       for (int e = 0; e < NE; ++e)
 #pragma unroll
         for (int f = 0; f < NF; ++f) val[e][f] = 0;
-      rys_add<LA, LB, LC, LD, R>(pp, qq, AC, ze, sq, ize, kq * sq, tabN, tab32N, val);
+      rys_add<LA, LB, LC, LD, R>(pp, qq, AQ, ze, sq, ize, kq * sq, tabN, tab32N, val);
       ++ng;
 #pragma unroll
       for (int m = 0; m < NM; ++m) {
