@@ -71,10 +71,16 @@ static void allow_lds(K kern, int bytes) {
   if (bytes > 64 * 1024) hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
+// blocks of a run-time-class launch: 64 / kpb per 64-ket chunk
+static unsigned hl_blocks(const WaveWork& ww) {
+  const int kpb = ww.kpb > 0 ? ww.kpb : 64;
+  return (unsigned)(ww.nwaves * (64 / kpb));
+}
+
 void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStream_t s) {
   if (ww.nwaves == 0) return;
   allow_lds(k_jkh, h.lds);
-  hipLaunchKernelGGL(k_jkh, dim3((unsigned)ww.nwaves), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+  hipLaunchKernelGGL(k_jkh, dim3(hl_blocks(ww)), dim3(HL_BLOCK), h.lds, s, p, ww, h);
 }
 
 void launch_jkh_multi(const JKParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s) {
@@ -100,7 +106,7 @@ void launch_gradh_multi(const GradParams& p, const HLTaskList& tl, long long nbl
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s) {
   if (ww.nwaves == 0) return;
   allow_lds(k_gradh, h.lds);
-  hipLaunchKernelGGL(k_gradh, dim3((unsigned)ww.nwaves), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+  hipLaunchKernelGGL(k_gradh, dim3(hl_blocks(ww)), dim3(HL_BLOCK), h.lds, s, p, ww, h);
 }
 
 }  // namespace oqph

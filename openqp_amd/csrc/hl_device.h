@@ -424,9 +424,13 @@ __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& 
 // screens its kets lane-parallel with OpenQP's tests, then evaluates and
 // digests the surviving quartets one after another, all 64 lanes on each.
 // One 64-ket chunk (wave) of one class's wavefront tasks, by one block.
-__device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& ww, const HLDims& h, long long wave,
+__device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& ww, const HLDims& h, long long blk,
                                           double* lds) {
   const HLTabs tb = hl_tables(h, lds);
+  // block blk takes kets [sub * kpb, (sub + 1) * kpb) of 64-ket chunk `wave`
+  const int kpb = ww.kpb > 0 ? ww.kpb : 64, per = 64 / kpb;
+  const long long wave = blk / per;
+  const int sub = (int)(blk - wave * per);
   if (wave >= ww.nwaves) return;  // block-uniform
   int lo = 0, hi = ww.nseg;
   while (hi - lo > 1) {
@@ -435,8 +439,8 @@ __device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& w
   }
   const int seg = __builtin_amdgcn_readfirstlane(lo);
   const int lane = threadIdx.x;
-  const int k = (int)(wave - ww.choff[seg]) * 64 + lane;
-  const bool active = k < ww.seg_n[seg];
+  const int k = (int)(wave - ww.choff[seg]) * 64 + sub * kpb + lane;
+  const bool active = lane < kpb && k < ww.seg_n[seg];
   const int ip = __builtin_amdgcn_readfirstlane(ww.seg_bra[seg]);
   const ShellPair P = prm.pairs[ip];
   int iq = 0;
@@ -601,6 +605,7 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
   ww.ket = tl.ket + tk.ket0;
   ww.nseg = tk.nseg;
   ww.nwaves = tk.nwaves;
+  ww.kpb = tk.kpb;
   const HLDims h = tk.h;
   jkh_chunk(prm, ww, h, b - tk.wave0, lds);
 }
@@ -634,12 +639,16 @@ __global__ __launch_bounds__(HL_BLOCK) void k_schwarzh(const ShellPair* pairs, c
 // need (T, LDS) and contract them with their Gamma entries; per-atom sums
 // go through LDS.
 // One chunk (wave) of one gradient class's wavefront tasks, by one block.
-__device__ __forceinline__ void gradh_chunk(const GradParams& prm, const WaveWork& ww, const HLDims& h, long long wave,
+__device__ __forceinline__ void gradh_chunk(const GradParams& prm, const WaveWork& ww, const HLDims& h, long long blk,
                                             double* lds) {
   const HLTabs tb = hl_tables(h, lds);
   double* s_de = lds + h.offB;  // 3 natm
   for (int i = threadIdx.x; i < 3 * prm.natm; i += HL_BLOCK) s_de[i] = 0.0;
   __syncthreads();
+  // block blk takes kets [sub * kpb, (sub + 1) * kpb) of 64-ket chunk `wave`
+  const int kpb = ww.kpb > 0 ? ww.kpb : 64, per = 64 / kpb;
+  const long long wave = blk / per;
+  const int sub = (int)(blk - wave * per);
   if (wave >= ww.nwaves) return;  // block-uniform
   int lo = 0, hi = ww.nseg;
   while (hi - lo > 1) {
@@ -648,12 +657,12 @@ __device__ __forceinline__ void gradh_chunk(const GradParams& prm, const WaveWor
   }
   const int seg = __builtin_amdgcn_readfirstlane(lo);
   const int lane = threadIdx.x;
-  const int k = (int)(wave - ww.choff[seg]) * 64 + lane;
-  const bool active = k < ww.seg_n[seg];
+  const int k = (int)(wave - ww.choff[seg]) * 64 + sub * kpb + lane;
+  const bool active = lane < kpb && k < ww.seg_n[seg];
   const int ip = __builtin_amdgcn_readfirstlane(ww.seg_bra[seg]);
   const ShellPair P = prm.pairs[ip];
   const int iq = active ? ww.ket[ww.seg_k0[seg] + k] : 0;
-  unsigned long long* ctr = prm.counters + 8 * (int)(wave & (OQPH_CTR_SLOTS - 1));
+  unsigned long long* ctr = prm.counters + 8 * (int)(blk & (OQPH_CTR_SLOTS - 1));
   unsigned long long todo = __ballot(active);
   double* G = lds + h.offG;
   double* Gam = lds + h.offA;
@@ -840,6 +849,7 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
   ww.ket = tl.ket + tk.ket0;
   ww.nseg = tk.nseg;
   ww.nwaves = tk.nwaves;
+  ww.kpb = tk.kpb;
   const HLDims h = tk.h;
   gradh_chunk(prm, ww, h, b - tk.wave0, lds);
 }

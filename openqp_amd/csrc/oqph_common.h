@@ -69,6 +69,9 @@ struct WaveWork {
   const int* ket;
   int nseg;
   long long nwaves;
+  // run-time-class kernels: kets of a 64-ket chunk per block (64, 32, 16 or
+  // 8; 0 means 64): a launch takes nwaves * 64 / kpb blocks
+  int kpb;
 };
 
 // One quartet class (la lb|lc ld), la >= lb, lc >= ld, for the kernels that
@@ -91,12 +94,13 @@ struct HLDims {
 };
 
 // One class of a multi-class launch of the run-time-class kernels: its
-// layout, the offsets of its wavefront tasks in the launch's lists, and its
-// range of blocks [wave0, wave0 + nwaves).
+// layout, the offsets of its wavefront tasks in the launch's lists, its first
+// block wave0, its 64-ket chunks nwaves (nwaves * 64 / kpb blocks).
 struct HLTask {
   HLDims h;
-  long long wave0, nwaves;
+  long long wave0, nwaves;  // first block, 64-ket chunks
   int seg0, ch0, nseg, ket0;
+  int kpb;                  // kets per block (WaveWork::kpb)
 };
 struct HLTaskList {
   const HLTask* task;

@@ -264,6 +264,10 @@ struct oqph_session {
   int grad_hl_ng = 0;
   // OQPH_JK_HL_NG=n: the same for the J/K build (k_jkh)
   int jk_hl_ng = 0;
+  // OQPH_HL_KPB=64|32|16|8: kets per block of every run-time-class launch
+  // (0, default: chosen per launch by hl_kpb)
+  int hl_kpb = 0;
+  int ncu = 0;  // compute units of the device
   std::vector<WLaunch> launches;
   unsigned long long list_gen = 0;       // bumped whenever the host work lists change
   unsigned long long list_uploaded = 0;  // list_gen of the copy in d_seg / d_choff / d_items
@@ -718,6 +722,8 @@ static int prepare_geometry(oqph_session* s) {
     int v = 0;
     HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
     s->lds_max = (size_t)v;
+    HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, s->device));
+    s->ncu = std::max(v, 1);
   }
   // Rys tables and AO transforms
   size_t nt;
@@ -739,6 +745,9 @@ static int prepare_geometry(oqph_session* s) {
     s->grad_hl_ng = gh ? atoi(gh) : 0;
     const char* jh = getenv("OQPH_JK_HL_NG");
     s->jk_hl_ng = jh ? atoi(jh) : 0;
+    const char* kp = getenv("OQPH_HL_KPB");
+    const int kpv = kp ? atoi(kp) : 0;
+    s->hl_kpb = (kpv == 64 || kpv == 32 || kpv == 16 || kpv == 8) ? kpv : 0;
     const char* e = getenv("OQPH_JK_ROWS");
     s->rows_mode = (e && (e[0] == '0' || e[0] == '2' || e[0] == '3')) ? e[0] - '0' : 1;
     const char* hm = getenv("OQPH_HL_MULTI");
@@ -875,6 +884,21 @@ static int prepare_attenuated(oqph_session* s, double mu) {
     return rc;
   s->att_mu = mu;
   return 0;
+}
+
+// Kets per block of a run-time-class launch (k_jkh, k_gradh and their
+// multi-class forms): their blocks are one wavefront, LDS-limited to per_cu
+// per CU, and walk a chunk's surviving quartets one after another, so a
+// launch of few 64-ket chunks (a small molecule's f or (dd|dd) classes) would
+// leave most of the device idle; such launches split each chunk over 2, 4 or
+// 8 blocks until they have two rounds of resident blocks.
+static int hl_kpb(const oqph_session* s, long long chunks, int lds) {
+  if (s->hl_kpb > 0) return s->hl_kpb;
+  const long long per_cu = std::min<long long>(16, (long long)(s->lds_max / (size_t)std::max(lds, 1)));
+  const long long target = 2 * std::max<long long>(per_cu, 1) * s->ncu;
+  int kpb = 64;
+  while (kpb > 8 && chunks * (64 / kpb) < target) kpb /= 2;
+  return kpb;
 }
 
 // gradient pair data and coarse-screened quartet lists (grd2.F90:219-328):
@@ -1030,6 +1054,7 @@ static int prepare_grad(oqph_session* s) {
                 L.hd.lds, s->lds_max);
         return 1;
       }
+      L.ww.kpb = hl_kpb(s, L.ww.nwaves, L.hd.lds);
     }
     s->glaunches.push_back(L);
   }
@@ -1063,6 +1088,13 @@ static int prepare_grad(oqph_session* s) {
       M.tl.nseg = (int)nseg;
       M.tl.choff = s->d_gchoff.p;
       M.tl.ket = s->d_gcls.p;
+      long long chunks = 0;
+      for (size_t i : t) {
+        chunks += s->glaunches[i].ww.nwaves;
+        M.lds = std::max(M.lds, s->glaunches[i].hd.lds);
+      }
+      const int kpb = hl_kpb(s, chunks, M.lds);
+      M.ww.kpb = kpb;
       long long w0 = 0;
       for (size_t i : t) {
         const auto& L = s->glaunches[i];
@@ -1074,9 +1106,9 @@ static int prepare_grad(oqph_session* s) {
         k.ch0 = (int)(L.ww.choff - s->d_gchoff.p);
         k.nseg = L.ww.nseg;
         k.ket0 = (int)(L.ww.ket - s->d_gcls.p);
+        k.kpb = kpb;
         tasks.push_back(k);
-        w0 += L.ww.nwaves;
-        M.lds = std::max(M.lds, L.hd.lds);
+        w0 += L.ww.nwaves * (64 / kpb);
         drop[i] = 1;
       }
       M.nblocks = w0;
@@ -1101,7 +1133,8 @@ static int prepare_grad(oqph_session* s) {
   std::stable_sort(s->glaunches.begin(), s->glaunches.end(),
                    [](const oqph_session::GLaunch& x, const oqph_session::GLaunch& y) {
                      auto cost = [](const oqph_session::GLaunch& L) {
-                       return (double)(L.multi ? L.nblocks : L.ww.nwaves) *
+                       // (64-ket chunks of the launch)
+                       return (double)(L.multi ? L.nblocks * (L.ww.kpb > 0 ? L.ww.kpb : 64) / 64 : L.ww.nwaves) *
                               (double)flops_prim_host(L.la, L.lb, L.lc, L.ld) * (L.hl ? 4.0 : 1.0);
                      };
                      return cost(x) > cost(y);
@@ -1407,6 +1440,7 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
                 s->lds_max);
         return 1;
       }
+      L.ww.kpb = hl_kpb(s, L.ww.nwaves, L.hd.lds);
     }
     s->launches.push_back(L);
   }
@@ -1443,6 +1477,12 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
       M.task0 = (int)s->h_tasks.size();
       M.ntask = (int)t.size();
       size_t lds = 0;
+      long long chunks = 0;
+      for (size_t i : t) {
+        chunks += s->launches[i].ww.nwaves;
+        lds = std::max(lds, (size_t)s->launches[i].hd.lds);
+      }
+      const int kpb = hl_kpb(s, chunks, (int)lds);
       long long w0 = 0;
       for (size_t i : t) {
         const auto& L = s->launches[i];
@@ -1454,9 +1494,9 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
         k.ch0 = (int)L.ch0;
         k.nseg = L.ww.nseg;
         k.ket0 = (int)s->cls_off[class_of(L.lc, L.ld)];
+        k.kpb = kpb;
         s->h_tasks.push_back(k);
-        w0 += L.ww.nwaves;
-        lds = std::max(lds, (size_t)L.hd.lds);
+        w0 += L.ww.nwaves * (64 / kpb);
         M.slots += L.slots;
         drop[i] = 1;
       }
