@@ -27,7 +27,8 @@ def per_dispatch(path):
 
 
 def bench_name(kernel):
-    m = re.match(r"void oqph::(k_\w+)<(\d), (\d), (\d), (\d), (true|false)>", kernel)
+    # closed-shell instances only: <la, lb, lc, ld, pure[, open-shell pass]>
+    m = re.match(r"void oqph::(k_\w+)<(\d), (\d), (\d), (\d), (true|false)(, false)?>", kernel)
     return f"{m.group(1)}<{m.group(2)},{m.group(3)},{m.group(4)},{m.group(5)}>" if m else None
 
 
