@@ -145,7 +145,8 @@ int oqph_fock_jk_noalloc(oqph_session *s, const double *d_dev, double *f_dev, do
  * class code la*1000+lb*100+lc*10+ld (plus 10000 when the launch used the
  * bra-row kernel k_jkb rather than k_jk, 20000 for the run-time-class kernel
  * k_jkh of the classes with an f shell, 30000 for the grouped bra-row kernel
- * k_jkg), device ms, model FLOPs, computed
+ * k_jkg, 60000 for its tiled form; a multi-class launch of k_jkh reports
+ * 40000 + its number of classes), device ms, model FLOPs, computed
  * quartets, primitive quartets.  Returns the number of launches.
  * No counterpart in the seam: measurement hooks for benchmarks. */
 int oqph_session_set_profiling(oqph_session *s, int on);

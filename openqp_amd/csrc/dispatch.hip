@@ -31,6 +31,9 @@ namespace oqph {
   bool launch_jkg_##a##b##c##d(bool, const JKParams&, const BraWork&, const ColMap&, const ChunkView&, size_t,  \
                                hipStream_t);                                                                \
   int jkg_info_##a##b##c##d(bool, size_t*, size_t);                                                         \
+  bool launch_jkt_##a##b##c##d(bool, const JKParams&, const BraWork&, const ColMap&, const ChunkView&, size_t,  \
+                               hipStream_t);                                                                \
+  int jkt_info_##a##b##c##d(bool, size_t*, size_t);                                                         \
   bool jkb_info_##a##b##c##d(bool, int*, size_t*, size_t);                                                 \
   bool jk_res_##a##b##c##d(int, bool, KernelRes*);
 OQPH_JK_CLASSES(OQPH_DECL_JK)
@@ -83,6 +86,29 @@ bool launch_jkg(int la, int lb, int lc, int ld, bool pure, const JKParams& p, co
   }
 }
 
+bool launch_jkt(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
+                const ChunkView& cv, size_t lds, hipStream_t s) {
+  if (bw.nitem == 0) return true;
+  switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): return launch_jkt_##a##b##c##d(pure, p, bw, cm, cv, lds, s);
+    OQPH_JK_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
+    default: return false;
+  }
+}
+
+// members per tile of the class's tiled bra-row kernel; 0: none
+int jkt_info(int la, int lb, int lc, int ld, bool pure, size_t* static_lds, size_t lds_max) {
+  switch (OQPH_CODE(la, lb, lc, ld)) {
+#define OQPH_CASE(a, b, c, d) \
+  case OQPH_CODE(a, b, c, d): return jkt_info_##a##b##c##d(pure, static_lds, lds_max);
+    OQPH_JK_CLASSES(OQPH_CASE)
+#undef OQPH_CASE
+    default: return 0;
+  }
+}
+
 // members per chunk of the class's grouped bra-row kernel; 0: none
 int jkg_info(int la, int lb, int lc, int ld, bool pure, size_t* static_lds, size_t lds_max) {
   switch (OQPH_CODE(la, lb, lc, ld)) {
@@ -106,12 +132,12 @@ bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* stat
 }
 
 // Compiled resources of one class-templated kernel: kind 0 = k_jk, 1 = k_jkb,
-// 2 = k_grad, 3 = k_schwarz (lc, ld ignored), 4 = k_jkg; false when there is no such
+// 2 = k_grad, 3 = k_schwarz (lc, ld ignored), 4 = k_jkg, 6 = tiled k_jkg; false when there is no such
 // kernel.  hipFuncGetAttributes reads the code object, nothing is launched.
 // kind | 16 (k_jk, k_jkb, k_jkg): the open-shell variant
 bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, KernelRes* r) {
   const int base = kind & 15;
-  if (base == 0 || base == 1 || base == 3 || base == 4) {
+  if (base == 0 || base == 1 || base == 3 || base == 4 || base == 6) {
     if (base == 3) { lc = la; ld = lb; }
     switch (OQPH_CODE(la, lb, lc, ld)) {
 #define OQPH_CASE(a, b, c, d) \

@@ -53,8 +53,21 @@ static const void* jk_fn(int kind, bool pure, bool two) {
       return nullptr;
     } else {
       if (two)
-        return pure ? (const void*)k_jkg<XA, XB, XC, XD, true, true> : (const void*)k_jkg<XA, XB, XC, XD, false, true>;
-      return pure ? (const void*)k_jkg<XA, XB, XC, XD, true, false> : (const void*)k_jkg<XA, XB, XC, XD, false, false>;
+        return pure ? (const void*)k_jkg<XA, XB, XC, XD, true, true, false>
+                    : (const void*)k_jkg<XA, XB, XC, XD, false, true, false>;
+      return pure ? (const void*)k_jkg<XA, XB, XC, XD, true, false, false>
+                  : (const void*)k_jkg<XA, XB, XC, XD, false, false, false>;
+    }
+  }
+  if (kind == 6) {
+    if constexpr (Big<XA, XB, XC, XD>::value || JKT<XA, XB, XC, XD>::nm < 2) {
+      return nullptr;
+    } else {
+      if (two)
+        return pure ? (const void*)k_jkg<XA, XB, XC, XD, true, true, true>
+                    : (const void*)k_jkg<XA, XB, XC, XD, false, true, true>;
+      return pure ? (const void*)k_jkg<XA, XB, XC, XD, true, false, true>
+                  : (const void*)k_jkg<XA, XB, XC, XD, false, false, true>;
     }
   }
   return nullptr;
@@ -149,11 +162,52 @@ bool OQPH_JKG_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWo
     const int bs = JKG<XA, XB, XC, XD>::threads;
     const dim3 g((unsigned)bw.nitem), b(bs);
     if (p.F2) {
-      if (pure) hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true, true>), g, b, lds, s, p, bw, cm, cv);
-      else hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false, true>), g, b, lds, s, p, bw, cm, cv);
+      if (pure) hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true, true, false>), g, b, lds, s, p, bw, cm, cv);
+      else hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false, true, false>), g, b, lds, s, p, bw, cm, cv);
     } else {
-      if (pure) hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true, false>), g, b, lds, s, p, bw, cm, cv);
-      else hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false, false>), g, b, lds, s, p, bw, cm, cv);
+      if (pure) hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true, false, false>), g, b, lds, s, p, bw, cm, cv);
+      else hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false, false, false>), g, b, lds, s, p, bw, cm, cv);
+    }
+    return true;
+  }
+}
+
+#define OQPH_JKT_LAUNCHER_(a, b, c, d) launch_jkt_##a##b##c##d
+#define OQPH_JKT_LAUNCHER(a, b, c, d) OQPH_JKT_LAUNCHER_(a, b, c, d)
+#define OQPH_JKT_INFO_(a, b, c, d) jkt_info_##a##b##c##d
+#define OQPH_JKT_INFO(a, b, c, d) OQPH_JKT_INFO_(a, b, c, d)
+
+// Tiled bra-row kernel of this class (bra pairs sharing their first shell):
+// members per tile (0: none) and static LDS bytes; raises its dynamic-LDS
+// limit to lds_max.
+int OQPH_JKT_INFO(XA, XB, XC, XD)(bool pure, size_t* static_lds, size_t lds_max) {
+  if constexpr (Big<XA, XB, XC, XD>::value || JKT<XA, XB, XC, XD>::nm < 2) {
+    return 0;
+  } else {
+    for (int two = 0; two < 2; ++two) {
+      hipFuncAttributes a;
+      if (hipFuncGetAttributes(&a, jk_fn(6, pure, two != 0)) != hipSuccess) return 0;
+      if (a.localSizeBytes > OQPH_PRIVATE_BUDGET) return 0;
+    }
+    size_t st = 0;
+    if (!allow_dyn_lds(6, pure, lds_max, &st)) return 0;
+    *static_lds = st;
+    return JKT<XA, XB, XC, XD>::nm;
+  }
+}
+
+bool OQPH_JKT_LAUNCHER(XA, XB, XC, XD)(bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
+                                       const ChunkView& cv, size_t lds, hipStream_t s) {
+  if constexpr (Big<XA, XB, XC, XD>::value || JKT<XA, XB, XC, XD>::nm < 2) {
+    return false;
+  } else {
+    const dim3 g((unsigned)bw.nitem), b(JKG<XA, XB, XC, XD>::threads);
+    if (p.F2) {
+      if (pure) hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true, true, true>), g, b, lds, s, p, bw, cm, cv);
+      else hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false, true, true>), g, b, lds, s, p, bw, cm, cv);
+    } else {
+      if (pure) hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, true, false, true>), g, b, lds, s, p, bw, cm, cv);
+      else hipLaunchKernelGGL((k_jkg<XA, XB, XC, XD, false, false, true>), g, b, lds, s, p, bw, cm, cv);
     }
     return true;
   }
@@ -173,12 +227,12 @@ static bool func_res(const void* f, KernelRes* r) {
 }
 
 // Compiled resources of this class's kernels: kind 0 = k_jk, 1 = k_jkb,
-// 3 = k_schwarz (diagonal classes only), 4 = k_jkg; kind | 16: the
-// open-shell variant.
+// 3 = k_schwarz (diagonal classes only), 4 = k_jkg, 6 = tiled k_jkg; kind |
+// 16: the open-shell variant.
 bool OQPH_JK_RES(XA, XB, XC, XD)(int kind, bool pure, KernelRes* r) {
   const bool two = (kind & 16) != 0;
   kind &= 15;
-  if (kind == 0 || kind == 1 || kind == 4) {
+  if (kind == 0 || kind == 1 || kind == 4 || kind == 6) {
     const void* f = jk_fn(kind, pure, two);
     return f && func_res(f, r);
   }

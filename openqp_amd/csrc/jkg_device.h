@@ -1,4 +1,4 @@
-// jkg_device.h -- the grouped bra-row J/K kernel k_jkg.
+// jkg_device.h -- the grouped bra-row J/K kernel k_jkg, and its tiled form.
 //
 // A block takes one chunk: up to JKG::nm shell pairs of one class that share
 // their two centres and every primitive pair's exponents (segmented shells
@@ -11,6 +11,14 @@
 // of int2.F90:763-800 run per member and ket; the digestion, K rows in LDS
 // (the members' distinct shells share rows) and J_ab in registers are those
 // of k_jkb, and J_cd is summed over the members before its one atomic.
+//
+// Tiled form (TILE = true): the members are bra pairs of one class that share
+// their first shell (its centre A), each with its own primitive pairs; the
+// quadrature runs per member, and what the members share is the ket: its
+// pair record and primitive pairs are read once for all of them, and its J_cd
+// takes one global atomic per tile instead of one per bra.  Global FP64
+// atomics bound the J_cd digestion of the bra-row kernel: (H2O)64/cc-pVDZ
+// builds without the J_cd atomics took 352 instead of 425 ms.
 #pragma once
 #include "jk_device.h"
 
@@ -31,6 +39,74 @@ struct JKG {
 #endif
   static constexpr int threads = 512;
 };
+
+// Members per tile of the tiled form: the (e0|f0) accumulators and AO blocks
+// of all members stay in registers.
+template <int LA, int LB, int LC, int LD>
+struct JKT {
+  static constexpr int NEF = Dims<LA, LB, LC, LD>::NE * Dims<LA, LB, LC, LD>::NF;
+  static constexpr int nm = NEF <= 4 ? 4 : (NEF <= 27 ? 2 : 1);
+};
+
+// Contracted (e0|f0) of every live member of a tile against one ket: bp
+// holds the members' primitive pairs (LDS), member m's at [moff[m],
+// moff[m + 1]); np[m] counts the primitive quartets its screens keep.
+template <int LA, int LB, int LC, int LD, int NM, typename R>
+__device__ __forceinline__ void contract_tile(const ShellPair& Q, const double (&Ac)[3],
+                                              const PrimJK* __restrict__ prims, const PrimF32* __restrict__ prims32,
+                                              const PrimV<R>* bp, const int* moff, int nmem, unsigned live,
+                                              const double* __restrict__ tabN, const float* __restrict__ tab32N,
+                                              R qcut, R imu2,
+                                              R (&acc)[NM][Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF],
+                                              int (&np)[NM]) {
+  using Dm = Dims<LA, LB, LC, LD>;
+  constexpr int NE = Dm::NE, NF = Dm::NF;
+  constexpr bool F32 = sizeof(R) == 4;
+#pragma unroll
+  for (int m = 0; m < NM; ++m) {
+    np[m] = 0;
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) acc[m][e][f] = 0;
+  }
+  R AC[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) AC[d] = (R)(Ac[d] - Q.A[d]);  // formed in FP64
+  const R qcut2 = qcut * qcut;
+  auto ket = [&](int i) -> PrimV<R> {
+    if constexpr (F32) return prim_v<float>(prims32[i]);
+    else return prim_v<double>(prims[i]);
+  };
+  // small classes, FP32: the next ket primitive pair is loaded while the current one runs
+  constexpr bool PREF = F32 && NE * NF <= 9;
+  const int nq = Q.nprim;
+  PrimV<R> qn{};
+  if (PREF && nq > 0) qn = ket(Q.prim0);
+  for (int iq = 0; iq < nq; ++iq) {
+    PrimV<R> qq;
+    if constexpr (PREF) {
+      qq = qn;
+      if (iq + 1 < nq) qn = ket(Q.prim0 + iq + 1);
+    } else {
+      qq = ket(Q.prim0 + iq);
+    }
+    if (fabs(qq.Kg) < qcut) continue;
+    R AQ[3];
+    ket_offset<LA, LB, LC, LD, R>(AC, qq, AQ);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      if (m >= nmem) break;  // uniform
+      if (!((live >> m) & 1u)) continue;
+      const int p1 = moff[m + 1];
+      for (int ip = moff[m]; ip < p1; ++ip) {
+        const PrimV<R> pp = bp[ip];
+        if (fabs(pp.Kg) < qcut) continue;
+        np[m] += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AQ, qcut2, imu2, tabN, tab32N, acc[m]) ? 1 : 0;
+      }
+    }
+  }
+}
 
 // Cartesian -> final AO block of one quartet from its contracted (e0|f0):
 // horizontal transfer and AO transforms, as in quartet_block.
@@ -142,10 +218,12 @@ __device__ __forceinline__ int contract_group(const ShellPair& Q, const double (
   return ng;
 }
 
-template <int LA, int LB, int LC, int LD, bool PURE, bool TWO>
+// TILE: the tiled form; its item[0] is the tile's offset in cv.pool (the
+// launch's work lists) and cv.hdr, cv.cp, cv.cp32 are unused.
+template <int LA, int LB, int LC, int LD, bool PURE, bool TWO, bool TILE>
 __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm, ChunkView cv) {
   using Dm = Dims<LA, LB, LC, LD>;
-  constexpr int NM = JKG<LA, LB, LC, LD>::nm;
+  constexpr int NM = TILE ? JKT<LA, LB, LC, LD>::nm : JKG<LA, LB, LC, LD>::nm;
   constexpr int NE = Dm::NE, NF = Dm::NF;
   constexpr int NR = Dm::NROOT;
   constexpr bool T64 = RysLds<NR>::f64;
@@ -159,16 +237,21 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
   __shared__ double s_dab[NM * OA * OB];  // each member's bra density block
   __shared__ double s_mq[NM], s_mthr[NM], s_mdab[NM];
   __shared__ int s_mi[NM][12];  // id, row a, row b, slot si, slot sj, si, sj, sa, sb, ao_a, ao_b
+  __shared__ int s_moff[NM + 1];  // tiled form: each member's first primitive pair in the LDS copy
+  __shared__ double s_mab[NM][3];  // tiled form: each member's A - B
   extern __shared__ double s_dyn[];
   const int* item = bw.item + (size_t)blockIdx.x * (1 + OQPH_NBKT);
   const int chunk = __builtin_amdgcn_readfirstlane(item[0]);
-  const int* ch = cv.pool + cv.hdr[chunk];
+  const int* ch = TILE ? cv.pool + chunk : cv.pool + cv.hdr[chunk];
   const int nmem = __builtin_amdgcn_readfirstlane(ch[0]);
   const int cprim0 = __builtin_amdgcn_readfirstlane(ch[1]);
+  // primitive records in LDS: the chunk's shared records (stride doubles
+  // each), or the tile members' own primitive pairs (ch[2] of them, a
+  // PrimV<double> and a PrimV<float> each)
   const int ncp = __builtin_amdgcn_readfirstlane(ch[2]);
   const int nrow = __builtin_amdgcn_readfirstlane(ch[3]);
   const int nshl = __builtin_amdgcn_readfirstlane(ch[4]);
-  const int stride = __builtin_amdgcn_readfirstlane(ch[5]);
+  const int stride = TILE ? 6 : __builtin_amdgcn_readfirstlane(ch[5]);
   const int* rowao = ch + ch[6];
   const int* shl = ch + ch[7];
   const int ncol = cm.ncol, nk = nrow * ncol, nsh = prm.nsh;
@@ -215,10 +298,23 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
       // host forms it (statistics of skipped quartets only)
       s_mthr[m] = prm.cutoff / (4.0 * prm.maxden * P.Q);
       s_mdab[m] = 4.0 * prm.dsh[P.si * nsh + P.sj];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) s_mab[m][d] = P.A[d] - P.B[d];
     } else {
       s_mi[m][0] = -1;
       s_mq[m] = 0.0;
     }
+  }
+  if constexpr (TILE) {
+    if (threadIdx.x == 0) {
+      int o = 0;
+      for (int m = 0; m < NM; ++m) {
+        s_moff[m] = o;
+        if (m < nmem) o += prm.pairs[ch[OQPH_JKG_HDR + OQPH_JKG_MEM * m]].nprim;
+      }
+      s_moff[NM] = o;
+    }
+    __syncthreads();
   }
   for (int i = threadIdx.x; i < NM * OA * OB; i += blockDim.x) {
     const int m = i / (OA * OB), ab = i - m * (OA * OB);
@@ -242,9 +338,21 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
       s_ds[i] = prm.dsh[shl[sl] * nsh + c];
     }
   }
-  for (int i = threadIdx.x; i < ncp * stride; i += blockDim.x) s_cp[i] = cv.cp[cprim0 + i];
-  if (prm.mixed)
-    for (int i = threadIdx.x; i < ncp * stride; i += blockDim.x) s_cp32[i] = cv.cp32[cprim0 + i];
+  if constexpr (TILE) {
+    PrimV<double>* bp = (PrimV<double>*)s_cp;
+    PrimV<float>* bp32 = (PrimV<float>*)s_cp32;
+    for (int m = 0; m < nmem; ++m) {
+      const ShellPair& P = prm.pairs[ch[OQPH_JKG_HDR + OQPH_JKG_MEM * m]];
+      for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) {
+        bp[s_moff[m] + i] = prim_v<double>(prm.prims[P.prim0 + i]);
+        if (prm.mixed) bp32[s_moff[m] + i] = prim_v<float>(prm.prims32[P.prim0 + i]);
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < ncp * stride; i += blockDim.x) s_cp[i] = cv.cp[cprim0 + i];
+    if (prm.mixed)
+      for (int i = threadIdx.x; i < ncp * stride; i += blockDim.x) s_cp32[i] = cv.cp32[cprim0 + i];
+  }
   __syncthreads();
   const double* tabN = T64 ? (const double*)s_t64 : prm.rys + RYS_OFF[NR];
   const float* tab32N = T32 ? (const float*)s_t32 : prm.rys32 + RYS32_OFF[NR];
@@ -326,9 +434,14 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
         if constexpr (T32) {
           if (lowp) {
             float accf[NM][NE][NF];
-            n_grp += contract_group<LA, LB, LC, LD, NM, float>(Q, Ac, prm.prims, prm.prims32, s_cp32, ncp, stride,
-                                                                nmem, live, tabN, tab32N, (float)(1.0e-4 * c),
-                                                                (float)prm.inv_mu2, accf, np);
+            if constexpr (TILE)
+              contract_tile<LA, LB, LC, LD, NM, float>(Q, Ac, prm.prims, prm.prims32, (const PrimV<float>*)s_cp32,
+                                                       s_moff, nmem, live, tabN, tab32N, (float)(1.0e-4 * c),
+                                                       (float)prm.inv_mu2, accf, np);
+            else
+              n_grp += contract_group<LA, LB, LC, LD, NM, float>(Q, Ac, prm.prims, prm.prims32, s_cp32, ncp, stride,
+                                                                  nmem, live, tabN, tab32N, (float)(1.0e-4 * c),
+                                                                  (float)prm.inv_mu2, accf, np);
             float ABf[3], CDf[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -339,6 +452,9 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
             for (int m = 0; m < NM; ++m) {
               if (m >= nmem) break;  // uniform
               float Vf[NBLK];
+              if constexpr (TILE)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) ABf[d] = (float)s_mab[m][d];
               block_from_acc<LA, LB, LC, LD, PURE, float>(ABf, CDf, prm.xform, accf[m], Vf);
 #pragma unroll
               for (int i = 0; i < NOUT; ++i) Vall[m][i] = (double)Vf[i];
@@ -347,12 +463,20 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
         }
         if (!lowp) {
           double acc[NM][NE][NF];
-          n_grp += contract_group<LA, LB, LC, LD, NM, double>(Q, Ac, prm.prims, prm.prims32, s_cp, ncp, stride,
-                                                               nmem, live, tabN, tab32N, 1.0e-4 * c, prm.inv_mu2,
-                                                               acc, np);
+          if constexpr (TILE)
+            contract_tile<LA, LB, LC, LD, NM, double>(Q, Ac, prm.prims, prm.prims32, (const PrimV<double>*)s_cp,
+                                                      s_moff, nmem, live, tabN, tab32N, 1.0e-4 * c, prm.inv_mu2, acc,
+                                                      np);
+          else
+            n_grp += contract_group<LA, LB, LC, LD, NM, double>(Q, Ac, prm.prims, prm.prims32, s_cp, ncp, stride,
+                                                                 nmem, live, tabN, tab32N, 1.0e-4 * c, prm.inv_mu2,
+                                                                 acc, np);
 #pragma unroll
           for (int m = 0; m < NM; ++m) {
             if (m >= nmem) break;  // uniform
+            if constexpr (TILE)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) AB[d] = s_mab[m][d];
             block_from_acc<LA, LB, LC, LD, PURE, double>(AB, CD, prm.xform, acc[m], Vall[m]);
           }
         }
@@ -512,7 +636,7 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
     unsigned long long v[6] = {n_comp, n_skip, n_f32,
                                (unsigned long long)n_prim * flops_prim<LA, LB, LC, LD>() +
                                    (unsigned long long)n_valid * (two ? 24ull : 12ull) * NOUT,
-                               n_prim, n_grp};
+                               n_prim, TILE ? n_prim : n_grp};
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       unsigned long long t = v[i];

@@ -164,6 +164,12 @@ struct ChunkView {
 };
 #define OQPH_JKG_HDR 8
 #define OQPH_JKG_MEM 5
+// Tiled form of the grouped kernel: bra pairs of one class sharing their
+// first shell, each with its own primitive pairs, at most OQPH_JKT_MAXP of
+// them per tile (the LDS copy).  Its tiles use the chunk header layout above
+// ([1], [5] unused, [2] the members' primitive pairs) and live in the
+// launch's work lists.
+#define OQPH_JKT_MAXP 192
 
 // LDS column space of the K rows of one launch: the AOs of the shells of
 // angular momentum lc (C space) and, when ld != lc, of ld (D space, after C).

@@ -36,6 +36,9 @@ bool jkb_info(int la, int lb, int lc, int ld, bool pure, int* nrow, size_t* stat
 bool launch_jkg(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
                 const ChunkView& cv, size_t lds, hipStream_t s);
 int jkg_info(int la, int lb, int lc, int ld, bool pure, size_t* static_lds, size_t lds_max);
+bool launch_jkt(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const BraWork& bw, const ColMap& cm,
+                const ChunkView& cv, size_t lds, hipStream_t s);
+int jkt_info(int la, int lb, int lc, int ld, bool pure, size_t* static_lds, size_t lds_max);
 bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,
                     const double* tab, const double* xf, double qcut, double imu2, double* Q, hipStream_t s);
 bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, KernelRes* r);
@@ -226,9 +229,11 @@ struct oqph_session {
   std::map<int, std::unique_ptr<ChunkSet>> chunksets;  // key: bra class * 16 + cap
   int jkg_mode = 1;  // OQPH_JKG: 0 = segmented bra-row kernel only, 1 = default class pairs, 2 = all
   int jkg_cap = 0;   // OQPH_JKG_CAP: at most this many members per chunk (0: the kernel's)
+  int jkt_mode = 1;  // OQPH_JKT: 0 = no tiles, 1 = default class pairs, 2 = every class pair with a tiled kernel
   struct WLaunch {
     int la, lb, lc, ld;
     bool grouped = false;  // grouped bra-row kernel (k_jkg; bw, cm, cv, lds)
+    bool tiled = false;    // its tiled form (tiles in the work lists; bw, cm, lds)
     ChunkView cv{};
     bool rows;   // bra-row kernel (bw, cm, lds) instead of wavefront tasks (ww)
     bool hl;     // run-time-class kernel on the wavefront tasks (f shells), layout hd
@@ -676,6 +681,21 @@ static int jkg_cap_for(const oqph_session* s, int la, int lb, int lc, int ld) {
 
 // Chunk sets of every (bra class, cap) a launch can use, built with the
 // geometry so that builds never allocate them.
+// Members per tile of a (bra class, ket class) launch of the tiled kernel; 0
+// when the launch keeps the segmented bra-row kernel for its single bras.  By
+// default only (ps|ps) is tiled: there the J_cd atomics it saves (three per
+// quartet) outweigh its larger register file ((H2O)64/cc-pVDZ: 58 -> 38 ms
+// mixed, 61 -> 51 ms FP64); (ps|ss), (pp|ss), (ds|ss), (pp|ps) and (dp|ss) ran
+// 10-130 % slower tiled, (ds|ps) 8 % faster mixed but 18 % slower in FP64.
+// OQPH_JKT=2 tiles every class pair that has a tiled kernel.
+static int jkt_cap_for(const oqph_session* s, int la, int lb, int lc, int ld) {
+  if (!s->jkt_mode) return 0;
+  if (s->jkt_mode == 1 && !(la == 1 && lb == 0 && lc == 1 && ld == 0)) return 0;
+  size_t st = 0;
+  const int nm = jkt_info(la, lb, lc, ld, s->pure, &st, s->lds_max);
+  return nm >= 2 ? nm : 0;
+}
+
 static int prepare_chunks(oqph_session* s, const std::vector<PrimPair>& prims) {
   s->chunksets.clear();
   if (!s->jkg_mode) return 0;
@@ -754,6 +774,8 @@ static int prepare_geometry(oqph_session* s) {
     s->hl_multi = (hm && hm[0] == '0') ? 0 : 1;
     const char* g = getenv("OQPH_JKG");
     s->jkg_mode = (g && (g[0] == '0' || g[0] == '2')) ? g[0] - '0' : 1;
+    const char* jt = getenv("OQPH_JKT");
+    s->jkt_mode = (jt && (jt[0] == '0' || jt[0] == '2')) ? jt[0] - '0' : 1;
     const char* gc = getenv("OQPH_JKG_CAP");
     s->jkg_cap = gc ? atoi(gc) : 0;
   }
@@ -1186,6 +1208,7 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
     bool drows = false;
     bool hl = false;  // run-time-class kernel (k_jkh)
     bool grouped = false;  // grouped bra-row kernel (k_jkg) on chunk set cv
+    bool tiled = false;    // tiled form of k_jkg (tiles in h_items)
     ChunkView cv{};
     long long slots = 0;
   };
@@ -1347,6 +1370,112 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
         }
         // longest blocks first
         std::stable_sort(its.begin(), its.end(), [](const It& x, const It& y) { return x.cost > y.cost; });
+        // tiled form: the bras that share their first shell, in tiles of up
+        // to tcap (consecutive in cost order, so their ket prefixes are alike)
+        const int tcap = jkt_cap_for(s, la, lb, lc, ld);
+        if (tcap >= 2 && !its.empty()) {
+          size_t st = 0;
+          jkt_info(la, lb, lc, ld, s->pure, &st, s->lds_max);
+          std::map<int, std::vector<int>> bysa;  // first shell -> positions in its (cost order)
+          std::vector<int> saorder;
+          for (size_t i = 0; i < its.size(); ++i) {
+            const int sa = s->pairs[s->cls[cb][its[i].p]].sa;
+            auto it = bysa.find(sa);
+            if (it == bysa.end()) {
+              saorder.push_back(sa);
+              bysa[sa] = {(int)i};
+            } else {
+              it->second.push_back((int)i);
+            }
+          }
+          struct Tile {
+            std::vector<int> mem;  // positions in its
+            long long cost = 0;
+            int nprim = 0;
+          };
+          std::vector<Tile> tiles;
+          bool ok = true;
+          for (int sa : saorder) {
+            Tile t;
+            for (int i : bysa[sa]) {
+              const int np = s->pairs[s->cls[cb][its[i].p]].nprim;
+              if (np > OQPH_JKT_MAXP) ok = false;
+              if (!t.mem.empty() && ((int)t.mem.size() == tcap || t.nprim + np > OQPH_JKT_MAXP)) {
+                tiles.push_back(t);
+                t = Tile{};
+              }
+              t.mem.push_back(i);
+              t.cost += its[i].cost;
+              t.nprim += np;
+            }
+            if (!t.mem.empty()) tiles.push_back(t);
+          }
+          std::stable_sort(tiles.begin(), tiles.end(), [](const Tile& x, const Tile& y) { return x.cost > y.cost; });
+          // headers (chunk layout) into the work lists, then the items
+          std::vector<int> hoff;
+          const size_t h0 = s->h_items.size();
+          size_t need_d = 0, need_n = 0;
+          const size_t nd2 = two ? 2 : 1;
+          for (const Tile& t : tiles) {
+            std::vector<int> shl, row0;
+            int nrow = 0;
+            auto slot = [&](int sh) {
+              for (size_t u = 0; u < shl.size(); ++u)
+                if (shl[u] == sh) return (int)u;
+              shl.push_back(sh);
+              row0.push_back(nrow);
+              nrow += s->nao[sh];
+              return (int)shl.size() - 1;
+            };
+            std::vector<int> ent;
+            for (int i : t.mem) {
+              const int p = its[i].p;
+              const ShellPair& sp = s->pairs[s->cls[cb][p]];
+              const int ta = slot(sp.sa), tb = slot(sp.sb);
+              const int ti = slot(sp.si), tj = slot(sp.sj);
+              ent.insert(ent.end(), {(int)s->cls_off[cb] + p, row0[ta], row0[tb], ti, tj});
+            }
+            const int nm = (int)t.mem.size();
+            const int hdr_len = OQPH_JKG_HDR + OQPH_JKG_MEM * nm;
+            hoff.push_back((int)s->h_items.size());
+            s->h_items.insert(s->h_items.end(), {nm, 0, t.nprim, nrow, (int)shl.size(), 0, hdr_len, hdr_len + nrow});
+            s->h_items.insert(s->h_items.end(), ent.begin(), ent.end());
+            for (size_t u = 0; u < shl.size(); ++u)
+              for (int k = 0; k < s->nao[shl[u]]; ++k) s->h_items.push_back(s->aooff[shl[u]] + k);
+            s->h_items.insert(s->h_items.end(), shl.begin(), shl.end());
+            const size_t rows_b = (size_t)nrow * ncol * sizeof(double);
+            const size_t prims_b = (size_t)t.nprim * 6 * (sizeof(double) + sizeof(float));  // PrimV<double> + PrimV<float>
+            need_n = std::max(need_n, nd2 * rows_b + prims_b);
+            need_d = std::max(need_d, 2 * nd2 * rows_b + shl.size() * (size_t)s->nsh * sizeof(double) + prims_b);
+          }
+          const bool gd = s->rows_mode != 3 && st + need_d <= s->lds_max;
+          if (ok && (gd || st + need_n <= s->lds_max)) {
+            const size_t i0 = s->h_items.size();
+            long long slots = 0;
+            for (size_t ti = 0; ti < tiles.size(); ++ti) {
+              s->h_items.push_back(hoff[ti]);
+              for (int b = 0; b < NBKT; ++b) {
+                int m = 0;
+                for (int i : tiles[ti].mem) m = std::max(m, cnt[(size_t)its[i].p * NBKT + b]);
+                s->h_items.push_back(m);
+                slots += 64ll * ((m + 63) / 64);
+              }
+            }
+            Pend q{la, lb, lc, ld, ck, 0, 0, 0, 0};
+            q.rows = true;
+            q.tiled = true;
+            q.item0 = i0;
+            q.nitem = (int)tiles.size();
+            q.ncolC = ncC;
+            q.ncol = ncol;
+            q.lds = gd ? need_d : need_n;
+            q.drows = gd;
+            q.slots = slots;
+            pend.push_back(q);
+            continue;
+          }
+          s->h_items.resize(h0);  // does not fit: the segmented bra-row kernel below
+        }
         const size_t i0 = s->h_items.size();
         long long slots = 0;
         for (const It& t : its) {
@@ -1409,6 +1538,7 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
     L.la = p.la; L.lb = p.lb; L.lc = p.lc; L.ld = p.ld;
     L.rows = p.rows;
     L.grouped = p.grouped;
+    L.tiled = p.tiled;
     L.cv = p.cv;
     L.slots = p.slots;
     L.seg0 = p.seg0;
@@ -1579,6 +1709,7 @@ static void bind_lists(const oqph_session* s, oqph_session::WLaunch& L, const in
   }
   if (L.rows) {
     L.bw.item = items + L.item0;
+    if (L.tiled) L.cv.pool = items;  // the tiles' headers live in the same lists
     return;
   }
   L.ww.seg_bra = seg + L.seg0;
@@ -1744,6 +1875,7 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     if (L.multi) launch_jkh_multi(p, L.tl, L.nblocks, (int)L.lds, ls);
     else if (L.hl) launch_jkh(L.hd, p, L.ww, ls);
     else if (L.grouped) ok = launch_jkg(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.cv, L.lds, ls);
+    else if (L.tiled) ok = launch_jkt(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.cv, L.lds, ls);
     else if (L.rows) ok = launch_jkb(L.la, L.lb, L.lc, L.ld, s->pure, p, L.bw, L.cm, L.lds, ls);
     else ok = launch_jk(L.la, L.lb, L.lc, L.ld, s->pure, p, L.ww, ls);
     if (!ok) {
@@ -1972,7 +2104,8 @@ int oqph_last_profile(oqph_session* s, int maxn, int* cls, double* ms, double* f
     const unsigned long long* c = i * 8 + 8 <= (int)s->prof_ctr.size() ? &s->prof_ctr[8 * i] : nullptr;
     // a multi-class launch reports 40000 + its number of classes
     if (cls) cls[i] = L.multi ? 40000 + L.ntask
-                              : (L.hl ? 20000 : L.grouped ? 30000 : L.rows ? 10000 : 0) + L.la * 1000 + L.lb * 100 +
+                              : (L.tiled ? 60000 : L.hl ? 20000 : L.grouped ? 30000 : L.rows ? 10000 : 0) +
+                                    L.la * 1000 + L.lb * 100 +
                                     L.lc * 10 + L.ld;
     if (ms) ms[i] = i < (int)s->prof_ms.size() ? s->prof_ms[i] : 0.0;
     if (flops) flops[i] = c ? (double)c[3] : 0.0;

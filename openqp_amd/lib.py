@@ -152,18 +152,19 @@ class FockBuilder:
         """Per-launch (class code, ms, model FLOPs, quartets, primitive quartets) of the last build.
 
         cls = la*1000 + lb*100 + lc*10 + ld; kernel is "k_jkb" (bra-row kernel), "k_jkg"
-        (grouped bra-row kernel), "k_jk" (wavefront tasks) or "k_jkh" (run-time class,
+        (grouped bra-row kernel), "k_jkt" (its tiled form: bras sharing their first
+        shell), "k_jk" (wavefront tasks) or "k_jkh" (run-time class,
         the classes with an f shell); a launch of several run-time classes has
         kernel "k_jkh_multi" and cls 40000 + its number of classes."""
         n = 64
         cls = np.zeros(n, np.int32); ms = np.zeros(n); fl = np.zeros(n)
         q = np.zeros(n, np.int64); pq = np.zeros(n, np.int64)
         m = load().oqph_last_profile(self._s, n, cls, ms, fl, q, pq)
-        kern = lambda c: ("k_jkh_multi" if c >= 40000 else "k_jkg" if c >= 30000 else "k_jkh" if c >= 20000
-                          else "k_jkb" if c >= 10000 else "k_jk")
+        kern = lambda c: ("k_jkt" if c >= 60000 else "k_jkh_multi" if c >= 40000 else "k_jkg" if c >= 30000
+                          else "k_jkh" if c >= 20000 else "k_jkb" if c >= 10000 else "k_jk")
         ctr = np.zeros(8 * n, np.uint64)
         load().oqph_last_counters(self._s, n, ctr)
-        return [dict(cls=int(cls[i]) if cls[i] >= 40000 else int(cls[i]) % 10000, kernel=kern(int(cls[i])),
+        return [dict(cls=int(cls[i]) if 40000 <= cls[i] < 50000 else int(cls[i]) % 10000, kernel=kern(int(cls[i])),
                      ms=float(ms[i]),
                      flops=float(fl[i]), quartets=int(q[i]), prims=int(pq[i]), prims_eval=int(ctr[8 * i + 5]),
                      screened=int(ctr[8 * i + 1]), slots=int(ctr[8 * i + 6]))

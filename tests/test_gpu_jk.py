@@ -291,3 +291,54 @@ def test_uhf_equal_spins_is_rhf(L, oracle):
     assert np.abs(fa - g).max() <= 1e-12
     assert np.abs(fbb - g).max() <= 1e-12
     fb.close()
+
+
+@pytest.mark.parametrize("basis,nw,nocc", [("cc-pvdz", 3, 15), ("6-31g(d)", 4, 20)])
+def test_tiled_kernel_modes(L, oracle, basis, nw, nocc):
+    """The tiled bra-row kernel (bras sharing their first shell: one ket read
+    and one J_cd atomic per tile) against the oracle and against the
+    segmented kernels: OQPH_JKT=0 (off), 1 (default class pairs), 2 (every
+    class pair with a tiled kernel).  Same G to 1e-12, same computed and
+    screened counts; the open-shell pass and mixed precision as well."""
+    from openqp_amd.autograd import packed_trace
+    _say("tiled kernel modes", basis, nw)
+    bas = small_system(basis, nw=nw, seed=17)
+    D, dp = density_for(oracle, bas, nocc, seed=23)
+    fo, so = oracle.fock_jk(bas, dp)
+    da, db = _spin_densities(oracle, bas, nocc, nocc - 1, seeds=(31, 32))
+    Q = oracle.schwarz(bas)
+    ra, rb, su = oracle.fock_jk_uhf(bas, da, db, se=1.0, sc=1.0, cutoff=5e-11, Q=Q)
+    out, kinds = {}, {}
+    old = os.environ.get("OQPH_JKT")
+    try:
+        for mode in ("0", "1", "2"):
+            os.environ["OQPH_JKT"] = mode
+            fb = L.FockBuilder(bas)
+            f = fb.fock_jk(dp)
+            st = (fb.last_stats.quartets_computed, fb.last_stats.quartets_screened)
+            kinds[mode] = {p["kernel"] for p in fb.last_profile()}
+            fa, fbb = fb.fock_jk_uhf(da, db)
+            nu = fb.last_stats.quartets_computed
+            fb.set_precision(1, 1e-4)
+            fm = fb.fock_jk(dp)
+            out[mode] = (f, st, fa, fbb, nu, fm)
+            fb.close()
+    finally:
+        if old is None:
+            os.environ.pop("OQPH_JKT", None)
+        else:
+            os.environ["OQPH_JKT"] = old
+    for mode in ("0", "1", "2"):
+        f, st, fa, fbb, nu, fm = out[mode]
+        err = np.abs(f - fo).max()
+        eu = max(np.abs(fa - ra).max(), np.abs(fbb - rb).max())
+        dE = abs(packed_trace(dp, fm, bas.nbf) - packed_trace(dp, fo, bas.nbf))
+        _say("  OQPH_JKT", mode, "max |f - f_oracle| =", err, "UHF", eu, "mixed dE2", dE, "kernels", sorted(kinds[mode]))
+        assert err <= TOL and eu <= TOL
+        assert st[0] == int(so[1]) and nu == int(su[1])
+        assert dE <= 1e-6
+    for mode in ("1", "2"):
+        assert np.abs(out["0"][0] - out[mode][0]).max() <= 1e-12
+        assert out["0"][1] == out[mode][1]
+    assert "k_jkt" not in kinds["0"]
+    assert "k_jkt" in kinds["2"]
