@@ -320,6 +320,33 @@ void launch_sumdiff(const double* a, const double* b, double* p, double* m, doub
   hipLaunchKernelGGL(k_sumdiff, dim3(1024), dim3(256), 0, s, a, b, p, m, c, n);
 }
 
+// F += the ket-side Coulomb blocks: pair i's block (its AO block, a-major,
+// entries [joff[i], joff[i + 1])) into the packed matrix, diagonal entries
+// doubled as the J/K digestion doubles them.  The two entries (x, y) and
+// (y, x) of a diagonal shell pair land on one packed element, hence atomics.
+__global__ void k_jscatter(const ShellPair* pairs, const int* joff, int npair, int pure, const double* Jk, double* F) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npair) return;
+  const ShellPair P = pairs[i];
+  const int na = (pure && P.la >= 2) ? 2 * P.la + 1 : (P.la + 1) * (P.la + 2) / 2;
+  const int nb = (pure && P.lb >= 2) ? 2 * P.lb + 1 : (P.lb + 1) * (P.lb + 2) / 2;
+  const double* J = Jk + joff[i];
+  for (int a = 0; a < na; ++a)
+    for (int b = 0; b < nb; ++b) {
+      const double v = J[a * nb + b];
+      if (v == 0.0) continue;
+      const int x = P.ao_a + a, y = P.ao_b + b;
+      unsafeAtomicAdd(F + tri_idx_(x, y), x == y ? 2.0 * v : v);
+    }
+}
+
+void launch_jscatter(const ShellPair* pairs, const int* joff, int npair, bool pure, const double* Jk, double* F,
+                     hipStream_t s) {
+  if (npair <= 0) return;
+  hipLaunchKernelGGL(k_jscatter, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, s, pairs, joff, npair,
+                     pure ? 1 : 0, Jk, F);
+}
+
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s) {
   hipLaunchKernelGGL(k_max_reduce, dim3(256), dim3(256), 0, s, x, n, out);
 }

@@ -440,14 +440,16 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
       const double v = cj * jab * (x == y ? 2.0 : 1.0);
       if (v != 0.0) fatomic(F + tri_idx(x, y), v);
     });
+    {  // J_cd into the ket's block of the ket-side Coulomb buffer
+      double* Jd = prm.Jk + ww.jbase + (long long)(iq - ww.kofs) * (OC * OD);
 #pragma unroll
-    for (int c = 0; c < OC; ++c)
+      for (int c = 0; c < OC; ++c)
 #pragma unroll
-      for (int d = 0; d < OD; ++d) {
-        const int x = oc + c, y = od + d;
-        const double v = cj * Jcd[c * OD + d] * (x == y ? 2.0 : 1.0);
-        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
-      }
+        for (int d = 0; d < OD; ++d) {
+          const double v = cj * Jcd[c * OD + d];
+          if (v != 0.0) fatomic(Jd + c * OD + d, v);
+        }
+    }
 #pragma unroll
     for (int a = 0; a < OA; ++a) {
 #pragma unroll
@@ -627,6 +629,8 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
           fatomic(F + tri_idx(x, y), v);
         }
       }
+    // J_cd into the ket's block of the ket-side Coulomb buffer
+    double* Jd = prm.Jk + ww.jbase + (long long)(iq - ww.kofs) * (OC * OD);
 #pragma unroll
     for (int cc = 0; cc < OC; ++cc)
 #pragma unroll
@@ -634,9 +638,8 @@ __global__ __launch_bounds__(JK_BLOCK) void k_jk(JKParams prm, WaveWork ww) {
         double s = 0.0;
 #pragma unroll
         for (int ab = 0; ab < OA * OB; ++ab) s += V[ab * OC * OD + cc * OD + d] * Dab[ab];
-        const int x = oc + cc, y = od + d;
-        const double v = cj * s * (x == y ? 2.0 : 1.0);
-        if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+        const double v = cj * s;
+        if (v != 0.0) fatomic(Jd + cc * OD + d, v);
       }
   }
   // K blocks: K_ac += V D_bd, K_ad += V D_bc, K_bc += V D_ad, K_bd += V D_ac,
@@ -921,6 +924,8 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
           for (int cd = 0; cd < OC * OD; ++cd) t += V[ab * OC * OD + cd] * Dcd[cd];
           jab[ab] += cj * t;
         }
+        // J_cd into the ket's block of the ket-side Coulomb buffer
+        double* Jd = prm.Jk + bw.jbase + (long long)(iq - bw.kofs) * (OC * OD);
 #pragma unroll
         for (int cc = 0; cc < OC; ++cc)
 #pragma unroll
@@ -928,9 +933,8 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
             double t = 0.0;
 #pragma unroll
             for (int ab = 0; ab < OA * OB; ++ab) t += V[ab * OC * OD + cc * OD + d] * Dab[ab];
-            const int x = oc + cc, y = od + d;
-            const double v = cj * t * (x == y ? 2.0 : 1.0);
-            if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+            const double v = cj * t;
+            if (v != 0.0) fatomic(Jd + cc * OD + d, v);
           }
       }
       // K rows a (0..OA-1) and b (OA..OA+OB-1) in LDS; the density in those

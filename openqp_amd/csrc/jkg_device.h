@@ -16,9 +16,7 @@
 // their first shell (its centre A), each with its own primitive pairs; the
 // quadrature runs per member, and what the members share is the ket: its
 // pair record and primitive pairs are read once for all of them, and its J_cd
-// takes one global atomic per tile instead of one per bra.  Global FP64
-// atomics bound the J_cd digestion of the bra-row kernel: (H2O)64/cc-pVDZ
-// builds without the J_cd atomics took 352 instead of 425 ms.
+// takes one atomic per tile instead of one per bra.
 #pragma once
 #include "jk_device.h"
 
@@ -619,14 +617,15 @@ __global__ __launch_bounds__(512) void k_jkg(JKParams prm, BraWork bw, ColMap cm
         }
 #undef VV
       }
-      // J_cd of all members: one atomic per element
+      // J_cd of all members: one atomic per element, into the ket's block of
+      // the ket-side Coulomb buffer
+      double* Jd = prm.Jk + bw.jbase + (long long)(iq - bw.kofs) * (OC * OD);
 #pragma unroll
       for (int cc = 0; cc < OC; ++cc)
 #pragma unroll
         for (int d = 0; d < OD; ++d) {
-          const int x = oc + cc, y = od + d;
-          const double v = tcd[cc * OD + d] * (x == y ? 2.0 : 1.0);
-          if (v != 0.0) fatomic(F + tri_idx(x, y), v);
+          const double v = tcd[cc * OD + d];
+          if (v != 0.0) fatomic(Jd + cc * OD + d, v);
         }
     }
   }

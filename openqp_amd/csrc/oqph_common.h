@@ -72,6 +72,10 @@ struct WaveWork {
   // run-time-class kernels: kets of a 64-ket chunk per block (64, 32, 16 or
   // 8; 0 means 64): a launch takes nwaves * 64 / kpb blocks
   int kpb;
+  // J/K class kernels: the ket class's block of the ket-side Coulomb buffer
+  // (JKParams::Jk) starts at jbase; ket pair id iq owns the OC*OD entries
+  // from jbase + (iq - kofs) * OC*OD
+  int jbase, kofs;
 };
 
 // One quartet class (la lb|lc ld), la >= lb, lc >= ld, for the kernels that
@@ -136,6 +140,7 @@ struct BraWork {
   const int* ket;
   int kb[OQPH_NBKT + 1];
   int nitem;
+  int jbase, kofs;  // as WaveWork
 };
 
 // Grouped bra-row work (k_jkg).  Segmented shells contracted from one set of
@@ -200,6 +205,12 @@ struct JKParams {
   const double* xform;  // [2][2][10][10] canonical Cartesian -> AO transforms (L = 2, 3)
   const double* D;      // packed density (ntri)
   double* F;            // packed G (ntri), accumulated with atomics
+  // Ket-side Coulomb buffer: the class kernels add each quartet's J_cd into
+  // the ket pair's own block (pairs in class-list order, so the kets of a
+  // wavefront hit consecutive addresses and their atomics coalesce), and one
+  // pass after the launches adds the blocks into F (with the diagonal
+  // doubling).  Random FP64 atomics into F were the J_cd bottleneck.
+  double* Jk;
   const double* dsh;    // nsh*nsh shell-pair max |D|
   unsigned long long* counters;  // [0] computed quartets, [1] skipped by fine screen, [2] fp32 quartets
   int nsh;

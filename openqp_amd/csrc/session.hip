@@ -45,6 +45,8 @@ bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, Kerne
 void launch_shell_density(const double* D, const double* D2, const int* ao_off, const int* nao, int nsh, double* dsh,
                           hipStream_t s);
 void launch_sumdiff(const double* a, const double* b, double* p, double* m, double c, long long n, hipStream_t s);
+void launch_jscatter(const ShellPair* pairs, const int* joff, int npair, bool pure, const double* Jk, double* F,
+                     hipStream_t s);
 void launch_max_reduce(const double* x, long long n, unsigned long long* out, hipStream_t s);
 bool kernels_support(int lmax);
 bool jk_template(int la, int lb, int lc, int ld);
@@ -192,6 +194,11 @@ struct oqph_session {
   DBuf<double> d_rys, d_xform;
   DBuf<float> d_rys32;
   DBuf<int> d_cls;                           // concatenated class lists
+  // ket-side Coulomb buffer (JKParams::Jk): device pair i owns the entries
+  // [joff[i], joff[i + 1]) (its AO block); class c's block starts at jbase[c]
+  std::vector<int> h_joff, jbase;
+  DBuf<int> d_joff;
+  DBuf<double> d_jk;
   DBuf<int> d_kbs;                           // per class: bucket starts (NBKT+1)
   std::vector<std::vector<int>> kbs;
   std::vector<size_t> cls_off;
@@ -229,7 +236,7 @@ struct oqph_session {
   std::map<int, std::unique_ptr<ChunkSet>> chunksets;  // key: bra class * 16 + cap
   int jkg_mode = 1;  // OQPH_JKG: 0 = segmented bra-row kernel only, 1 = default class pairs, 2 = all
   int jkg_cap = 0;   // OQPH_JKG_CAP: at most this many members per chunk (0: the kernel's)
-  int jkt_mode = 1;  // OQPH_JKT: 0 = no tiles, 1 = default class pairs, 2 = every class pair with a tiled kernel
+  int jkt_mode = 0;  // OQPH_JKT: 0 (default) = no tiles, 1 = (ps|ps), 2 = every class pair with a tiled kernel
   struct WLaunch {
     int la, lb, lc, ld;
     bool grouped = false;  // grouped bra-row kernel (k_jkg; bw, cm, cv, lds)
@@ -682,12 +689,12 @@ static int jkg_cap_for(const oqph_session* s, int la, int lb, int lc, int ld) {
 // Chunk sets of every (bra class, cap) a launch can use, built with the
 // geometry so that builds never allocate them.
 // Members per tile of a (bra class, ket class) launch of the tiled kernel; 0
-// when the launch keeps the segmented bra-row kernel for its single bras.  By
-// default only (ps|ps) is tiled: there the J_cd atomics it saves (three per
-// quartet) outweigh its larger register file ((H2O)64/cc-pVDZ: 58 -> 38 ms
-// mixed, 61 -> 51 ms FP64); (ps|ss), (pp|ss), (ds|ss), (pp|ps) and (dp|ss) ran
-// 10-130 % slower tiled, (ds|ps) 8 % faster mixed but 18 % slower in FP64.
-// OQPH_JKT=2 tiles every class pair that has a tiled kernel.
+// when the launch keeps the segmented bra-row kernel for its single bras.
+// Tiles are off by default: they were written to share each ket's J_cd atomic
+// among the bras of a tile ((ps|ps) on (H2O)64/cc-pVDZ 58 -> 38 ms mixed), but
+// the ket-side Coulomb buffer removed that cost for every class ((ps|ps) 28
+// ms untiled, 29 ms tiled; FP64 36 vs 47 ms).  OQPH_JKT=1 tiles (ps|ps),
+// OQPH_JKT=2 every class pair that has a tiled kernel.
 static int jkt_cap_for(const oqph_session* s, int la, int lb, int lc, int ld) {
   if (!s->jkt_mode) return 0;
   if (s->jkt_mode == 1 && !(la == 1 && lb == 0 && lc == 1 && ld == 0)) return 0;
@@ -775,7 +782,7 @@ static int prepare_geometry(oqph_session* s) {
     const char* g = getenv("OQPH_JKG");
     s->jkg_mode = (g && (g[0] == '0' || g[0] == '2')) ? g[0] - '0' : 1;
     const char* jt = getenv("OQPH_JKT");
-    s->jkt_mode = (jt && (jt[0] == '0' || jt[0] == '2')) ? jt[0] - '0' : 1;
+    s->jkt_mode = (jt && (jt[0] == '1' || jt[0] == '2')) ? jt[0] - '0' : 0;
     const char* gc = getenv("OQPH_JKG_CAP");
     s->jkg_cap = gc ? atoi(gc) : 0;
   }
@@ -835,6 +842,14 @@ static int prepare_geometry(oqph_session* s) {
       }
     }
     s->h_dpairs = dp;
+    s->h_joff.assign(dp.size() + 1, 0);
+    for (size_t i = 0; i < dp.size(); ++i)
+      s->h_joff[i + 1] = s->h_joff[i] + nao_of(dp[i].la, s->pure) * nao_of(dp[i].lb, s->pure);
+    s->jbase.assign(11, 0);
+    for (int c = 0; c <= 10; ++c) s->jbase[c] = s->h_joff[s->cls_off[c]];
+    if ((rc = s->d_joff.ensure(s->h_joff.size())) ||
+        (rc = upload_sync(s->d_joff.p, s->h_joff.data(), s->h_joff.size() * sizeof(int), s->stream)))
+      return rc;
     std::vector<int> ident(all.size());
     for (size_t i = 0; i < all.size(); ++i) ident[i] = (int)i;
     if ((rc = s->d_pairs.ensure(std::max<size_t>(dp.size(), 1))) ||
@@ -1546,6 +1561,8 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
     L.item0 = p.item0;
     if (p.rows) {
       L.bw.ket = s->d_cls.p + s->cls_off[p.ck];
+      L.bw.jbase = s->jbase[p.ck];
+      L.bw.kofs = (int)s->cls_off[p.ck];
       for (int b = 0; b <= NBKT; ++b) L.bw.kb[b] = s->kbs[p.ck][b];
       L.bw.nitem = p.nitem;
       L.cm.colidx = s->d_colidx.p;
@@ -1560,6 +1577,8 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
       continue;
     }
     L.ww.ket = s->d_cls.p + s->cls_off[p.ck];
+    L.ww.jbase = s->jbase[p.ck];
+    L.ww.kofs = (int)s->cls_off[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
     L.hl = p.hl;
@@ -1805,6 +1824,8 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   long long* lchoff;
   int* litems;
   HLTask* ltasks;
+  const size_t njk = std::max<size_t>(s->h_joff.empty() ? 0 : (size_t)s->h_joff.back(), 1);
+  double* Jk = nullptr;
   if (ss) {
     const ListSizes n = list_sizes(s);
     ctr = (unsigned long long*)carve(nc * std::max<size_t>(nl, 1) * sizeof(unsigned long long));
@@ -1812,9 +1833,11 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
     lchoff = (long long*)carve(n.choff * sizeof(long long));
     litems = (int*)carve(n.items * sizeof(int));
     ltasks = (HLTask*)carve(n.tasks * sizeof(HLTask));
+    Jk = (double*)carve(njk * sizeof(double));
     if (off > ss->bytes) {
       set_err("no-allocation build: this basis and density need %zu bytes of scratch (shell-pair maxima, "
-              "counters and work lists); the static scratch holds %zu", off, ss->bytes);
+              "counters, work lists and the ket-side Coulomb buffer); the static scratch holds %zu", off,
+              ss->bytes);
       return 1;
     }
     if (*ss->serial != s->serial || *ss->gen != s->list_gen) {
@@ -1825,6 +1848,8 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   } else {
     if ((rc = session_lists(s, stream))) return rc;
     if ((rc = s->d_ctr.ensure(8 + nc * nl))) return rc;
+    if ((rc = s->d_jk.ensure(njk))) return rc;
+    Jk = s->d_jk.p;
     ctr = s->d_ctr.p + 8;
     lseg = s->d_seg.p;
     lchoff = s->d_choff.p;
@@ -1850,6 +1875,8 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   p.rys = s->d_rys.p;
   p.rys32 = s->d_rys32.p;
   p.xform = s->d_xform.p;
+  HIPCHK(hipMemsetAsync(Jk, 0, njk * sizeof(double), stream));
+  p.Jk = Jk;
   p.D = dD;
   p.F = dF;
   p.D2 = dF2 ? dD2 : nullptr;
@@ -1886,6 +1913,8 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   }
   HIPCHK(hipGetLastError());
   if ((rc = streams_join(s, stream, nst))) return rc;
+  // the ket-side Coulomb blocks into F
+  launch_jscatter(s->d_pairs.p, s->d_joff.p, (int)(s->h_joff.size() - 1), s->pure != 0, Jk, dF, stream);
   HIPCHK(hipEventRecord(s->ev1, stream));
   if (st || profile) {
     std::vector<unsigned long long> raw(nc * nl);

@@ -297,7 +297,7 @@ def test_uhf_equal_spins_is_rhf(L, oracle):
 def test_tiled_kernel_modes(L, oracle, basis, nw, nocc):
     """The tiled bra-row kernel (bras sharing their first shell: one ket read
     and one J_cd atomic per tile) against the oracle and against the
-    segmented kernels: OQPH_JKT=0 (off), 1 (default class pairs), 2 (every
+    segmented kernels: OQPH_JKT=0 (off, the default), 1 ((ps|ps)), 2 (every
     class pair with a tiled kernel).  Same G to 1e-12, same computed and
     screened counts; the open-shell pass and mixed precision as well."""
     from openqp_amd.autograd import packed_trace
@@ -341,4 +341,4 @@ def test_tiled_kernel_modes(L, oracle, basis, nw, nocc):
         assert np.abs(out["0"][0] - out[mode][0]).max() <= 1e-12
         assert out["0"][1] == out[mode][1]
     assert "k_jkt" not in kinds["0"]
-    assert "k_jkt" in kinds["2"]
+    assert "k_jkt" in kinds["1"] and "k_jkt" in kinds["2"]
