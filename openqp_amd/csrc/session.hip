@@ -274,8 +274,11 @@ struct oqph_session {
   // per quartet run on the run-time-class kernel k_gradh (0: templates wherever
   // they exist and fit)
   int grad_hl_ng = 0;
-  // OQPH_JK_HL_NG=n: the same for the J/K build (k_jkh)
-  int jk_hl_ng = 0;
+  // OQPH_JK_HL_NG=n: the same for the J/K build (k_jkh).  Default 300: the
+  // d classes above 300 Cartesian components ((dp|dp), (dd|pp), (dd|dp))
+  // run on k_jkh, whose multi-class launches now beat their spilling
+  // templates ((H2O)64/cc-pVDZ 366 -> 359 ms mixed, (H2O)16/cc-pVTZ 450 -> 435)
+  int jk_hl_ng = 300;
   // OQPH_HL_KPB=64|32|16|8: kets per block of every run-time-class launch
   // (0, default: chosen per launch by hl_kpb)
   int hl_kpb = 0;
@@ -771,7 +774,7 @@ static int prepare_geometry(oqph_session* s) {
     const char* gh = getenv("OQPH_GRAD_HL_NG");
     s->grad_hl_ng = gh ? atoi(gh) : 0;
     const char* jh = getenv("OQPH_JK_HL_NG");
-    s->jk_hl_ng = jh ? atoi(jh) : 0;
+    s->jk_hl_ng = jh ? atoi(jh) : 300;
     const char* kp = getenv("OQPH_HL_KPB");
     const int kpv = kp ? atoi(kp) : 0;
     s->hl_kpb = (kpv == 64 || kpv == 32 || kpv == 16 || kpv == 8) ? kpv : 0;

@@ -342,3 +342,39 @@ def test_tiled_kernel_modes(L, oracle, basis, nw, nocc):
         assert out["0"][1] == out[mode][1]
     assert "k_jkt" not in kinds["0"]
     assert "k_jkt" in kinds["1"] and "k_jkt" in kinds["2"]
+
+
+def test_large_d_classes_template_and_runtime(L, oracle):
+    """(dp|dp), (dd|pp) and (dd|dp) run on the run-time-class kernel by
+    default (more than 300 Cartesian components per quartet); OQPH_JK_HL_NG=0
+    puts them back on their class templates.  Both against the oracle, same G
+    to 1e-12 and the same quartet counts."""
+    _say("large d classes: templates vs run-time class")
+    bas = small_system("cc-pvdz", nw=3, seed=29)
+    D, dp = density_for(oracle, bas, 15, seed=30)
+    fo, so = oracle.fock_jk(bas, dp)
+    out, kinds = {}, {}
+    old = os.environ.get("OQPH_JK_HL_NG")
+    try:
+        for mode in ("0", None):
+            if mode is None:
+                os.environ.pop("OQPH_JK_HL_NG", None)
+            else:
+                os.environ["OQPH_JK_HL_NG"] = mode
+            fb = L.FockBuilder(bas)
+            out[mode] = (fb.fock_jk(dp), fb.last_stats.quartets_computed, fb.last_stats.quartets_screened)
+            kinds[mode] = sorted({(p["kernel"], p["cls"]) for p in fb.last_profile()})
+            fb.close()
+    finally:
+        if old is None:
+            os.environ.pop("OQPH_JK_HL_NG", None)
+        else:
+            os.environ["OQPH_JK_HL_NG"] = old
+    for mode in ("0", None):
+        err = np.abs(out[mode][0] - fo).max()
+        _say("  OQPH_JK_HL_NG", mode, "max |f - f_oracle| =", err)
+        assert err <= TOL
+        assert out[mode][1] == int(so[1])
+    assert np.abs(out["0"][0] - out[None][0]).max() <= 1e-12
+    assert out["0"][1:] == out[None][1:]
+    assert ("k_jk", 2121) in kinds["0"] and ("k_jk", 2121) not in kinds[None]
