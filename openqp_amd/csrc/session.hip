@@ -272,8 +272,9 @@ struct oqph_session {
   bool f_templates = true;
   // OQPH_GRAD_HL_NG=n: gradient classes with more than n Cartesian components
   // per quartet run on the run-time-class kernel k_gradh (0: templates wherever
-  // they exist and fit)
-  int grad_hl_ng = 0;
+  // they exist and fit).  Default 300, as for the J/K build: (H2O)16/cc-pVTZ
+  // gradient 548 -> 507 ms, C20H42 unchanged (150: 526 and 24.6 ms, 80: slower)
+  int grad_hl_ng = 300;
   // OQPH_JK_HL_NG=n: the same for the J/K build (k_jkh).  Default 300: the
   // d classes above 300 Cartesian components ((dp|dp), (dd|pp), (dd|dp))
   // run on k_jkh, whose multi-class launches now beat their spilling
@@ -772,7 +773,7 @@ static int prepare_geometry(oqph_session* s) {
     const char* ft = getenv("OQPH_F_TEMPLATES");
     s->f_templates = !(ft && ft[0] == '0');
     const char* gh = getenv("OQPH_GRAD_HL_NG");
-    s->grad_hl_ng = gh ? atoi(gh) : 0;
+    s->grad_hl_ng = gh ? atoi(gh) : 300;
     const char* jh = getenv("OQPH_JK_HL_NG");
     s->jk_hl_ng = jh ? atoi(jh) : 300;
     const char* kp = getenv("OQPH_HL_KPB");

@@ -72,3 +72,22 @@ def test_scf_kat_h2o_631gs():
     _say("SCF H2O/6-31G*", res.energy, res.iterations, res.converged)
     assert res.converged
     assert abs(res.energy - (-76.01074651325918)) < 1e-8
+
+
+@pytest.mark.parametrize("mode", ["0", "300"])
+def test_grad2_template_and_runtime_classes(oracle, monkeypatch, mode):
+    """The gradient classes above 300 Cartesian components ((dp|dp), (dd|pp)
+    and up) run on the run-time-class kernel by default; OQPH_GRAD_HL_NG=0
+    keeps them on their class templates.  Both against the oracle, 1e-8."""
+    from openqp_amd import lib as L
+    monkeypatch.setenv("OQPH_GRAD_HL_NG", mode)
+    _say("grad2, OQPH_GRAD_HL_NG", mode)
+    bas = small_system("cc-pvdz", nw=2, seed=5)
+    D, dp = density_for(oracle, bas, 10, seed=8)
+    go, st = oracle.grad2(bas, dp)
+    fb = L.FockBuilder(bas)
+    g = fb.grad2(dp)
+    err = np.abs(g - go).max()
+    _say("  max |g - g_oracle| =", err)
+    assert err <= 1e-8
+    fb.close()
