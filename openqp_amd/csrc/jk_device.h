@@ -86,7 +86,8 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
                                          const float* __restrict__ tab32N, R qcut, R imu2,
                                          R (&acc)[Dims<LA, LB, LC, LD>::NE][Dims<LA, LB, LC, LD>::NF],
                                          int& nprim_used, const PrimV<double>* sbra = nullptr,
-                                         const PrimV<float>* sbra32 = nullptr) {
+                                         const PrimV<float>* sbra32 = nullptr, int nbra64 = 0,
+                                         int nbra32 = 0) {
   using Dm = Dims<LA, LB, LC, LD>;
   constexpr bool F32 = sizeof(R) == 4;
 #pragma unroll
@@ -103,7 +104,9 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
     else return prim_v<double>(prims[i]);
   };
   if (UNIFORM_BRA && sbra) {
-    const int npp = __builtin_amdgcn_readfirstlane(P.nprim);
+    // the LDS copies hold only the bra primitive pairs with |K| >= qcut
+    // (nbra64 FP64, nbra32 FP32 records), so the loop needs no test for them
+    const int npp = __builtin_amdgcn_readfirstlane(sbra32 ? nbra32 : nbra64);
     // small classes, FP32: the next ket primitive pair is loaded while the
     // current one runs against the bra's primitives (where registers are
     // short, i.e. the larger classes and FP64, it costs more than it hides)
@@ -122,12 +125,15 @@ __device__ __forceinline__ void contract(const ShellPair& P, const ShellPair& Q,
       if (fabs(qq.Kg) < qcut) continue;
       R AQ[3];
       ket_offset<LA, LB, LC, LD, R>(AC, qq, AQ);
-      for (int ip = 0; ip < npp; ++ip) {
-        PrimV<R> pp;
-        if constexpr (F32) pp = sbra32 ? sbra32[ip] : prim_v<float>(sbra[ip]);
-        else pp = sbra[ip];
-        if (fabs(pp.Kg) < qcut) continue;
-        nprim_used += rys_accumulate<LA, LB, LC, LD, R>(pp, qq, AQ, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
+      auto run = [&](auto bra) {
+        for (int ip = 0; ip < npp; ++ip)
+          nprim_used += rys_accumulate<LA, LB, LC, LD, R>(bra(ip), qq, AQ, qcut2, imu2, tabN, tab32N, acc) ? 1 : 0;
+      };
+      if constexpr (F32) {
+        if (sbra32) run([&](int i) { return sbra32[i]; });
+        else run([&](int i) { return prim_v<float>(sbra[i]); });
+      } else {
+        run([&](int i) { return sbra[i]; });
       }
     }
     return;
@@ -252,11 +258,12 @@ __device__ __forceinline__ void quartet_block(const ShellPair& P, const ShellPai
                                               const float* __restrict__ tab32N, const double* __restrict__ xf,
                                               R qcut, R imu2, R (&V)[Dims<LA, LB, LC, LD>::NBLK], int& nprim_used,
                                               const PrimV<double>* sbra = nullptr,
-                                              const PrimV<float>* sbra32 = nullptr) {
+                                              const PrimV<float>* sbra32 = nullptr, int nbra64 = 0,
+                                              int nbra32 = 0) {
   using Dm = Dims<LA, LB, LC, LD>;
   R acc[Dm::NE][Dm::NF];
   contract<LA, LB, LC, LD, R, UNIFORM_BRA>(P, Q, prims, prims32, tabN, tab32N, qcut, imu2, acc, nprim_used, sbra,
-                                           sbra32);
+                                           sbra32, nbra64, nbra32);
   R AB[3], CD[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -800,11 +807,41 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   const int ip = __builtin_amdgcn_readfirstlane(item[0]);
   const ShellPair P = prm.pairs[ip];
   const bool bra_lds = P.nprim <= JKB_MAX_BRA_PRIMS;  // block-uniform
-  if (bra_lds)
-    for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra[i] = prim_v<double>(prm.prims[P.prim0 + i]);
-  if constexpr (B32)
-    if (bra_lds && prm.mixed)
-      for (int i = threadIdx.x; i < P.nprim; i += blockDim.x) s_bra32[i] = prim_v<float>(prm.prims32[P.prim0 + i]);
+  // LDS copies of the bra primitive pairs that pass the primitive screen
+  // |K| >= 1e-4 c (FP64 test for the FP64 copy, FP32 test for the FP32 copy),
+  // compacted by the first wavefront: the quadrature loops need no test
+  __shared__ int s_nbra[2];
+  if (bra_lds && threadIdx.x < 64) {
+    const int lane0 = (int)__lane_id();
+    const unsigned long long below = (1ull << lane0) - 1ull;
+    const double qc64 = 1.0e-4 * prm.cutoff;
+    const float qc32 = (float)qc64;
+    int n64 = 0, n32 = 0;
+    for (int i0 = 0; i0 < P.nprim; i0 += 64) {
+      const int i = i0 + lane0;
+      bool ok64 = false, ok32 = false;
+      PrimV<double> r{};
+      PrimV<float> r32{};
+      if (i < P.nprim) {
+        r = prim_v<double>(prm.prims[P.prim0 + i]);
+        ok64 = !(fabs(r.Kg) < qc64);
+        if (B32 && prm.mixed) {
+          r32 = prim_v<float>(prm.prims32[P.prim0 + i]);
+          ok32 = !(fabsf(r32.Kg) < qc32);
+        }
+      }
+      const unsigned long long m64 = __ballot(ok64), m32 = __ballot(ok32);
+      if (ok64) s_bra[n64 + __popcll(m64 & below)] = r;
+      if constexpr (B32)
+        if (ok32) s_bra32[n32 + __popcll(m32 & below)] = r32;
+      n64 += __popcll(m64);
+      n32 += __popcll(m32);
+    }
+    if (lane0 == 0) {
+      s_nbra[0] = n64;
+      s_nbra[1] = n32;
+    }
+  }
   // the density entries the digestion reads in the bra rows a and b, gathered
   // once per block instead of once per quartet
   if (drows) {
@@ -825,6 +862,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
   const float* tab32N = T32 ? (const float*)s_t32 : prm.rys32 + RYS32_OFF[NR];
   const PrimV<double>* sbra = bra_lds ? (const PrimV<double>*)s_bra : nullptr;
   const PrimV<float>* sbra32 = (B32 && bra_lds && prm.mixed) ? (const PrimV<float>*)s_bra32 : nullptr;
+  const int nbra64 = bra_lds ? s_nbra[0] : 0, nbra32 = bra_lds ? s_nbra[1] : 0;
   const int lane = (int)__lane_id(), wv = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
   const int oa = P.ao_a, ob = P.ao_b;
   const double* Dp = prm.D;
@@ -884,7 +922,8 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
             float Vf[NBLK];
             quartet_block<LA, LB, LC, LD, PURE, float, true>(P, Q, prm.prims, prm.prims32, tabN, tab32N,
                                                              prm.xform, (float)(1.0e-4 * prm.cutoff),
-                                                             (float)prm.inv_mu2, Vf, nprim, sbra, sbra32);
+                                                             (float)prm.inv_mu2, Vf, nprim, sbra, sbra32, nbra64,
+                                                             nbra32);
 #pragma unroll
             for (int i = 0; i < NOUT; ++i) V[i] = (double)Vf[i];
           }
@@ -892,7 +931,7 @@ __attribute__((amdgpu_waves_per_eu(JKB<LA, LB, LC, LD>::min_waves))) void k_jkb(
         if (!lowp)
           quartet_block<LA, LB, LC, LD, PURE, double, true>(P, Q, prm.prims, prm.prims32, tabN, tab32N,
                                                             prm.xform, 1.0e-4 * prm.cutoff, prm.inv_mu2, V, nprim,
-                                                            sbra);
+                                                            sbra, nullptr, nbra64, nbra32);
       }
       n_f32 += (lowp && !skip) ? 1 : 0;
       n_valid += nprim ? 1 : 0;
