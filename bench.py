@@ -92,6 +92,25 @@ def cpu_baseline(bas, dp, gpu_quartets, target_s=15.0):
             "quartets_per_s": round(int(st[1]) / dt, 1)}
 
 
+# mixed-precision 2e gradient: template quartets with Q_ij Q_kl below this run
+# in FP32 (max |dgrad| against FP64 below 1e-7 Eh/bohr on both gradient workloads)
+GRAD_FP32_CUT = 1e-2
+
+
+def mixed_gradient_rate(fb, dp, steps, warmup):
+    """Seconds per 2e-gradient evaluation in mixed precision, and its gradient;
+    leaves the builder's gradient in FP64."""
+    fb.set_grad_precision(1, GRAD_FP32_CUT)
+    for _ in range(max(warmup, 1)):
+        fb.grad2(dp)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = fb.grad2(dp)
+    dt = (time.perf_counter() - t0) / steps
+    fb.set_grad_precision(0, 0.0)
+    return dt, g
+
+
 def gradient_leg(steps, warmup):
     """2e nuclear gradient evals/s on C20H42 / 6-31G(d) (384 bf, Cartesian d) at
     the converged FP64 density of an SCF run on the device Fock builder."""
@@ -113,12 +132,16 @@ def gradient_leg(steps, warmup):
         g = fb.grad2(dp)
     dt = (time.perf_counter() - t0) / steps
     st = fb.last_stats
+    st = (st.kernel_ms, st.quartets_computed, st.model_flops)
+    dtm, gm = mixed_gradient_rate(fb, dp, steps, warmup)
     out = {"workload": "c20h42_631gd", "nbf": bas.nbf, "natm": bas.natm, "scf_energy": res.energy,
            "scf_iterations": res.iterations, "scf_converged": res.converged, "scf_wall_s": round(t_scf, 3),
            "scf_fock_ms_mean": round(float(np.mean(res.fock_ms)), 3),
            "evals_per_s": round(1.0 / dt, 4), "ms_per_eval": round(dt * 1e3, 3),
-           "kernel_ms": round(st.kernel_ms, 3), "quartets": st.quartets_computed,
-           "model_tflops": round(st.model_flops / (st.kernel_ms * 1e-3) / 1e12, 3) if st.kernel_ms else None,
+           "kernel_ms": round(st[0], 3), "quartets": st[1],
+           "model_tflops": round(st[2] / (st[0] * 1e-3) / 1e12, 3) if st[0] else None,
+           "mixed_evals_per_s": round(1.0 / dtm, 4), "mixed_fp32_cutoff": GRAD_FP32_CUT,
+           "mixed_max_dgrad_vs_fp64": float(np.abs(gm - g).max()),
            "max_abs_grad": float(np.abs(g).max()), "translational_sum": float(np.abs(g.sum(axis=0)).max())}
     fb.close()
     return out
@@ -167,10 +190,11 @@ def fshell_leg(fp32_cutoff, steps):
     fb.grad2(dp)
     t0 = time.perf_counter()
     for _ in range(steps):
-        fb.grad2(dp)
+        g2 = fb.grad2(dp)
     dtg = (time.perf_counter() - t0) / steps
     # last_stats is overwritten by every call: keep the values
     g_kernel_ms, g_quartets = fb.last_stats.kernel_ms, fb.last_stats.quartets_computed
+    dtgm, g2m = mixed_gradient_rate(fb, dp, steps, 1)
 
     def builds(precision):
         fb.set_precision(precision, fp32_cutoff)
@@ -196,6 +220,8 @@ def fshell_leg(fp32_cutoff, steps):
             "quartets_per_build": q64, "fp32_quartets": q32,
             "gradient_evals_per_s": round(1.0 / dtg, 4), "gradient_kernel_ms": round(g_kernel_ms, 3),
             "gradient_quartets": g_quartets,
+            "mixed_gradient_evals_per_s": round(1.0 / dtgm, 4),
+            "mixed_gradient_max_dgrad_vs_fp64": float(np.abs(g2m - g2).max()),
             "scf_energy_mixed": rmx.energy, "scf_iterations_mixed": rmx.iterations,
             "scf_wall_s_mixed": round(wmx, 3),
             "mixed_dE_scf_hartree": abs(rmx.energy - r64.energy),
@@ -386,6 +412,7 @@ def main():
         "E2_fp64_hartree": e64,
         "fp32_cutoff_sweep": sweep,
         "gradient_evals_per_s": grad["evals_per_s"] if grad else None,
+        "mixed_gradient_evals_per_s": grad["mixed_evals_per_s"] if grad else None,
         "gradient": grad,
         "scf": scf,
         "fshell": fsh,

@@ -81,6 +81,12 @@ int oqph_session_set_thresholds(oqph_session *s, double int2e_cutoff, double gra
 /* policy 0 = FP64 everywhere; 1 = mixed: quartets whose bound
  * Q_ij Q_kl max|D| is below fp32_cutoff run in FP32 (FP64 accumulation). */
 int oqph_session_set_precision(oqph_session *s, int policy, double fp32_cutoff);
+/* Precision of the 2e gradient (no counterpart in the reference, which works
+ * in FP64; for callers that accept an error of about 1e-7 Eh/bohr for speed):
+ * policy 0 = FP64 (default); 1 = mixed: quartet classes on the class templates
+ * run the quartets with Q_ij Q_kl below fp32_cutoff in FP32 arithmetic (FP64
+ * screens and per-atom sums). */
+int oqph_session_set_grad_precision(oqph_session *s, int policy, double fp32_cutoff);
 /* Make s the session used by routec_fock_jk / routec_grad2. */
 int oqph_set_default_session(oqph_session *s);
 /* Build the geometry-dependent device state now (pair records, Schwarz

@@ -7,19 +7,26 @@ namespace oqph {
 #define OQPH_GR_LAUNCHER_(a, b, c, d) launch_grad_##a##b##c##d
 #define OQPH_GR_LAUNCHER(a, b, c, d) OQPH_GR_LAUNCHER_(a, b, c, d)
 
-void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const WaveWork& ww, hipStream_t s) {
+// f32: the FP32 form (quartets the host assigned to FP32)
+void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32) {
   const int bs = 256;  // four wavefronts
   const long long nb = (ww.nwaves + 3) / 4;
-  hipLaunchKernelGGL((k_grad<XA, XB, XC, XD>), dim3((unsigned)nb), dim3(bs), sizeof(double) * 3 * p.natm, s, p, ww);
+  if (f32)
+    hipLaunchKernelGGL((k_grad<XA, XB, XC, XD, float>), dim3((unsigned)nb), dim3(bs), sizeof(double) * 3 * p.natm, s,
+                       p, ww);
+  else
+    hipLaunchKernelGGL((k_grad<XA, XB, XC, XD, double>), dim3((unsigned)nb), dim3(bs), sizeof(double) * 3 * p.natm,
+                       s, p, ww);
 }
 
 #define OQPH_GR_RES_(a, b, c, d) grad_res_##a##b##c##d
 #define OQPH_GR_RES(a, b, c, d) OQPH_GR_RES_(a, b, c, d)
 
-// Compiled resources of this class's gradient kernel.
-bool OQPH_GR_RES(XA, XB, XC, XD)(KernelRes* r) {
+// Compiled resources of this class's gradient kernel (f32: its FP32 form).
+bool OQPH_GR_RES(XA, XB, XC, XD)(KernelRes* r, bool f32) {
   hipFuncAttributes a;
-  if (hipFuncGetAttributes(&a, (const void*)k_grad<XA, XB, XC, XD>) != hipSuccess) return false;
+  const void* f = f32 ? (const void*)k_grad<XA, XB, XC, XD, float> : (const void*)k_grad<XA, XB, XC, XD, double>;
+  if (hipFuncGetAttributes(&a, f) != hipSuccess) return false;
   r->private_bytes = (int)a.localSizeBytes;
   r->lds_bytes = (int)a.sharedSizeBytes;
   r->regs = a.numRegs;

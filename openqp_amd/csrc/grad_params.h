@@ -13,6 +13,7 @@ struct GradParams {
   const ShellPair* pairs;
   const PrimPair* prims;
   const double* rys;
+  const float* rys32;     // FP32 root fits (the FP32 form of k_grad)
   const double* Dc;       // ncf x ncf raw Cartesian effective density
   const int* coff;        // Cartesian offset per shell
   const int* shell_atom;  // atom of each shell

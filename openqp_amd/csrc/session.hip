@@ -59,7 +59,7 @@ void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prim
 struct GradParams;
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s);
 void launch_gradh_multi(const GradParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s);
-bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s);
+bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32);
 void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
                          const double* xf, int ncf, double* Dc, hipStream_t s);
 const double* rys_table_host(size_t* n);
@@ -219,6 +219,7 @@ struct oqph_session {
     HLTaskList tl{};
     long long nblocks = 0;
     int lds = 0;
+    bool f32 = false;  // the FP32 form of the class template (mixed-precision gradient)
   };
   // Grouped bra-row kernel (k_jkg): per shell, the shell whose primitive set
   // it is contracted from (fam[i] = i for a family's master); per bra class
@@ -306,6 +307,12 @@ struct oqph_session {
   DBuf<double> d_Dt;  // total density of a UHF build
   int ncf = 0;
   double gcut_ready = -1.0;
+  // gradient precision (oqph_session_set_grad_precision): 1 = quartets with
+  // Q_ij Q_kl below grad_fp32_cut run in the FP32 form of their template
+  int grad_policy = 0;
+  double grad_fp32_cut = 1.0e-4;
+  int gpolicy_ready = -1;
+  double gf32_ready = -1.0;
   std::vector<GLaunch> glaunches;  // gradient wavefront tasks
   // optional per-launch profile of the last build (HIP events on the build stream)
   bool profile = false;
@@ -948,7 +955,9 @@ static int hl_kpb(const oqph_session* s, long long chunks, int lds) {
 static int prepare_grad(oqph_session* s) {
   int rc = prepare_geometry(s);
   if (rc) return rc;
-  if (s->gcut_ready == s->grad_cutoff) return 0;
+  if (s->gcut_ready == s->grad_cutoff && s->gpolicy_ready == s->grad_policy &&
+      (s->grad_policy == 0 || s->gf32_ready == s->grad_fp32_cut))
+    return 0;
   // the per-atom gradient sums are staged in LDS (3 natm doubles per block)
   if (24 * (size_t)s->natm + 16384 > s->lds_max) {
     set_err("gradient: %d atoms need %zu bytes of LDS per block for the per-atom sums (device: %zu)", s->natm,
@@ -1009,7 +1018,7 @@ static int prepare_grad(oqph_session* s) {
   // once, with the bra the later entry of the (bucket, Q) order
   std::vector<int> sbra, sk0, sn;
   std::vector<long long> choff;
-  struct Pend { int la, lb, lc, ld, ck; size_t seg0, ch0; int nseg; long long nwaves; };
+  struct Pend { int la, lb, lc, ld, ck; size_t seg0, ch0; int nseg; long long nwaves; bool f32; };
   std::vector<Pend> pend;
   const int ncls = class_of(s->lmax, s->lmax) + 1;
   for (int cb = 0; cb < ncls; ++cb)
@@ -1023,7 +1032,7 @@ static int prepare_grad(oqph_session* s) {
       for (int b = 0; b < NBKT; ++b)
         for (int k = kb[b]; k < kb[b + 1]; ++k) kmax[b] = std::max(kmax[b], gp[cls[ck][k]].nprim);
       struct Seg { int bra, k0, n; long long cost; };
-      std::vector<Seg> segs;
+      std::vector<Seg> segs, segs32;  // FP64 and, in the mixed-precision gradient, FP32 kets
       int bp = 0;
       for (int p = 0; p < (int)QB.size(); ++p) {
         while (bp + 1 < NBKT && p >= bb[bp + 1]) ++bp;
@@ -1037,15 +1046,37 @@ static int prepare_grad(oqph_session* s) {
             if (b > bp) k = 0;
             else if (b == bp) k = std::min(k, p - bb[bp] + 1);
           }
-          if (k > 0) segs.push_back({(int)coff[cb] + p, kb[b], k, (long long)gp[bra].nprim * kmax[b]});
+          if (k <= 0) continue;
+          // mixed precision: the kets with Q_p Q_k below grad_fp32_cut (a
+          // suffix of the prefix) go to the FP32 form; the FP64 part is
+          // rounded up to whole wavefronts, so the split adds no partly
+          // filled wavefront
+          int khi = k;
+          if (s->grad_policy == 1) {
+            khi = (int)(std::lower_bound(QK.begin() + kb[b], QK.begin() + kb[b] + k, s->grad_fp32_cut / QB[p],
+                                         [](double a, double t) { return a >= t; }) - (QK.begin() + kb[b]));
+            khi = std::min(k, (khi + 63) / 64 * 64);
+          }
+          const long long cost = (long long)gp[bra].nprim * kmax[b];
+          if (khi > 0) segs.push_back({(int)coff[cb] + p, kb[b], khi, cost});
+          if (k > khi) segs32.push_back({(int)coff[cb] + p, kb[b] + khi, k - khi, cost});
         }
       }
-      if (segs.empty()) continue;
+      if (segs.empty() && segs32.empty()) continue;
+      int la = 0, lb = 0, lc = 0, ld = 0;
+      for (int x = 0; x <= 3; ++x)
+        for (int y = 0; y <= x; ++y) {
+          if (class_of(x, y) == cb) { la = x; lb = y; }
+          if (class_of(x, y) == ck) { lc = x; ld = y; }
+        }
+      for (int f32 = 0; f32 < 2; ++f32) {
+      std::vector<Seg>& segs_ = f32 ? segs32 : segs;
+      if (segs_.empty()) continue;
       // longest wavefronts first
-      std::stable_sort(segs.begin(), segs.end(), [](const Seg& x, const Seg& y) { return x.cost > y.cost; });
+      std::stable_sort(segs_.begin(), segs_.end(), [](const Seg& x, const Seg& y) { return x.cost > y.cost; });
       const size_t seg0 = sbra.size(), ch0 = choff.size();
       long long waves = 0;
-      for (const Seg& g : segs) {
+      for (const Seg& g : segs_) {
         sbra.push_back(g.bra);
         sk0.push_back(g.k0);
         sn.push_back(g.n);
@@ -1053,13 +1084,8 @@ static int prepare_grad(oqph_session* s) {
         waves += (g.n + 63) / 64;
       }
       choff.push_back(waves);
-      int la = 0, lb = 0, lc = 0, ld = 0;
-      for (int x = 0; x <= 3; ++x)
-        for (int y = 0; y <= x; ++y) {
-          if (class_of(x, y) == cb) { la = x; lb = y; }
-          if (class_of(x, y) == ck) { lc = x; ld = y; }
-        }
-      pend.push_back({la, lb, lc, ld, ck, seg0, ch0, (int)segs.size(), waves});
+      pend.push_back({la, lb, lc, ld, ck, seg0, ch0, (int)segs_.size(), waves, f32 != 0});
+      }
     }
   const size_t nseg = sbra.size();
   if ((rc = s->d_gseg.ensure(3 * nseg + 1)) || (rc = s->d_gchoff.ensure(choff.size() + 1))) return rc;
@@ -1087,6 +1113,11 @@ static int prepare_grad(oqph_session* s) {
       KernelRes kr{};
       L.hl = !kernel_resources(2, p.la, p.lb, p.lc, p.ld, s->pure, &kr) || kr.private_bytes > OQPH_PRIVATE_BUDGET ||
              (size_t)kr.lds_bytes + 24 * (size_t)s->natm > s->lds_max;
+    }
+    if (!L.hl && p.f32) {  // the FP32 form where its resources fit as well (else the FP64 form)
+      KernelRes kr{};
+      L.f32 = kernel_resources(2 | 16, p.la, p.lb, p.lc, p.ld, s->pure, &kr) &&
+              kr.private_bytes <= OQPH_PRIVATE_BUDGET && (size_t)kr.lds_bytes + 24 * (size_t)s->natm <= s->lds_max;
     }
     if (L.hl) {
       L.hd = hl_dims(p.la, p.lb, p.lc, p.ld, s->pure, 1, s->natm);
@@ -1200,6 +1231,8 @@ static int prepare_grad(oqph_session* s) {
     return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   s->gcut_ready = s->grad_cutoff;
+  s->gpolicy_ready = s->grad_policy;
+  s->gf32_ready = s->grad_fp32_cut;
   return 0;
 }
 
@@ -2120,6 +2153,16 @@ int oqph_session_set_precision(oqph_session* s, int policy, double fp32_cut) {
   return 0;
 }
 
+int oqph_session_set_grad_precision(oqph_session* s, int policy, double fp32_cut) {
+  if (!s || policy < 0 || policy > 1 || fp32_cut < 0.0) {
+    set_err("set_grad_precision: invalid argument");
+    return 4;
+  }
+  s->grad_policy = policy;
+  s->grad_fp32_cut = fp32_cut;
+  return 0;
+}
+
 int oqph_session_nbf(oqph_session* s) { return s ? s->nbf : 0; }
 
 int oqph_session_set_profiling(oqph_session* s, int on) {
@@ -2318,6 +2361,7 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   p.pairs = s->d_gpairs.p;
   p.prims = s->d_gprims.p;
   p.rys = s->d_rys.p;
+  p.rys32 = s->d_rys32.p;
   p.Dc = s->d_Dc.p;
   p.coff = s->d_coff.p;
   p.shell_atom = s->d_satom.p;
@@ -2339,7 +2383,7 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
       launch_gradh_multi(p, L.tl, L.nblocks, L.lds, ls);
     } else if (L.hl) {
       launch_gradh(L.hd, p, L.ww, ls);
-    } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, ls)) {
+    } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, ls, L.f32)) {
       set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }

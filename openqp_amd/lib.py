@@ -30,7 +30,7 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_grad2", "oqph_fock_jk_device", "oqph_schwarz", "oqph_session_nbf", "oqph_last_error",
            "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e",
            "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources",
-           "oqph_fock_jk_noalloc", "oqph_last_counters"]
+           "oqph_fock_jk_noalloc", "oqph_last_counters", "oqph_session_set_grad_precision"]
 
 
 class OqphError(RuntimeError):
@@ -60,6 +60,7 @@ def load(path: str | None = None):
     L.oqph_session_set_geometry.argtypes = [vp, C.c_int, _pd, _pd]
     L.oqph_session_set_thresholds.argtypes = [vp, C.c_double, C.c_double]
     L.oqph_session_set_precision.argtypes = [vp, C.c_int, C.c_double]
+    L.oqph_session_set_grad_precision.argtypes = [vp, C.c_int, C.c_double]
     L.oqph_set_default_session.argtypes = [vp]
     L.oqph_fock_jk.argtypes = [vp, _pd, _pd, C.c_int, C.c_double, C.c_double, C.POINTER(Stats)]
     L.oqph_fock_jk_inc.argtypes = [vp, _pd, _pd, _pd, _pd, C.c_int, C.c_double, C.c_double, C.POINTER(Stats)]
@@ -124,6 +125,10 @@ class FockBuilder:
 
     def set_precision(self, policy: int, fp32_cutoff: float = 0.0):
         _check(load().oqph_session_set_precision(self._s, policy, fp32_cutoff), "set_precision")
+
+    def set_grad_precision(self, policy: int, fp32_cutoff: float = 1e-4):
+        """2e-gradient precision: 0 = FP64; 1 = quartets with Q_ij Q_kl < fp32_cutoff in FP32."""
+        _check(load().oqph_session_set_grad_precision(self._s, policy, fp32_cutoff), "set_grad_precision")
 
     def set_share(self, rank: int, size: int):
         """Restrict builds to OpenQP MPI rank `rank` of `size` (int2.F90:759-761)."""

@@ -125,7 +125,8 @@ class RHF:
             res.fock_ms.append((time.perf_counter() - t0) * 1e3)
             F = self.H + B.unpack_tri(g, nbf)
             E = 0.5 * (np.sum(D * F) + np.sum(D * self.H)) + self.enuc
-            err = self.X.T @ (F @ D @ self.S - self.S @ D @ F) @ self.X
+            FDS = F @ D @ self.S  # S D F is its transpose (F, D, S symmetric)
+            err = self.X.T @ (FDS - FDS.T) @ self.X
             diis_err = float(np.abs(err).max())
             res.history.append((it, E, diis_err))
             if it > 1 and diis_err < self.conv and abs(E - e_prev) < self.econv:

@@ -91,3 +91,27 @@ def test_grad2_template_and_runtime_classes(oracle, monkeypatch, mode):
     _say("  max |g - g_oracle| =", err)
     assert err <= 1e-8
     fb.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cut,tol", [(1e-2, 1e-6), (1e30, 1e-5)])
+def test_grad2_mixed_precision(oracle, cut, tol):
+    """Mixed-precision gradient: the template quartets with Q_ij Q_kl below
+    `cut` run in FP32 (1e30: all of them).  Against the FP64 oracle; the FP64
+    gradient of the same session is unchanged after switching back."""
+    from openqp_amd import lib as L
+    _say("grad2 mixed precision, cut", cut)
+    bas = small_system("6-31g(d)", nw=3, seed=11)
+    D, dp = density_for(oracle, bas, 15, seed=4)
+    go, st = oracle.grad2(bas, dp)
+    fb = L.FockBuilder(bas)
+    g64 = fb.grad2(dp)
+    fb.set_grad_precision(1, cut)
+    g = fb.grad2(dp)
+    err = np.abs(g - go).max()
+    _say("  max |g_mixed - g_oracle| =", err, " FP64:", np.abs(g64 - go).max())
+    assert err <= tol
+    assert np.abs(g - g64).max() > 0.0  # the FP32 form ran
+    fb.set_grad_precision(0, 0.0)
+    assert np.abs(fb.grad2(dp) - g64).max() <= 1e-12
+    fb.close()
