@@ -87,6 +87,16 @@ int oqph_session_set_precision(oqph_session *s, int policy, double fp32_cutoff);
  * run the quartets with Q_ij Q_kl below fp32_cutoff in FP32 arithmetic (FP64
  * screens and per-atom sums). */
 int oqph_session_set_grad_precision(oqph_session *s, int policy, double fp32_cutoff);
+/* MRSF response integral pass: replaces int2_driver%run on int2_mrsf_data_t
+ * (tdhf_mrsf_energy.F90:742-752; digestion tdhf_mrsf_lib.F90:278-308, pass 1,
+ * FP64, Tamm-Dancoff).  d3 and f3 are the reference's d3(nf, nmat, nbf, nbf)
+ * and f3(nf, nmat, nbf, nbf) arrays (Fortran order; MRSF: nmat = 7,
+ * ncoul = 4): f3(v,c,:,:) = sc J[M] (c < ncoul) - se K[M] for
+ * M = d3(v,c,:,:), J[M]_ij = sum_kl (ij|kl) M_kl, K[M]_ik = sum_jl (ij|kl)
+ * M_jl, screened on the last component as shell_den_screen_mrsf
+ * (tdhf_mrsf_lib.F90:176-204).  f3 is overwritten. */
+int oqph_mrsf_int2(oqph_session *s, int nf, int nmat, int ncoul, const double *d3, double *f3, double se,
+                   double sc, oqph_stats *stats);
 /* Make s the session used by routec_fock_jk / routec_grad2. */
 int oqph_set_default_session(oqph_session *s);
 /* Build the geometry-dependent device state now (pair records, Schwarz

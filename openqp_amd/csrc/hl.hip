@@ -77,16 +77,27 @@ static unsigned hl_blocks(const WaveWork& ww) {
   return (unsigned)(ww.nwaves * (64 / kpb));
 }
 
+// p.Dg set: the response-pass instances
 void launch_jkh(const HLDims& h, const JKParams& p, const WaveWork& ww, hipStream_t s) {
   if (ww.nwaves == 0) return;
-  allow_lds(k_jkh, h.lds);
-  hipLaunchKernelGGL(k_jkh, dim3(hl_blocks(ww)), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+  if (p.Dg) {
+    allow_lds(k_jkh<true>, h.lds);
+    hipLaunchKernelGGL(k_jkh<true>, dim3(hl_blocks(ww)), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+  } else {
+    allow_lds(k_jkh<false>, h.lds);
+    hipLaunchKernelGGL(k_jkh<false>, dim3(hl_blocks(ww)), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+  }
 }
 
 void launch_jkh_multi(const JKParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s) {
   if (nblocks == 0) return;
-  allow_lds(k_jkh_multi, lds);
-  hipLaunchKernelGGL(k_jkh_multi, dim3((unsigned)nblocks), dim3(HL_BLOCK), lds, s, p, tl);
+  if (p.Dg) {
+    allow_lds(k_jkh_multi<true>, lds);
+    hipLaunchKernelGGL(k_jkh_multi<true>, dim3((unsigned)nblocks), dim3(HL_BLOCK), lds, s, p, tl);
+  } else {
+    allow_lds(k_jkh_multi<false>, lds);
+    hipLaunchKernelGGL(k_jkh_multi<false>, dim3((unsigned)nblocks), dim3(HL_BLOCK), lds, s, p, tl);
+  }
 }
 
 void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prims, const int* list, int n,

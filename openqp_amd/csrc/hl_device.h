@@ -423,7 +423,126 @@ __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& 
 // Wavefront tasks as k_jk: a block takes one 64-ket chunk of one segment,
 // screens its kets lane-parallel with OpenQP's tests, then evaluates and
 // digests the surviving quartets one after another, all 64 lanes on each.
+// Response-pass digestion of one quartet (ab|cd) for every non-symmetric
+// density M_v (int2_mrsf_data_t_update, tdhf_mrsf_lib.F90:278-308): Coulomb
+// with the symmetrised blocks M + M^T into F_ab, F_ba, F_cd, F_dc; exchange
+// into the eight targets F_ac, F_ca, F_ad, F_da, F_bc, F_cb, F_bd, F_db, each
+// from its own block of M (F_ac from M_bd, F_ca from M_db, ...).  deg holds
+// the shell-level degeneracy factors; the outputs are full nbf x nbf.  The
+// lanes run over (density, output element) pairs and read M from memory, so
+// small classes keep the wavefront busy across the densities.
+__device__ __forceinline__ void hl_general(const HLDims& h, const double* V, const ShellPair& P, const ShellPair& Q,
+                                           double deg, const JKParams& prm, int lane) {
+  const int nbf = prm.nbf, nd = prm.gnd;
+  const size_t n2 = (size_t)nbf * nbf;
+  const int oa = P.ao_a, ob = P.ao_b, oc = Q.ao_a, od = Q.ao_b;
+  const int OA = h.oa, OB = h.ob, OC = h.oc, OD = h.od;
+  const double cut = prm.cutoff;
+  const double cj = deg * prm.sc, ck = -deg * prm.se;
+  auto val = [&](int a, int b, int c, int d) {
+    const double v = V[((a * OB + b) * OC + c) * OD + d];
+    return fabs(v) < cut ? 0.0 : v;  // OpenQP element cutoff on the AO value
+  };
+  auto put = [&](double* Fv, int x, int y, double w) {
+    if (w != 0.0) fatomic(Fv + (size_t)x * nbf + y, w);
+  };
+  // Coulomb
+  for (int t = lane; t < nd * OA * OB; t += HL_BLOCK) {
+    const int v = t / (OA * OB), e = t - v * (OA * OB);
+    if (!prm.gcoul[v]) continue;
+    const int a = e / OB, b = e - a * OB;
+    const double* M = prm.Dg + (size_t)v * n2;
+    double s = 0.0;
+    for (int c = 0; c < OC; ++c)
+      for (int d = 0; d < OD; ++d)
+        s += val(a, b, c, d) * (M[(size_t)(oc + c) * nbf + od + d] + M[(size_t)(od + d) * nbf + oc + c]);
+    double* Fv = prm.Fg + (size_t)v * n2;
+    put(Fv, oa + a, ob + b, cj * s);
+    put(Fv, ob + b, oa + a, cj * s);
+  }
+  for (int t = lane; t < nd * OC * OD; t += HL_BLOCK) {
+    const int v = t / (OC * OD), e = t - v * (OC * OD);
+    if (!prm.gcoul[v]) continue;
+    const int c = e / OD, d = e - c * OD;
+    const double* M = prm.Dg + (size_t)v * n2;
+    double s = 0.0;
+    for (int a = 0; a < OA; ++a)
+      for (int b = 0; b < OB; ++b)
+        s += val(a, b, c, d) * (M[(size_t)(oa + a) * nbf + ob + b] + M[(size_t)(ob + b) * nbf + oa + a]);
+    double* Fv = prm.Fg + (size_t)v * n2;
+    put(Fv, oc + c, od + d, cj * s);
+    put(Fv, od + d, oc + c, cj * s);
+  }
+  // exchange: F_ac / F_ca
+  for (int t = lane; t < nd * OA * OC; t += HL_BLOCK) {
+    const int v = t / (OA * OC), e = t - v * (OA * OC);
+    const int a = e / OC, c = e - a * OC;
+    const double* M = prm.Dg + (size_t)v * n2;
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = 0; b < OB; ++b)
+      for (int d = 0; d < OD; ++d) {
+        const double w = val(a, b, c, d);
+        s1 += w * M[(size_t)(ob + b) * nbf + od + d];
+        s2 += w * M[(size_t)(od + d) * nbf + ob + b];
+      }
+    double* Fv = prm.Fg + (size_t)v * n2;
+    put(Fv, oa + a, oc + c, ck * s1);
+    put(Fv, oc + c, oa + a, ck * s2);
+  }
+  // F_ad / F_da
+  for (int t = lane; t < nd * OA * OD; t += HL_BLOCK) {
+    const int v = t / (OA * OD), e = t - v * (OA * OD);
+    const int a = e / OD, d = e - a * OD;
+    const double* M = prm.Dg + (size_t)v * n2;
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = 0; b < OB; ++b)
+      for (int c = 0; c < OC; ++c) {
+        const double w = val(a, b, c, d);
+        s1 += w * M[(size_t)(ob + b) * nbf + oc + c];
+        s2 += w * M[(size_t)(oc + c) * nbf + ob + b];
+      }
+    double* Fv = prm.Fg + (size_t)v * n2;
+    put(Fv, oa + a, od + d, ck * s1);
+    put(Fv, od + d, oa + a, ck * s2);
+  }
+  // F_bc / F_cb
+  for (int t = lane; t < nd * OB * OC; t += HL_BLOCK) {
+    const int v = t / (OB * OC), e = t - v * (OB * OC);
+    const int b = e / OC, c = e - b * OC;
+    const double* M = prm.Dg + (size_t)v * n2;
+    double s1 = 0.0, s2 = 0.0;
+    for (int a = 0; a < OA; ++a)
+      for (int d = 0; d < OD; ++d) {
+        const double w = val(a, b, c, d);
+        s1 += w * M[(size_t)(oa + a) * nbf + od + d];
+        s2 += w * M[(size_t)(od + d) * nbf + oa + a];
+      }
+    double* Fv = prm.Fg + (size_t)v * n2;
+    put(Fv, ob + b, oc + c, ck * s1);
+    put(Fv, oc + c, ob + b, ck * s2);
+  }
+  // F_bd / F_db
+  for (int t = lane; t < nd * OB * OD; t += HL_BLOCK) {
+    const int v = t / (OB * OD), e = t - v * (OB * OD);
+    const int b = e / OD, d = e - b * OD;
+    const double* M = prm.Dg + (size_t)v * n2;
+    double s1 = 0.0, s2 = 0.0;
+    for (int a = 0; a < OA; ++a)
+      for (int c = 0; c < OC; ++c) {
+        const double w = val(a, b, c, d);
+        s1 += w * M[(size_t)(oa + a) * nbf + oc + c];
+        s2 += w * M[(size_t)(oc + c) * nbf + oa + a];
+      }
+    double* Fv = prm.Fg + (size_t)v * n2;
+    put(Fv, ob + b, od + d, ck * s1);
+    put(Fv, od + d, ob + b, ck * s2);
+  }
+}
+
 // One 64-ket chunk (wave) of one class's wavefront tasks, by one block.
+// GEN: the response pass over non-symmetric densities (prm.Dg), a separate
+// instance so the Fock-build kernels keep their register allocation.
+template <bool GEN>
 __device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& ww, const HLDims& h, long long blk,
                                           double* lds) {
   const HLTabs tb = hl_tables(h, lds);
@@ -478,11 +597,15 @@ __device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& w
     const double* V = hl_block(h, tb, P, Q, prm.prims, prm.rys, prm.xform, 1.0e-4 * cut, prm.inv_mu2, lds, nprim);
     prims_used += (unsigned long long)nprim;
     if (nprim == 0) continue;
-    flops += (unsigned long long)(nprim * fprim + (prm.F2 ? 24 : 12) * nout);
+    flops += (unsigned long long)(nprim * fprim + (GEN ? 20 * prm.gnd : prm.F2 ? 24 : 12) * nout);
     double deg = 1.0;
     if (P.sa == P.sb) deg *= 0.5;
     if (Q.sa == Q.sb) deg *= 0.5;
     if (ip == jq) deg *= 0.5;
+    if constexpr (GEN) {
+      hl_general(h, V, P, Q, deg, prm, lane);
+      continue;
+    }
     const double cj = 2.0 * deg * prm.sc, ck = -0.5 * deg * prm.se;
     const int oc = Q.ao_a, od = Q.ao_b;
     const int OA = h.oa, OB = h.ob, OC = h.oc, OD = h.od;
@@ -576,9 +699,10 @@ __device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& w
   }
 }
 
+template <bool GEN>
 __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_jkh(JKParams prm, WaveWork ww, HLDims h) {
   extern __shared__ double lds[];
-  jkh_chunk(prm, ww, h, (long long)blockIdx.x, lds);
+  jkh_chunk<GEN>(prm, ww, h, (long long)blockIdx.x, lds);
 }
 
 // Several run-time classes in one launch: block b takes wave b - wave0 of the
@@ -586,6 +710,7 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
 // cost per wave, so the most expensive chunks start first and the launch ends
 // on the cheapest.  The task tables hold offsets; the list bases are launch
 // arguments (the lists may live in different buffers from build to build).
+template <bool GEN>
 __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_HL_WAVES))) void k_jkh_multi(
     JKParams prm, HLTaskList tl) {
   extern __shared__ double lds[];
@@ -607,7 +732,7 @@ __global__ __launch_bounds__(HL_BLOCK) __attribute__((amdgpu_waves_per_eu(OQPH_H
   ww.nwaves = tk.nwaves;
   ww.kpb = tk.kpb;
   const HLDims h = tk.h;
-  jkh_chunk(prm, ww, h, b - tk.wave0, lds);
+  jkh_chunk<GEN>(prm, ww, h, b - tk.wave0, lds);
 }
 
 // --------------------------------------------------------- Schwarz bounds

@@ -227,4 +227,14 @@ struct JKParams {
   // for the packed density D2 into F2, from the same integrals.
   const double* D2;
   double* F2;
+  // Response pass over non-symmetric densities (run-time-class kernel only):
+  // when Dg is set, D, F, D2 and F2 are unused and every quartet adds, for
+  // each of the gnd full nbf x nbf densities M_v, sc J[M_v] (if gcoul[v]) -
+  // se K[M_v] into the full matrix F_v, J[M]_ij = sum_kl (ij|kl) M_kl and
+  // K[M]_ik = sum_jl (ij|kl) M_jl.
+  const double* Dg;
+  double* Fg;
+  const int* gcoul;
+  int gnd;
+  int nbf;
 };

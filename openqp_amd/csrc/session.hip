@@ -205,10 +205,15 @@ struct oqph_session {
   DBuf<int> d_aooff, d_nao;
   // per build
   DBuf<double> d_D, d_F, d_dsh, d_tmp;
+  DBuf<double> d_gD, d_gF;  // response-pass densities and outputs (nd x nbf x nbf)
+  DBuf<int> d_gcoul;
   DBuf<long long> d_off;
   DBuf<unsigned long long> d_ctr;
   double cached_maxden = -1.0;
   bool cached_two = false;  // lists prepared for the open-shell pass (K rows of two densities in LDS)
+  // response pass over non-symmetric densities (oqph_mrsf_int2): every class
+  // on the run-time-class kernel
+  bool general = false, cached_general = false;
   std::vector<long long> h_off;
   struct GLaunch {
     int la, lb, lc, ld;
@@ -1242,7 +1247,7 @@ static int prepare_grad(oqph_session* s) {
 // entry of the (bucket, Q) order).  Non-empty (p, b) segments are stored with
 // a prefix over their 64-ket chunks.
 static int prepare_lists(oqph_session* s, double maxden, bool two) {
-  if (maxden == s->cached_maxden && two == s->cached_two) return 0;
+  if (maxden == s->cached_maxden && two == s->cached_two && s->general == s->cached_general) return 0;
   s->launches.clear();
   s->h_seg_bra.clear();
   s->h_seg_k0.clear();
@@ -1320,7 +1325,7 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
       // run-time-class kernel for the f classes without a class template
       // templates whose private segment exceeds the budget run on k_jkh instead
       KernelRes kr0{}, kr1{};
-      const bool tmpl = jk_template(la, lb, lc, ld) && !(!s->f_templates && (la == 3 || lc == 3)) &&
+      const bool tmpl = !s->general && jk_template(la, lb, lc, ld) && !(!s->f_templates && (la == 3 || lc == 3)) &&
                         !(s->jk_hl_ng > 0 && ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > s->jk_hl_ng);
       const int variant = two ? 16 : 0;  // the open-shell kernels when two
       const bool fit0 = tmpl && kernel_resources(0 | variant, la, lb, lc, ld, s->pure, &kr0) &&
@@ -1701,6 +1706,7 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
   ++s->list_gen;
   s->cached_maxden = maxden;
   s->cached_two = two;
+  s->cached_general = s->general;
   return 0;
 }
 
@@ -1984,6 +1990,85 @@ static int build_device(oqph_session* s, const double* dD, double* dF, double se
   return 0;
 }
 
+// Response pass over nd full nbf x nbf densities (device slabs dD, outputs
+// dF): every class on k_jkh / k_jkh_multi with the general digestion; dsh and
+// maxden come from the caller (the reference's screening density).
+static int build_general(oqph_session* s, int nd, const double* dD, double* dF, const int* dcoul, double maxden,
+                         double se, double sc, oqph_stats* st, hipStream_t stream) {
+  int rc;
+  if ((rc = prepare_share(s))) return rc;
+  const size_t n2 = (size_t)s->nbf * s->nbf;
+  HIPCHK(hipMemsetAsync(dF, 0, (size_t)nd * n2 * sizeof(double), stream));
+  if (st) memset(st, 0, sizeof(*st));
+  if (maxden <= 0.0) return 0;
+  s->general = true;
+  rc = prepare_lists(s, maxden, false);
+  s->general = false;
+  if (rc) return rc;
+  const size_t nl = s->launches.size();
+  const size_t nc = 8 * (size_t)OQPH_CTR_SLOTS;
+  if ((rc = session_lists(s, stream))) return rc;
+  if ((rc = s->d_ctr.ensure(8 + nc * nl))) return rc;
+  unsigned long long* ctr = s->d_ctr.p + 8;
+  for (auto& L : s->launches) bind_lists(s, L, s->d_seg.p, s->d_choff.p, s->d_items.p, s->d_tasks.p);
+  HIPCHK(hipMemsetAsync(ctr, 0, nc * nl * sizeof(unsigned long long), stream));
+  JKParams p{};
+  p.pairs = s->d_pairs.p;
+  p.prims = s->d_prims.p;
+  p.prims32 = s->d_prims32.p;
+  p.own = s->share_size > 1 ? s->d_own.p : nullptr;
+  p.rys = s->d_rys.p;
+  p.rys32 = s->d_rys32.p;
+  p.xform = s->d_xform.p;
+  p.dsh = s->d_dsh.p;
+  p.nsh = s->nsh;
+  p.cutoff = s->cutoff;
+  p.maxden = maxden;
+  p.sc = sc;
+  p.se = se;
+  p.Dg = dD;
+  p.Fg = dF;
+  p.gcoul = dcoul;
+  p.gnd = nd;
+  p.nbf = s->nbf;
+  HIPCHK(hipEventRecord(s->ev0, stream));
+  const int nst = s->nstreams;
+  if ((rc = streams_fork(s, stream, nst))) return rc;
+  for (size_t i = 0; i < nl; ++i) {
+    auto& L = s->launches[i];
+    const hipStream_t ls = stream_of(s, stream, i, nst);
+    p.counters = ctr + nc * i;
+    if (L.multi) launch_jkh_multi(p, L.tl, L.nblocks, (int)L.lds, ls);
+    else if (L.hl) launch_jkh(L.hd, p, L.ww, ls);
+    else {
+      set_err("response pass: class (%d %d|%d %d) is not on the run-time-class kernel", L.la, L.lb, L.lc, L.ld);
+      return 1;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  if ((rc = streams_join(s, stream, nst))) return rc;
+  HIPCHK(hipEventRecord(s->ev1, stream));
+  std::vector<unsigned long long> raw(nc * nl);
+  if (nl) HIPCHK(hipMemcpyAsync(raw.data(), ctr, nc * nl * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  if (st) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    for (size_t k = 0; k < raw.size(); ++k) {
+      const unsigned long long c = raw[k];
+      switch (k % 8) {
+        case 0: st->quartets_computed += (long long)c; break;
+        case 1: st->quartets_screened += (long long)c; break;
+        case 3: st->model_flops += (double)c; break;
+        case 4: st->prim_quartets += (long long)c; break;
+        default: break;
+      }
+    }
+    st->kernel_ms = ms;
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------------ C ABI
 extern "C" {
 
@@ -2046,6 +2131,7 @@ int oqph_session_destroy(oqph_session* s) {
   s->d_am.release(); s->d_pure.release(); s->d_gseg.release(); s->d_gchoff.release(); s->d_Dc.release(); s->d_de.release();
   s->d_aooff.release(); s->d_nao.release(); s->d_colidx.release(); s->d_colao.release(); s->d_items.release(); s->d_Dt.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
   s->d_tmp.release(); s->d_off.release(); s->d_ctr.release();
+  s->d_gD.release(); s->d_gF.release(); s->d_gcoul.release();
   for (auto e : s->pev) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
@@ -2329,6 +2415,64 @@ int oqph_fock_jk_inc(oqph_session* s, const double* d, double* d_old, double* f,
     f_old[i] = f[i];
     d_old[i] = d[i];
   }
+  return 0;
+}
+
+int oqph_mrsf_int2(oqph_session* s, int nf, int nmat, int ncoul, const double* d3, double* f3, double se,
+                   double sc, oqph_stats* st) {
+  if (!s || !d3 || !f3 || nf < 1 || nmat < 1 || ncoul < 0 || ncoul > nmat) {
+    set_err("mrsf_int2: invalid argument");
+    return 4;
+  }
+  if (int fc = fault_code()) {
+    set_err("OQPH_FAULT injected decline");
+    return fc;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  int rc = prepare_geometry(s);
+  if (rc) return rc;
+  const int nbf = s->nbf, nsh = s->nsh;
+  const size_t n2 = (size_t)nbf * nbf, nd = (size_t)nf * nmat;
+  // slabs q = v nmat + c, each in the reference's (mu, nu) column order
+  std::vector<double> h(nd * n2);
+  size_t x = 0;
+  for (int nu = 0; nu < nbf; ++nu)
+    for (int mu = 0; mu < nbf; ++mu)
+      for (int c = 0; c < nmat; ++c)
+        for (int v = 0; v < nf; ++v) h[((size_t)v * nmat + c) * n2 + (size_t)nu * nbf + mu] = d3[x++];
+  std::vector<int> coul(nd);
+  for (size_t q = 0; q < nd; ++q) coul[q] = (int)(q % nmat) < ncoul;
+  // shell_den_screen_mrsf (tdhf_mrsf_lib.F90:176-204): the last component
+  // only, dsh(i, j) = dsh(j, i) = max |M(mu, nu)|, mu in shell j, nu in
+  // shell i >= j
+  std::vector<double> dsh((size_t)nsh * nsh, 0.0);
+  double maxden = 0.0;
+  for (int si = 0; si < nsh; ++si)
+    for (int sj = 0; sj <= si; ++sj) {
+      double dm = 0.0;
+      for (int v = 0; v < nf; ++v) {
+        const double* M = h.data() + ((size_t)v * nmat + nmat - 1) * n2;
+        for (int nu = s->aooff[si]; nu < s->aooff[si] + s->nao[si]; ++nu)
+          for (int mu = s->aooff[sj]; mu < s->aooff[sj] + s->nao[sj]; ++mu) dm = std::max(dm, fabs(M[(size_t)nu * nbf + mu]));
+      }
+      dsh[(size_t)si * nsh + sj] = dsh[(size_t)sj * nsh + si] = dm;
+      maxden = std::max(maxden, dm);
+    }
+  if ((rc = s->d_dsh.ensure((size_t)nsh * nsh)) || (rc = s->d_gD.ensure(nd * n2)) ||
+      (rc = s->d_gF.ensure(nd * n2)) || (rc = s->d_gcoul.ensure(nd)))
+    return rc;
+  if ((rc = upload(s->d_gD.p, h.data(), nd * n2 * sizeof(double), s->stream)) ||
+      (rc = upload(s->d_dsh.p, dsh.data(), dsh.size() * sizeof(double), s->stream)) ||
+      (rc = upload(s->d_gcoul.p, coul.data(), nd * sizeof(int), s->stream)))
+    return rc;
+  if ((rc = build_general(s, (int)nd, s->d_gD.p, s->d_gF.p, s->d_gcoul.p, maxden, se, sc, st, s->stream))) return rc;
+  HIPCHK(hipMemcpyAsync(h.data(), s->d_gF.p, nd * n2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  x = 0;
+  for (int nu = 0; nu < nbf; ++nu)
+    for (int mu = 0; mu < nbf; ++mu)
+      for (int c = 0; c < nmat; ++c)
+        for (int v = 0; v < nf; ++v) f3[x++] = h[((size_t)v * nmat + c) * n2 + (size_t)nu * nbf + mu];
   return 0;
 }
 

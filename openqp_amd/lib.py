@@ -30,7 +30,7 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_grad2", "oqph_fock_jk_device", "oqph_schwarz", "oqph_session_nbf", "oqph_last_error",
            "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e",
            "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources",
-           "oqph_fock_jk_noalloc", "oqph_last_counters", "oqph_session_set_grad_precision"]
+           "oqph_fock_jk_noalloc", "oqph_last_counters", "oqph_session_set_grad_precision", "oqph_mrsf_int2"]
 
 
 class OqphError(RuntimeError):
@@ -70,6 +70,8 @@ def load(path: str | None = None):
     L.oqph_fock_jk_noalloc.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.POINTER(Stats),
                                        C.c_void_p]
     L.oqph_schwarz.argtypes = [vp, _pd]
+    L.oqph_mrsf_int2.argtypes = [vp, C.c_int, C.c_int, C.c_int, _pd, _pd, C.c_double, C.c_double,
+                                 C.POINTER(Stats)]
     L.oqph_session_prepare.argtypes = [vp]
     L.oqph_session_set_share.argtypes = [vp, C.c_int, C.c_int]
     L.oqph_fock_jk_cam.argtypes = [vp, _pd, _pd, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -129,6 +131,19 @@ class FockBuilder:
     def set_grad_precision(self, policy: int, fp32_cutoff: float = 1e-4):
         """2e-gradient precision: 0 = FP64; 1 = quartets with Q_ij Q_kl < fp32_cutoff in FP32."""
         _check(load().oqph_session_set_grad_precision(self._s, policy, fp32_cutoff), "set_grad_precision")
+
+    def mrsf_int2(self, d3, ncoul: int = 4, se: float = 1.0, sc: float = 1.0) -> np.ndarray:
+        """MRSF response integral pass (int2_mrsf_data_t): d3 of shape (nbf, nbf,
+        nmat, nf) is the reference's Fortran d3(nf, nmat, nbf, nbf) in C order;
+        returns f3 of the same shape."""
+        d3 = np.ascontiguousarray(d3, np.float64)
+        nbf2, nbf1, nmat, nf = d3.shape
+        if nbf1 != self.bas.nbf or nbf2 != self.bas.nbf:
+            raise OqphError(f"mrsf_int2: d3 has {nbf1}x{nbf2} AOs, the basis {self.bas.nbf}")
+        f3 = np.empty_like(d3)
+        _check(load().oqph_mrsf_int2(self._s, nf, nmat, ncoul, d3, f3, se, sc, C.byref(self.last_stats)),
+               "mrsf_int2")
+        return f3
 
     def set_share(self, rank: int, size: int):
         """Restrict builds to OpenQP MPI rank `rank` of `size` (int2.F90:759-761)."""

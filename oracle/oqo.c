@@ -693,6 +693,71 @@ static void digest_urohf(const ibuf *b, const double *d, double *f, long ntri, d
   }
 }
 
+/* MRSF response digestion (int2_mrsf_data_t_update, pass 1 FP64 with the
+ * Tamm-Dancoff approximation, tdhf_mrsf_lib.F90:278-308).  d3 and f3 are the
+ * reference's d3(nf, nmat, nbf, nbf) / f3 arrays in Fortran order; components
+ * c < ncoul (the reference: 1..4 of 7) take the Coulomb term with the
+ * symmetrised density ds = d3 + d3^T (precomputed there,
+ * tdhf_mrsf_lib.F90:81-90), every component the eight exchange targets. */
+typedef struct {
+  int nf, nmat, ncoul;
+  const double *d3;
+} mrsf_t;
+
+static void digest_mrsf(const ibuf *b, const mrsf_t *m, long nbf, double *f3, double se, double sc) {
+  const long nf = m->nf, nm = m->nmat;
+  const double *d3 = m->d3;
+#define OQO_D3(v, c, i, j) (((((j) - 1) * nbf + ((i) - 1)) * nm + (c)) * nf + (v))
+  for (int n = 0; n < b->n; ++n) {
+    long i = b->id[4 * n], j = b->id[4 * n + 1], k = b->id[4 * n + 2], l = b->id[4 * n + 3];
+    double val = b->v[n];
+    double xval = val * se, cval = val * sc;
+    for (int c = 0; c < m->ncoul; ++c)
+      for (int v = 0; v < nf; ++v) {
+        double dskl = d3[OQO_D3(v, c, k, l)] + d3[OQO_D3(v, c, l, k)];
+        double dsij = d3[OQO_D3(v, c, i, j)] + d3[OQO_D3(v, c, j, i)];
+        f3[OQO_D3(v, c, i, j)] += cval * dskl;
+        f3[OQO_D3(v, c, j, i)] += cval * dskl;
+        f3[OQO_D3(v, c, k, l)] += cval * dsij;
+        f3[OQO_D3(v, c, l, k)] += cval * dsij;
+      }
+    for (int c = 0; c < nm; ++c)
+      for (int v = 0; v < nf; ++v) {
+        f3[OQO_D3(v, c, i, k)] -= xval * d3[OQO_D3(v, c, j, l)];
+        f3[OQO_D3(v, c, k, i)] -= xval * d3[OQO_D3(v, c, l, j)];
+        f3[OQO_D3(v, c, i, l)] -= xval * d3[OQO_D3(v, c, j, k)];
+        f3[OQO_D3(v, c, l, i)] -= xval * d3[OQO_D3(v, c, k, j)];
+        f3[OQO_D3(v, c, j, k)] -= xval * d3[OQO_D3(v, c, i, l)];
+        f3[OQO_D3(v, c, k, j)] -= xval * d3[OQO_D3(v, c, l, i)];
+        f3[OQO_D3(v, c, j, l)] -= xval * d3[OQO_D3(v, c, i, k)];
+        f3[OQO_D3(v, c, l, j)] -= xval * d3[OQO_D3(v, c, k, i)];
+      }
+  }
+#undef OQO_D3
+}
+
+/* shell_den_screen_mrsf (tdhf_mrsf_lib.F90:176-204): dsh(i, j) = dsh(j, i) =
+ * max |d3(:, nmat, mu, nu)| over mu in shell j, nu in shell i, i >= j (the
+ * last component only, and only that block of each shell pair) */
+static void shell_density_mrsf(const oqo_basis *bs, const mrsf_t *m, double *dsh) {
+  int nsh = bs->nsh;
+  long nbf = bs->nbf, nf = m->nf, nm = m->nmat;
+  for (int si = 0; si < nsh; ++si) {
+    int mini = bs->aooff[si], ni = nao_of(bs, si);
+    for (int sj = 0; sj <= si; ++sj) {
+      int minj = bs->aooff[sj], nj = nao_of(bs, sj);
+      double dm = 0.0;
+      for (int nu = mini; nu < mini + ni; ++nu)
+        for (int mu = minj; mu < minj + nj; ++mu)
+          for (int v = 0; v < nf; ++v) {
+            double x = fabs(m->d3[((nu * nbf + mu) * nm + (nm - 1)) * nf + v]);
+            if (x > dm) dm = x;
+          }
+      dsh[si * nsh + sj] = dsh[sj * nsh + si] = dm;
+    }
+  }
+}
+
 /* storeints (int2.F90:1741-1865): element cutoff, canonical AO order, degeneracy halving */
 static void store_block(const oqo_basis *bs, const int *ids, const double *blk, double cutoff, ibuf *b,
                         long *nint) {
@@ -788,14 +853,15 @@ static double max6(double a, double b, double c, double d, double e, double f) {
 static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                       const int *nprim, const int *poff, const int *aooff, const double *ex,
                       const double *cc, const double *xyz, const double *Q, const double *Qmap, const double *d,
-                      double *f, double se, double sc, double cutoff, double imu2, long *stats, int rank, int size);
+                      double *f, double se, double sc, double cutoff, double imu2, long *stats, int rank, int size,
+                      const mrsf_t *mr);
 
 int oqo_fock_jk(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                 const int *nprim, const int *poff, const int *aooff, const double *ex,
                 const double *cc, const double *xyz, const double *Q, const double *d, double *f,
                 double se, double sc, double cutoff, long *stats) {
   return fock_jk_nf(1, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, NULL, d, f, se, sc,
-                    cutoff, 0.0, stats, 0, 1);
+                    cutoff, 0.0, stats, 0, 1, NULL);
 }
 
 /* Same build restricted to the quartets rank `rank` of `size` computes under
@@ -807,7 +873,7 @@ int oqo_fock_jk_part(int nsh, int nbf, int natm, const int *am, const int *harm,
                      const double *cc, const double *xyz, const double *Q, const double *d, double *f,
                      double se, double sc, double cutoff, long *stats, int rank, int size) {
   return fock_jk_nf(1, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, NULL, d, f, se, sc,
-                    cutoff, 0.0, stats, rank, size);
+                    cutoff, 0.0, stats, rank, size, NULL);
 }
 
 /* General form: nfocks 1 (RHF) or 2 (UHF/ROHF); Q screens the quartets, Qmap
@@ -820,7 +886,20 @@ int oqo_fock_jk_ex(int nsh, int nbf, int natm, const int *am, const int *harm, c
                    double *f, int nfocks, double se, double sc, double cutoff, double imu2, long *stats, int rank,
                    int size) {
   return fock_jk_nf(nfocks, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, Qmap, d, f, se, sc,
-                    cutoff, imu2, stats, rank, size);
+                    cutoff, imu2, stats, rank, size, NULL);
+}
+
+/* MRSF response integral pass (int2_driver%run on int2_mrsf_data_t,
+ * tdhf_mrsf_energy.F90:742-752, tdhf_mrsf_lib.F90:58-308): f3 (same shape as
+ * d3, Fortran order) = sum over the screened quartets of the digestion above;
+ * no final scaling (int2_mrsf_data_t_parallel_stop only reduces). */
+int oqo_mrsf_int2(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                  const int *nprim, const int *poff, const int *aooff, const double *ex,
+                  const double *cc, const double *xyz, const double *Q, int nf, int nmat, int ncoul,
+                  const double *d3, double *f3, double se, double sc, double cutoff, long *stats) {
+  mrsf_t m = {nf, nmat, ncoul, d3};
+  return fock_jk_nf(1, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, NULL, NULL, f3, se, sc,
+                    cutoff, 0.0, stats, 0, 1, &m);
 }
 
 /* UHF/ROHF build (nfocks = 2): d and f hold the alpha then the beta packed
@@ -832,7 +911,7 @@ int oqo_fock_jk_uhf(int nsh, int nbf, int natm, const int *am, const int *harm, 
                     const double *cc, const double *xyz, const double *Q, const double *d, double *f,
                     double se, double sc, double cutoff, long *stats) {
   return fock_jk_nf(2, nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, NULL, d, f, se, sc,
-                    cutoff, 0.0, stats, 0, 1);
+                    cutoff, 0.0, stats, 0, 1, NULL);
 }
 
 /* int2_build_shell_pair_map (int2.F90:864-921): 1-based position of pair
@@ -870,14 +949,16 @@ void oqo_pair_map(int nsh, const int *am, const double *Q, long *out) {
 static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
                       const int *nprim, const int *poff, const int *aooff, const double *ex,
                       const double *cc, const double *xyz, const double *Q, const double *Qmap, const double *d,
-                      double *f, double se, double sc, double cutoff, double imu2, long *stats, int rank, int size) {
+                      double *f, double se, double sc, double cutoff, double imu2, long *stats, int rank, int size,
+                      const mrsf_t *mr) {
   build_tables();
   oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
   oqo_cut cut = make_cut(cutoff);
   cut.imu2 = imu2;
   long ntri = (long)nbf * (nbf + 1) / 2;
   double *dsh = (double *)malloc(sizeof(double) * nsh * nsh);
-  shell_density(&bs, d, nfocks, ntri, dsh);
+  if (mr) shell_density_mrsf(&bs, mr, dsh);
+  else shell_density(&bs, d, nfocks, ntri, dsh);
   double maxden = 0.0;
   for (int x = 0; x < nsh * nsh; ++x)
     if (fabs(dsh[x]) > maxden) maxden = fabs(dsh[x]);
@@ -888,7 +969,7 @@ static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, con
 #ifdef _OPENMP
   nthr = omp_get_max_threads();
 #endif
-  const long nf = (long)nfocks * ntri;
+  const long nf = mr ? (long)mr->nf * mr->nmat * nbf * nbf : (long)nfocks * ntri;
   double *fthr = (double *)calloc((size_t)nthr * nf, sizeof(double));
 #pragma omp parallel reduction(+ : nschwz, nquart, nints)
   {
@@ -925,7 +1006,8 @@ static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, con
             int ids[4] = {i, j, k, l};
             b.n = 0;
             store_block(&bs, ids, out, cutoff, &b, &nints);
-            if (nfocks == 1) digest_rhf(&b, d, fl, se, sc);
+            if (mr) digest_mrsf(&b, mr, nbf, fl, se, sc);
+            else if (nfocks == 1) digest_rhf(&b, d, fl, se, sc);
             else digest_urohf(&b, d, fl, ntri, se, sc);
           }
         }
@@ -937,10 +1019,11 @@ static int fock_jk_nf(int nfocks, int nsh, int nbf, int natm, const int *am, con
   for (long x = 0; x < nf; ++x) {
     double s = 0.0;
     for (int t = 0; t < nthr; ++t) s += fthr[(size_t)t * nf + x];
-    f[x] = 0.5 * s;
+    f[x] = mr ? s : 0.5 * s;
   }
-  for (int fk = 0; fk < nfocks; ++fk)
-    for (long i = 1; i <= nbf; ++i) f[fk * ntri + tri(i, i) - 1] *= 2.0;
+  if (!mr)
+    for (int fk = 0; fk < nfocks; ++fk)
+      for (long i = 1; i <= nbf; ++i) f[fk * ntri + tri(i, i) - 1] *= 2.0;
   free(fthr);
   free(dsh);
   free(pos);

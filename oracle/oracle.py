@@ -48,6 +48,8 @@ def lib():
                                          np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
         L.oqo_fock_jk_part.argtypes = L.oqo_fock_jk.argtypes + [C.c_int, C.c_int]
         L.oqo_fock_jk_uhf.argtypes = L.oqo_fock_jk.argtypes
+        L.oqo_mrsf_int2.argtypes = _BAS + [_pd, C.c_int, C.c_int, C.c_int, _pd, _pd, C.c_double, C.c_double,
+                                           C.c_double, _pl]
         L.oqo_eri_tensor.argtypes = _BAS + [_pd]
         L.oqo_int1e.argtypes = _BAS + [_pd, _pd, _pd, _pd]
         if hasattr(L, "oqo_grad2"):
@@ -165,6 +167,22 @@ def fock_jk_uhf(bas, d_alpha, d_beta, se=1.0, sc=1.0, cutoff=5e-11, Q=None):
     stats = np.zeros(3, dtype=np.int64)
     lib().oqo_fock_jk_uhf(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)), d, f, se, sc, cutoff, stats)
     return f[:bas.ntri], f[bas.ntri:], stats
+
+
+def mrsf_int2(bas, d3, ncoul=4, se=1.0, sc=1.0, cutoff=5e-11, Q=None):
+    """MRSF response integral pass (int2_mrsf_data_t): d3 of shape
+    (nbf, nbf, nmat, nf) is the reference's Fortran d3(nf, nmat, nbf, nbf)
+    read in C order, i.e. d3[nu, mu, c, v] = d3_F(v, c, mu, nu); returns f3 of
+    the same shape and the stats."""
+    if Q is None:
+        Q = schwarz(bas)
+    d3 = np.ascontiguousarray(d3, np.float64)
+    nbf2, nbf1, nmat, nf = d3.shape
+    f3 = np.zeros_like(d3)
+    stats = np.zeros(3, dtype=np.int64)
+    lib().oqo_mrsf_int2(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)), nf, nmat, ncoul, d3, f3, se, sc,
+                        cutoff, stats)
+    return f3, stats
 
 
 def grad2(bas, d_packed, hfscale=1.0, coulscale=1.0, cutoff=1e-10, Q=None):

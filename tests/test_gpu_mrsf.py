@@ -1,0 +1,77 @@
+"""MRSF response integral pass on the device (oqph_mrsf_int2) against the
+oracle's restatement of int2_mrsf_data_t (tdhf_mrsf_lib.F90:58-308)."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import small_system
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _d3(bas, nf, seed):
+    """Non-symmetric transition densities d3 (C order [nu, mu, c, v]) of the
+    size of MRSF's: C_o x C_v^T products of random orbitals."""
+    rng = np.random.default_rng(seed)
+    n = bas.nbf
+    d3 = np.zeros((n, n, 7, nf))
+    for v in range(nf):
+        for c in range(7):
+            a = rng.standard_normal((n, 3)) * 0.3
+            b = rng.standard_normal((n, 3)) * 0.3
+            d3[:, :, c, v] = a @ b.T
+    return d3
+
+
+@pytest.mark.parametrize("name,nw,pure", [("6-31g(d)", 2, None), ("cc-pvdz", 2, None), ("cc-pvtz", 1, None),
+                                          ("cc-pvtz", 1, 0)])
+def test_mrsf_int2_matches_oracle(oracle, name, nw, pure):
+    from openqp_amd import lib as L
+    _say("mrsf_int2", name, nw, "pure" if pure is None else "cartesian")
+    from openqp_amd import basis as B, molecules as M
+    if pure is None:
+        bas = small_system(name, nw=nw, seed=2)
+    else:
+        Z, xyz = M.h2o_kat()
+        bas = B.build_basis(name, Z, xyz, harmonic=pure)
+    d3 = _d3(bas, 2, seed=5)
+    fo, so = oracle.mrsf_int2(bas, d3, ncoul=4, se=0.8, sc=0.8)
+    fb = L.FockBuilder(bas)
+    f3 = fb.mrsf_int2(d3, ncoul=4, se=0.8, sc=0.8)
+    st = fb.last_stats
+    err = np.abs(f3 - fo).max()
+    _say("  max |f3 - f3_oracle| =", err, " quartets", st.quartets_computed, "oracle", so[1])
+    assert err <= 1e-10
+    assert st.quartets_computed == so[1]
+    # the Fock build of the same session still works after the response pass
+    D = d3[:, :, 6, 0] + d3[:, :, 6, 0].T
+    from openqp_amd import basis as B
+    dp = B.pack_tri(D)
+    g = fb.fock_jk(dp)
+    go, _ = oracle.fock_jk(bas, dp)
+    assert np.abs(g - go).max() <= 1e-10
+    fb.close()
+
+
+def test_mrsf_int2_symmetric_density_is_exchange_build(oracle):
+    """A symmetric exchange-only component reproduces the RHF exchange build
+    (se = 2, sc = 0 gives packed -K) on the device."""
+    from openqp_amd import basis as B, lib as L
+    bas = small_system("cc-pvdz", nw=2, seed=4)
+    n = bas.nbf
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((n, 4)) * 0.4
+    D = A @ A.T
+    d3 = np.zeros((n, n, 7, 1))
+    d3[:, :, 6, 0] = D
+    fb = L.FockBuilder(bas)
+    f3 = fb.mrsf_int2(d3, se=1.0, sc=1.0)
+    fr, _ = oracle.fock_jk(bas, B.pack_tri(D), se=2.0, sc=0.0)
+    assert np.abs(B.pack_tri(f3[:, :, 6, 0]) - fr).max() <= 1e-10
+    fb.close()

@@ -441,3 +441,36 @@ def test_rank_shares_sum_to_full_build(oracle):
         nq += s[1]
     assert np.abs(tot - full).max() < 1e-12
     assert nq == st[1]
+
+
+@pytest.mark.parametrize("name", ["6-31g(d)", "cc-pvdz"])
+def test_mrsf_int2_matches_brute_force(oracle, name):
+    """MRSF response integral pass (int2_mrsf_data_t_update, non-symmetric
+    transition densities): f3(v,c) = sc J[M] [c < 4] - se K[M] for every
+    component M = d3(v,c,:,:), J[M]_ij = sum_kl (ij|kl) M_kl, K[M]_ik =
+    sum_jl (ij|kl) M_jl, against the nbf^4 contraction, unscreened.  A
+    dropped transposed exchange target or a Coulomb term on the exchange-only
+    components fails it."""
+    bas = small_system(name, nw=1)
+    n = bas.nbf
+    rng = np.random.default_rng(3)
+    nf, nmat = 2, 7
+    d3 = rng.standard_normal((n, n, nmat, nf))  # C order [nu, mu, c, v]
+    eri = oracle.eri_tensor(bas).reshape(n, n, n, n)
+    f3, st = oracle.mrsf_int2(bas, d3, ncoul=4, se=0.7, sc=0.7, cutoff=1e-30)
+    for v in range(nf):
+        for c in range(nmat):
+            Mt = d3[:, :, c, v]  # [nu, mu]
+            M = Mt.T
+            J = np.einsum("ijkl,kl->ij", eri, M)
+            K = np.einsum("ijkl,jl->ik", eri, M)
+            ref = (0.7 * J if c < 4 else 0.0) - 0.7 * K
+            assert np.abs(f3[:, :, c, v].T - ref).max() < 1e-12, (v, c)
+    # a symmetric density: the same quartets as the RHF build, and its G
+    D = d3[:, :, 6, 0] + d3[:, :, 6, 0].T
+    d3s = np.zeros((n, n, 7, 1))
+    d3s[:, :, 6, 0] = D
+    f3s, sts = oracle.mrsf_int2(bas, d3s, se=1.0, sc=1.0)
+    fr, str_ = oracle.fock_jk(bas, B.pack_tri(D), se=2.0, sc=0.0)
+    assert np.abs(B.pack_tri(f3s[:, :, 6, 0]) - fr).max() < 1e-12
+    assert list(sts) == list(str_)
