@@ -2461,9 +2461,10 @@ int oqph_mrsf_int2(oqph_session* s, int nf, int nmat, int ncoul, const double* d
   if ((rc = s->d_dsh.ensure((size_t)nsh * nsh)) || (rc = s->d_gD.ensure(nd * n2)) ||
       (rc = s->d_gF.ensure(nd * n2)) || (rc = s->d_gcoul.ensure(nd)))
     return rc;
-  if ((rc = upload(s->d_gD.p, h.data(), nd * n2 * sizeof(double), s->stream)) ||
-      (rc = upload(s->d_dsh.p, dsh.data(), dsh.size() * sizeof(double), s->stream)) ||
-      (rc = upload(s->d_gcoul.p, coul.data(), nd * sizeof(int), s->stream)))
+  // synchronous: h, dsh and coul are host locals that an early return frees
+  if ((rc = upload_sync(s->d_gD.p, h.data(), nd * n2 * sizeof(double), s->stream)) ||
+      (rc = upload_sync(s->d_dsh.p, dsh.data(), dsh.size() * sizeof(double), s->stream)) ||
+      (rc = upload_sync(s->d_gcoul.p, coul.data(), nd * sizeof(int), s->stream)))
     return rc;
   if ((rc = build_general(s, (int)nd, s->d_gD.p, s->d_gF.p, s->d_gcoul.p, maxden, se, sc, st, s->stream))) return rc;
   HIPCHK(hipMemcpyAsync(h.data(), s->d_gF.p, nd * n2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));

@@ -27,9 +27,16 @@ def per_dispatch(path):
 
 
 def bench_name(kernel):
-    # closed-shell instances only: <la, lb, lc, ld, pure[, open-shell pass]>
-    m = re.match(r"void oqph::(k_\w+)<(\d), (\d), (\d), (\d), (true|false)(, false)?>", kernel)
-    return f"{m.group(1)}<{m.group(2)},{m.group(3)},{m.group(4)},{m.group(5)}>" if m else None
+    # <la, lb, lc, ld, pure[, open-shell pass[, tiled]]>: closed-shell
+    # instances only; the tiled form of k_jkg is reported as k_jkt
+    m = re.match(r"void oqph::(k_\w+)<(\d), (\d), (\d), (\d), (true|false)((?:, (?:true|false))*)>", kernel)
+    if not m:
+        return None
+    flags = [x.strip() for x in m.group(7).split(",") if x.strip()]
+    if flags and flags[0] == "true":
+        return None
+    kern = "k_jkt" if m.group(1) == "k_jkg" and len(flags) > 1 and flags[1] == "true" else m.group(1)
+    return f"{kern}<{m.group(2)},{m.group(3)},{m.group(4)},{m.group(5)}>"
 
 
 def copy_scale(path, nbytes):
