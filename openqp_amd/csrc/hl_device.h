@@ -190,21 +190,19 @@ __device__ __forceinline__ HLPrim hl_prim(const ShellPair& P, const ShellPair& Q
 }
 
 // 2D integrals of one primitive quartet by the vertical recursion: lane
-// r < nr fills G[(r*3 + d)][i][k], i < ni <= 8, k < nk, for root r and the
-// three axes d (the x table carries the prefactor and the weight); the last
-// two ket columns stay in registers.  The axis loop is unrolled, so every
-// axis component is a compile-time field of HLPrim.
-template <int D>
-__device__ __forceinline__ void hl_vrr_axis(const HLDims& h, const HLPrim& s, double t2, double wr, double* g) {
+// 3 r + d < 3 nr fills G[(r*3 + d)][i][k], i < ni <= 8, k < nk, for root r
+// and axis d (the x table carries the prefactor and the weight); the last
+// two ket columns stay in registers.
+__device__ __forceinline__ void hl_vrr_axis(const HLDims& h, const HLPrim& s, int d, double t2, double wr, double* g) {
   const double B00 = s.h2ze * t2, B10 = s.h2z - s.h2z * s.rz * t2, B01 = s.h2e - s.h2e * s.re * t2;
-  const double pa = D == 0 ? s.PAx : (D == 1 ? s.PAy : s.PAz);
-  const double pq = D == 0 ? s.PQx : (D == 1 ? s.PQy : s.PQz);
-  const double qc = D == 0 ? s.QCx : (D == 1 ? s.QCy : s.QCz);
+  const double pa = d == 0 ? s.PAx : (d == 1 ? s.PAy : s.PAz);
+  const double pq = d == 0 ? s.PQx : (d == 1 ? s.PQy : s.PQz);
+  const double qc = d == 0 ? s.QCx : (d == 1 ? s.QCy : s.QCz);
   const double C00 = pa - s.rz * t2 * pq;
   const double D00 = qc + s.re * t2 * pq;
   const int ni = h.ni, nk = h.nk;
   double c1[8], c2[8];
-  c1[0] = (D == 0) ? s.pref * wr : 1.0;
+  c1[0] = (d == 0) ? s.pref * wr : 1.0;
   g[0] = c1[0];
 #pragma unroll
   for (int i = 1; i < 8; ++i) {
@@ -232,16 +230,28 @@ __device__ __forceinline__ void hl_vrr_axis(const HLDims& h, const HLPrim& s, do
     }
   }
 }
+// AXES: lane 3 r + d takes root r, axis d (shorter chain, more registers);
+// otherwise lane r takes the three axes of root r one after another
+template <bool AXES = true>
 __device__ __forceinline__ void hl_vrr(const HLDims& h, const HLPrim& s, const double* __restrict__ rys, double* G) {
-  const int r = threadIdx.x;
-  if (r >= h.nr) return;
+  const int lane = threadIdx.x;
+  if constexpr (!AXES) {
+    if (lane >= h.nr) return;
+    double t2, wr;
+    hl_root(h.nr, s.T, rys, lane, t2, wr);
+    const int nrk = h.ni * h.nk;
+    double* g = G + 3 * lane * nrk;
+    hl_vrr_axis(h, s, 0, t2, wr, g);
+    hl_vrr_axis(h, s, 1, t2, wr, g + nrk);
+    hl_vrr_axis(h, s, 2, t2, wr, g + 2 * nrk);
+    return;
+  }
+  if (lane >= 3 * h.nr) return;
+  const int r = lane / 3, d = lane - 3 * r;
   double t2, wr;
   hl_root(h.nr, s.T, rys, r, t2, wr);
   const int nrk = h.ni * h.nk;
-  double* g = G + 3 * r * nrk;
-  hl_vrr_axis<0>(h, s, t2, wr, g);
-  hl_vrr_axis<1>(h, s, t2, wr, g + nrk);
-  hl_vrr_axis<2>(h, s, t2, wr, g + 2 * nrk);
+  hl_vrr_axis(h, s, d, t2, wr, G + (3 * r + d) * nrk);
 }
 
 // one index of a 4-index block through the canonical-Cartesian -> AO
@@ -285,6 +295,7 @@ __device__ __forceinline__ void hl_xform_inplace(double* buf, int left, int n, i
 // pointer); nprim = surviving primitive quartets, 0 when none survived (the
 // block is then not formed).  Primitive screens as contract(): |K| < qcut
 // per pair and K_ab^2 K_cd^2 < qcut^2 (zeta + eta) per quartet.
+template <bool AXES = true>
 __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& P, const ShellPair& Q,
                             const PrimJK* prims, const double* __restrict__ rys, const double* __restrict__ xf,
                             double qcut, double imu2, double* lds, int& nprim) {
@@ -310,7 +321,7 @@ __device__ double* hl_block(const HLDims& h, const HLTabs& tb, const ShellPair& 
       ++nprim;
       const HLPrim s = hl_prim(P, Q, pp, qq, imu2);
       __syncthreads();
-      hl_vrr(h, s, rys, G);
+      hl_vrr<AXES>(h, s, rys, G);
       __syncthreads();
       HLStep st(lane, nf);
       for (int idx = lane; idx < nef; idx += HL_BLOCK, st.next()) {
@@ -594,7 +605,9 @@ __device__ __forceinline__ void jkh_chunk(const JKParams& prm, const WaveWork& w
     const int jq = __builtin_amdgcn_readfirstlane(__shfl(iq, l));
     const ShellPair Q = prm.pairs[jq];
     int nprim = 0;
-    const double* V = hl_block(h, tb, P, Q, prm.prims, prm.rys, prm.xform, 1.0e-4 * cut, prm.inv_mu2, lds, nprim);
+    // the response pass keeps one lane per root (its densities' digestion
+    // needs the registers)
+    const double* V = hl_block<!GEN>(h, tb, P, Q, prm.prims, prm.rys, prm.xform, 1.0e-4 * cut, prm.inv_mu2, lds, nprim);
     prims_used += (unsigned long long)nprim;
     if (nprim == 0) continue;
     flops += (unsigned long long)(nprim * fprim + (GEN ? 20 * prm.gnd : prm.F2 ? 24 : 12) * nout);
