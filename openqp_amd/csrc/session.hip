@@ -273,6 +273,9 @@ struct oqph_session {
   // K rows fit in LDS, 3 = as 2 without the LDS density rows, otherwise
   // (default) the bra-row kernel where it fits and also saves atomics
   int rows_mode = 1;
+  // OQPH_LAUNCH_ORDER: 1 (default) = the launches of a build or gradient in
+  // decreasing per-quartet cost (run-time classes first), 0 = class order
+  int launch_order = 1;
   // OQPH_F_TEMPLATES=0: every class with an f shell on the run-time-class
   // kernels (k_jkh, k_gradh) instead of the class templates where they exist
   bool f_templates = true;
@@ -788,6 +791,8 @@ static int prepare_geometry(oqph_session* s) {
     s->grad_hl_ng = gh ? atoi(gh) : 300;
     const char* jh = getenv("OQPH_JK_HL_NG");
     s->jk_hl_ng = jh ? atoi(jh) : 300;
+    const char* lo = getenv("OQPH_LAUNCH_ORDER");
+    s->launch_order = lo ? atoi(lo) : 1;
     const char* kp = getenv("OQPH_HL_KPB");
     const int kpv = kp ? atoi(kp) : 0;
     s->hl_kpb = (kpv == 64 || kpv == 32 || kpv == 16 || kpv == 8) ? kpv : 0;
@@ -952,6 +957,14 @@ static int hl_kpb(const oqph_session* s, long long chunks, int lds) {
   int kpb = 64;
   while (kpb > 8 && chunks * (64 / kpb) < target) kpb /= 2;
   return kpb;
+}
+
+// Per-quartet cost of a class (Cartesian block x roots) for the launch order:
+// launches whose wavefronts are long and few start first, so the many short
+// ones fill the device behind them instead of leaving them as the tail.
+static long long class_weight(int la, int lb, int lc, int ld, bool runtime) {
+  const long long w = (long long)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) * ((la + lb + lc + ld) / 2 + 1);
+  return runtime ? w + (1ll << 40) : w;
 }
 
 // gradient pair data and coarse-screened quartet lists (grd2.F90:219-328):
@@ -1235,6 +1248,12 @@ static int prepare_grad(oqph_session* s) {
       (rc = upload_sync(s->d_pure.p, pu.data(), s->nsh * sizeof(int), s->stream)))
     return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->launch_order == 1)
+    std::stable_sort(s->glaunches.begin(), s->glaunches.end(),
+                     [](const oqph_session::GLaunch& x, const oqph_session::GLaunch& y) {
+                       return class_weight(x.la, x.lb, x.lc, x.ld, x.hl || x.multi) >
+                              class_weight(y.la, y.lb, y.lc, y.ld, y.hl || y.multi);
+                     });
   s->gcut_ready = s->grad_cutoff;
   s->gpolicy_ready = s->grad_policy;
   s->gf32_ready = s->grad_fp32_cut;
@@ -1703,6 +1722,12 @@ static int prepare_lists(oqph_session* s, double maxden, bool two) {
       s->launches.swap(keep);
     }
   }
+  if (s->launch_order == 1)
+    std::stable_sort(s->launches.begin(), s->launches.end(),
+                     [](const oqph_session::WLaunch& x, const oqph_session::WLaunch& y) {
+                       return class_weight(x.la, x.lb, x.lc, x.ld, x.hl || x.multi) >
+                              class_weight(y.la, y.lb, y.lc, y.ld, y.hl || y.multi);
+                     });
   ++s->list_gen;
   s->cached_maxden = maxden;
   s->cached_two = two;
