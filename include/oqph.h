@@ -97,6 +97,19 @@ int oqph_session_set_grad_precision(oqph_session *s, int policy, double fp32_cut
  * (tdhf_mrsf_lib.F90:176-204).  f3 is overwritten. */
 int oqph_mrsf_int2(oqph_session *s, int nf, int nmat, int ncoul, const double *d3, double *f3, double se,
                    double sc, oqph_stats *stats);
+/* MRSF sigma session, the reference's routec_sig ABI (routec_sig.F90:27-57,
+ * bound by routec_sig_load :128-210): (A - B) X in the MO frame for the
+ * singlet (kind 1) / triplet (kind 3) MRSF response of two open shells
+ * (nocca = noccb + 2), replacing mrsfcbc -> int2 (int2_mrsf_data_t) ->
+ * mrsfmntoia + mrsfesum (tdhf_mrsf_energy.F90:686-790).  init takes the MO
+ * coefficients and MO Fock matrices (nbf x nbf, column-major) and uses the
+ * default session's basis and geometry; 0 = ready.  iter: bvec_mo and
+ * sigma_mo are (nocca (nbf - noccb)) x nv_new column-major; info 0 = done. */
+int routec_sig_init(const int *nbf, const double *mo_a, const double *mo_b, const double *fmo_a,
+                    const double *fmo_b, const int *nocca, const int *noccb, const int *kind);
+void routec_sig_set_scale(const double *s);
+void routec_sig_iter(const double *bvec_mo, const int *nv_new, double *sigma_mo, int *info);
+void routec_sig_free(void);
 /* Make s the session used by routec_fock_jk / routec_grad2. */
 int oqph_set_default_session(oqph_session *s);
 /* Build the geometry-dependent device state now (pair records, Schwarz

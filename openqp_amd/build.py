@@ -56,10 +56,11 @@ DEPS = {
     "int1e.hip": ["../../include/oqph.h", "ao_xform.h", "rys_table.inc", "rys_table_f32.inc"],
     "hl.hip": ["hl_device.h", "jk_device.h", "grad_params.h", "eri_device.h", "oqph_common.h", "rys_table.inc",
                "rys_table_f32.inc"],
+    "sigma.hip": ["../../include/oqph.h"],
 }
 
 
-EXTRA = {"int1e.hip": ["-fopenmp"]}
+EXTRA = {"int1e.hip": ["-fopenmp"], "sigma.hip": ["-fopenmp"]}
 
 
 def _key(unit: str, defs) -> str:
@@ -73,7 +74,7 @@ def _key(unit: str, defs) -> str:
 
 def _jobs():
     units = [("session.hip", [], "session.o"), ("dispatch.hip", [], "dispatch.o"), ("int1e.hip", [], "int1e.o"),
-             ("hl.hip", [], "hl.o")]
+             ("hl.hip", [], "hl.o"), ("sigma.hip", [], "sigma.o")]
     heavy, light = [], []
     per_class = [(c, "jk_class.hip", "jk") for c in CLASSES + F_JK] + \
                 [(c, "grad_class.hip", "gr") for c in CLASSES + F_GRAD if c not in GRAD_HL]

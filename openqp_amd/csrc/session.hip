@@ -330,6 +330,11 @@ struct oqph_session {
 };
 
 static oqph_session* g_default = nullptr;
+namespace oqph {
+// the default session (sigma.hip) and the error text its entry points set
+oqph_session* default_session();
+void set_error_text(const char* msg);
+}  // namespace oqph
 static std::mutex g_mtx;
 static unsigned long long g_serial = 0;  // last session serial handed out (under g_mtx)
 
@@ -2093,6 +2098,12 @@ static int build_general(oqph_session* s, int nd, const double* dD, double* dF, 
   }
   return 0;
 }
+
+oqph_session* oqph::default_session() {
+  std::lock_guard<std::mutex> lk(g_mtx);
+  return g_default;
+}
+void oqph::set_error_text(const char* msg) { set_err("%s", msg); }
 
 // ------------------------------------------------------------------ C ABI
 extern "C" {

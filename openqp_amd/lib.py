@@ -30,7 +30,8 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_grad2", "oqph_fock_jk_device", "oqph_schwarz", "oqph_session_nbf", "oqph_last_error",
            "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e",
            "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources",
-           "oqph_fock_jk_noalloc", "oqph_last_counters", "oqph_session_set_grad_precision", "oqph_mrsf_int2"]
+           "oqph_fock_jk_noalloc", "oqph_last_counters", "oqph_session_set_grad_precision", "oqph_mrsf_int2",
+           "routec_sig_init", "routec_sig_set_scale", "routec_sig_iter", "routec_sig_free"]
 
 
 class OqphError(RuntimeError):
@@ -91,6 +92,12 @@ def load(path: str | None = None):
     dp = C.POINTER(C.c_double)
     L.routec_fock_jk.argtypes = [dp, dp, ip, ip, dp, dp, ip]
     L.routec_grad2.argtypes = [dp, dp, dp, ip, ip, dp, dp, ip]
+    L.routec_sig_init.argtypes = [ip, dp, dp, dp, dp, ip, ip, ip]
+    L.routec_sig_set_scale.argtypes = [dp]
+    L.routec_sig_iter.argtypes = [dp, ip, dp, ip]
+    L.routec_sig_set_scale.restype = None
+    L.routec_sig_iter.restype = None
+    L.routec_sig_free.restype = None
     _lib = L
     return L
 
@@ -314,3 +321,36 @@ def routec_fock_jk(d_packed, nbf: int, se: float = 1.0, sc: float = 1.0):
     L.routec_fock_jk(d.ctypes.data_as(C.POINTER(C.c_double)), f.ctypes.data_as(C.POINTER(C.c_double)),
                      C.byref(nb), C.byref(nf), C.byref(se_), C.byref(sc_), C.byref(info))
     return f, info.value
+
+
+class SigmaSession:
+    """The MRSF sigma seam (routec_sig_init / set_scale / iter / free) called by
+    reference exactly as OpenQP's routec_sig module does; uses the default
+    session (oqph_set_default_session) for the basis and geometry."""
+
+    def __init__(self, mo_a, mo_b, fmo_a, fmo_b, nocca: int, noccb: int, kind: int, scale: float = 1.0):
+        L = load()
+        self.nbf = int(np.asarray(mo_a).shape[0])
+        self.nocca, self.noccb = nocca, noccb
+        dp = C.POINTER(C.c_double)
+        f = [np.asfortranarray(x, np.float64) for x in (mo_a, mo_b, fmo_a, fmo_b)]
+        rc = L.routec_sig_init(C.byref(C.c_int(self.nbf)), *[x.ctypes.data_as(dp) for x in f],
+                               C.byref(C.c_int(nocca)), C.byref(C.c_int(noccb)), C.byref(C.c_int(kind)))
+        if rc != 0:
+            raise OqphError(f"routec_sig_init failed (code {rc}): {L.oqph_last_error().decode()}")
+        L.routec_sig_set_scale(C.byref(C.c_double(scale)))
+
+    def iter(self, bvec) -> np.ndarray:
+        """(A - B) X for the columns of bvec (nocca (nbf - noccb) x nv)."""
+        L = load()
+        b = np.asfortranarray(np.asarray(bvec, np.float64).reshape(self.nocca * (self.nbf - self.noccb), -1))
+        out = np.zeros_like(b, order="F")
+        info = C.c_int(-1)
+        dp = C.POINTER(C.c_double)
+        L.routec_sig_iter(b.ctypes.data_as(dp), C.byref(C.c_int(b.shape[1])), out.ctypes.data_as(dp), C.byref(info))
+        if info.value != 0:
+            raise OqphError(f"routec_sig_iter failed (info {info.value}): {L.oqph_last_error().decode()}")
+        return out
+
+    def close(self):
+        load().routec_sig_free()

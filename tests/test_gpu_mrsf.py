@@ -75,3 +75,41 @@ def test_mrsf_int2_symmetric_density_is_exchange_build(oracle):
     fr, _ = oracle.fock_jk(bas, B.pack_tri(D), se=2.0, sc=0.0)
     assert np.abs(B.pack_tri(f3[:, :, 6, 0]) - fr).max() <= 1e-10
     fb.close()
+
+
+@pytest.mark.parametrize("name,nw,kind", [("6-31g(d)", 1, 1), ("6-31g(d)", 1, 3), ("cc-pvdz", 2, 1), ("cc-pvdz", 2, 3)])
+def test_mrsf_sigma_session_matches_oracle(oracle, name, nw, kind):
+    """routec_sig_init / set_scale / iter / free through the C ABI, by
+    reference as OpenQP's routec_sig module calls them, against the oracle's
+    restatement of mrsfcbc -> int2 -> mrsfmntoia + mrsfesum; two iterations,
+    the second with fewer vectors; 1e-10 relative to the largest element."""
+    from oracle import mrsf_sigma as MS
+    from openqp_amd import lib as L
+    _say("MRSF sigma", name, nw, "kind", kind)
+    bas = small_system(name, nw=nw, seed=3)
+    n = bas.nbf
+    S, _, _ = L.int1e(bas)
+    rng = np.random.default_rng(7)
+    w, U = np.linalg.eigh(S)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    va = U @ np.diag(w ** -0.5) @ U.T @ Q
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    vb = U @ np.diag(w ** -0.5) @ U.T @ Q2
+    fa = rng.standard_normal((n, n)); fa = fa + fa.T
+    fb = rng.standard_normal((n, n)); fb = fb + fb.T
+    nocca = 5 * nw + 1
+    noccb = nocca - 2
+    fbld = L.FockBuilder(bas)
+    fbld.make_default()
+    sig = L.SigmaSession(va, vb, fa, fb, nocca, noccb, kind, scale=0.6)
+    try:
+        for nv in (3, 2):
+            B = rng.standard_normal((nocca * (n - noccb), nv))
+            got = sig.iter(B)
+            ref = MS.sigma(bas, va, vb, fa, fb, nocca, noccb, kind, 0.6, B)
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            _say("  nv", nv, "max rel |sigma - sigma_oracle| =", err)
+            assert err <= 1e-10
+    finally:
+        sig.close()
+        fbld.close()
