@@ -34,6 +34,18 @@ def test_routec_signature_matches_reference_seam():
             "const int *natm,") in txt
 
 
+def test_routec_sig_signatures_match_reference_seam():
+    """routec_sig.F90:31-57 binds routec_sig_init(nbf, mo_a, mo_b, fmo_a,
+    fmo_b, nocca, noccb, kind) -> int, routec_sig_set_scale(s),
+    routec_sig_iter(bvec_mo, nv_new, sigma_mo, info), routec_sig_free()."""
+    txt = " ".join(open(os.path.join(ROOT, "include", "oqph.h")).read().split())
+    assert ("int routec_sig_init(const int *nbf, const double *mo_a, const double *mo_b, const double *fmo_a, "
+            "const double *fmo_b, const int *nocca, const int *noccb, const int *kind);") in txt
+    assert "void routec_sig_set_scale(const double *s);" in txt
+    assert "void routec_sig_iter(const double *bvec_mo, const int *nv_new, double *sigma_mo, int *info);" in txt
+    assert "void routec_sig_free(void);" in txt
+
+
 def test_product_does_not_import_oracle():
     import subprocess
     import sys
