@@ -12,7 +12,8 @@ module oqph_bridge
   public :: oqph_session_create, oqph_session_destroy, oqph_session_set_basis, &
             oqph_session_set_geometry, oqph_session_set_thresholds, oqph_set_default_session, &
             oqph_session_set_precision, oqph_session_set_share, oqph_session_prepare, &
-            oqph_fock_jk, oqph_fock_jk_inc, oqph_fock_jk_cam, oqph_grad2
+            oqph_fock_jk, oqph_fock_jk_inc, oqph_fock_jk_cam, oqph_grad2, &
+            oqph_session_set_grad_precision, oqph_mrsf_int2
   public :: oqph_stats
 
   !> Build statistics (oqph_stats of include/oqph.h).
@@ -79,6 +80,27 @@ module oqph_bridge
       type(c_ptr), value :: s
       integer(c_int), value :: policy
       real(c_double), value :: fp32_cutoff
+    end function
+    !> 2e-gradient precision: 0 = FP64, 1 = quartets with Q_ij Q_kl below
+    !> fp32_cutoff in FP32 (no counterpart in OpenQP)
+    integer(c_int) function oqph_session_set_grad_precision(s, policy, fp32_cutoff) &
+        bind(C, name="oqph_session_set_grad_precision")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: s
+      integer(c_int), value :: policy
+      real(c_double), value :: fp32_cutoff
+    end function
+    !> MRSF response integral pass (int2_driver%run on int2_mrsf_data_t,
+    !> tdhf_mrsf_energy.F90:742-752): d3, f3 as the reference's (nf, nmat, nbf, nbf)
+    integer(c_int) function oqph_mrsf_int2(s, nf, nmat, ncoul, d3, f3, se, sc, stats) &
+        bind(C, name="oqph_mrsf_int2")
+      import :: c_ptr, c_int, c_double, oqph_stats
+      type(c_ptr), value :: s
+      integer(c_int), value :: nf, nmat, ncoul
+      real(c_double), intent(in) :: d3(*)
+      real(c_double), intent(out) :: f3(*)
+      real(c_double), value :: se, sc
+      type(oqph_stats), intent(out) :: stats
     end function
     !> OpenQP MPI share: rank of size (int2.F90:759-761)
     integer(c_int) function oqph_session_set_share(s, rank, size) bind(C, name="oqph_session_set_share")
