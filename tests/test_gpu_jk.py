@@ -378,3 +378,35 @@ def test_large_d_classes_template_and_runtime(L, oracle):
     assert np.abs(out["0"][0] - out[None][0]).max() <= 1e-12
     assert out["0"][1:] == out[None][1:]
     assert ("k_jk", 2121) in kinds["0"] and ("k_jk", 2121) not in kinds[None]
+
+
+def test_launch_order_modes(L, oracle):
+    """OQPH_LAUNCH_ORDER=1 (default: launches in decreasing per-quartet cost)
+    and 0 (class order) give the same G (1e-12), the same counts and the same
+    2e gradient (1e-12); both against the oracle."""
+    _say("launch order 1 vs 0")
+    bas = small_system("cc-pvdz", nw=3, seed=31)
+    D, dp = density_for(oracle, bas, 15, seed=32)
+    fo, so = oracle.fock_jk(bas, dp)
+    go, _ = oracle.grad2(bas, dp)
+    out = {}
+    old = os.environ.get("OQPH_LAUNCH_ORDER")
+    try:
+        for mode in ("1", "0"):
+            os.environ["OQPH_LAUNCH_ORDER"] = mode
+            fb = L.FockBuilder(bas)
+            f = fb.fock_jk(dp)
+            st = (fb.last_stats.quartets_computed, fb.last_stats.quartets_screened)
+            g = fb.grad2(dp)
+            out[mode] = (f, st, g)
+            fb.close()
+    finally:
+        if old is None:
+            os.environ.pop("OQPH_LAUNCH_ORDER", None)
+        else:
+            os.environ["OQPH_LAUNCH_ORDER"] = old
+    assert np.abs(out["1"][0] - out["0"][0]).max() <= 1e-12
+    assert out["1"][1] == out["0"][1] and out["1"][1][0] == so[1]
+    assert np.abs(out["1"][0] - fo).max() <= 1e-10
+    assert np.abs(out["1"][2] - out["0"][2]).max() <= 1e-12
+    assert np.abs(out["1"][2] - go).max() <= 1e-8
