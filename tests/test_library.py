@@ -52,3 +52,24 @@ def test_product_does_not_import_oracle():
     code = ("import sys; sys.path.insert(0, %r); import openqp_amd, openqp_amd.lib, openqp_amd.molecules; "
             "assert not any(m == 'oracle' or m.startswith('oracle.') for m in sys.modules)" % ROOT)
     subprocess.check_call([sys.executable, "-c", code])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "openqp_amd", "_lib", "liboqph.so")),
+                    reason="liboqph not built")
+def test_routec_sig_declines_without_session():
+    """routec_sig_init declines (nonzero) with no default session and
+    routec_sig_iter reports info != 0 before a successful init, so OpenQP keeps
+    its native sigma path (routec_sig.F90:213-224); no device call is made."""
+    lib = ctypes.CDLL(_lib_path())
+    i = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    a = (ctypes.c_double * 4)()
+    lib.routec_sig_init.argtypes = [ctypes.POINTER(i), dp, dp, dp, dp, ctypes.POINTER(i), ctypes.POINTER(i),
+                                    ctypes.POINTER(i)]
+    rc = lib.routec_sig_init(ctypes.byref(i(2)), a, a, a, a, ctypes.byref(i(2)), ctypes.byref(i(0)), ctypes.byref(i(1)))
+    assert rc != 0
+    lib.routec_sig_iter.argtypes = [dp, ctypes.POINTER(i), dp, ctypes.POINTER(i)]
+    lib.routec_sig_iter.restype = None
+    info = i(0)
+    lib.routec_sig_iter(a, ctypes.byref(i(1)), a, ctypes.byref(info))
+    assert info.value != 0
