@@ -169,6 +169,39 @@ def scf_leg():
             "fock_ms_total": round(float(fms.sum()), 3), "incremental": True, "guess": "extended Hueckel"}
 
 
+def sigma_leg(steps):
+    """MRSF sigma seam (routec_sig_*) on (H2O)32 / 6-31G(d): two trial vectors
+    per iteration (14 non-symmetric transition densities through the device
+    response pass), synthetic S-orthonormal MOs and a diagonal MO Fock matrix
+    (two open shells over the 160 doubly occupied orbitals)."""
+    from openqp_amd import basis as B, molecules as M
+    from openqp_amd import lib as L
+    bld, bname, nocc = M.WORKLOADS["h2o32_631gd"]
+    Z, xyz = bld(0)
+    bas = B.build_basis(bname, Z, xyz)
+    n = bas.nbf
+    S, _, _ = L.int1e(bas)
+    rng = np.random.default_rng(1)
+    w, U = np.linalg.eigh(S)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    va = U @ np.diag(w ** -0.5) @ U.T @ Q
+    fmo = np.diag(np.sort(rng.standard_normal(n)))
+    nocca, noccb, nv = nocc + 1, nocc - 1, 2
+    fb = L.FockBuilder(bas)
+    fb.make_default()
+    sg = L.SigmaSession(va, va, fmo, fmo, nocca, noccb, 1)
+    bv = rng.standard_normal((nocca * (n - noccb), nv)) * 0.01
+    sg.iter(bv)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sg.iter(bv)
+    dt = (time.perf_counter() - t0) / steps
+    sg.close()
+    fb.close()
+    return {"workload": "h2o32_631gd", "nbf": n, "trial_vectors": nv, "densities": 7 * nv,
+            "ms_per_iteration": round(dt * 1e3, 2)}
+
+
 def fshell_leg(fp32_cutoff, steps):
     """(H2O)16 / cc-pVTZ (928 bf, pure d and f shells): full direct RHF SCF to
     1e-8 Eh on the device builder in FP64 and in mixed precision, the total
@@ -242,6 +275,7 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--no-fshell", action="store_true")
+    ap.add_argument("--no-sigma", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-builds", type=int, default=3,
                     help="builds after the timed ones with per-launch HIP events (launches serialised): the "
@@ -382,6 +416,11 @@ def main():
         log("[bench] SCF: RHF on the (H2O)32/6-31G(d) workload, incremental builds")
         scf = scf_leg()
 
+    sig = None
+    if not a.no_sigma:
+        log("[bench] MRSF sigma: routec_sig_iter on the (H2O)32/6-31G(d) workload, two trial vectors")
+        sig = sigma_leg(2)
+
     fsh = None
     if not a.no_fshell:
         log("[bench] f shells: RHF SCF, builds and gradient on the (H2O)16/cc-pVTZ workload")
@@ -416,6 +455,7 @@ def main():
         "gradient": grad,
         "scf": scf,
         "fshell": fsh,
+        "mrsf_sigma": sig,
         "roofline": roof, "cpu_baseline": cpu,
         "per_class": {str(k): {"kernel": v["kernel"], "ms_fp64": round(v["ms"] / v["n"], 3),
                                "ms_mixed": round(profmx[k]["ms"] / profmx[k]["n"], 3) if k in profmx else None,
