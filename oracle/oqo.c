@@ -1229,10 +1229,103 @@ static void shell_xmap(int l, int pure, double *X /* [nao][ncart] */) {
  * grd2_rys.F90:375-376).  Derivatives: d/dA (a| = 2 alpha (a+1| - a (a-1|
  * per primitive, D by translational invariance.  de: [natm][3].
  */
+/* MRSF two-particle density (grd2_mrsf_compute_data_t_get_density,
+ * tdhf_mrsf_gradient.F90:404-601): the eleven Cartesian matrices
+ * m[0..10] = d2a, d2b, p2a, p2b, ball, bo2v, bo1v, bco1, bco2, o21v, co12
+ * (ncf x ncf, M(i,j) = m[i ncf + j]), scales hs, hs2, cs, qf[3], sgnk. */
+typedef struct {
+  const double *m[11];
+  double hs, hs2, cs, qf[3], sgnk;
+} mrsf_gam_t;
+
+static double mrsf_gamma(const mrsf_gam_t *g, long ncf, long i, long j, long k, long l) {
+#define M_(x, r, c) (g->m[x][(r) * ncf + (c)])
+  const int D2A = 0, D2B = 1, P2A = 2, P2B = 3, BALL = 4, BO2V = 5, BO1V = 6, BCO1 = 7, BCO2 = 8, O21V = 9, CO12 = 10;
+  double df1 = ((M_(D2A, i, j) + M_(P2A, i, j)) * M_(D2A, k, l) + M_(D2A, i, j) * M_(P2A, k, l)) * 4.0 * g->cs;
+  if (g->hs != 0.0 || g->hs2 != 0.0) {
+    double dq1 = (M_(D2A, i, k) + M_(P2A, i, k)) * M_(D2A, j, l) + M_(D2A, i, k) * M_(P2A, j, l) +
+                 (M_(D2A, i, l) + M_(P2A, i, l)) * M_(D2A, j, k) + M_(D2A, i, l) * M_(P2A, j, k) +
+                 (M_(D2B, i, k) + M_(P2B, i, k)) * M_(D2B, j, l) + M_(D2B, i, k) * M_(P2B, j, l) +
+                 (M_(D2B, i, l) + M_(P2B, i, l)) * M_(D2B, j, k) + M_(D2B, i, l) * M_(P2B, j, k);
+    double dt2 = M_(BALL, i, k) * M_(BALL, j, l) + M_(BALL, k, i) * M_(BALL, l, j) +
+                 M_(BALL, i, l) * M_(BALL, j, k) + M_(BALL, l, i) * M_(BALL, k, j);
+    df1 = df1 - g->hs * dq1 - g->hs2 * 2.0 * dt2;
+  }
+  /* the eight-term spin-pair products (:490-586) */
+#define PB(x) (M_(x, i, k) * M_(x, l, j) + M_(x, i, l) * M_(x, k, j) + M_(x, j, k) * M_(x, l, i) + \
+               M_(x, j, l) * M_(x, k, i) + M_(x, l, j) * M_(x, i, k) + M_(x, k, j) * M_(x, i, l) + \
+               M_(x, l, i) * M_(x, j, k) + M_(x, k, i) * M_(x, j, l))
+#define PC(x, y) (M_(x, i, k) * M_(y, j, l) + M_(x, i, l) * M_(y, j, k) + M_(x, j, k) * M_(y, i, l) + \
+                  M_(x, j, l) * M_(y, i, k) + M_(x, l, j) * M_(y, k, i) + M_(x, k, j) * M_(y, l, i) + \
+                  M_(x, l, i) * M_(y, k, j) + M_(x, k, i) * M_(y, l, j))
+#define PD(x, y) (M_(x, i, j) * M_(y, l, k) + M_(x, i, j) * M_(y, k, l) + M_(x, j, i) * M_(y, l, k) + \
+                  M_(x, j, i) * M_(y, k, l) + M_(x, l, k) * M_(y, i, j) + M_(x, k, l) * M_(y, i, j) + \
+                  M_(x, l, k) * M_(y, j, i) + M_(x, k, l) * M_(y, j, i))
+  if (g->qf[0] != 0.0) df1 += g->sgnk * g->qf[0] * PB(CO12);
+  if (g->qf[1] != 0.0) df1 += g->sgnk * g->qf[1] * PB(O21V);
+  if (g->qf[2] != 0.0)
+    df1 += g->sgnk * g->qf[2] *
+           (-PC(BCO1, BO2V) - PC(BCO2, BO1V) - PC(BO2V, BCO1) - PC(BO1V, BCO2) + PD(BCO1, BO2V) + PD(BCO2, BO1V) +
+            PD(BO2V, BCO1) + PD(BO1V, BCO2));
+#undef PB
+#undef PC
+#undef PD
+#undef M_
+  return df1;
+}
+
+static int grad2_core(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                      const int *nprim, const int *poff, const int *aooff, const double *ex,
+                      const double *cc, const double *xyz, const double *Q, const double *d, double *de,
+                      double hfscale, double coulscale, double cutoff, long *stats, const double *mrsf_ao,
+                      const mrsf_gam_t *mg_in);
+
 int oqo_grad2(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
               const int *nprim, const int *poff, const int *aooff, const double *ex,
               const double *cc, const double *xyz, const double *Q, const double *d, double *de,
               double hfscale, double coulscale, double cutoff, long *stats) {
+  return grad2_core(nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, d, de, hfscale, coulscale,
+                    cutoff, stats, NULL, NULL);
+}
+
+/* MRSF 2e gradient (mrsf_2e_grad -> grd2_driver with grd2_mrsf_compute_data_t,
+ * tdhf_mrsf_gradient.F90:204-337): d, p are the raw alpha/beta SCF and
+ * relaxed difference densities (nbf, nbf, 2) and spc the seven response
+ * densities (7, nbf, nbf), all in Fortran order; init forms the sums and
+ * differences (:341-353), each matrix goes to the Cartesian effective form
+ * (:361-390), and Gamma is mrsf_gamma. */
+int oqo_grad2_mrsf(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                   const int *nprim, const int *poff, const int *aooff, const double *ex,
+                   const double *cc, const double *xyz, const double *Q, const double *d, const double *p,
+                   const double *spc, double *de, double hfscale, double hfscale2, double coulscale,
+                   const double *spcscale, int mrst, double cutoff, long *stats) {
+  const size_t n2 = (size_t)nbf * nbf;
+  double *ao = (double *)malloc(sizeof(double) * 11 * n2);  /* row-major M(i, j) = ao[x n2 + i nbf + j] */
+  for (long i = 0; i < nbf; ++i)
+    for (long j = 0; j < nbf; ++j) {
+      const size_t f = (size_t)i + (size_t)nbf * j, r = (size_t)i * nbf + j;
+      const double da = d[f], db = d[f + n2], pa = p[f], pb = p[f + n2];
+      ao[0 * n2 + r] = da + db;
+      ao[1 * n2 + r] = da - db;
+      ao[2 * n2 + r] = pa + pb;
+      ao[3 * n2 + r] = pa - pb;
+      /* spc(c, i, j): ball = 7, bo2v = 1, bo1v = 2, bco1 = 3, bco2 = 4, o21v = 5, co12 = 6 */
+      static const int comp[7] = {6, 0, 1, 2, 3, 4, 5};
+      for (int x = 0; x < 7; ++x) ao[(4 + x) * n2 + r] = spc[comp[x] + 7 * f];
+    }
+  mrsf_gam_t mg = {{0}, hfscale, hfscale2, coulscale, {spcscale[0], spcscale[1], spcscale[2]},
+                   mrst == 3 ? -1.0 : 1.0};
+  int rc = grad2_core(nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz, Q, NULL, de, hfscale,
+                      coulscale, cutoff, stats, ao, &mg);
+  free(ao);
+  return rc;
+}
+
+static int grad2_core(int nsh, int nbf, int natm, const int *am, const int *harm, const int *atom,
+                      const int *nprim, const int *poff, const int *aooff, const double *ex,
+                      const double *cc, const double *xyz, const double *Q, const double *d, double *de,
+                      double hfscale, double coulscale, double cutoff, long *stats, const double *mrsf_ao,
+                      const mrsf_gam_t *mg_in) {
   build_tables();
   oqo_basis bs = {nsh, nbf, natm, am, harm, atom, nprim, poff, aooff, ex, cc, xyz};
   /* Cartesian offsets and Dc = X^T D X */
@@ -1240,8 +1333,37 @@ int oqo_grad2(int nsh, int nbf, int natm, const int *am, const int *harm, const 
   int ncf = 0;
   for (int s = 0; s < nsh; ++s) { coff[s] = ncf; ncf += NCART(am[s]); }
   double *Dc = (double *)calloc((size_t)ncf * ncf, sizeof(double));
+  /* MRSF: the eleven matrices in the same Cartesian form (a general X^T M X) */
+  double *Mc = NULL;
+  mrsf_gam_t mg;
+  if (mrsf_ao) {
+    mg = *mg_in;
+    Mc = (double *)calloc((size_t)11 * ncf * ncf, sizeof(double));
+    double Ya[15 * 15], Yb[15 * 15];
+    for (int x = 0; x < 11; ++x) {
+      const double *M = mrsf_ao + (size_t)x * nbf * nbf;
+      double *C = Mc + (size_t)x * ncf * ncf;
+      for (int a = 0; a < nsh; ++a) {
+        int na = nao_of(&bs, a), ca = NCART(am[a]);
+        shell_xmap(am[a], harm[a], Ya);
+        for (int b = 0; b < nsh; ++b) {
+          int nb = nao_of(&bs, b), cb = NCART(am[b]);
+          shell_xmap(am[b], harm[b], Yb);
+          for (int p = 0; p < ca; ++p)
+            for (int q = 0; q < cb; ++q) {
+              double s = 0.0;
+              for (int i = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j)
+                  s += Ya[i * ca + p] * M[(size_t)(aooff[a] + i) * nbf + aooff[b] + j] * Yb[j * cb + q];
+              C[(size_t)(coff[a] + p) * ncf + coff[b] + q] = s;
+            }
+        }
+      }
+      mg.m[x] = C;
+    }
+  }
   double Xa[15 * 15], Xb[15 * 15];
-  for (int a = 0; a < nsh; ++a) {
+  for (int a = 0; a < nsh && d; ++a) {
     int na = nao_of(&bs, a), ca = NCART(am[a]);
     shell_xmap(am[a], harm[a], Xa);
     for (int b = 0; b < nsh; ++b) {
@@ -1310,8 +1432,9 @@ int oqo_grad2(int nsh, int nbf, int natm, const int *am, const int *harm, const 
                 for (int c = 0; c < nc[2]; ++c)
                   for (int e = 0; e < nc[3]; ++e) {
                     size_t A = coff[i] + a, B_ = coff[j] + b, C = coff[k] + c, E = coff[l] + e;
-                    double g = 4.0 * coulscale * Dc[A * ncf + B_] * Dc[C * ncf + E] -
-                               hfscale * (Dc[A * ncf + C] * Dc[B_ * ncf + E] + Dc[A * ncf + E] * Dc[B_ * ncf + C]);
+                    double g = Mc ? mrsf_gamma(&mg, ncf, (long)A, (long)B_, (long)C, (long)E)
+                                  : 4.0 * coulscale * Dc[A * ncf + B_] * Dc[C * ncf + E] -
+                                        hfscale * (Dc[A * ncf + C] * Dc[B_ * ncf + E] + Dc[A * ncf + E] * Dc[B_ * ncf + C]);
                     G[((a * nc[1] + b) * nc[2] + c) * nc[3] + e] = g;
                     if (fabs(g) > gm) gm = fabs(g);
                   }
@@ -1379,7 +1502,7 @@ int oqo_grad2(int nsh, int nbf, int natm, const int *am, const int *harm, const 
     for (int t = 0; t < nthr; ++t) s += dethr[(size_t)t * natm * 3 + x];
     de[x] = s;
   }
-  free(dethr); free(Dc); free(coff);
+  free(dethr); free(Dc); free(coff); free(Mc);
   if (stats) { stats[0] = skip1; stats[1] = skip2; stats[2] = nint; }
   return 0;
 }

@@ -52,6 +52,8 @@ def lib():
                                            C.c_double, _pl]
         L.oqo_eri_tensor.argtypes = _BAS + [_pd]
         L.oqo_int1e.argtypes = _BAS + [_pd, _pd, _pd, _pd]
+        L.oqo_grad2_mrsf.argtypes = _BAS + [_pd, _pd, _pd, _pd, _pd, C.c_double, C.c_double, C.c_double, _pd,
+                                            C.c_int, C.c_double, _pl]
         if hasattr(L, "oqo_grad2"):
             L.oqo_grad2.argtypes = _BAS + [_pd, _pd, _pd, C.c_double, C.c_double, C.c_double,
                                            np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
@@ -183,6 +185,21 @@ def mrsf_int2(bas, d3, ncoul=4, se=1.0, sc=1.0, cutoff=5e-11, Q=None):
     lib().oqo_mrsf_int2(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)), nf, nmat, ncoul, d3, f3, se, sc,
                         cutoff, stats)
     return f3, stats
+
+
+def grad2_mrsf(bas, d, p, spc, hfscale=1.0, hfscale2=1.0, coulscale=1.0, spcscale=(1.0, 1.0, 1.0), mrst=1,
+               cutoff=1e-10, Q=None):
+    """MRSF 2e gradient (natm, 3): d, p (2, nbf, nbf) raw alpha/beta densities
+    as numpy [spin, j, i] = Fortran (i, j, spin); spc (nbf, nbf, 7) as numpy
+    [j, i, c] = Fortran spc(c, i, j)."""
+    if Q is None:
+        Q = schwarz(bas)
+    de = np.zeros(bas.natm * 3)
+    stats = np.zeros(3, dtype=np.int64)
+    lib().oqo_grad2_mrsf(*bas.carray(), np.ascontiguousarray(Q.reshape(-1)), np.ascontiguousarray(d, np.float64),
+                         np.ascontiguousarray(p, np.float64), np.ascontiguousarray(spc, np.float64), de, hfscale,
+                         hfscale2, coulscale, np.ascontiguousarray(spcscale, np.float64), mrst, cutoff, stats)
+    return de.reshape(bas.natm, 3), stats
 
 
 def grad2(bas, d_packed, hfscale=1.0, coulscale=1.0, cutoff=1e-10, Q=None):

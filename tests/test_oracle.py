@@ -526,3 +526,95 @@ def test_mrsf_sigma_restatement_is_symmetric(oracle, kind):
     keep[nocca - 2:, :2] = False  # the open-shell block carries the extra terms
     keep = keep.T.reshape(-1)
     assert np.abs(s[keep] - ref[keep]).max() < 1e-12
+
+
+def _mrsf_grad_inputs(n, rng, scale=0.3):
+    A = rng.standard_normal((n, n)) * scale
+    Bm = rng.standard_normal((n, n)) * scale
+    da, db = A @ A.T / n, Bm @ Bm.T / n
+    Pa = rng.standard_normal((n, n)) * 0.05
+    Pb = rng.standard_normal((n, n)) * 0.05
+    d = np.array([da.T, db.T])            # numpy [spin, j, i] = Fortran (i, j, spin)
+    p = np.array([(Pa + Pa.T).T, (Pb + Pb.T).T])
+    spcm = rng.standard_normal((7, n, n)) * 0.1   # M_c(i, j) = spcm[c, i, j]
+    spc = np.transpose(spcm, (2, 1, 0)).copy()     # numpy [j, i, c] = Fortran spc(c, i, j)
+    return d, p, spc, spcm
+
+
+def _mrsf_gamma_ao(d, p, spcm, hs, hs2, cs, qf, sgnk):
+    """tdhf_mrsf_gradient.F90:404-601 written with einsum on AO matrices
+    (an independent transcription of the C oracle's mrsf_gamma)."""
+    da, db = d[0].T, d[1].T
+    pa, pb = p[0].T, p[1].T
+    A, Bd, P, PB = da + db, da - db, pa + pb, pa - pb
+    ball, bo2v, bo1v, bco1, bco2, o21v, co12 = spcm[6], spcm[0], spcm[1], spcm[2], spcm[3], spcm[4], spcm[5]
+
+    def T(x, ab, y, cd):
+        return np.einsum(f"{ab},{cd}->ijkl", x, y)
+
+    g = 4.0 * cs * (T(A + P, "ij", A, "kl") + T(A, "ij", P, "kl"))
+    dq1 = (T(A + P, "ik", A, "jl") + T(A, "ik", P, "jl") + T(A + P, "il", A, "jk") + T(A, "il", P, "jk")
+           + T(Bd + PB, "ik", Bd, "jl") + T(Bd, "ik", PB, "jl") + T(Bd + PB, "il", Bd, "jk") + T(Bd, "il", PB, "jk"))
+    dt2 = T(ball, "ik", ball, "jl") + T(ball, "ki", ball, "lj") + T(ball, "il", ball, "jk") + T(ball, "li", ball, "kj")
+    g = g - hs * dq1 - hs2 * 2.0 * dt2
+
+    def pb_(x):
+        return (T(x, "ik", x, "lj") + T(x, "il", x, "kj") + T(x, "jk", x, "li") + T(x, "jl", x, "ki")
+                + T(x, "lj", x, "ik") + T(x, "kj", x, "il") + T(x, "li", x, "jk") + T(x, "ki", x, "jl"))
+
+    def pc_(x, y):
+        return (T(x, "ik", y, "jl") + T(x, "il", y, "jk") + T(x, "jk", y, "il") + T(x, "jl", y, "ik")
+                + T(x, "lj", y, "ki") + T(x, "kj", y, "li") + T(x, "li", y, "kj") + T(x, "ki", y, "lj"))
+
+    def pd_(x, y):
+        return (T(x, "ij", y, "lk") + T(x, "ij", y, "kl") + T(x, "ji", y, "lk") + T(x, "ji", y, "kl")
+                + T(x, "lk", y, "ij") + T(x, "kl", y, "ij") + T(x, "lk", y, "ji") + T(x, "kl", y, "ji"))
+
+    g = g + sgnk * qf[0] * pb_(co12) + sgnk * qf[1] * pb_(o21v)
+    g = g + sgnk * qf[2] * (-pc_(bco1, bo2v) - pc_(bco2, bo1v) - pc_(bo2v, bco1) - pc_(bo1v, bco2)
+                            + pd_(bco1, bo2v) + pd_(bco2, bo1v) + pd_(bo2v, bco1) + pd_(bo1v, bco2))
+    return g
+
+
+@pytest.mark.parametrize("mrst", [1, 3])
+def test_mrsf_grad2_matches_fd_of_gamma_energy(oracle, mrst):
+    """MRSF 2e gradient restatement (grd2_mrsf_compute_data_t): central
+    differences (1e-4 bohr) of E(R) = 1/8 sum_ijkl Gamma_ijkl (ij|kl)(R), with
+    Gamma from an independent einsum transcription of the reference formula
+    on fixed AO matrices; 6-31G(d) (Cartesian d) water, unscreened."""
+    bas = small_system("6-31g(d)", nw=1)
+    n = bas.nbf
+    rng = np.random.default_rng(21 + mrst)
+    d, p, spc, spcm = _mrsf_grad_inputs(n, rng)
+    hs, hs2, cs, qf = 0.7, 0.5, 1.0, (0.9, 1.1, 0.8)
+    sgnk = -1.0 if mrst == 3 else 1.0
+    g, _ = oracle.grad2_mrsf(bas, d, p, spc, hs, hs2, cs, qf, mrst, cutoff=1e-30)
+    gam = _mrsf_gamma_ao(d, p, spcm, hs, hs2, cs, qf, sgnk)
+    h = 1e-4
+    for atom, ax in [(0, 0), (1, 2), (2, 1)]:
+        E = []
+        for sgn in (+1, -1):
+            xyz = np.array(bas.xyz, dtype=float).reshape(-1, 3).copy()
+            xyz[atom, ax] += sgn * h
+            I = oracle.eri_tensor(bas.with_coords(xyz)).reshape(n, n, n, n)
+            E.append(np.sum(gam * I) / 8.0)
+        fd = (E[0] - E[1]) / (2 * h)
+        assert abs(g[atom, ax] - fd) < 1e-6 * max(1.0, abs(fd)), (atom, ax, g[atom, ax], fd)
+    assert np.abs(g.sum(axis=0)).max() < 1e-9  # translational invariance
+
+
+def test_mrsf_grad2_reduces_to_rhf(oracle):
+    """With Da = Db = D/2, no relaxed difference density and no response
+    densities the MRSF two-particle density is the RHF one: the MRSF gradient
+    equals oqo_grad2 of D."""
+    bas = small_system("cc-pvdz", nw=1)
+    n = bas.nbf
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((n, 5))
+    D = A @ A.T / 5
+    d = np.array([0.5 * D, 0.5 * D])
+    z = np.zeros((2, n, n))
+    g1, s1 = oracle.grad2_mrsf(bas, d, z, np.zeros((n, n, 7)), 0.6, 0.3, 1.0, (1.0, 1.0, 1.0), 1)
+    g0, s0 = oracle.grad2(bas, B.pack_tri(D), hfscale=0.6)
+    assert np.abs(g1 - g0).max() < 1e-12
+    assert list(s1) == list(s0)
