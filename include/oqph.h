@@ -48,6 +48,16 @@ void routec_fock_jk(const double *d, double *f, const int *nbf, const int *nfock
  * exchange*hfscale at fixed density. */
 void routec_grad2(const double *d, const double *xyz, double *de, const int *nbf, const int *natm,
                   const double *hfscale, const double *coulscale, int *info);
+/* MRSF 2e gradient, the reference's routec_grad2_mrsf_i (routec_bridge.F90:61-75,
+ * called from mrsf_2e_grad, tdhf_mrsf_gradient.F90:281-296): d and p are the
+ * raw alpha/beta SCF and relaxed difference densities (nbf, nbf, 2), spc the
+ * seven MRSF response densities (7, nbf, nbf), Fortran order; de (3, natm)
+ * receives the gradient of grd2_mrsf_compute_data_t's two-particle density
+ * (tdhf_mrsf_gradient.F90:404-601); spcscale = (CO-CO, OV-OV, CO-OV) spin-pair
+ * scales; mrst 1 singlet / 3 triplet.  Uses the default session. */
+void routec_grad2_mrsf(const double *d, const double *p, const double *spc, const double *xyz, double *de,
+                       const int *nbf, const int *natm, const double *hfscale, const double *hfscale2,
+                       const double *coulscale, const double *spcscale, const int *mrst, int *info);
 
 /* ---- Session (no counterpart in the seam, which passes no basis) ------- */
 
@@ -95,6 +105,10 @@ int oqph_session_set_grad_precision(oqph_session *s, int policy, double fp32_cut
  * M = d3(v,c,:,:), J[M]_ij = sum_kl (ij|kl) M_kl, K[M]_ik = sum_jl (ij|kl)
  * M_jl, screened on the last component as shell_den_screen_mrsf
  * (tdhf_mrsf_lib.F90:176-204).  f3 is overwritten. */
+/* The MRSF 2e gradient of routec_grad2_mrsf on session s (same arguments). */
+int oqph_grad2_mrsf(oqph_session *s, const double *d, const double *p, const double *spc, double *de,
+                    double hfscale, double hfscale2, double coulscale, const double *spcscale, int mrst,
+                    oqph_stats *stats);
 int oqph_mrsf_int2(oqph_session *s, int nf, int nmat, int ncoul, const double *d3, double *f3, double se,
                    double sc, oqph_stats *stats);
 /* MRSF sigma session, the reference's routec_sig ABI (routec_sig.F90:27-57,

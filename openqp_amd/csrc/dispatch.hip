@@ -185,8 +185,10 @@ bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const Wave
 // Cartesian-effective density Dc = X^T D X, one thread per shell pair block
 // (build_cart_density, source/basis_tools.F90:407-452, with the raw x^l
 // normalised Cartesians of the derivative kernels)
+// nbf > 0: D is a full row-major nbf x nbf matrix (not necessarily
+// symmetric) instead of the packed triangle
 __global__ void k_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off,
-                               const int* coff, const double* xf, int ncf, double* Dc) {
+                               const int* coff, const double* xf, int ncf, double* Dc, int nbf) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nsh * nsh) return;
   const int a = (int)(t / nsh), b = (int)(t % nsh);
@@ -202,7 +204,8 @@ __global__ void k_cart_density(const double* D, int nsh, const int* am, const in
         for (int j = 0; j < nb; ++j) {
           const double xb = lb >= 2 ? xf[(((lb - 2) * 2 + (pure[b] ? 1 : 0)) * 10 + j) * 10 + q] : (j == q ? 1.0 : 0.0);
           if (xb == 0.0) continue;
-          s += xa * D[tri_idx_(ao_off[a] + i, ao_off[b] + j)] * xb;
+          const int r = ao_off[a] + i, c = ao_off[b] + j;
+          s += xa * (nbf > 0 ? D[(long long)r * nbf + c] : D[tri_idx_(r, c)]) * xb;
         }
       }
       Dc[(long long)(coff[a] + p) * ncf + coff[b] + q] = s;
@@ -213,7 +216,15 @@ void launch_cart_density(const double* D, int nsh, const int* am, const int* pur
                          const double* xf, int ncf, double* Dc, hipStream_t s) {
   const long long n = (long long)nsh * nsh;
   hipLaunchKernelGGL(k_cart_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, nsh, am, pure, ao_off,
-                     coff, xf, ncf, Dc);
+                     coff, xf, ncf, Dc, 0);
+}
+
+// the same transform of a full (general) row-major nbf x nbf matrix
+void launch_cart_matrix(const double* M, int nbf, int nsh, const int* am, const int* pure, const int* ao_off,
+                        const int* coff, const double* xf, int ncf, double* Mc, hipStream_t s) {
+  const long long n = (long long)nsh * nsh;
+  hipLaunchKernelGGL(k_cart_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, M, nsh, am, pure, ao_off,
+                     coff, xf, ncf, Mc, nbf);
 }
 
 bool kernels_support(int lmax) { return lmax <= 3; }

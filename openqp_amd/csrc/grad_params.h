@@ -23,6 +23,13 @@ struct GradParams {
   double gcut;            // grad_cutoff
   double dabcut2;         // (two-particle density cutoff)^2
   double cs, hs;          // Coulomb and exchange scales
+  // MRSF two-particle density (run-time-class kernel only): when Mc is set,
+  // Gamma comes from the eleven Cartesian matrices Mc[x ncf^2 ..] (row-major,
+  // M(i, j) = Mc[x ncf^2 + i ncf + j]): d2a, d2b, p2a, p2b, ball, bo2v, bo1v,
+  // bco1, bco2, o21v, co12, with the response exchange scale hs2, the
+  // spin-pair scales qf[3] and the multiplicity sign sgnk; Dc is unused.
+  const double* Mc;
+  double hs2, qf0, qf1, qf2, sgnk;
 };
 
 }  // namespace oqph

@@ -110,14 +110,24 @@ void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prim
 
 void launch_gradh_multi(const GradParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s) {
   if (nblocks == 0) return;
-  allow_lds(k_gradh_multi, lds);
-  hipLaunchKernelGGL(k_gradh_multi, dim3((unsigned)nblocks), dim3(HL_BLOCK), lds, s, p, tl);
+  if (p.Mc) {  // the MRSF two-particle density
+    allow_lds(k_gradh_multi<true>, lds);
+    hipLaunchKernelGGL(k_gradh_multi<true>, dim3((unsigned)nblocks), dim3(HL_BLOCK), lds, s, p, tl);
+  } else {
+    allow_lds(k_gradh_multi<false>, lds);
+    hipLaunchKernelGGL(k_gradh_multi<false>, dim3((unsigned)nblocks), dim3(HL_BLOCK), lds, s, p, tl);
+  }
 }
 
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s) {
   if (ww.nwaves == 0) return;
-  allow_lds(k_gradh, h.lds);
-  hipLaunchKernelGGL(k_gradh, dim3(hl_blocks(ww)), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+  if (p.Mc) {
+    allow_lds(k_gradh<true>, h.lds);
+    hipLaunchKernelGGL(k_gradh<true>, dim3(hl_blocks(ww)), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+  } else {
+    allow_lds(k_gradh<false>, h.lds);
+    hipLaunchKernelGGL(k_gradh<false>, dim3(hl_blocks(ww)), dim3(HL_BLOCK), h.lds, s, p, ww, h);
+  }
 }
 
 }  // namespace oqph

@@ -60,6 +60,8 @@ struct GradParams;
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s);
 void launch_gradh_multi(const GradParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s);
 bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32);
+void launch_cart_matrix(const double* M, int nbf, int nsh, const int* am, const int* pure, const int* ao_off,
+                        const int* coff, const double* xf, int ncf, double* Mc, hipStream_t s);
 void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
                          const double* xf, int ncf, double* Dc, hipStream_t s);
 const double* rys_table_host(size_t* n);
@@ -318,6 +320,9 @@ struct oqph_session {
   // gradient precision (oqph_session_set_grad_precision): 1 = quartets with
   // Q_ij Q_kl below grad_fp32_cut run in the FP32 form of their template
   int grad_policy = 0;
+  // MRSF gradient (oqph_grad2_mrsf): every class on the run-time-class kernel
+  bool grad_all_hl = false, ghl_ready = false;
+  DBuf<double> d_gM, d_gMc;  // the eleven MRSF matrices, AO (row-major) and Cartesian
   double grad_fp32_cut = 1.0e-4;
   int gpolicy_ready = -1;
   double gf32_ready = -1.0;
@@ -979,7 +984,7 @@ static int prepare_grad(oqph_session* s) {
   int rc = prepare_geometry(s);
   if (rc) return rc;
   if (s->gcut_ready == s->grad_cutoff && s->gpolicy_ready == s->grad_policy &&
-      (s->grad_policy == 0 || s->gf32_ready == s->grad_fp32_cut))
+      (s->grad_policy == 0 || s->gf32_ready == s->grad_fp32_cut) && s->ghl_ready == s->grad_all_hl)
     return 0;
   // the per-atom gradient sums are staged in LDS (3 natm doubles per block)
   if (24 * (size_t)s->natm + 16384 > s->lds_max) {
@@ -1075,7 +1080,7 @@ static int prepare_grad(oqph_session* s) {
           // rounded up to whole wavefronts, so the split adds no partly
           // filled wavefront
           int khi = k;
-          if (s->grad_policy == 1) {
+          if (s->grad_policy == 1 && !s->grad_all_hl) {
             khi = (int)(std::lower_bound(QK.begin() + kb[b], QK.begin() + kb[b] + k, s->grad_fp32_cut / QB[p],
                                          [](double a, double t) { return a >= t; }) - (QK.begin() + kb[b]));
             khi = std::min(k, (khi + 63) / 64 * 64);
@@ -1130,7 +1135,7 @@ static int prepare_grad(oqph_session* s) {
     L.ww.ket = s->d_gcls.p + coff[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
-    L.hl = !grad_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3)) ||
+    L.hl = s->grad_all_hl || !grad_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3)) ||
            (s->grad_hl_ng > 0 && ncart(p.la) * ncart(p.lb) * ncart(p.lc) * ncart(p.ld) > s->grad_hl_ng);
     if (!L.hl) {  // templates whose private segment exceeds the budget run on k_gradh
       KernelRes kr{};
@@ -1260,6 +1265,7 @@ static int prepare_grad(oqph_session* s) {
                               class_weight(y.la, y.lb, y.lc, y.ld, y.hl || y.multi);
                      });
   s->gcut_ready = s->grad_cutoff;
+  s->ghl_ready = s->grad_all_hl;
   s->gpolicy_ready = s->grad_policy;
   s->gf32_ready = s->grad_fp32_cut;
   return 0;
@@ -2167,7 +2173,7 @@ int oqph_session_destroy(oqph_session* s) {
   s->d_am.release(); s->d_pure.release(); s->d_gseg.release(); s->d_gchoff.release(); s->d_Dc.release(); s->d_de.release();
   s->d_aooff.release(); s->d_nao.release(); s->d_colidx.release(); s->d_colao.release(); s->d_items.release(); s->d_Dt.release(); s->d_D.release(); s->d_F.release(); s->d_dsh.release();
   s->d_tmp.release(); s->d_off.release(); s->d_ctr.release();
-  s->d_gD.release(); s->d_gF.release(); s->d_gcoul.release();
+  s->d_gD.release(); s->d_gF.release(); s->d_gcoul.release(); s->d_gM.release(); s->d_gMc.release();
   for (auto e : s->pev) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
@@ -2513,20 +2519,11 @@ int oqph_mrsf_int2(oqph_session* s, int nf, int nmat, int ncoul, const double* d
   return 0;
 }
 
-int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, double coulscale, oqph_stats* st) {
-  if (!s || !d || !de) return 4;
-  if (int fc = fault_code()) {
-    set_err("OQPH_FAULT injected decline");
-    return fc;
-  }
-  HIPCHK(hipSetDevice(s->device));
-  int rc = prepare_grad(s);
-  if (rc) return rc;
-  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+// The gradient launches over the Cartesian density (p.Dc) or the MRSF
+// matrices (p.Mc) the caller set in p; de receives the natm x 3 sums.
+static int grad_run(oqph_session* s, GradParams& p, double* de, oqph_stats* st) {
+  int rc;
   hipStream_t stream = s->stream;
-  if ((rc = upload(s->d_D.p, d, ntri * sizeof(double), stream))) return rc;
-  launch_cart_density(s->d_D.p, s->nsh, s->d_am.p, s->d_pure.p, s->d_aooff.p, s->d_coff.p, s->d_xform.p, s->ncf,
-                      s->d_Dc.p, stream);
   const size_t nde = 3 * (size_t)s->natm;
   HIPCHK(hipMemsetAsync(s->d_de.p, 0, nde * OQPH_DE_SLOTS * sizeof(double), stream));
   const size_t nl = s->glaunches.size();
@@ -2538,12 +2535,10 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   double dabcut = 1.0e-11;                 // grd2.F90:224-226
   if (zbig > 1.0e6) dabcut /= 10;
   if (zbig > 1.0e7) dabcut /= 10;
-  GradParams p{};
   p.pairs = s->d_gpairs.p;
   p.prims = s->d_gprims.p;
   p.rys = s->d_rys.p;
   p.rys32 = s->d_rys32.p;
-  p.Dc = s->d_Dc.p;
   p.coff = s->d_coff.p;
   p.shell_atom = s->d_satom.p;
   p.de = s->d_de.p;
@@ -2551,8 +2546,6 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
   p.natm = s->natm;
   p.gcut = s->grad_cutoff;
   p.dabcut2 = dabcut * dabcut;
-  p.cs = coulscale;
-  p.hs = hfscale;
   HIPCHK(hipEventRecord(s->ev0, stream));
   const int nst = s->profile ? 1 : s->nstreams;
   if ((rc = streams_fork(s, stream, nst))) return rc;
@@ -2564,6 +2557,9 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
       launch_gradh_multi(p, L.tl, L.nblocks, L.lds, ls);
     } else if (L.hl) {
       launch_gradh(L.hd, p, L.ww, ls);
+    } else if (p.Mc) {
+      set_err("MRSF gradient: class (%d %d|%d %d) is not on the run-time-class kernel", L.la, L.lb, L.lc, L.ld);
+      return 1;
     } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, ls, L.f32)) {
       set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
@@ -2599,6 +2595,73 @@ int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, dou
     st->kernel_ms = ms;
   }
   return 0;
+}
+
+int oqph_grad2(oqph_session* s, const double* d, double* de, double hfscale, double coulscale, oqph_stats* st) {
+  if (!s || !d || !de) return 4;
+  if (int fc = fault_code()) {
+    set_err("OQPH_FAULT injected decline");
+    return fc;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  int rc = prepare_grad(s);
+  if (rc) return rc;
+  const size_t ntri = (size_t)s->nbf * (s->nbf + 1) / 2;
+  if ((rc = upload(s->d_D.p, d, ntri * sizeof(double), s->stream))) return rc;
+  launch_cart_density(s->d_D.p, s->nsh, s->d_am.p, s->d_pure.p, s->d_aooff.p, s->d_coff.p, s->d_xform.p, s->ncf,
+                      s->d_Dc.p, s->stream);
+  GradParams p{};
+  p.Dc = s->d_Dc.p;
+  p.cs = coulscale;
+  p.hs = hfscale;
+  return grad_run(s, p, de, st);
+}
+
+int oqph_grad2_mrsf(oqph_session* s, const double* d, const double* pd, const double* spc, double* de, double hfscale,
+                    double hfscale2, double coulscale, const double* spcscale, int mrst, oqph_stats* st) {
+  if (!s || !d || !pd || !spc || !de || !spcscale) return 4;
+  if (int fc = fault_code()) {
+    set_err("OQPH_FAULT injected decline");
+    return fc;
+  }
+  HIPCHK(hipSetDevice(s->device));
+  s->grad_all_hl = true;
+  int rc = prepare_grad(s);
+  s->grad_all_hl = false;
+  if (rc) return rc;
+  const int nbf = s->nbf;
+  const size_t n2 = (size_t)nbf * nbf;
+  // the sums and differences of the raw spin densities (grd2_mrsf_compute_data_t_init,
+  // tdhf_mrsf_gradient.F90:341-353) and the seven response densities, row-major
+  std::vector<double> h(11 * n2);
+  static const int comp[7] = {6, 0, 1, 2, 3, 4, 5};  // ball, bo2v, bo1v, bco1, bco2, o21v, co12 in spc(1:7)
+  for (int i = 0; i < nbf; ++i)
+    for (int j = 0; j < nbf; ++j) {
+      const size_t f = (size_t)i + (size_t)nbf * j, r = (size_t)i * nbf + j;
+      h[0 * n2 + r] = d[f] + d[f + n2];
+      h[1 * n2 + r] = d[f] - d[f + n2];
+      h[2 * n2 + r] = pd[f] + pd[f + n2];
+      h[3 * n2 + r] = pd[f] - pd[f + n2];
+      for (int x = 0; x < 7; ++x) h[(4 + x) * n2 + r] = spc[comp[x] + 7 * f];
+    }
+  const size_t nc2 = (size_t)s->ncf * s->ncf;
+  if ((rc = s->d_gM.ensure(11 * n2)) || (rc = s->d_gMc.ensure(11 * nc2))) return rc;
+  if ((rc = upload_sync(s->d_gM.p, h.data(), 11 * n2 * sizeof(double), s->stream))) return rc;
+  for (int x = 0; x < 11; ++x)
+    launch_cart_matrix(s->d_gM.p + x * n2, nbf, s->nsh, s->d_am.p, s->d_pure.p, s->d_aooff.p, s->d_coff.p,
+                       s->d_xform.p, s->ncf, s->d_gMc.p + x * nc2, s->stream);
+  GradParams p{};
+  p.Mc = s->d_gMc.p;
+  p.cs = coulscale;
+  p.hs = hfscale;
+  p.hs2 = hfscale2;
+  p.qf0 = spcscale[0];
+  p.qf1 = spcscale[1];
+  p.qf2 = spcscale[2];
+  p.sgnk = mrst == 3 ? -1.0 : 1.0;
+  rc = grad_run(s, p, de, st);
+  // the next ground-state gradient rebuilds its lists with the class templates
+  return rc;
 }
 
 // CAM build (int2_run_cam, source/integrals/int2.F90:538-584): pass 1 with
@@ -2727,6 +2790,36 @@ void routec_grad2(const double* d, const double* xyz, double* de, const int* nbf
     s->ready = false;
   }
   *info = oqph_grad2(s, d, de, *hfscale, *coulscale, nullptr);
+}
+
+void routec_grad2_mrsf(const double* d, const double* pd, const double* spc, const double* xyz, double* de,
+                       const int* nbf, const int* natm, const double* hfscale, const double* hfscale2,
+                       const double* coulscale, const double* spcscale, const int* mrst, int* info) {
+  if (!info) return;
+  *info = 1;
+  oqph_session* s;
+  {
+    std::lock_guard<std::mutex> lk(g_mtx);
+    s = g_default;
+  }
+  if (!s || !d || !pd || !spc || !xyz || !de || !nbf || !natm || !hfscale || !hfscale2 || !coulscale ||
+      !spcscale || !mrst) {
+    set_err("routec_grad2_mrsf: no default session");
+    return;
+  }
+  if (*nbf != s->nbf || *natm != s->natm) {
+    set_err("routec_grad2_mrsf: dimensions do not match the session");
+    *info = 4;
+    return;
+  }
+  bool moved = false;
+  for (int i = 0; i < 3 * s->natm; ++i)
+    if (xyz[i] != s->xyz[i]) moved = true;
+  if (moved) {
+    s->xyz.assign(xyz, xyz + 3 * s->natm);
+    s->ready = false;
+  }
+  *info = oqph_grad2_mrsf(s, d, pd, spc, de, *hfscale, *hfscale2, *coulscale, spcscale, *mrst, nullptr);
 }
 
 }  // extern "C"

@@ -31,7 +31,8 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_session_set_profiling", "oqph_last_profile", "oqph_int1e", "oqph_grad1e",
            "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources",
            "oqph_fock_jk_noalloc", "oqph_last_counters", "oqph_session_set_grad_precision", "oqph_mrsf_int2",
-           "routec_sig_init", "routec_sig_set_scale", "routec_sig_iter", "routec_sig_free"]
+           "routec_sig_init", "routec_sig_set_scale", "routec_sig_iter", "routec_sig_free", "oqph_grad2_mrsf",
+           "routec_grad2_mrsf"]
 
 
 class OqphError(RuntimeError):
@@ -93,6 +94,10 @@ def load(path: str | None = None):
     L.routec_fock_jk.argtypes = [dp, dp, ip, ip, dp, dp, ip]
     L.routec_grad2.argtypes = [dp, dp, dp, ip, ip, dp, dp, ip]
     L.routec_sig_init.argtypes = [ip, dp, dp, dp, dp, ip, ip, ip]
+    L.oqph_grad2_mrsf.argtypes = [vp, _pd, _pd, _pd, _pd, C.c_double, C.c_double, C.c_double, _pd, C.c_int,
+                                  C.POINTER(Stats)]
+    L.routec_grad2_mrsf.argtypes = [dp, dp, dp, dp, dp, ip, ip, dp, dp, dp, dp, ip, ip]
+    L.routec_grad2_mrsf.restype = None
     L.routec_sig_set_scale.argtypes = [dp]
     L.routec_sig_iter.argtypes = [dp, ip, dp, ip]
     L.routec_sig_set_scale.restype = None
@@ -138,6 +143,18 @@ class FockBuilder:
     def set_grad_precision(self, policy: int, fp32_cutoff: float = 1e-4):
         """2e-gradient precision: 0 = FP64; 1 = quartets with Q_ij Q_kl < fp32_cutoff in FP32."""
         _check(load().oqph_session_set_grad_precision(self._s, policy, fp32_cutoff), "set_grad_precision")
+
+    def grad2_mrsf(self, d, p, spc, hfscale: float = 1.0, hfscale2: float = 1.0, coulscale: float = 1.0,
+                   spcscale=(1.0, 1.0, 1.0), mrst: int = 1) -> np.ndarray:
+        """MRSF 2e gradient (natm, 3): d, p as numpy (2, nbf, nbf) = Fortran
+        (nbf, nbf, 2) raw alpha/beta densities, spc as numpy (nbf, nbf, 7) =
+        Fortran spc(7, nbf, nbf)."""
+        de = np.zeros(3 * self.bas.natm)
+        _check(load().oqph_grad2_mrsf(self._s, np.ascontiguousarray(d, np.float64), np.ascontiguousarray(p, np.float64),
+                                      np.ascontiguousarray(spc, np.float64), de, hfscale, hfscale2, coulscale,
+                                      np.ascontiguousarray(spcscale, np.float64), mrst, C.byref(self.last_stats)),
+               "grad2_mrsf")
+        return de.reshape(self.bas.natm, 3)
 
     def mrsf_int2(self, d3, ncoul: int = 4, se: float = 1.0, sc: float = 1.0) -> np.ndarray:
         """MRSF response integral pass (int2_mrsf_data_t): d3 of shape (nbf, nbf,

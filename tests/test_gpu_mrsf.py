@@ -113,3 +113,57 @@ def test_mrsf_sigma_session_matches_oracle(oracle, name, nw, kind):
     finally:
         sig.close()
         fbld.close()
+
+
+def _grad_inputs(n, rng):
+    A = rng.standard_normal((n, n)) * 0.3
+    Bm = rng.standard_normal((n, n)) * 0.3
+    d = np.array([(A @ A.T / n).T, (Bm @ Bm.T / n).T])
+    Pa = rng.standard_normal((n, n)) * 0.05
+    Pb = rng.standard_normal((n, n)) * 0.05
+    p = np.array([(Pa + Pa.T).T, (Pb + Pb.T).T])
+    spc = rng.standard_normal((n, n, 7)) * 0.1
+    return d, p, spc
+
+
+@pytest.mark.parametrize("name,nw,mrst", [("6-31g(d)", 1, 1), ("6-31g(d)", 1, 3), ("cc-pvdz", 2, 1)])
+def test_mrsf_grad2_matches_oracle(oracle, name, nw, mrst):
+    """MRSF 2e gradient (routec_grad2_mrsf, by reference through the C ABI,
+    and oqph_grad2_mrsf) against the oracle's grd2_mrsf restatement, 1e-8;
+    afterwards the ground-state gradient of the same session is unchanged."""
+    import ctypes as C
+    from openqp_amd import basis as B, lib as L
+    _say("MRSF gradient", name, nw, "mrst", mrst)
+    bas = small_system(name, nw=nw, seed=6)
+    n = bas.nbf
+    rng = np.random.default_rng(40 + mrst)
+    d, p, spc = _grad_inputs(n, rng)
+    hs, hs2, cs, qf = 0.7, 0.5, 1.0, (0.9, 1.1, 0.8)
+    go, so = oracle.grad2_mrsf(bas, d, p, spc, hs, hs2, cs, qf, mrst)
+    fb = L.FockBuilder(bas)
+    fb.make_default()
+    g = fb.grad2_mrsf(d, p, spc, hs, hs2, cs, qf, mrst)
+    err = np.abs(g - go).max()
+    _say("  max |g - g_oracle| =", err, "quartets", fb.last_stats.quartets_computed, "oracle", so[2])
+    assert err <= 1e-8
+    assert fb.last_stats.quartets_computed == so[2]
+    # the reference's by-reference ABI
+    lib = L.load()
+    dp_ = C.POINTER(C.c_double)
+    xyz = np.ascontiguousarray(np.asarray(bas.xyz, np.float64).reshape(-1))
+    de = np.zeros(3 * bas.natm)
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (d, p, spc)]
+    sc = np.array(qf, np.float64)
+    info = C.c_int(-1)
+    lib.routec_grad2_mrsf(*[a.ctypes.data_as(dp_) for a in arrs], xyz.ctypes.data_as(dp_), de.ctypes.data_as(dp_),
+                          C.byref(C.c_int(n)), C.byref(C.c_int(bas.natm)), C.byref(C.c_double(hs)),
+                          C.byref(C.c_double(hs2)), C.byref(C.c_double(cs)), sc.ctypes.data_as(dp_),
+                          C.byref(C.c_int(mrst)), C.byref(info))
+    assert info.value == 0
+    assert np.abs(de.reshape(-1, 3) - go).max() <= 1e-8
+    # ground-state gradient after the MRSF one: class templates again, same result as the oracle
+    D = d[0].T + d[1].T
+    dpk = B.pack_tri(0.5 * (D + D.T))
+    g0, _ = oracle.grad2(bas, dpk)
+    assert np.abs(fb.grad2(dpk) - g0).max() <= 1e-8
+    fb.close()

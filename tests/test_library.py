@@ -34,6 +34,15 @@ def test_routec_signature_matches_reference_seam():
             "const int *natm,") in txt
 
 
+def test_routec_grad2_mrsf_signature_matches_reference_seam():
+    """routec_bridge.F90:61-75 binds routec_grad2_mrsf(d, p, spc, xyz, de, nbf,
+    natm, hfscale, hfscale2, coulscale, spcscale, mrst, info), all by reference."""
+    txt = " ".join(open(os.path.join(ROOT, "include", "oqph.h")).read().split())
+    assert ("void routec_grad2_mrsf(const double *d, const double *p, const double *spc, const double *xyz, "
+            "double *de, const int *nbf, const int *natm, const double *hfscale, const double *hfscale2, "
+            "const double *coulscale, const double *spcscale, const int *mrst, int *info);") in txt
+
+
 def test_routec_sig_signatures_match_reference_seam():
     """routec_sig.F90:31-57 binds routec_sig_init(nbf, mo_a, mo_b, fmo_a,
     fmo_b, nocca, noccb, kind) -> int, routec_sig_set_scale(s),
