@@ -197,9 +197,19 @@ def sigma_leg(steps):
         sg.iter(bv)
     dt = (time.perf_counter() - t0) / steps
     sg.close()
+    # MRSF 2e gradient (routec_grad2_mrsf's work) at synthetic densities of the same system
+    A = va[:, :nocca]
+    d = np.array([(A @ A.T).T, (A[:, :noccb] @ A[:, :noccb].T).T])
+    p = 0.01 * d
+    spc = rng.standard_normal((n, n, 7)) * 1e-3
+    fb.grad2_mrsf(d, p, spc)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fb.grad2_mrsf(d, p, spc)
+    dtg = (time.perf_counter() - t0) / steps
     fb.close()
     return {"workload": "h2o32_631gd", "nbf": n, "trial_vectors": nv, "densities": 7 * nv,
-            "ms_per_iteration": round(dt * 1e3, 2)}
+            "ms_per_iteration": round(dt * 1e3, 2), "mrsf_grad2_ms": round(dtg * 1e3, 2)}
 
 
 def fshell_leg(fp32_cutoff, steps):

@@ -13,7 +13,7 @@ module oqph_bridge
             oqph_session_set_geometry, oqph_session_set_thresholds, oqph_set_default_session, &
             oqph_session_set_precision, oqph_session_set_share, oqph_session_prepare, &
             oqph_fock_jk, oqph_fock_jk_inc, oqph_fock_jk_cam, oqph_grad2, &
-            oqph_session_set_grad_precision, oqph_mrsf_int2
+            oqph_session_set_grad_precision, oqph_mrsf_int2, oqph_grad2_mrsf
   public :: oqph_stats
 
   !> Build statistics (oqph_stats of include/oqph.h).
@@ -100,6 +100,18 @@ module oqph_bridge
       real(c_double), intent(in) :: d3(*)
       real(c_double), intent(out) :: f3(*)
       real(c_double), value :: se, sc
+      type(oqph_stats), intent(out) :: stats
+    end function
+    !> MRSF 2e gradient (routec_grad2_mrsf on session s): d, p (nbf, nbf, 2),
+    !> spc (7, nbf, nbf), spcscale(3)
+    integer(c_int) function oqph_grad2_mrsf(s, d, p, spc, de, hfscale, hfscale2, coulscale, spcscale, mrst, stats) &
+        bind(C, name="oqph_grad2_mrsf")
+      import :: c_ptr, c_int, c_double, oqph_stats
+      type(c_ptr), value :: s
+      real(c_double), intent(in) :: d(*), p(*), spc(*), spcscale(*)
+      real(c_double), intent(inout) :: de(*)
+      real(c_double), value :: hfscale, hfscale2, coulscale
+      integer(c_int), value :: mrst
       type(oqph_stats), intent(out) :: stats
     end function
     !> OpenQP MPI share: rank of size (int2.F90:759-761)
