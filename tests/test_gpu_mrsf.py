@@ -167,3 +167,46 @@ def test_mrsf_grad2_matches_oracle(oracle, name, nw, mrst):
     g0, _ = oracle.grad2(bas, dpk)
     assert np.abs(fb.grad2(dpk) - g0).max() <= 1e-8
     fb.close()
+
+
+def test_mrsf_seams_on_h2o8_631g(oracle):
+    """The three MRSF paths on the (H2O)8/6-31G workload (104 bf, the
+    CPU-runnable benchmark system) against the oracle: response pass
+    (1e-10, same counts), sigma iteration (1e-10 relative) and MRSF 2e gradient
+    (1e-8, same counts)."""
+    from oracle import mrsf_sigma as MS
+    from openqp_amd import basis as B, lib as L, molecules as M
+    bld, bname, nocc = M.WORKLOADS["h2o8_631g"]
+    Z, xyz = bld(0)
+    bas = B.build_basis(bname, Z, xyz)
+    n = bas.nbf
+    _say("MRSF seams on (H2O)8/6-31G, nbf", n)
+    fb = L.FockBuilder(bas)
+    fb.make_default()
+    d3 = _d3(bas, 2, seed=11)
+    fo, so = oracle.mrsf_int2(bas, d3, ncoul=4, se=0.5, sc=0.5)
+    f3 = fb.mrsf_int2(d3, ncoul=4, se=0.5, sc=0.5)
+    _say("  response pass max |df3| =", np.abs(f3 - fo).max())
+    assert np.abs(f3 - fo).max() <= 1e-10
+    assert fb.last_stats.quartets_computed == so[1]
+    S, _, _ = L.int1e(bas)
+    rng = np.random.default_rng(12)
+    w, U = np.linalg.eigh(S)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    va = U @ np.diag(w ** -0.5) @ U.T @ Q
+    fa = np.diag(np.sort(rng.standard_normal(n)))
+    nocca, noccb = nocc + 1, nocc - 1
+    sig = L.SigmaSession(va, va, fa, fa, nocca, noccb, 1, scale=0.5)
+    bv = rng.standard_normal((nocca * (n - noccb), 2))
+    got = sig.iter(bv)
+    ref = MS.sigma(bas, va, va, fa, fa, nocca, noccb, 1, 0.5, bv)
+    sig.close()
+    _say("  sigma max rel =", np.abs(got - ref).max() / np.abs(ref).max())
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+    d, p, spc = _grad_inputs(n, rng)
+    go, sg = oracle.grad2_mrsf(bas, d, p, spc, 0.5, 0.5, 1.0, (1.0, 1.0, 1.0), 1)
+    g = fb.grad2_mrsf(d, p, spc, 0.5, 0.5, 1.0, (1.0, 1.0, 1.0), 1)
+    _say("  MRSF gradient max |dg| =", np.abs(g - go).max())
+    assert np.abs(g - go).max() <= 1e-8
+    assert fb.last_stats.quartets_computed == sg[2]
+    fb.close()
