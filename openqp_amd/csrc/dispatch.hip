@@ -38,8 +38,8 @@ namespace oqph {
   bool jk_res_##a##b##c##d(int, bool, KernelRes*);
 OQPH_JK_CLASSES(OQPH_DECL_JK)
 #define OQPH_DECL_GR(a, b, c, d)                                              \
-  void launch_grad_##a##b##c##d(const GradParams&, const WaveWork&, hipStream_t, bool); \
-  bool grad_res_##a##b##c##d(KernelRes*, bool);
+  void launch_grad_##a##b##c##d(const GradParams&, const WaveWork&, hipStream_t, bool, bool); \
+  bool grad_res_##a##b##c##d(KernelRes*, bool, bool);
 OQPH_GRAD_CLASSES(OQPH_DECL_GR)
 #define OQPH_DECL_SCH(a, b)                                                                                  \
   void launch_schwarz_##a##b(bool, const ShellPair*, const PrimJK*, const int*, int, const double*, const double*, \
@@ -150,7 +150,7 @@ bool kernel_resources(int kind, int la, int lb, int lc, int ld, bool pure, Kerne
   if (base == 2) {  // kind 2 | 16: the FP32 form
     switch (OQPH_CODE(la, lb, lc, ld)) {
 #define OQPH_CASE(a, b, c, d) \
-  case OQPH_CODE(a, b, c, d): return grad_res_##a##b##c##d(r, (kind & 16) != 0);
+  case OQPH_CODE(a, b, c, d): return grad_res_##a##b##c##d(r, (kind & 16) != 0, (kind & 32) != 0);
       OQPH_GRAD_CLASSES(OQPH_CASE)
 #undef OQPH_CASE
       default: return false;
@@ -171,11 +171,12 @@ bool launch_schwarz(int la, int lb, bool pure, const ShellPair* pairs, const Pri
   }
 }
 
-bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32) {
+bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32,
+                 bool mrsf) {
   if (ww.nwaves == 0) return true;
   switch (OQPH_CODE(la, lb, lc, ld)) {
 #define OQPH_CASE(a, b, c, d) \
-  case OQPH_CODE(a, b, c, d): launch_grad_##a##b##c##d(p, ww, s, f32); return true;
+  case OQPH_CODE(a, b, c, d): launch_grad_##a##b##c##d(p, ww, s, f32, mrsf); return true;
     OQPH_GRAD_CLASSES(OQPH_CASE)
 #undef OQPH_CASE
     default: return false;

@@ -7,10 +7,21 @@ namespace oqph {
 #define OQPH_GR_LAUNCHER_(a, b, c, d) launch_grad_##a##b##c##d
 #define OQPH_GR_LAUNCHER(a, b, c, d) OQPH_GR_LAUNCHER_(a, b, c, d)
 
-// f32: the FP32 form (quartets the host assigned to FP32)
-void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32) {
+// the MRSF form is instantiated for the classes up to 27 Cartesian
+// components (its two-particle density is gathered per element)
+constexpr bool MRSF_T = ncart(XA) * ncart(XB) * ncart(XC) * ncart(XD) <= 27;
+
+// f32: the FP32 form (quartets the host assigned to FP32); mrsf: the MRSF
+// two-particle density (FP64)
+void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32, bool mrsf) {
   const int bs = 256;  // four wavefronts
   const long long nb = (ww.nwaves + 3) / 4;
+  if constexpr (MRSF_T)
+    if (mrsf) {
+      hipLaunchKernelGGL((k_grad<XA, XB, XC, XD, double, true>), dim3((unsigned)nb), dim3(bs),
+                         sizeof(double) * 3 * p.natm, s, p, ww);
+      return;
+    }
   if (f32)
     hipLaunchKernelGGL((k_grad<XA, XB, XC, XD, float>), dim3((unsigned)nb), dim3(bs), sizeof(double) * 3 * p.natm, s,
                        p, ww);
@@ -22,10 +33,15 @@ void OQPH_GR_LAUNCHER(XA, XB, XC, XD)(const GradParams& p, const WaveWork& ww, h
 #define OQPH_GR_RES_(a, b, c, d) grad_res_##a##b##c##d
 #define OQPH_GR_RES(a, b, c, d) OQPH_GR_RES_(a, b, c, d)
 
-// Compiled resources of this class's gradient kernel (f32: its FP32 form).
-bool OQPH_GR_RES(XA, XB, XC, XD)(KernelRes* r, bool f32) {
+// Compiled resources of this class's gradient kernel (f32: its FP32 form;
+// mrsf: its MRSF form, false when the class has none).
+bool OQPH_GR_RES(XA, XB, XC, XD)(KernelRes* r, bool f32, bool mrsf) {
   hipFuncAttributes a;
   const void* f = f32 ? (const void*)k_grad<XA, XB, XC, XD, float> : (const void*)k_grad<XA, XB, XC, XD, double>;
+  if (mrsf) {
+    if constexpr (MRSF_T) f = (const void*)k_grad<XA, XB, XC, XD, double, true>;
+    else return false;
+  }
   if (hipFuncGetAttributes(&a, f) != hipSuccess) return false;
   r->private_bytes = (int)a.localSizeBytes;
   r->lds_bytes = (int)a.sharedSizeBytes;

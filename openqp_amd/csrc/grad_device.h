@@ -59,7 +59,8 @@ __host__ __device__ constexpr long long flops_grad_prim() {
   return flops_grad_prim_rt(LA, LB, LC, LD);
 }
 
-template <int LA, int LB, int LC, int LD, typename R>
+// MRSF: Gamma from the MRSF matrices (prm.Mc) instead of the density
+template <int LA, int LB, int LC, int LD, typename R, bool MRSF = false>
 __global__ __launch_bounds__(256) void k_grad(GradParams prm, WaveWork ww) {
   extern __shared__ double s_de[];
   constexpr bool F32 = sizeof(R) == 4;
@@ -118,9 +119,13 @@ __global__ __launch_bounds__(256) void k_grad(GradParams prm, WaveWork ww) {
         for (int c = 0; c < NC; ++c)
 #pragma unroll
           for (int d = 0; d < ND; ++d) {
-            const double g = 4.0 * prm.cs * Dc[(oa + a) * n + ob + b] * Dc[(oc + c) * n + od + d] -
-                             prm.hs * (Dc[(oa + a) * n + oc + c] * Dc[(ob + b) * n + od + d] +
-                                       Dc[(oa + a) * n + od + d] * Dc[(ob + b) * n + oc + c]);
+            double g;
+            if constexpr (MRSF)
+              g = mrsf_gamma(prm, oa + a, ob + b, oc + c, od + d);
+            else
+              g = 4.0 * prm.cs * Dc[(oa + a) * n + ob + b] * Dc[(oc + c) * n + od + d] -
+                  prm.hs * (Dc[(oa + a) * n + oc + c] * Dc[(ob + b) * n + od + d] +
+                            Dc[(oa + a) * n + od + d] * Dc[(ob + b) * n + oc + c]);
             Gam[((a * NB + b) * NC + c) * ND + d] = (R)g;
             gm = fmax(gm, fabs(g));
           }

@@ -59,7 +59,8 @@ void launch_schwarzh(const HLDims& h, const ShellPair* pairs, const PrimJK* prim
 struct GradParams;
 void launch_gradh(const HLDims& h, const GradParams& p, const WaveWork& ww, hipStream_t s);
 void launch_gradh_multi(const GradParams& p, const HLTaskList& tl, long long nblocks, int lds, hipStream_t s);
-bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32);
+bool launch_grad(int la, int lb, int lc, int ld, const GradParams& p, const WaveWork& ww, hipStream_t s, bool f32,
+                 bool mrsf);
 void launch_cart_matrix(const double* M, int nbf, int nsh, const int* am, const int* pure, const int* ao_off,
                         const int* coff, const double* xf, int ncf, double* Mc, hipStream_t s);
 void launch_cart_density(const double* D, int nsh, const int* am, const int* pure, const int* ao_off, const int* coff,
@@ -320,7 +321,8 @@ struct oqph_session {
   // gradient precision (oqph_session_set_grad_precision): 1 = quartets with
   // Q_ij Q_kl below grad_fp32_cut run in the FP32 form of their template
   int grad_policy = 0;
-  // MRSF gradient (oqph_grad2_mrsf): every class on the run-time-class kernel
+  // MRSF gradient (oqph_grad2_mrsf): the MRSF form of the class templates
+  // where it exists (up to 27 Cartesian components), else the run-time-class kernel
   bool grad_all_hl = false, ghl_ready = false;
   DBuf<double> d_gM, d_gMc;  // the eleven MRSF matrices, AO (row-major) and Cartesian
   double grad_fp32_cut = 1.0e-4;
@@ -1135,11 +1137,12 @@ static int prepare_grad(oqph_session* s) {
     L.ww.ket = s->d_gcls.p + coff[p.ck];
     L.ww.nseg = p.nseg;
     L.ww.nwaves = p.nwaves;
-    L.hl = s->grad_all_hl || !grad_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3)) ||
+    L.hl = !grad_template(p.la, p.lb, p.lc, p.ld) || (!s->f_templates && (p.la == 3 || p.lc == 3)) ||
            (s->grad_hl_ng > 0 && ncart(p.la) * ncart(p.lb) * ncart(p.lc) * ncart(p.ld) > s->grad_hl_ng);
     if (!L.hl) {  // templates whose private segment exceeds the budget run on k_gradh
       KernelRes kr{};
-      L.hl = !kernel_resources(2, p.la, p.lb, p.lc, p.ld, s->pure, &kr) || kr.private_bytes > OQPH_PRIVATE_BUDGET ||
+      const int kind = s->grad_all_hl ? (2 | 32) : 2;  // the MRSF form in the MRSF gradient
+      L.hl = !kernel_resources(kind, p.la, p.lb, p.lc, p.ld, s->pure, &kr) || kr.private_bytes > OQPH_PRIVATE_BUDGET ||
              (size_t)kr.lds_bytes + 24 * (size_t)s->natm > s->lds_max;
     }
     if (!L.hl && p.f32) {  // the FP32 form where its resources fit as well (else the FP64 form)
@@ -2557,10 +2560,7 @@ static int grad_run(oqph_session* s, GradParams& p, double* de, oqph_stats* st) 
       launch_gradh_multi(p, L.tl, L.nblocks, L.lds, ls);
     } else if (L.hl) {
       launch_gradh(L.hd, p, L.ww, ls);
-    } else if (p.Mc) {
-      set_err("MRSF gradient: class (%d %d|%d %d) is not on the run-time-class kernel", L.la, L.lb, L.lc, L.ld);
-      return 1;
-    } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, ls, L.f32)) {
+    } else if (!launch_grad(L.la, L.lb, L.lc, L.ld, p, L.ww, ls, L.f32, p.Mc != nullptr)) {
       set_err("no gradient kernel for class (%d %d|%d %d)", L.la, L.lb, L.lc, L.ld);
       return 1;
     }
