@@ -1,0 +1,169 @@
+"""Oracle of the semilocal XC build (oracle/xc.py), pinned on the CPU.
+
+* the molecular grid integrates products of basis functions: sum_g w phi_m
+  phi_n equals the analytic overlap S (oracle int1e) to the grid error;
+* the complex-step derivatives of every functional component equal central
+  differences;
+* Vxc is the gradient of E_xc: dE_xc/dD along a random symmetric direction
+  (central differences) equals sum(Vxc * dD), closed shell and both spins;
+* self-consistent Kohn-Sham energies reproduce the reference's own answers:
+  B3LYP5 / 6-31G* closed-shell H2O, examples/TDDFT/H2O_B3LYP5-TDDFT_ENERGY.json
+  (E = -76.37092191486505), and the triplet UKS B3LYP (libxc's B3LYP, VWN RPA
+  fit) / cc-pVDZ H2O of examples/TRAH/H2O_UHF-DFT_GRAD_TRAH.json
+  (E = -76.1477466020111), each to 5e-6 Eh.  The grids differ (the reference's
+  Lebedev grid against this product grid), so the bar is the grid error: the
+  two answers here land 8e-7 and 5e-7 Eh from the reference's; a dropped or
+  mis-scaled functional term moves them by 1e-3 Eh or more.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from openqp_amd import basis as B, molecules as M
+
+X = pytest.importorskip("oracle.xc")
+
+# the geometry and answer of the UKS known-answer deck (bohr)
+UKS_Z = np.array([8, 1, 1], dtype=np.int32)
+UKS_XYZ = np.array([[0.0, 0.0, 0.0], [0.0, -1.430522676341708, 1.109269235155327],
+                    [0.0, 1.430522676341708, 1.109269235155327]])
+UKS_E = -76.1477466020111
+RKS_E = -76.37092191486505
+
+
+@pytest.fixture(scope="module")
+def h2o(oracle):
+    Z, xyz = M.h2o_kat()
+    bas = B.build_basis("6-31g(d)", Z, xyz)
+    pts, w, _ = X.molecular_grid(bas.zn, bas.xyz, 96, 15)
+    ao = X.ao_values(bas, pts, oracle)
+    return Z, xyz, bas, (pts, w), ao
+
+
+def _ks_energy_fock(oracle, bas, Z, xyz, name, Ds, grid, ao, H, Q, restricted):
+    a = X.hf_exchange(name)
+    if restricted:
+        D = Ds[0] + Ds[1]
+        g, _ = oracle.fock_jk(bas, B.pack_tri(D), se=a, Q=Q)
+        G = B.unpack_tri(g, bas.nbf)
+        V, exc, nel = X.vxc(bas, D, name, grid, oracle, 1, ao)
+        return 0.5 * np.sum(D * (2 * H + G)) + exc + B.nuclear_repulsion(Z, xyz), [H + G + V[0]] * 2, nel
+    ga, gb, _ = oracle.fock_jk_uhf(bas, B.pack_tri(Ds[0]), B.pack_tri(Ds[1]), se=a, Q=Q)
+    G = [B.unpack_tri(ga, bas.nbf), B.unpack_tri(gb, bas.nbf)]
+    V, exc, nel = X.vxc(bas, Ds, name, grid, oracle, 2, ao)
+    E = sum(0.5 * np.sum(Ds[s] * (2 * H + G[s])) for s in range(2)) + exc + B.nuclear_repulsion(Z, xyz)
+    return E, [H + G[0] + V[0], H + G[1] + V[1]], nel
+
+
+def ks_scf(oracle, bas, Z, xyz, name, nocc, grid, ao, restricted, C0=None, conv=1e-9, maxit=100):
+    """Roothaan iterations with Pulay DIIS on F D S - S D F from the core
+    guess or from the orbitals C0; returns E, N_elec and the last orbitals."""
+    S, T, V = oracle.int1e(bas)
+    H = T + V
+    Q = oracle.schwarz(bas)
+    C = sl.eigh(H, S)[1] if C0 is None else C0
+    Ds = [C[:, :n] @ C[:, :n].T for n in nocc]
+    hist = []
+    Eold = 0.0
+    for it in range(maxit):
+        E, Fs, nel = _ks_energy_fock(oracle, bas, Z, xyz, name, Ds, grid, ao, H, Q, restricted)
+        errs = [F @ D @ S - S @ D @ F for F, D in zip(Fs, Ds)]
+        err = max(np.abs(e).max() for e in errs)
+        if err < conv and abs(E - Eold) < 1e-9:
+            break
+        Eold = E
+        hist = (hist + [(Fs, errs)])[-8:]
+        n = len(hist)
+        Bm = -np.ones((n + 1, n + 1))
+        Bm[n, n] = 0.0
+        for i in range(n):
+            for j in range(n):
+                Bm[i, j] = sum(np.sum(x * y) for x, y in zip(hist[i][1], hist[j][1]))
+        rhs = np.zeros(n + 1)
+        rhs[n] = -1.0
+        c = np.linalg.lstsq(Bm, rhs, rcond=None)[0][:n]
+        Fs = [sum(c[i] * hist[i][0][k] for i in range(n)) for k in range(len(Fs))]
+        Ds = []
+        for F, n in zip(Fs, nocc):
+            C = sl.eigh(F, S)[1]
+            Ds.append(C[:, :n] @ C[:, :n].T)
+    return E, nel, C
+
+
+def test_grid_integrates_overlap(oracle, h2o):
+    Z, xyz, bas, (pts, w), (phi, _) = h2o
+    S, _, _ = oracle.int1e(bas)
+    assert np.abs(phi.T @ (phi * w[:, None]) - S).max() < 5e-6
+    # the angular grid alone: exact for spherical harmonics up to degree 29
+    v, aw = X.angular_grid(15)
+    assert abs(aw.sum() - 4 * np.pi) < 1e-12
+    assert abs(np.sum(aw * v[:, 2] ** 28) - 4 * np.pi / 29) < 1e-12
+    # int x^a y^b z^c dOmega = 2 G((a+1)/2) G((b+1)/2) G((c+1)/2) / G((a+b+c+3)/2), a, b, c even
+    g = math.gamma
+    exact = 2 * g(1.5) * g(2.5) * g(3.5) / g(7.5)
+    assert abs(np.sum(aw * v[:, 0] ** 2 * v[:, 1] ** 4 * v[:, 2] ** 6) - exact) < 1e-13
+
+
+@pytest.mark.parametrize("comp", ["lda_x", "b88", "vwn5", "vwn_rpa", "lyp"])
+def test_component_derivatives(comp):
+    rng = np.random.default_rng(3)
+    args = [rng.uniform(0.05, 3.0, 16), rng.uniform(0.05, 3.0, 16)] + [rng.uniform(1e-3, 2.0, 16) for _ in range(3)]
+    f = X.COMPONENTS[comp][0]
+    _, d = X.derivatives(f, args)
+    for k in range(5):
+        h = 1e-5 * np.maximum(args[k], 1e-3)
+        up = [a.copy() for a in args]
+        dn = [a.copy() for a in args]
+        up[k] += h
+        dn[k] -= h
+        fd = (f(*up) - f(*dn)) / (2 * h)
+        assert np.allclose(d[k], fd, rtol=1e-6, atol=1e-9), (comp, k)
+
+
+@pytest.mark.parametrize("name,nfocks", [("b3lyp5", 1), ("b3lyp", 2), ("svwn5", 1)])
+def test_vxc_is_gradient_of_exc(oracle, h2o, name, nfocks):
+    Z, xyz, bas, grid, ao = h2o
+    S, _, _ = oracle.int1e(bas)
+    D = M.random_density(S, 5, seed=4)
+    rng = np.random.default_rng(5)
+    Ds = D if nfocks == 1 else [0.6 * D, 0.4 * D]
+    V, _, _ = X.vxc(bas, Ds, name, grid, oracle, nfocks, ao)
+    for s in range(nfocks):
+        P = rng.standard_normal(D.shape)
+        P = 1e-4 * (P + P.T)
+        def E(t):
+            if nfocks == 1:
+                return X.vxc(bas, D + t * P, name, grid, oracle, 1, ao)[1]
+            dd = [x.copy() for x in Ds]
+            dd[s] = dd[s] + t * P
+            return X.vxc(bas, dd, name, grid, oracle, 2, ao)[1]
+        fd = (E(1.0) - E(-1.0)) / 2
+        an = np.sum(V[s] * P)
+        assert abs(fd - an) < 1e-7 * max(1.0, abs(an)), (name, s, fd, an)
+
+
+def test_rks_b3lyp5_energy_matches_reference(oracle, h2o):
+    """examples/TDDFT/H2O_B3LYP5-TDDFT_ENERGY.json: E = -76.37092191486505."""
+    Z, xyz, bas, grid, ao = h2o
+    E, nel, _ = ks_scf(oracle, bas, Z, xyz, "b3lyp5", (5, 5), grid, ao, True)
+    assert abs(nel - 10.0) < 1e-5
+    assert abs(E - RKS_E) < 5e-6, E
+
+
+def test_uks_b3lyp_triplet_energy_matches_reference(oracle):
+    """examples/TRAH/H2O_UHF-DFT_GRAD_TRAH.json: B3LYPV1R (libxc's B3LYP),
+    UHF triplet, cc-pVDZ (pure), E = -76.1477466020111."""
+    bas = B.build_basis("cc-pvdz", UKS_Z, UKS_XYZ)
+    assert bas.nbf == 24
+    pts, w, _ = X.molecular_grid(bas.zn, bas.xyz, 96, 15)
+    ao = X.ao_values(bas, pts, oracle)
+    # guess: the closed-shell orbitals with the lowest virtual in alpha (from
+    # the core guess the iterations settle on an excited triplet, a saddle)
+    _, _, C = ks_scf(oracle, bas, UKS_Z, UKS_XYZ, "b3lypv1r", (5, 5), (pts, w), ao, True, conv=1e-5)
+    E, nel, _ = ks_scf(oracle, bas, UKS_Z, UKS_XYZ, "b3lypv1r", (6, 4), (pts, w), ao, False, C0=C)
+    assert abs(nel - 10.0) < 1e-5
+    assert abs(E - UKS_E) < 5e-6, E
