@@ -169,6 +169,35 @@ def scf_leg():
             "fock_ms_total": round(float(fms.sum()), 3), "incremental": True, "guess": "extended Hueckel"}
 
 
+def xc_leg(steps, name="b3lyp5"):
+    """Semilocal Vxc build (routec_vxc's work) on the (H2O)64/cc-pVDZ workload:
+    B3LYP5 on the default grid (96 radial shells x 450 angular points per
+    atom, SSF partition), closed shell, S-orthonormal random density; the grid
+    build (first call, per geometry) is timed apart from the builds."""
+    from openqp_amd import basis as B, molecules as M
+    from openqp_amd import lib as L
+    bld, bname, nocc = M.WORKLOADS["h2o64_ccpvdz"]
+    Z, xyz = bld(0)
+    bas = B.build_basis(bname, Z, xyz)
+    S, _, _ = L.int1e(bas)
+    dp = B.pack_tri(M.random_density(S, nocc, seed=3))
+    fb = L.FockBuilder(bas)
+    fb.set_xc(name)
+    t0 = time.perf_counter()
+    npts = len(fb.xc_grid())
+    tg = time.perf_counter() - t0
+    fb.vxc(dp, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        V, exc, nel = fb.vxc(dp, 1)
+    dt = (time.perf_counter() - t0) / steps
+    log(f"[bench] Vxc {name}: {dt * 1e3:.1f} ms/build over {npts} points (grid build {tg * 1e3:.0f} ms), "
+        f"N_elec {nel:.6f}")
+    return {"workload": "h2o64_ccpvdz", "functional": name, "nbf": bas.nbf, "grid_points": npts,
+            "grid": "96 x (15 x 30), SSF", "ms_per_build": round(dt * 1e3, 2), "grid_build_ms": round(tg * 1e3, 1),
+            "points_per_s": round(npts / dt, 1), "E_xc": exc, "N_elec": nel, "N_expected": 2 * nocc}
+
+
 def sigma_leg(steps):
     """MRSF sigma seam (routec_sig_*) on (H2O)32 / 6-31G(d): two trial vectors
     per iteration (14 non-symmetric transition densities through the device
@@ -286,6 +315,7 @@ def main():
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--no-fshell", action="store_true")
     ap.add_argument("--no-sigma", action="store_true")
+    ap.add_argument("--no-xc", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-builds", type=int, default=3,
                     help="builds after the timed ones with per-launch HIP events (launches serialised): the "
@@ -431,6 +461,11 @@ def main():
         log("[bench] MRSF sigma: routec_sig_iter on the (H2O)32/6-31G(d) workload, two trial vectors")
         sig = sigma_leg(2)
 
+    xcl = None
+    if not a.no_xc:
+        log("[bench] Vxc: B3LYP5 on the (H2O)64/cc-pVDZ workload")
+        xcl = xc_leg(3)
+
     fsh = None
     if not a.no_fshell:
         log("[bench] f shells: RHF SCF, builds and gradient on the (H2O)16/cc-pVTZ workload")
@@ -466,6 +501,7 @@ def main():
         "scf": scf,
         "fshell": fsh,
         "mrsf_sigma": sig,
+        "vxc": xcl,
         "roofline": roof, "cpu_baseline": cpu,
         "per_class": {str(k): {"kernel": v["kernel"], "ms_fp64": round(v["ms"] / v["n"], 3),
                                "ms_mixed": round(profmx[k]["ms"] / profmx[k]["n"], 3) if k in profmx else None,

@@ -124,6 +124,32 @@ int routec_sig_init(const int *nbf, const double *mo_a, const double *mo_b, cons
 void routec_sig_set_scale(const double *s);
 void routec_sig_iter(const double *bvec_mo, const int *nv_new, double *sigma_mo, int *info);
 void routec_sig_free(void);
+/* ---- semilocal exchange-correlation (Route-C Vxc seam) ----------------- */
+
+/* Vxc build.  Replaces routec_vxc (source/modules/routec_bridge.F90:43-50,
+ * caller calc_jk_xc, source/scf_addons.F90:2104-2111, in place of
+ * calc_dft_xc_density, :1857-1942).  d: packed (ntri, nfocks), the total
+ * density (nfocks = 1) or the alpha then the beta density (nfocks = 2);
+ * fxc: packed Vxc per spin, overwritten; eexc: E_xc; totele: the grid
+ * electron count.  Uses the default session's functional and grid
+ * (oqph_session_set_xc); info = 1 (decline) when there is none, when nbf
+ * does not match or on a device error. */
+void routec_vxc(const double *d, double *fxc, const int *nbf, const int *nfocks, double *eexc, double *totele,
+                int *info);
+/* Functional and grid of the session's Vxc builds: names as the reference's
+ * libxc table (source/dftlib/libxc.F90:300-460): slater, svwn5, blyp,
+ * b3lyp5 (= b3lypv5), b3lyp (= b3lypv1r, VWN RPA fit), bhhlyp; grid nrad
+ * radial shells x (ntheta x 2 ntheta) angular points per atom.  1: unknown
+ * name; 4: grid out of range. */
+int oqph_session_set_xc(oqph_session *s, const char *functional, int nrad, int ntheta);
+/* Exact-exchange fraction and GGA flag of a functional name (1: unknown). */
+int oqph_xc_functional_info(const char *functional, double *hfex, int *gga);
+/* Session form of routec_vxc (same arguments, host buffers, return code). */
+int oqph_vxc(oqph_session *s, const double *d, int nfocks, double *fxc, double *exc, double *nel);
+/* The session's XC grid: returns the point count (negative on error) and
+ * copies up to cap points as (x, y, z, w) into pts (may be NULL). */
+long long oqph_xc_grid(oqph_session *s, double *pts, long long cap);
+
 /* Make s the session used by routec_fock_jk / routec_grad2. */
 int oqph_set_default_session(oqph_session *s);
 /* Build the geometry-dependent device state now (pair records, Schwarz

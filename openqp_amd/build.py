@@ -51,12 +51,13 @@ def _cost(c):
 DEPS = {
     "jk_class.hip": ["jk_device.h", "jkg_device.h", "eri_device.h", "oqph_common.h", "rys_table.inc", "rys_table_f32.inc"],
     "dispatch.hip": ["eri_device.h", "grad_device.h", "grad_params.h", "oqph_common.h", "rys_table.inc", "rys_table_f32.inc"],
-    "session.hip": ["oqph_common.h", "grad_params.h", "ao_xform.h", "../../include/oqph.h"],
+    "session.hip": ["oqph_common.h", "grad_params.h", "ao_xform.h", "session_view.h", "../../include/oqph.h"],
     "grad_class.hip": ["grad_device.h", "grad_params.h", "eri_device.h", "oqph_common.h", "rys_table.inc", "rys_table_f32.inc"],
     "int1e.hip": ["../../include/oqph.h", "ao_xform.h", "rys_table.inc", "rys_table_f32.inc"],
     "hl.hip": ["hl_device.h", "jk_device.h", "grad_params.h", "eri_device.h", "oqph_common.h", "rys_table.inc",
                "rys_table_f32.inc"],
     "sigma.hip": ["../../include/oqph.h"],
+    "xc.hip": ["../../include/oqph.h", "ao_xform.h", "session_view.h"],
 }
 
 
@@ -74,7 +75,7 @@ def _key(unit: str, defs) -> str:
 
 def _jobs():
     units = [("session.hip", [], "session.o"), ("dispatch.hip", [], "dispatch.o"), ("int1e.hip", [], "int1e.o"),
-             ("hl.hip", [], "hl.o"), ("sigma.hip", [], "sigma.o")]
+             ("hl.hip", [], "hl.o"), ("sigma.hip", [], "sigma.o"), ("xc.hip", [], "xc.o")]
     heavy, light = [], []
     per_class = [(c, "jk_class.hip", "jk") for c in CLASSES + F_JK] + \
                 [(c, "grad_class.hip", "gr") for c in CLASSES + F_GRAD if c not in GRAD_HL]
@@ -110,7 +111,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda j: _compile(j, verbose), _jobs()))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "-fopenmp", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "-fopenmp", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

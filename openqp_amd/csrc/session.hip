@@ -27,6 +27,7 @@
 #include "../../include/oqph.h"
 #include "oqph_common.h"
 #include "grad_params.h"
+#include "session_view.h"
 
 namespace oqph {
 bool launch_jk(int la, int lb, int lc, int ld, bool pure, const JKParams& p, const WaveWork& cw, hipStream_t s);
@@ -2108,6 +2109,26 @@ static int build_general(oqph_session* s, int nd, const double* dD, double* dF, 
   return 0;
 }
 
+bool oqph::session_view(oqph_session* s, SessionView* v) {
+  if (!s || s->nsh == 0 || s->natm == 0 || (int)s->xyz.size() != 3 * s->natm) return false;
+  v->device = s->device;
+  v->stream = s->stream;
+  v->nsh = s->nsh;
+  v->nbf = s->nbf;
+  v->natm = s->natm;
+  v->am = s->am.data();
+  v->harm = s->harm.data();
+  v->atom = s->atom.data();
+  v->nprim = s->nprim.data();
+  v->poff = s->poff.data();
+  v->aooff = s->aooff.data();
+  v->ex = s->ex.data();
+  v->cc = s->cc.data();
+  v->xyz = s->xyz.data();
+  v->zn = s->zn.data();
+  return true;
+}
+
 oqph_session* oqph::default_session() {
   std::lock_guard<std::mutex> lk(g_mtx);
   return g_default;
@@ -2170,6 +2191,7 @@ int oqph_session_destroy(oqph_session* s) {
     if (g_default == s) g_default = nullptr;
   }
   hipSetDevice(s->device);
+  oqph::xc_release(s);
   s->d_pairs.release(); s->d_prims.release(); s->d_rys.release(); s->d_cls.release();
   s->d_xform.release(); s->d_rys32.release(); s->d_kbs.release(); s->d_seg.release(); s->d_choff.release();
   s->d_gpairs.release(); s->d_gprims.release(); s->d_gcls.release(); s->d_coff.release(); s->d_satom.release();

@@ -32,7 +32,8 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_session_prepare", "oqph_session_set_share", "oqph_fock_jk_cam", "oqph_kernel_resources",
            "oqph_fock_jk_noalloc", "oqph_last_counters", "oqph_session_set_grad_precision", "oqph_mrsf_int2",
            "routec_sig_init", "routec_sig_set_scale", "routec_sig_iter", "routec_sig_free", "oqph_grad2_mrsf",
-           "routec_grad2_mrsf"]
+           "routec_grad2_mrsf", "routec_vxc", "oqph_session_set_xc", "oqph_xc_functional_info", "oqph_vxc",
+           "oqph_xc_grid"]
 
 
 class OqphError(RuntimeError):
@@ -103,6 +104,13 @@ def load(path: str | None = None):
     L.routec_sig_set_scale.restype = None
     L.routec_sig_iter.restype = None
     L.routec_sig_free.restype = None
+    L.routec_vxc.argtypes = [dp, dp, ip, ip, dp, dp, ip]
+    L.routec_vxc.restype = None
+    L.oqph_session_set_xc.argtypes = [vp, C.c_char_p, C.c_int, C.c_int]
+    L.oqph_xc_functional_info.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.oqph_vxc.argtypes = [vp, _pd, C.c_int, _pd, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.oqph_xc_grid.argtypes = [vp, C.c_void_p, C.c_longlong]
+    L.oqph_xc_grid.restype = C.c_longlong
     _lib = L
     return L
 
@@ -225,6 +233,31 @@ class FockBuilder:
         f = np.zeros_like(d)
         _check(load().oqph_fock_jk(self._s, d, f, 1, se, sc, C.byref(self.last_stats)), "fock_jk")
         return f
+
+    def set_xc(self, functional: str, nrad: int = 96, ntheta: int = 15):
+        """Select the XC functional and the grid (nrad radial shells, ntheta x
+        2 ntheta angular points per shell) of this session's Vxc builds."""
+        _check(load().oqph_session_set_xc(self._s, functional.encode(), nrad, ntheta), "set_xc")
+
+    def vxc(self, d, nfocks: int = 1):
+        """Packed Vxc per spin, E_xc and the grid electron count for the packed
+        density d (nfocks = 1: total density; 2: alpha then beta)."""
+        d = np.ascontiguousarray(d, np.float64).reshape(-1)
+        f = np.zeros_like(d)
+        exc, nel = C.c_double(), C.c_double()
+        _check(load().oqph_vxc(self._s, d, nfocks, f, C.byref(exc), C.byref(nel)), "vxc")
+        n = d.size // nfocks
+        return [f[i * n:(i + 1) * n] for i in range(nfocks)], exc.value, nel.value
+
+    def xc_grid(self) -> np.ndarray:
+        """The session's XC grid: (npoints, 4) rows x, y, z (bohr), weight."""
+        L = load()
+        n = L.oqph_xc_grid(self._s, None, 0)
+        if n < 0:
+            raise OqphError(f"xc_grid failed (code {n}): {L.oqph_last_error().decode()}")
+        out = np.zeros((n, 4))
+        L.oqph_xc_grid(self._s, out.ctypes.data_as(C.c_void_p), n)
+        return out
 
     def fock_jk_uhf(self, d_alpha, d_beta, se: float = 1.0, sc: float = 1.0):
         """Packed UHF/ROHF (F_alpha, F_beta) = sc J[Da + Db] - se K[Ds] (fock_jk with nfocks = 2)."""
@@ -371,3 +404,11 @@ class SigmaSession:
 
     def close(self):
         load().routec_sig_free()
+
+
+def xc_functional_info(functional: str):
+    """(exact-exchange fraction, is GGA) of a functional name the library knows."""
+    hf, gga = C.c_double(), C.c_int()
+    if load().oqph_xc_functional_info(functional.encode(), C.byref(hf), C.byref(gga)) != 0:
+        raise OqphError(f"unknown functional {functional!r}")
+    return hf.value, bool(gga.value)

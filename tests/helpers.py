@@ -32,7 +32,7 @@ def density_for(oracle, bas, nocc, seed=0):
 def header_functions():
     txt = open(os.path.join(ROOT, "include", "oqph.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"^\s*(?:int|void|const char \*)\s*\**\s*(\w+)\s*\(", txt, flags=re.M)
+    names = re.findall(r"^\s*(?:int|void|long long|const char \*)\s*\**\s*(\w+)\s*\(", txt, flags=re.M)
     return sorted(set(names))
 
 

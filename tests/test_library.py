@@ -82,3 +82,40 @@ def test_routec_sig_declines_without_session():
     info = i(0)
     lib.routec_sig_iter(a, ctypes.byref(i(1)), a, ctypes.byref(info))
     assert info.value != 0
+
+
+def test_routec_vxc_signature_matches_reference_seam():
+    """routec_bridge.F90:43-50 binds routec_vxc(d, fxc, nbf, nfocks, eexc,
+    totele, info), all by reference."""
+    txt = " ".join(open(os.path.join(ROOT, "include", "oqph.h")).read().split())
+    assert ("void routec_vxc(const double *d, double *fxc, const int *nbf, const int *nfocks, double *eexc, "
+            "double *totele, int *info);") in txt
+
+
+def test_routec_vxc_declines_without_session():
+    """With no default session routec_vxc reports info = 1, so calc_jk_xc keeps
+    its native grid path (scf_addons.F90:2104-2111); no device call is made."""
+    lib = ctypes.CDLL(_lib_path())
+    i = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    a = (ctypes.c_double * 4)()
+    e, n = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    lib.routec_vxc.argtypes = [dp, dp, ctypes.POINTER(i), ctypes.POINTER(i), dp, dp, ctypes.POINTER(i)]
+    lib.routec_vxc.restype = None
+    info = i(0)
+    lib.routec_vxc(a, a, ctypes.byref(i(2)), ctypes.byref(i(1)), ctypes.byref(e), ctypes.byref(n), ctypes.byref(info))
+    assert info.value == 1
+
+
+def test_xc_functional_table():
+    """Exact-exchange fractions of the libxc compositions (libxc.F90:300-460)."""
+    lib = ctypes.CDLL(_lib_path())
+    lib.oqph_xc_functional_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_int)]
+    want = {b"b3lyp5": (0.2, 1), b"B3LYPV5": (0.2, 1), b"b3lypv1r": (0.2, 1), b"bhhlyp": (0.5, 1),
+            b"blyp": (0.0, 1), b"svwn5": (0.0, 0), b"slater": (0.0, 0)}
+    for name, (hf, gga) in want.items():
+        h, g = ctypes.c_double(), ctypes.c_int()
+        assert lib.oqph_xc_functional_info(name, ctypes.byref(h), ctypes.byref(g)) == 0
+        assert (h.value, g.value) == (hf, gga), name
+    assert lib.oqph_xc_functional_info(b"m06-2x", None, None) == 1
