@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--no-fshell", action="store_true")
     ap.add_argument("--no-sigma", action="store_true")
     ap.add_argument("--no-xc", action="store_true")
+    ap.add_argument("--xc-only", action="store_true",
+                    help="run only the Vxc leg (for profiling it) and print its record")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-builds", type=int, default=3,
                     help="builds after the timed ones with per-launch HIP events (launches serialised): the "
@@ -326,6 +328,9 @@ def main():
               "shell pairs would be split by cost and G all-reduced once per build, which is not "
               "implemented here", file=sys.stderr)
         sys.exit(2)
+    if a.xc_only:
+        print(json.dumps({"vxc": xc_leg(a.steps)}))
+        return
 
     import torch
     from openqp_amd import basis as B, molecules as M

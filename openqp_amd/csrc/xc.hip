@@ -10,7 +10,9 @@
 // (a = 0.64) computed on the device, points with weight <= 1e-20 dropped.
 // The kept points are sorted into cubic boxes (5 bohr) and cut into batches
 // of at most 32768 points; a batch sees only the shells whose most diffuse
-// primitive reaches its bounding sphere (|c| exp(-alpha r^2) >= 1e-16).
+// primitive reaches its bounding sphere (|c| exp(-alpha r^2) >= 1e-12).
+// Per-batch matrices are stored point-fastest (column-major npts x nb), so
+// the basis-function and per-point kernels (one thread per point) coalesce.
 //
 // Per batch and spin: phi, grad phi (k_ao) -> X = phi D_b (rocBLAS dgemm on
 // the gathered D_b) -> per point rho, grad rho, the functional and its
@@ -49,7 +51,8 @@ constexpr double RHO_FLOOR = 1e-30;  // spin densities are floored here
 constexpr double SSF_A = 0.64;
 constexpr double BOX = 5.0;          // bohr
 constexpr int MAX_BATCH = 32768;
-constexpr double AO_EPS = 1e-16;
+constexpr int KCHUNK = 4096;         // points per partial V_b (split of the long GEMM dimension)
+constexpr double AO_EPS = 1e-12;
 
 void err(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 void err(const char* fmt, ...) {
@@ -334,15 +337,15 @@ struct AOArgs {
   const int* col;     // their first column
   const int *am, *harm, *atom, *nprim, *poff;
   const double *ex, *cc, *xyz, *xf;
-  double *phi, *dphi;  // npts x nb, 3 x npts x nb
+  double *phi, *dphi;  // column-major npts x nb (point fastest), 3 of them
   int grad;
 };
 
 __global__ void k_ao(AOArgs a) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)a.npts * a.nsh) return;
-  const int q = (int)(t % a.nsh);
-  const int p = (int)(t / a.nsh);
+  const int p = (int)(t % a.npts);
+  const int q = (int)(t / a.npts);
   const int sh = a.shl[q], l = a.am[sh], A = a.atom[sh];
   const double dx = a.pts[4 * (size_t)p] - a.xyz[3 * A], dy = a.pts[4 * (size_t)p + 1] - a.xyz[3 * A + 1],
                dz = a.pts[4 * (size_t)p + 2] - a.xyz[3 * A + 2];
@@ -373,14 +376,14 @@ __global__ void k_ao(AOArgs a) {
       gy[i] = (ly ? ly * px[lx] * py[ly - 1] * pz[lz] * R : 0.0) + m * dR * dy;
       gz[i] = (lz ? lz * px[lx] * py[ly] * pz[lz - 1] * R : 0.0) + m * dR * dz;
     }
-  const size_t row = (size_t)p * a.nb + a.col[q], plane = (size_t)a.npts * a.nb;
+  const size_t ld = (size_t)a.npts, row = (size_t)a.col[q] * ld + p, plane = (size_t)a.npts * a.nb;
   if (l < 2) {
     for (int o = 0; o < nc; ++o) {
-      a.phi[row + o] = c[o];
+      a.phi[row + o * ld] = c[o];
       if (a.grad) {
-        a.dphi[row + o] = gx[o];
-        a.dphi[plane + row + o] = gy[o];
-        a.dphi[2 * plane + row + o] = gz[o];
+        a.dphi[row + o * ld] = gx[o];
+        a.dphi[plane + row + o * ld] = gy[o];
+        a.dphi[2 * plane + row + o * ld] = gz[o];
       }
     }
     return;
@@ -397,11 +400,11 @@ __global__ void k_ao(AOArgs a) {
       vy += f * gy[k];
       vz += f * gz[k];
     }
-    a.phi[row + o] = v;
+    a.phi[row + o * ld] = v;
     if (a.grad) {
-      a.dphi[row + o] = vx;
-      a.dphi[plane + row + o] = vy;
-      a.dphi[2 * plane + row + o] = vz;
+      a.dphi[row + o * ld] = vx;
+      a.dphi[plane + row + o * ld] = vy;
+      a.dphi[2 * plane + row + o * ld] = vz;
     }
   }
 }
@@ -421,36 +424,50 @@ __device__ inline double wsum(double v) {
   return v;
 }
 
-// one wavefront per point
-__global__ __launch_bounds__(256) void k_xc_point(PointArgs a) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long long p = (long long)blockIdx.x * 4 + wv;
-  __shared__ double red[4][2];
-  double e = 0.0, nel = 0.0;
-  if (p < a.npts) {
-    const int n = a.nb;
-    const size_t plane = (size_t)a.npts * n;
-    const double* phi = a.phi + p * n;
-    double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-    for (int s = 0; s < a.ns; ++s) {
-      const double* X = a.X + s * plane + p * n;
-      double r = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-      for (int m = lane; m < n; m += 64) {
-        const double x = X[m];
-        r += x * phi[m];
-        if (a.gga) {
-          g0 += x * a.dphi[p * n + m];
-          g1 += x * a.dphi[plane + p * n + m];
-          g2 += x * a.dphi[2 * plane + p * n + m];
-        }
-      }
-      rho[s] = wsum(r);
+// 64 points per block (one per lane) x XC_SLICES wavefronts that split the
+// basis functions; matrices point-fastest: element (p, m) at m npts + p
+#define XC_SLICES 8
+__global__ __launch_bounds__(64 * XC_SLICES) void k_xc_point(PointArgs a) {
+  const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const long long p = (long long)blockIdx.x * 64 + lane;
+  __shared__ double part[XC_SLICES][2][4][64];  // per slice: rho, grad rho per spin
+  __shared__ double coef[2][4][64];             // per spin: c0, v (3)
+  __shared__ double red[2][64];
+  const bool live = p < a.npts;
+  const int n = a.nb;
+  const size_t ld = (size_t)a.npts, plane = ld * n;
+  const int m0 = (int)((long long)n * sl / XC_SLICES), m1 = (int)((long long)n * (sl + 1) / XC_SLICES);
+  for (int s = 0; s < a.ns; ++s) {
+    double r = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (live) {
+      const double* X = a.X + s * plane + p;
+      const double* phi = a.phi + p;
+      const double* gp = a.dphi + p;
       if (a.gga) {
-        g[s][0] = 2.0 * wsum(g0);
-        g[s][1] = 2.0 * wsum(g1);
-        g[s][2] = 2.0 * wsum(g2);
+        for (int m = m0; m < m1; ++m) {
+          const double x = X[m * ld];
+          r += x * phi[m * ld];
+          g0 += x * gp[m * ld];
+          g1 += x * gp[plane + m * ld];
+          g2 += x * gp[2 * plane + m * ld];
+        }
+      } else {
+        for (int m = m0; m < m1; ++m) r += X[m * ld] * phi[m * ld];
       }
     }
+    part[sl][s][0][lane] = r;
+    part[sl][s][1][lane] = g0;
+    part[sl][s][2][lane] = g1;
+    part[sl][s][3][lane] = g2;
+  }
+  __syncthreads();
+  if (sl == 0) {
+    double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int s = 0; s < a.ns; ++s)
+      for (int j = 0; j < XC_SLICES; ++j) {
+        rho[s] += part[j][s][0][lane];
+        for (int k = 0; k < 3; ++k) g[s][k] += 2.0 * part[j][s][k + 1][lane];
+      }
     double ra, rb, ga[3], gb[3];
     if (a.ns == 1) {
       ra = rb = 0.5 * rho[0];
@@ -463,10 +480,9 @@ __global__ __launch_bounds__(256) void k_xc_point(PointArgs a) {
         gb[k] = g[1][k];
       }
     }
-    const double w = a.pts[4 * p + 3], rt = ra + rb;
-    nel = w * rt;
-    double c0[2] = {0.0, 0.0}, vec[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-    if (rt >= RHO_CUT) {
+    const double w = live ? a.pts[4 * p + 3] : 0.0, rt = ra + rb;
+    double e = 0.0, c0[2] = {0.0, 0.0}, vec[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    if (live && rt >= RHO_CUT) {
       const double saa = ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2];
       const double sab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
       const double sbb = gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2];
@@ -480,25 +496,33 @@ __global__ __launch_bounds__(256) void k_xc_point(PointArgs a) {
         vec[1][k] = w * (2.0 * F.d[4] * gb[k] + F.d[3] * ga[k]);
       }
     }
-    for (int s = 0; s < a.ns; ++s) {
-      double* Y = a.Y + s * plane + p * n;
-      for (int m = lane; m < n; m += 64) {
-        double y = c0[s] * phi[m];
-        if (a.gga)
-          y += vec[s][0] * a.dphi[p * n + m] + vec[s][1] * a.dphi[plane + p * n + m] +
-               vec[s][2] * a.dphi[2 * plane + p * n + m];
-        Y[m] = y;
-      }
+    for (int s = 0; s < 2; ++s) {
+      coef[s][0][lane] = c0[s];
+      for (int k = 0; k < 3; ++k) coef[s][k + 1][lane] = vec[s][k];
     }
-  }
-  if (lane == 0) {
-    red[wv][0] = e;
-    red[wv][1] = nel;
+    red[0][lane] = e;
+    red[1][lane] = w * rt;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(a.acc, red[0][0] + red[1][0] + red[2][0] + red[3][0]);
-    atomicAdd(a.acc + 1, red[0][1] + red[1][1] + red[2][1] + red[3][1]);
+  if (live)
+    for (int s = 0; s < a.ns; ++s) {
+      const double c = coef[s][0][lane], v0 = coef[s][1][lane], v1 = coef[s][2][lane], v2 = coef[s][3][lane];
+      double* Y = a.Y + s * plane + p;
+      const double* phi = a.phi + p;
+      const double* gp = a.dphi + p;
+      if (a.gga) {
+        for (int m = m0; m < m1; ++m)
+          Y[m * ld] = c * phi[m * ld] + v0 * gp[m * ld] + v1 * gp[plane + m * ld] + v2 * gp[2 * plane + m * ld];
+      } else {
+        for (int m = m0; m < m1; ++m) Y[m * ld] = c * phi[m * ld];
+      }
+    }
+  if (sl == 0) {
+    const double e = wsum(red[0][lane]), nel = wsum(red[1][lane]);
+    if (lane == 0) {
+      atomicAdd(a.acc, e);
+      atomicAdd(a.acc + 1, nel);
+    }
   }
 }
 
@@ -518,12 +542,16 @@ __global__ void k_gather(const double* full, int nbf, const int* idx, int nb, do
   out[t] = full[(size_t)idx[i] * nbf + idx[j]];
 }
 
-// V[idx[m], idx[n]] += Vb(m, n) + Vb(n, m); Vb column-major nb x nb
-__global__ void k_scatter(const double* Vb, int nb, const int* idx, int nbf, double* V) {
+// V[idx[m], idx[n]] += sum over the nparts partials of Vb(m, n) + Vb(n, m);
+// each partial column-major nb x nb
+__global__ void k_scatter(const double* Vb, int nb, int nparts, const int* idx, int nbf, double* V) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nb * nb) return;
   const int m = (int)(t % nb), n = (int)(t / nb);
-  V[(size_t)idx[m] * nbf + idx[n]] += Vb[(size_t)n * nb + m] + Vb[(size_t)m * nb + n];
+  const size_t nb2 = (size_t)nb * nb;
+  double v = 0.0;
+  for (int k = 0; k < nparts; ++k) v += Vb[k * nb2 + (size_t)n * nb + m] + Vb[k * nb2 + (size_t)m * nb + n];
+  V[(size_t)idx[m] * nbf + idx[n]] += v;
 }
 
 __global__ void k_pack(const double* full, int nbf, double* pk) {
@@ -796,7 +824,7 @@ int build_grid(XCState* x, const oqph::SessionView& v) {
        upload(x->d_nprim, v.nprim, (size_t)v.nsh, st) && upload(x->d_poff, v.poff, (size_t)v.nsh, st) &&
        upload(x->d_ex, v.ex, (size_t)nprim_tot, st) && upload(x->d_cc, v.cc, (size_t)nprim_tot, st) &&
        x->d_phi.ensure(mp * mb) && x->d_dphi.ensure(3 * mp * mb) && x->d_X.ensure(2 * mp * mb) &&
-       x->d_Y.ensure(2 * mp * mb) && x->d_Db.ensure(mb * mb) && x->d_Vb.ensure(mb * mb) &&
+       x->d_Y.ensure(2 * mp * mb) && x->d_Db.ensure(mb * mb) && x->d_Vb.ensure((mp / KCHUNK + 1) * mb * mb) &&
        hipStreamSynchronize(st) == hipSuccess;
   if (!ok) return 3;
   x->grid_ready = true;
@@ -879,24 +907,37 @@ int vxc_run(oqph_session* s, const double* d, int nfocks, double* fxc, double* e
     for (int sp = 0; sp < nfocks; ++sp) {
       hipLaunchKernelGGL(k_gather, dim3(nblk((long long)nb * nb, 256)), dim3(256), 0, st, x->d_D.p + sp * nn, nbf,
                          x->d_idx.p + b.ao0, nb, x->d_Db.p);
-      // X^T (nb x np, column-major) = D_b (nb x nb) phi^T (nb x np)
-      if (rocblas_dgemm(x->blas, rocblas_operation_none, rocblas_operation_none, nb, np, nb, &one, x->d_Db.p, nb,
-                        x->d_phi.p, nb, &zero, x->d_X.p + sp * plane, nb) != rocblas_status_success) {
+      // X (np x nb, column-major) = phi (np x nb) D_b (nb x nb)
+      if (rocblas_dgemm(x->blas, rocblas_operation_none, rocblas_operation_none, np, nb, nb, &one, x->d_phi.p, np,
+                        x->d_Db.p, nb, &zero, x->d_X.p + sp * plane, np) != rocblas_status_success) {
         err("oqph_vxc: rocblas_dgemm failed");
         return 2;
       }
     }
     PointArgs pa{x->d_pts.p + 4 * b.p0, np, nb, nfocks, x->f.gga, x->d_phi.p, x->d_dphi.p, x->d_X.p, x->d_Y.p,
                  x->d_acc.p, x->f};
-    hipLaunchKernelGGL(k_xc_point, dim3(nblk(np, 4)), dim3(256), 0, st, pa);
+    hipLaunchKernelGGL(k_xc_point, dim3(nblk(np, 64)), dim3(64 * XC_SLICES), 0, st, pa);
+    // V_b = phi^T Y, V_b(m, n) = sum_p phi(p, m) Y(p, n), as partial sums over
+    // KCHUNK-point slices (one strided-batched GEMM plus the remainder): the
+    // nb x nb result alone gives too few tiles to fill the device
+    const int nfull = np / KCHUNK, rem = np % KCHUNK, nparts = nfull + (rem > 0 ? 1 : 0);
+    const long long nb2 = (long long)nb * nb;
     for (int sp = 0; sp < nfocks; ++sp) {
-      // V_b (nb x nb) = sum_p phi[p][m] Y[p][n]
-      if (rocblas_dgemm(x->blas, rocblas_operation_none, rocblas_operation_transpose, nb, nb, np, &one, x->d_phi.p, nb,
-                        x->d_Y.p + sp * plane, nb, &zero, x->d_Vb.p, nb) != rocblas_status_success) {
+      const double* Ys = x->d_Y.p + sp * plane;
+      if (nfull > 0 &&
+          rocblas_dgemm_strided_batched(x->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, KCHUNK,
+                                        &one, x->d_phi.p, np, KCHUNK, Ys, np, KCHUNK, &zero, x->d_Vb.p, nb, nb2,
+                                        nfull) != rocblas_status_success) {
+        err("oqph_vxc: rocblas_dgemm_strided_batched failed");
+        return 2;
+      }
+      if (rem > 0 && rocblas_dgemm(x->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, rem, &one,
+                                   x->d_phi.p + (size_t)nfull * KCHUNK, np, Ys + (size_t)nfull * KCHUNK, np, &zero,
+                                   x->d_Vb.p + nfull * nb2, nb) != rocblas_status_success) {
         err("oqph_vxc: rocblas_dgemm failed");
         return 2;
       }
-      hipLaunchKernelGGL(k_scatter, dim3(nblk((long long)nb * nb, 256)), dim3(256), 0, st, x->d_Vb.p, nb,
+      hipLaunchKernelGGL(k_scatter, dim3(nblk(nb2, 256)), dim3(256), 0, st, x->d_Vb.p, nb, nparts,
                          x->d_idx.p + b.ao0, nbf, x->d_V.p + sp * nn);
     }
   }
