@@ -191,11 +191,16 @@ def xc_leg(steps, name="b3lyp5"):
     for _ in range(steps):
         V, exc, nel = fb.vxc(dp, 1)
     dt = (time.perf_counter() - t0) / steps
+    st = fb.xc_grid_stats()
+    gemm_flops = 4.0 * st["sum_np_nb2"]
     log(f"[bench] Vxc {name}: {dt * 1e3:.1f} ms/build over {npts} points (grid build {tg * 1e3:.0f} ms), "
-        f"N_elec {nel:.6f}")
+        f"N_elec {nel:.6f}, {st['batches']} batches, GEMM model {gemm_flops / 1e12:.2f} TFLOP")
     return {"workload": "h2o64_ccpvdz", "functional": name, "nbf": bas.nbf, "grid_points": npts,
             "grid": "96 x (15 x 30), SSF", "ms_per_build": round(dt * 1e3, 2), "grid_build_ms": round(tg * 1e3, 1),
-            "points_per_s": round(npts / dt, 1), "E_xc": exc, "N_elec": nel, "N_expected": 2 * nocc}
+            "points_per_s": round(npts / dt, 1), "E_xc": exc, "N_elec": nel, "N_expected": 2 * nocc,
+            "batches": st["batches"], "mean_basis_functions_per_point": round(st["sum_np_nb"] / npts, 1),
+            "gemm_model_tflop": round(gemm_flops / 1e12, 3),
+            "gemm_model_tflops_over_build": round(gemm_flops / dt / 1e12, 2)}
 
 
 def sigma_leg(steps):

@@ -149,6 +149,11 @@ int oqph_vxc(oqph_session *s, const double *d, int nfocks, double *fxc, double *
 /* The session's XC grid: returns the point count (negative on error) and
  * copies up to cap points as (x, y, z, w) into pts (may be NULL). */
 long long oqph_xc_grid(oqph_session *s, double *pts, long long cap);
+/* Work of one Vxc build on the session's grid: out[0] points, [1] batches,
+ * [2] sum over batches of points x basis functions, [3] of points x basis
+ * functions^2 (the two GEMMs per spin take 4 out[3] flops), [4] the largest
+ * batch's basis-function count. */
+int oqph_xc_grid_stats(oqph_session *s, long long *out);
 
 /* Make s the session used by routec_fock_jk / routec_grad2. */
 int oqph_set_default_session(oqph_session *s);

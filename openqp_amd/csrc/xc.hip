@@ -554,7 +554,8 @@ __global__ void k_scatter(const double* Vb, int nb, int nparts, const int* idx, 
   V[(size_t)idx[m] * nbf + idx[n]] += v;
 }
 
-__global__ void k_pack(const double* full, int nbf, double* pk) {
+// packed lower triangle of V0 + V1 (V1 may be null)
+__global__ void k_pack(const double* full, const double* full1, int nbf, double* pk) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long ntri = (long long)nbf * (nbf + 1) / 2;
   if (t >= ntri) return;
@@ -562,7 +563,7 @@ __global__ void k_pack(const double* full, int nbf, double* pk) {
   while ((long long)i * (i + 1) / 2 > t) --i;
   while ((long long)(i + 1) * (i + 2) / 2 <= t) ++i;
   const int j = (int)(t - (long long)i * (i + 1) / 2);
-  pk[t] = full[(size_t)i * nbf + j];
+  pk[t] = full[(size_t)i * nbf + j] + (full1 ? full1[(size_t)i * nbf + j] : 0.0);
 }
 
 inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -597,6 +598,26 @@ struct Batch {
   int npts, sh0, nsh, ao0, nb;
 };
 
+// a stream with its own batch buffers, GEMM handle and Vxc accumulator;
+// batches alternate between the lanes so that the GEMMs of one overlap the
+// memory-bound kernels of the other
+constexpr int XC_LANES = 2;
+struct Lane {
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  rocblas_handle blas = nullptr;
+  Buf<double> phi, dphi, X, Y, Db, Vb, V;
+  void release() {
+    for (auto* b : {&phi, &dphi, &X, &Y, &Db, &Vb, &V}) b->release();
+    if (blas) rocblas_destroy_handle(blas);
+    if (done) hipEventDestroy(done);
+    if (st) hipStreamDestroy(st);
+    blas = nullptr;
+    done = nullptr;
+    st = nullptr;
+  }
+};
+
 struct XCState {
   XCFunc f{};
   bool configured = false;
@@ -608,17 +629,17 @@ struct XCState {
   std::vector<double> grid;  // kept points (x, y, z, w), atom-major
   std::vector<Batch> batches;
   int max_nb = 0, max_pts = 0;
-  Buf<double> d_pts, d_xf, d_phi, d_dphi, d_X, d_Y, d_Db, d_Vb, d_D, d_V, d_pk, d_acc;
+  Buf<double> d_pts, d_xf, d_D, d_pk, d_acc;
   Buf<int> d_shl, d_col, d_idx, d_am, d_harm, d_atom, d_nprim, d_poff;
   Buf<double> d_ex, d_cc, d_xyz;
-  rocblas_handle blas = nullptr;
+  Lane lane[XC_LANES];
+  hipEvent_t ready = nullptr;
   void release() {
-    for (auto* b : {&d_pts, &d_xf, &d_phi, &d_dphi, &d_X, &d_Y, &d_Db, &d_Vb, &d_D, &d_V, &d_pk, &d_acc, &d_ex,
-                    &d_cc, &d_xyz})
-      b->release();
+    for (auto* b : {&d_pts, &d_xf, &d_D, &d_pk, &d_acc, &d_ex, &d_cc, &d_xyz}) b->release();
     for (auto* b : {&d_shl, &d_col, &d_idx, &d_am, &d_harm, &d_atom, &d_nprim, &d_poff}) b->release();
-    if (blas) rocblas_destroy_handle(blas);
-    blas = nullptr;
+    for (auto& l : lane) l.release();
+    if (ready) hipEventDestroy(ready);
+    ready = nullptr;
     grid_ready = false;
   }
 };
@@ -822,10 +843,11 @@ int build_grid(XCState* x, const oqph::SessionView& v) {
        upload(x->d_idx, idx.data(), idx.size(), st) && upload(x->d_am, v.am, (size_t)v.nsh, st) &&
        upload(x->d_harm, harm.data(), harm.size(), st) && upload(x->d_atom, v.atom, (size_t)v.nsh, st) &&
        upload(x->d_nprim, v.nprim, (size_t)v.nsh, st) && upload(x->d_poff, v.poff, (size_t)v.nsh, st) &&
-       upload(x->d_ex, v.ex, (size_t)nprim_tot, st) && upload(x->d_cc, v.cc, (size_t)nprim_tot, st) &&
-       x->d_phi.ensure(mp * mb) && x->d_dphi.ensure(3 * mp * mb) && x->d_X.ensure(2 * mp * mb) &&
-       x->d_Y.ensure(2 * mp * mb) && x->d_Db.ensure(mb * mb) && x->d_Vb.ensure((mp / KCHUNK + 1) * mb * mb) &&
-       hipStreamSynchronize(st) == hipSuccess;
+       upload(x->d_ex, v.ex, (size_t)nprim_tot, st) && upload(x->d_cc, v.cc, (size_t)nprim_tot, st);
+  for (auto& l : x->lane)
+    ok = ok && l.phi.ensure(mp * mb) && l.dphi.ensure(3 * mp * mb) && l.X.ensure(2 * mp * mb) &&
+         l.Y.ensure(2 * mp * mb) && l.Db.ensure(mb * mb) && l.Vb.ensure((mp / KCHUNK + 1) * mb * mb);
+  ok = ok && hipStreamSynchronize(st) == hipSuccess;
   if (!ok) return 3;
   x->grid_ready = true;
   return 0;
@@ -878,72 +900,96 @@ int vxc_run(oqph_session* s, const double* d, int nfocks, double* fxc, double* e
   hipStream_t st = v.stream;
   int rc = ensure_grid(x, v);
   if (rc) return rc;
-  if (!x->blas) {
-    if (rocblas_create_handle(&x->blas) != rocblas_status_success) {
-      x->blas = nullptr;
+  const int nbf = v.nbf;
+  const size_t ntri = (size_t)nbf * (nbf + 1) / 2, nn = (size_t)nbf * nbf;
+  if (!x->ready && hipEventCreateWithFlags(&x->ready, hipEventDisableTiming) != hipSuccess) {
+    x->ready = nullptr;
+    err("oqph_vxc: hipEventCreate failed");
+    return 2;
+  }
+  for (auto& l : x->lane) {
+    if (!l.st && hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking) != hipSuccess) {
+      l.st = nullptr;
+      err("oqph_vxc: hipStreamCreate failed");
+      return 2;
+    }
+    if (!l.done && hipEventCreateWithFlags(&l.done, hipEventDisableTiming) != hipSuccess) {
+      l.done = nullptr;
+      err("oqph_vxc: hipEventCreate failed");
+      return 2;
+    }
+    if (!l.blas && rocblas_create_handle(&l.blas) != rocblas_status_success) {
+      l.blas = nullptr;
       err("oqph_vxc: rocblas_create_handle failed");
       return 2;
     }
+    rocblas_set_stream(l.blas, l.st);
+    if (!l.V.ensure(nn * nfocks)) return 3;
   }
-  rocblas_set_stream(x->blas, st);
-  const int nbf = v.nbf;
-  const size_t ntri = (size_t)nbf * (nbf + 1) / 2, nn = (size_t)nbf * nbf;
-  if (!upload(x->d_pk, d, ntri * nfocks, st) || !x->d_D.ensure(nn * nfocks) || !x->d_V.ensure(nn * nfocks) ||
-      !x->d_acc.ensure(2))
-    return 3;
+  if (!upload(x->d_pk, d, ntri * nfocks, st) || !x->d_D.ensure(nn * nfocks) || !x->d_acc.ensure(2)) return 3;
   for (int sp = 0; sp < nfocks; ++sp)
     hipLaunchKernelGGL(k_unpack, dim3(nblk((long long)nn, 256)), dim3(256), 0, st, x->d_pk.p + sp * ntri, nbf,
                        x->d_D.p + sp * nn);
-  hipMemsetAsync(x->d_V.p, 0, nn * nfocks * sizeof(double), st);
   hipMemsetAsync(x->d_acc.p, 0, 2 * sizeof(double), st);
+  hipEventRecord(x->ready, st);
+  for (auto& l : x->lane) {
+    hipStreamWaitEvent(l.st, x->ready, 0);
+    hipMemsetAsync(l.V.p, 0, nn * nfocks * sizeof(double), l.st);
+  }
   const double one = 1.0, zero = 0.0;
-  for (const Batch& b : x->batches) {
+  for (size_t ib = 0; ib < x->batches.size(); ++ib) {
+    const Batch& b = x->batches[ib];
+    Lane& L = x->lane[ib % XC_LANES];
+    hipStream_t ls = L.st;
     const int np = b.npts, nb = b.nb;
     AOArgs aa{x->d_pts.p + 4 * b.p0, np, b.nsh, nb, x->d_shl.p + b.sh0, x->d_col.p + b.sh0, x->d_am.p,
               x->d_harm.p, x->d_atom.p, x->d_nprim.p, x->d_poff.p, x->d_ex.p, x->d_cc.p, x->d_xyz.p, x->d_xf.p,
-              x->d_phi.p, x->d_dphi.p, x->f.gga};
-    hipLaunchKernelGGL(k_ao, dim3(nblk((long long)np * b.nsh, 256)), dim3(256), 0, st, aa);
+              L.phi.p, L.dphi.p, x->f.gga};
+    hipLaunchKernelGGL(k_ao, dim3(nblk((long long)np * b.nsh, 256)), dim3(256), 0, ls, aa);
     const size_t plane = (size_t)np * nb;
     for (int sp = 0; sp < nfocks; ++sp) {
-      hipLaunchKernelGGL(k_gather, dim3(nblk((long long)nb * nb, 256)), dim3(256), 0, st, x->d_D.p + sp * nn, nbf,
-                         x->d_idx.p + b.ao0, nb, x->d_Db.p);
+      hipLaunchKernelGGL(k_gather, dim3(nblk((long long)nb * nb, 256)), dim3(256), 0, ls, x->d_D.p + sp * nn, nbf,
+                         x->d_idx.p + b.ao0, nb, L.Db.p);
       // X (np x nb, column-major) = phi (np x nb) D_b (nb x nb)
-      if (rocblas_dgemm(x->blas, rocblas_operation_none, rocblas_operation_none, np, nb, nb, &one, x->d_phi.p, np,
-                        x->d_Db.p, nb, &zero, x->d_X.p + sp * plane, np) != rocblas_status_success) {
+      if (rocblas_dgemm(L.blas, rocblas_operation_none, rocblas_operation_none, np, nb, nb, &one, L.phi.p, np, L.Db.p,
+                        nb, &zero, L.X.p + sp * plane, np) != rocblas_status_success) {
         err("oqph_vxc: rocblas_dgemm failed");
         return 2;
       }
     }
-    PointArgs pa{x->d_pts.p + 4 * b.p0, np, nb, nfocks, x->f.gga, x->d_phi.p, x->d_dphi.p, x->d_X.p, x->d_Y.p,
-                 x->d_acc.p, x->f};
-    hipLaunchKernelGGL(k_xc_point, dim3(nblk(np, 64)), dim3(64 * XC_SLICES), 0, st, pa);
+    PointArgs pa{x->d_pts.p + 4 * b.p0, np, nb, nfocks, x->f.gga, L.phi.p, L.dphi.p, L.X.p, L.Y.p, x->d_acc.p, x->f};
+    hipLaunchKernelGGL(k_xc_point, dim3(nblk(np, 64)), dim3(64 * XC_SLICES), 0, ls, pa);
     // V_b = phi^T Y, V_b(m, n) = sum_p phi(p, m) Y(p, n), as partial sums over
     // KCHUNK-point slices (one strided-batched GEMM plus the remainder): the
     // nb x nb result alone gives too few tiles to fill the device
     const int nfull = np / KCHUNK, rem = np % KCHUNK, nparts = nfull + (rem > 0 ? 1 : 0);
     const long long nb2 = (long long)nb * nb;
     for (int sp = 0; sp < nfocks; ++sp) {
-      const double* Ys = x->d_Y.p + sp * plane;
+      const double* Ys = L.Y.p + sp * plane;
       if (nfull > 0 &&
-          rocblas_dgemm_strided_batched(x->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, KCHUNK,
-                                        &one, x->d_phi.p, np, KCHUNK, Ys, np, KCHUNK, &zero, x->d_Vb.p, nb, nb2,
+          rocblas_dgemm_strided_batched(L.blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, KCHUNK,
+                                        &one, L.phi.p, np, KCHUNK, Ys, np, KCHUNK, &zero, L.Vb.p, nb, nb2,
                                         nfull) != rocblas_status_success) {
         err("oqph_vxc: rocblas_dgemm_strided_batched failed");
         return 2;
       }
-      if (rem > 0 && rocblas_dgemm(x->blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, rem, &one,
-                                   x->d_phi.p + (size_t)nfull * KCHUNK, np, Ys + (size_t)nfull * KCHUNK, np, &zero,
-                                   x->d_Vb.p + nfull * nb2, nb) != rocblas_status_success) {
+      if (rem > 0 && rocblas_dgemm(L.blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, rem, &one,
+                                   L.phi.p + (size_t)nfull * KCHUNK, np, Ys + (size_t)nfull * KCHUNK, np, &zero,
+                                   L.Vb.p + nfull * nb2, nb) != rocblas_status_success) {
         err("oqph_vxc: rocblas_dgemm failed");
         return 2;
       }
-      hipLaunchKernelGGL(k_scatter, dim3(nblk(nb2, 256)), dim3(256), 0, st, x->d_Vb.p, nb, nparts,
-                         x->d_idx.p + b.ao0, nbf, x->d_V.p + sp * nn);
+      hipLaunchKernelGGL(k_scatter, dim3(nblk(nb2, 256)), dim3(256), 0, ls, L.Vb.p, nb, nparts, x->d_idx.p + b.ao0,
+                         nbf, L.V.p + sp * nn);
     }
   }
+  for (auto& l : x->lane) {
+    hipEventRecord(l.done, l.st);
+    hipStreamWaitEvent(st, l.done, 0);
+  }
   for (int sp = 0; sp < nfocks; ++sp)
-    hipLaunchKernelGGL(k_pack, dim3(nblk((long long)ntri, 256)), dim3(256), 0, st, x->d_V.p + sp * nn, nbf,
-                       x->d_pk.p + sp * ntri);
+    hipLaunchKernelGGL(k_pack, dim3(nblk((long long)ntri, 256)), dim3(256), 0, st, x->lane[0].V.p + sp * nn,
+                       XC_LANES > 1 ? x->lane[1].V.p + sp * nn : nullptr, nbf, x->d_pk.p + sp * ntri);
   double acc[2];
   if (hipMemcpyAsync(fxc, x->d_pk.p, ntri * nfocks * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipMemcpyAsync(acc, x->d_acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1013,6 +1059,26 @@ extern "C" long long oqph_xc_grid(oqph_session* s, double* pts, long long cap) {
   const long long n = (long long)x->grid.size() / 4;
   if (pts) memcpy(pts, x->grid.data(), (size_t)std::min(n, cap) * 4 * sizeof(double));
   return n;
+}
+
+extern "C" int oqph_xc_grid_stats(oqph_session* s, long long* out) {
+  oqph::SessionView v;
+  if (!s || !out || !oqph::session_view(s, &v)) return 4;
+  XCState* x = state(s, false);
+  if (!x || !x->configured) return 1;
+  hipSetDevice(v.device);
+  if (ensure_grid(x, v)) return 2;
+  long long a = 0, b2 = 0;
+  for (const Batch& b : x->batches) {
+    a += (long long)b.npts * b.nb;
+    b2 += (long long)b.npts * b.nb * b.nb;
+  }
+  out[0] = (long long)x->grid.size() / 4;
+  out[1] = (long long)x->batches.size();
+  out[2] = a;
+  out[3] = b2;
+  out[4] = x->max_nb;
+  return 0;
 }
 
 extern "C" void routec_vxc(const double* d, double* fxc, const int* nbf, const int* nfocks, double* eexc,

@@ -8,12 +8,13 @@ module oqph_bridge
   use, intrinsic :: iso_c_binding
   implicit none
   private
-  public :: routec_fock_jk, routec_grad2
+  public :: routec_fock_jk, routec_grad2, routec_vxc
   public :: oqph_session_create, oqph_session_destroy, oqph_session_set_basis, &
             oqph_session_set_geometry, oqph_session_set_thresholds, oqph_set_default_session, &
             oqph_session_set_precision, oqph_session_set_share, oqph_session_prepare, &
             oqph_fock_jk, oqph_fock_jk_inc, oqph_fock_jk_cam, oqph_grad2, &
-            oqph_session_set_grad_precision, oqph_mrsf_int2, oqph_grad2_mrsf
+            oqph_session_set_grad_precision, oqph_mrsf_int2, oqph_grad2_mrsf, &
+            oqph_session_set_xc, oqph_vxc
   public :: oqph_stats
 
   !> Build statistics (oqph_stats of include/oqph.h).
@@ -163,6 +164,29 @@ module oqph_bridge
       real(c_double), intent(inout) :: de(*)
       real(c_double), value :: hfscale, coulscale
       type(oqph_stats), intent(out) :: stats
+    end function
+    !> Semilocal Vxc (routec_bridge.F90:43-50), by reference like the seam.
+    subroutine routec_vxc(d, fxc, nbf, nfocks, eexc, totele, info) bind(C, name="routec_vxc")
+      import :: c_double, c_int
+      real(c_double), intent(in) :: d(*)
+      real(c_double), intent(inout) :: fxc(*)
+      integer(c_int), intent(in) :: nbf, nfocks
+      real(c_double), intent(inout) :: eexc, totele
+      integer(c_int), intent(out) :: info
+    end subroutine
+    integer(c_int) function oqph_session_set_xc(s, functional, nrad, ntheta) bind(C, name="oqph_session_set_xc")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: s
+      character(kind=c_char), intent(in) :: functional(*)  ! NUL-terminated
+      integer(c_int), value :: nrad, ntheta
+    end function
+    integer(c_int) function oqph_vxc(s, d, nfocks, fxc, exc, nel) bind(C, name="oqph_vxc")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: s
+      real(c_double), intent(in) :: d(*)
+      integer(c_int), value :: nfocks
+      real(c_double), intent(inout) :: fxc(*)
+      real(c_double), intent(out) :: exc, nel
     end function
   end interface
 end module oqph_bridge

@@ -33,7 +33,7 @@ EXPORTS = ["routec_fock_jk", "routec_grad2", "oqph_session_create", "oqph_sessio
            "oqph_fock_jk_noalloc", "oqph_last_counters", "oqph_session_set_grad_precision", "oqph_mrsf_int2",
            "routec_sig_init", "routec_sig_set_scale", "routec_sig_iter", "routec_sig_free", "oqph_grad2_mrsf",
            "routec_grad2_mrsf", "routec_vxc", "oqph_session_set_xc", "oqph_xc_functional_info", "oqph_vxc",
-           "oqph_xc_grid"]
+           "oqph_xc_grid", "oqph_xc_grid_stats"]
 
 
 class OqphError(RuntimeError):
@@ -111,6 +111,7 @@ def load(path: str | None = None):
     L.oqph_vxc.argtypes = [vp, _pd, C.c_int, _pd, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.oqph_xc_grid.argtypes = [vp, C.c_void_p, C.c_longlong]
     L.oqph_xc_grid.restype = C.c_longlong
+    L.oqph_xc_grid_stats.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
     _lib = L
     return L
 
@@ -258,6 +259,13 @@ class FockBuilder:
         out = np.zeros((n, 4))
         L.oqph_xc_grid(self._s, out.ctypes.data_as(C.c_void_p), n)
         return out
+
+    def xc_grid_stats(self) -> dict:
+        """Points, batches and GEMM work (sum of points x basis functions^k) of a Vxc build."""
+        out = np.zeros(5, dtype=np.int64)
+        _check(load().oqph_xc_grid_stats(self._s, out), "xc_grid_stats")
+        return {"points": int(out[0]), "batches": int(out[1]), "sum_np_nb": int(out[2]), "sum_np_nb2": int(out[3]),
+                "max_nb": int(out[4])}
 
     def fock_jk_uhf(self, d_alpha, d_beta, se: float = 1.0, sc: float = 1.0):
         """Packed UHF/ROHF (F_alpha, F_beta) = sc J[Da + Db] - se K[Ds] (fock_jk with nfocks = 2)."""
