@@ -15,6 +15,11 @@ Restates the semantics of OpenQP's scf_driver for closed shells
   ill-conditioned (reciprocal condition number below 1e-12), so the
   extrapolation coefficients stay meaningful near convergence.
 
+Kohn-Sham (``functional=``, closed shell): F = H + G_se[D] + Vxc[D] with the
+exact-exchange scale se of the functional and Vxc from the builder's XC
+build (the routec_vxc work), E = 0.5 (Tr D (H + G) + Tr D H) + E_xc + E_nuc,
+as calc_jk_xc assembles them (source/scf_addons.F90:1972-2145).
+
 Initial guess: OpenQP's extended-Hueckel guess (``guess="huckel"``,
 source/modules/guess_huckel.F90:33; openqp_amd/guess.py) or the core
 Hamiltonian (``guess="core"``).
@@ -41,14 +46,23 @@ class SCFResult:
     orbital_energies: np.ndarray
     fock_ms: list = field(default_factory=list)
     history: list = field(default_factory=list)
+    exc: float = 0.0             # E_xc (Kohn-Sham)
+    nelec_grid: float = 0.0      # the XC grid's electron count (Kohn-Sham)
 
 
 class RHF:
     def __init__(self, bas: B.Basis, builder: FockBuilder | None = None, nocc: int | None = None,
                  conv: float = 1e-6, econv: float = 1e-8, max_iter: int = 100, incremental: bool = True,
-                 diis_size: int = 8, guess: str = "huckel"):
+                 diis_size: int = 8, guess: str = "huckel", functional: str | None = None,
+                 grid: tuple = (96, 15)):
         self.bas = bas
         self.builder = builder or FockBuilder(bas)
+        self.functional = functional
+        self.hfex = 1.0
+        if functional is not None:
+            from .lib import xc_functional_info
+            self.hfex = xc_functional_info(functional)[0]
+            self.builder.set_xc(functional, *grid)
         self.nocc = nocc if nocc is not None else int(round(bas.zn.sum())) // 2
         self.conv, self.econv, self.max_iter = conv, econv, max_iter
         self.incremental = incremental
@@ -119,12 +133,18 @@ class RHF:
                 if it > 1 and (it % 10 == 0 or diis_err < 1.0e-4):
                     d_old[:] = 0.0
                     f_old[:] = 0.0
-                g = self.builder.fock_jk_inc(dp, d_old, f_old)
+                g = self.builder.fock_jk_inc(dp, d_old, f_old, se=self.hfex)
             else:
-                g = self.builder.fock_jk(dp)
+                g = self.builder.fock_jk(dp, se=self.hfex)
+            G = B.unpack_tri(g, nbf)
+            E = 0.5 * (np.sum(D * G) + 2.0 * np.sum(D * self.H)) + self.enuc
+            F = self.H + G
+            if self.functional is not None:
+                vx, exc, nel = self.builder.vxc(dp, 1)
+                F = F + B.unpack_tri(vx[0], nbf)
+                E += exc
+                res.exc, res.nelec_grid = exc, nel
             res.fock_ms.append((time.perf_counter() - t0) * 1e3)
-            F = self.H + B.unpack_tri(g, nbf)
-            E = 0.5 * (np.sum(D * F) + np.sum(D * self.H)) + self.enuc
             FDS = F @ D @ self.S  # S D F is its transpose (F, D, S symmetric)
             err = self.X.T @ (FDS - FDS.T) @ self.X
             diis_err = float(np.abs(err).max())

@@ -143,3 +143,14 @@ def test_ks_scf_b3lyp5_on_device_matches_reference():
         D = 2 * C_[:, :5] @ C_[:, :5].T
     _say(f"[ks scf] E {E!r} after {it} iterations, dE {E + 76.37092191486505:.2e}")
     assert abs(E - (-76.37092191486505)) < 5e-6
+
+
+def test_host_kohn_sham_scf_on_device():
+    """openqp_amd.scf.RHF(functional="b3lyp5") on the device builder (Hueckel
+    guess, incremental J/K with exchange 0.2, device Vxc) reaches the
+    reference's B3LYP5/6-31G* H2O energy to the grid error."""
+    from openqp_amd.scf import RHF
+    bas = small_system("6-31g(d)")
+    r = RHF(bas, nocc=5, conv=1e-7, econv=1e-10, functional="b3lyp5").run()
+    _say(f"[host ks] E {r.energy!r} in {r.iterations} iterations")
+    assert r.converged and abs(r.energy - (-76.37092191486505)) < 5e-6
