@@ -8,7 +8,7 @@
 // alpha = 0.6, per-element xi) times a Gauss-Legendre (cos theta) x uniform
 // (phi) product angular grid, Stratmann-Scuseria-Frisch partition weights
 // (a = 0.64) computed on the device, points with weight <= 1e-20 dropped.
-// The kept points are sorted into cubic boxes (5 bohr) and cut into batches
+// The kept points are sorted into cubic boxes (4 bohr) and cut into batches
 // of at most 32768 points; a batch sees only the shells whose most diffuse
 // primitive reaches its bounding sphere (|c| exp(-alpha r^2) >= 1e-12).
 // Per-batch matrices are stored point-fastest (column-major npts x nb), so
@@ -49,7 +49,7 @@ constexpr double PI = 3.14159265358979323846;
 constexpr double RHO_CUT = 1e-10;    // points below this total density contribute nothing
 constexpr double RHO_FLOOR = 1e-30;  // spin densities are floored here
 constexpr double SSF_A = 0.64;
-constexpr double BOX = 5.0;          // bohr
+constexpr double BOX = 4.0;          // bohr
 constexpr int MAX_BATCH = 32768;
 constexpr int KCHUNK = 4096;         // points per partial V_b (split of the long GEMM dimension)
 constexpr double AO_EPS = 1e-12;
@@ -623,7 +623,8 @@ struct XCState {
   bool configured = false;
   int nrad = 96, ntheta = 15;
   // cache key of the grid
-  std::vector<double> key_xyz, key_ex;
+  std::vector<double> key_xyz, key_ex, key_cc;
+  std::vector<int> key_am;
   int key_nbf = -1, key_nrad = 0, key_ntheta = 0;
   bool grid_ready = false;
   std::vector<double> grid;  // kept points (x, y, z, w), atom-major
@@ -859,7 +860,9 @@ bool grid_current(XCState* x, const oqph::SessionView& v) {
     return false;
   int np = 0;
   for (int s = 0; s < v.nsh; ++s) np = std::max(np, v.poff[s] + v.nprim[s]);
-  return x->key_ex.size() == (size_t)np && std::equal(x->key_ex.begin(), x->key_ex.end(), v.ex);
+  return x->key_ex.size() == (size_t)np && std::equal(x->key_ex.begin(), x->key_ex.end(), v.ex) &&
+         std::equal(x->key_cc.begin(), x->key_cc.end(), v.cc) && x->key_am.size() == (size_t)v.nsh &&
+         std::equal(x->key_am.begin(), x->key_am.end(), v.am);
 }
 
 int ensure_grid(XCState* x, const oqph::SessionView& v) {
@@ -874,6 +877,8 @@ int ensure_grid(XCState* x, const oqph::SessionView& v) {
   int np = 0;
   for (int s = 0; s < v.nsh; ++s) np = std::max(np, v.poff[s] + v.nprim[s]);
   x->key_ex.assign(v.ex, v.ex + np);
+  x->key_cc.assign(v.cc, v.cc + np);
+  x->key_am.assign(v.am, v.am + v.nsh);
   return 0;
 }
 
