@@ -780,9 +780,12 @@ int build_grid(XCState* x, const oqph::SessionView& v) {
     for (int k = 0; k < 3; ++k) b[k] = std::min(nbx[k] - 1, (long long)((x->grid[4 * t + k] - lo[k]) / BOX));
     key[t] = (b[0] * nbx[1] + b[1]) * nbx[2] + b[2];
   }
-  std::vector<long long> order(n);
-  for (long long t = 0; t < n; ++t) order[t] = t;
-  std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) { return key[a] < key[b]; });
+  // counting sort by box (stable: atom-major order within a box)
+  const long long nkeys = nbx[0] * nbx[1] * nbx[2];
+  std::vector<long long> start(nkeys + 1, 0), order(n);
+  for (long long t = 0; t < n; ++t) ++start[key[t] + 1];
+  for (long long k = 0; k < nkeys; ++k) start[k + 1] += start[k];
+  for (long long t = 0; t < n; ++t) order[start[key[t]]++] = t;
   std::vector<double> sorted((size_t)n * 4);
   for (long long t = 0; t < n; ++t) memcpy(&sorted[4 * t], &x->grid[4 * order[t]], 4 * sizeof(double));
   // batches and their shells

@@ -170,3 +170,82 @@ class RHF:
         res.energy, res.iterations = E, self.max_iter
         res.density, res.fock = D, F
         return res
+
+
+class UHF(RHF):
+    """Unrestricted host loop (UHF, or UKS with ``functional=``): the alpha and
+    beta Fock matrices from one nfocks = 2 build (int2_urohf_data_t semantics,
+    source/integrals/int2.F90:1488-1578) plus the alpha/beta Vxc,
+    E = 0.5 sum_s (Tr D_s (H + G_s) + Tr D_s H) + E_xc + E_nuc, Pulay DIIS on
+    the stacked alpha and beta commutators.  Full builds every iteration.
+
+    Starts from orbitals C0 when given (alpha and beta share them), else from
+    the guess; ``nalpha``/``nbeta`` default to the multiplicity's occupations."""
+
+    def __init__(self, bas: B.Basis, builder: FockBuilder | None = None, mult: int = 1,
+                 nalpha: int | None = None, nbeta: int | None = None, **kw):
+        nel = int(round(bas.zn.sum()))
+        na = nalpha if nalpha is not None else (nel + mult - 1) // 2
+        nb = nbeta if nbeta is not None else nel - na
+        super().__init__(bas, builder, nocc=nb, **kw)
+        self.nalpha, self.nbeta = na, nb
+
+    def _orbitals(self, F):
+        e, Co = np.linalg.eigh(self.X.T @ F @ self.X)
+        return self.X @ Co, e
+
+    def run(self, C0: np.ndarray | None = None) -> SCFResult:
+        nbf = self.bas.nbf
+        if C0 is None:
+            if self.guess == "huckel":
+                from .guess import huckel_guess
+                C0, _ = huckel_guess(self.bas, self.nalpha)
+            else:
+                C0, _ = self._orbitals(self.H)
+        Ds = [C0[:, :n] @ C0[:, :n].T for n in (self.nalpha, self.nbeta)]
+        errs, focks = [], []
+        e_prev = 0.0
+        res = SCFResult(0.0, False, 0, Ds, self.H, None, None)
+        for it in range(1, self.max_iter + 1):
+            t0 = time.perf_counter()
+            dps = [B.pack_tri(D) for D in Ds]
+            ga, gb = self.builder.fock_jk_uhf(dps[0], dps[1], se=self.hfex)
+            G = [B.unpack_tri(ga, nbf), B.unpack_tri(gb, nbf)]
+            E = sum(0.5 * (np.sum(D * g) + 2.0 * np.sum(D * self.H)) for D, g in zip(Ds, G)) + self.enuc
+            Fs = [self.H + g for g in G]
+            if self.functional is not None:
+                vx, exc, nel = self.builder.vxc(np.concatenate(dps), 2)
+                Fs = [F + B.unpack_tri(v, nbf) for F, v in zip(Fs, vx)]
+                E += exc
+                res.exc, res.nelec_grid = exc, nel
+            res.fock_ms.append((time.perf_counter() - t0) * 1e3)
+            err = []
+            for F, D in zip(Fs, Ds):
+                FDS = F @ D @ self.S
+                err.append(self.X.T @ (FDS - FDS.T) @ self.X)
+            err = np.stack(err)
+            diis_err = float(np.abs(err).max())
+            res.history.append((it, E, diis_err))
+            if it > 1 and diis_err < self.conv and abs(E - e_prev) < self.econv:
+                res.energy, res.converged, res.iterations = E, True, it
+                res.density, res.fock = Ds, Fs
+                res.orbitals, res.orbital_energies = zip(*[self._orbitals(F) for F in Fs])
+                return res
+            e_prev = E
+            focks.append(Fs)
+            errs.append(err)
+            if len(focks) > self.diis_size:
+                focks.pop(0)
+                errs.pop(0)
+            if len(focks) > 1:
+                c = self.diis_coefficients(errs)
+                Fx = [sum(ci * f[s] for ci, f in zip(c, focks) if ci != 0.0) for s in range(2)]
+            else:
+                Fx = Fs
+            Ds = []
+            for F, n in zip(Fx, (self.nalpha, self.nbeta)):
+                C, _ = self._orbitals(F)
+                Ds.append(C[:, :n] @ C[:, :n].T)
+        res.energy, res.iterations = E, self.max_iter
+        res.density, res.fock = Ds, Fs
+        return res

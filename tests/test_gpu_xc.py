@@ -154,3 +154,21 @@ def test_host_kohn_sham_scf_on_device():
     r = RHF(bas, nocc=5, conv=1e-7, econv=1e-10, functional="b3lyp5").run()
     _say(f"[host ks] E {r.energy!r} in {r.iterations} iterations")
     assert r.converged and abs(r.energy - (-76.37092191486505)) < 5e-6
+
+
+def test_host_uks_triplet_on_device():
+    """openqp_amd.scf.UHF(mult=3, functional="b3lypv1r") on the device builder
+    (single-pass alpha/beta J/K, alpha/beta Vxc) from the closed-shell KS
+    orbitals: the reference's examples/TRAH/H2O_UHF-DFT_GRAD_TRAH.json energy
+    -76.1477466020111 Eh to the grid error."""
+    from openqp_amd.lib import FockBuilder
+    from openqp_amd.scf import RHF, UHF
+    Z = np.array([8, 1, 1], dtype=np.int32)
+    xyz = np.array([[0.0, 0.0, 0.0], [0.0, -1.430522676341708, 1.109269235155327],
+                    [0.0, 1.430522676341708, 1.109269235155327]])
+    bas = B.build_basis("cc-pvdz", Z, xyz)
+    fb = FockBuilder(bas)
+    r0 = RHF(bas, fb, nocc=5, conv=1e-5, functional="b3lypv1r").run()
+    r = UHF(bas, fb, mult=3, conv=1e-7, econv=1e-10, functional="b3lypv1r").run(r0.orbitals)
+    _say(f"[host uks] E {r.energy!r} in {r.iterations} iterations")
+    assert r.converged and abs(r.energy - (-76.1477466020111)) < 5e-6

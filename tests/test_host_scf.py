@@ -31,6 +31,10 @@ class _CPUBuilder:
         d_old[:] = d
         return f
 
+    def fock_jk_uhf(self, da, db, se=1.0):
+        fa, fb, _ = self.O.fock_jk_uhf(self.bas, da, db, se=se, Q=self.Q)
+        return fa, fb
+
     def set_xc(self, name, nrad, ntheta):
         from oracle import xc as X
         self.xc, self.X = name, X
@@ -39,8 +43,10 @@ class _CPUBuilder:
         self.ao = X.ao_values(self.bas, pts, self.O)
 
     def vxc(self, dp, nfocks=1):
-        V, exc, nel = self.X.vxc(self.bas, B.unpack_tri(dp, self.bas.nbf), self.xc, self.grid, self.O, 1, self.ao)
-        return [B.pack_tri(V[0])], exc, nel
+        n = dp.size // nfocks
+        ds = [B.unpack_tri(dp[i * n:(i + 1) * n], self.bas.nbf) for i in range(nfocks)]
+        V, exc, nel = self.X.vxc(self.bas, ds[0] if nfocks == 1 else ds, self.xc, self.grid, self.O, nfocks, self.ao)
+        return [B.pack_tri(v) for v in V], exc, nel
 
 
 def test_huckel_orbitals_orthonormal_and_occupied_space(oracle):
@@ -126,3 +132,21 @@ def test_kohn_sham_b3lyp5_through_the_host_loop(oracle):
     assert r.converged
     assert abs(r.nelec_grid - 10.0) < 1e-5
     assert abs(r.energy - (-76.37092191486505)) < 5e-6
+
+
+def test_uks_triplet_through_the_host_loop(oracle):
+    """UKS triplet B3LYP (libxc's B3LYP, VWN RPA fit) / cc-pVDZ H2O of
+    examples/TRAH/H2O_UHF-DFT_GRAD_TRAH.json (-76.1477466020111 Eh) through the
+    unrestricted host loop, from the closed-shell Kohn-Sham orbitals (from the
+    core or Hueckel guess DIIS settles on an excited triplet), to the grid
+    error (5e-6; measured 5e-7); N_elec on the grid 10."""
+    from openqp_amd.scf import RHF, UHF
+    Z = np.array([8, 1, 1], dtype=np.int32)
+    xyz = np.array([[0.0, 0.0, 0.0], [0.0, -1.430522676341708, 1.109269235155327],
+                    [0.0, 1.430522676341708, 1.109269235155327]])
+    bas = B.build_basis("cc-pvdz", Z, xyz)
+    r0 = RHF(bas, _CPUBuilder(oracle, bas), nocc=5, conv=1e-5, functional="b3lypv1r").run()
+    r = UHF(bas, _CPUBuilder(oracle, bas), mult=3, conv=1e-7, econv=1e-10, functional="b3lypv1r").run(r0.orbitals)
+    assert r.converged
+    assert abs(r.nelec_grid - 10.0) < 1e-5
+    assert abs(r.energy - (-76.1477466020111)) < 5e-6, r.energy
