@@ -10,7 +10,7 @@
 // (a = 0.64) computed on the device, points with weight <= 1e-20 dropped.
 // The kept points are sorted into cubic boxes (4 bohr) and cut into batches
 // of at most 32768 points; a batch sees only the shells whose most diffuse
-// primitive reaches its bounding sphere (|c| exp(-alpha r^2) >= 1e-12).
+// primitive reaches its bounding box (|c| exp(-alpha r^2) >= 1e-12).
 // Per-batch matrices are stored point-fastest (column-major npts x nb), so
 // the basis-function and per-point kernels (one thread per point) coalesce.
 //
@@ -805,22 +805,23 @@ int build_grid(XCState* x, const oqph::SessionView& v) {
   for (long long t0 = 0; t0 < n;) {
     long long t1 = t0;
     while (t1 < n && key[order[t1]] == key[order[t0]] && t1 - t0 < MAX_BATCH) ++t1;
-    double c[3] = {0, 0, 0};
+    // the batch's bounding box; a shell is kept when its centre lies within
+    // its cutoff radius of the box
+    double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
     for (long long t = t0; t < t1; ++t)
-      for (int k = 0; k < 3; ++k) c[k] += sorted[4 * t + k];
-    for (int k = 0; k < 3; ++k) c[k] /= (double)(t1 - t0);
-    double rad2 = 0.0;
-    for (long long t = t0; t < t1; ++t) {
-      double d2 = 0.0;
-      for (int k = 0; k < 3; ++k) d2 += (sorted[4 * t + k] - c[k]) * (sorted[4 * t + k] - c[k]);
-      rad2 = std::max(rad2, d2);
-    }
-    const double R = sqrt(rad2);
+      for (int k = 0; k < 3; ++k) {
+        blo[k] = std::min(blo[k], sorted[4 * t + k]);
+        bhi[k] = std::max(bhi[k], sorted[4 * t + k]);
+      }
     Batch b{t0, (int)(t1 - t0), (int)shl.size(), 0, (int)idx.size(), 0};
     for (int s = 0; s < v.nsh; ++s) {
       const int A = v.atom[s];
-      const double dx = v.xyz[3 * A] - c[0], dy = v.xyz[3 * A + 1] - c[1], dz = v.xyz[3 * A + 2] - c[2];
-      if (sqrt(dx * dx + dy * dy + dz * dz) - R > rcut[s]) continue;
+      double d2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double a = v.xyz[3 * A + k], o = a < blo[k] ? blo[k] - a : (a > bhi[k] ? a - bhi[k] : 0.0);
+        d2 += o * o;
+      }
+      if (sqrt(d2) > rcut[s]) continue;
       const int l = v.am[s], no = (v.harm[s] && l >= 2) ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
       shl.push_back(s);
       col.push_back(b.nb);
