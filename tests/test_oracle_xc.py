@@ -167,3 +167,18 @@ def test_uks_b3lyp_triplet_energy_matches_reference(oracle):
     E, nel, _ = ks_scf(oracle, bas, UKS_Z, UKS_XYZ, "b3lypv1r", (6, 4), (pts, w), ao, False, C0=C)
     assert abs(nel - 10.0) < 1e-5
     assert abs(E - UKS_E) < 5e-6, E
+
+
+def test_rks_bhhlyp_hcn_energy_matches_reference(oracle):
+    """examples/XAS/delta-CHP-MRSF/HCN_DFT.json: BHHLYP / 6-31G* HCN,
+    E = -93.36417693628827 (0.5 B88 + LYP with half exact exchange); measured
+    1.5e-6 Eh from it."""
+    Z = np.array([6, 7, 1], dtype=np.int32)
+    xyz = np.array([[0.0, 0.0, -0.9561900807038921], [0.0, 0.0, 1.2421698940480084],
+                    [0.0, 0.0, -2.9580449946604572]])
+    bas = B.build_basis("6-31g(d)", Z, xyz)
+    pts, w, _ = X.molecular_grid(bas.zn, bas.xyz, 96, 15)
+    ao = X.ao_values(bas, pts, oracle)
+    E, nel, _ = ks_scf(oracle, bas, Z, xyz, "bhhlyp", (7, 7), (pts, w), ao, True)
+    assert abs(nel - 14.0) < 1e-5
+    assert abs(E - (-93.36417693628827)) < 5e-6, E
