@@ -954,6 +954,13 @@ int vxc_run(oqph_session* s, const double* d, int nfocks, double* fxc, double* e
     hipStreamWaitEvent(l.st, x->ready, 0);
     hipMemsetAsync(l.V.p, 0, nn * nfocks * sizeof(double), l.st);
   }
+  // on an error, let the work already queued on the lanes finish before the
+  // buffers can be reused
+  auto drain = [&](int code) {
+    for (auto& l : x->lane) hipStreamSynchronize(l.st);
+    hipStreamSynchronize(st);
+    return code;
+  };
   const double one = 1.0, zero = 0.0;
   for (size_t ib = 0; ib < x->batches.size(); ++ib) {
     const Batch& b = x->batches[ib];
@@ -972,7 +979,7 @@ int vxc_run(oqph_session* s, const double* d, int nfocks, double* fxc, double* e
       if (rocblas_dgemm(L.blas, rocblas_operation_none, rocblas_operation_none, np, nb, nb, &one, L.phi.p, np, L.Db.p,
                         nb, &zero, L.X.p + sp * plane, np) != rocblas_status_success) {
         err("oqph_vxc: rocblas_dgemm failed");
-        return 2;
+        return drain(2);
       }
     }
     PointArgs pa{x->d_pts.p + 4 * b.p0, np, nb, nfocks, x->f.gga, L.phi.p, L.dphi.p, L.X.p, L.Y.p, x->d_acc.p, x->f};
@@ -989,13 +996,13 @@ int vxc_run(oqph_session* s, const double* d, int nfocks, double* fxc, double* e
                                         &one, L.phi.p, np, KCHUNK, Ys, np, KCHUNK, &zero, L.Vb.p, nb, nb2,
                                         nfull) != rocblas_status_success) {
         err("oqph_vxc: rocblas_dgemm_strided_batched failed");
-        return 2;
+        return drain(2);
       }
       if (rem > 0 && rocblas_dgemm(L.blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, rem, &one,
                                    L.phi.p + (size_t)nfull * KCHUNK, np, Ys + (size_t)nfull * KCHUNK, np, &zero,
                                    L.Vb.p + nfull * nb2, nb) != rocblas_status_success) {
         err("oqph_vxc: rocblas_dgemm failed");
-        return 2;
+        return drain(2);
       }
       hipLaunchKernelGGL(k_scatter, dim3(nblk(nb2, 256)), dim3(256), 0, ls, L.Vb.p, nb, nparts, x->d_idx.p + b.ao0,
                          nbf, L.V.p + sp * nn);
