@@ -14,6 +14,15 @@ cutoff, FP64 accumulation) are both timed; ``value`` is the mixed rate when
 its energy error |dE2| = |0.5 Tr D (G_mixed - G_fp64)| is within 1 uEh,
 otherwise the FP64 rate.
 
+Further legs, each reported in its own field of the same JSON line and each
+switched off by its --no-* flag:
+- the 2e gradient on C20H42/6-31G(d) (FP64 and mixed);
+- a full RHF SCF of (H2O)32/6-31G(d);
+- the MRSF sigma and MRSF 2e gradient seams on (H2O)32/6-31G(d);
+- the semilocal Vxc build (B3LYP5) on the (H2O)64/cc-pVDZ workload above, with
+  ``--xc-only`` running that leg alone;
+- the f-shell leg on (H2O)16/cc-pVTZ.
+
 Prints one JSON line (rank 0, N = 1).
 """
 from __future__ import annotations
