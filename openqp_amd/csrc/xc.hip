@@ -52,7 +52,7 @@ constexpr double RHO_FLOOR = 1e-30;  // spin densities are floored here
 constexpr double SSF_A = 0.64;
 constexpr double BOX = 4.0;          // bohr
 constexpr int MAX_BATCH = 32768;
-constexpr int KCHUNK = 4096;         // points per partial V_b (split of the long GEMM dimension)
+constexpr int KCHUNK = 2048;         // points per partial V_b (split of the long GEMM dimension)
 constexpr double AO_EPS = 1e-12;
 
 void err(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
